@@ -1,0 +1,184 @@
+/*
+ * shape_atrophy.h -- C ABI of the MI355X (gfx950) forward shape-evolution
+ * engine.
+ *
+ * The reference exposes this path as four MATLAB MEX functions, called only
+ * by CUDA/deformation.m and CUDA/computeObjfcn.m:
+ *
+ *   deformation_2D_frame / deformation_3D_frame       (2D|3D_frame/deformation.cu:14)
+ *   computeObjfcn_2D_frame / computeObjfcn_3D_frame   (2D|3D_frame/computeObjfcn.cu:14)
+ *
+ * Each takes the mesh index tables (elmObj), the initial landmarks, the model
+ * parameters and a matrix of candidate tau variables, and loops over the
+ * candidate columns (deformation.cu:152-156, computeObjfcn.cu:162-166).  The
+ * structs below carry exactly those positional arguments (the argument
+ * number of each field is given) and the two entry points
+ * sa_deformation / sa_compute_objfcn replace the two MEX families one to one;
+ * the dimension is a field instead of part of the function name.
+ *
+ * Layouts are the reference's: every table is column-major (MATLAB) with
+ * 0-based int32 indices, landmark matrices are lmkNum x dim column-major,
+ * lmkStk is (lmkNum*dim) x timeNum, tauMat lmkNum x timeNum.
+ *
+ * Pointers are host pointers unless `on_device` is set in sa_mesh, in which
+ * case every array of sa_mesh, sa_target and lmkIniMat is a device pointer
+ * (what mxGPUGetDataReadOnly yields, deformation.cu:67-77).  Outputs are
+ * always host memory (the MEX copies results back, deformation.cu:163-164).
+ *
+ * Return value: 0 on success, a negative SA_E* code on a hard error (the
+ * mexErrMsgIdAndTxt cases of deformation.cu:116-129); sa_last_error() then
+ * describes it.  A candidate whose velocity solve fails gets objective
+ * DBL_MAX and its trajectory stops, as deform.cu:68-73 does.
+ */
+#ifndef SHAPE_ATROPHY_H
+#define SHAPE_ATROPHY_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SA_OK 0
+#define SA_E_ARG -1        /* bad sizes / null pointers / unsupported option */
+#define SA_E_ALLOC -2      /* device allocation failed ("objfcn*:cudaMalloc") */
+#define SA_E_DEVICE -3     /* HIP runtime error */
+#define SA_E_NOTFOUND -4   /* unknown session / request handle */
+
+/* elasticity model: 'frame' = the reference's anisotropic frame operator;
+   'isotropic' = per-layer Lame pairs (lambda, mu) scaled by (1 - 0.5 u), the
+   branch CUDA/deformation.m:20-24 reserves but leaves empty. */
+#define SA_ELA_FRAME 0
+#define SA_ELA_ISOTROPIC 1
+
+/* arithmetic of the velocity solve */
+#define SA_PREC_FP64 0     /* plain FP64 everywhere */
+#define SA_PREC_MIXED 1    /* FP32 kernel products inside FP64 iterative refinement */
+
+typedef struct sa_mesh {
+    int dim;               /* 2 or 3 */
+    int on_device;         /* 0: host pointers, 1: device pointers */
+    int lmkNum;            /* rows of lmkIniMat (arg 2) */
+    int elmNum;            /* rows of elmVtxMat (arg 3) */
+    int vtxElmCols;        /* columns of vtxElmMat (arg 4) = 1 + 2 * max adjacency */
+    int bdrElmNum;         /* rows of bdrVtxMat (arg 7) */
+    int vtxBdrCols;        /* columns of vtxBdrMat (arg 8) */
+    int btmElmNum;         /* rows of btmVtxMat (arg 9) */
+    int btmLmkNum;         /* rows of vtxBtmMat (arg 10) */
+    int vtxBtmCols;        /* columns of vtxBtmMat */
+    int nzrNum;            /* rows of femVtxMat (arg 11) */
+    int femIfoCols;        /* columns of femIfoMat (arg 12) */
+    int vfdElmNum;         /* rows of vfdVtxMat (computeObjfcn arg 13); 0 for deformation */
+    const int *elmVtxMat;  /* arg 3  elmNum x (dim+1) */
+    const int *vtxElmMat;  /* arg 4  lmkNum x vtxElmCols */
+    const int *frmVtxMat;  /* arg 5  tanVtxMat (2D, elmNum x 4) | nmlVtxMat (3D, elmNum x 6) */
+    const int *tsvVtxMat;  /* arg 6  elmNum x 4 (2D) | 6 (3D) */
+    const int *bdrVtxMat;  /* arg 7  (unused by the path, accepted for call compatibility) */
+    const int *vtxBdrMat;  /* arg 8  (unused by the path) */
+    const int *btmVtxMat;  /* arg 9  btmElmNum x dim */
+    const int *vtxBtmMat;  /* arg 10 btmLmkNum x vtxBtmCols */
+    const int *femVtxMat;  /* arg 11 nzrNum x 2, sorted (row, col) */
+    const int *femIfoMat;  /* arg 12 nzrNum x femIfoCols */
+    const int *vfdVtxMat;  /* computeObjfcn arg 13, vfdElmNum x dim */
+    const int *elmLyrVec;  /* isotropic model only: material layer of each element (may be NULL) */
+    int ctlNum;            /* > 0: RKHS kernel on these control landmarks (build-defined form) */
+    const int *ctlIdxVec;  /* ctlNum landmark indices */
+} sa_mesh;
+
+typedef struct sa_params {
+    int elaType;                 /* SA_ELA_FRAME | SA_ELA_ISOTROPIC */
+    double ldmWgt;               /* arg 13 (deformation numbering) */
+    int knlOrder;                /* arg 14: -1 Gaussian, 0..4 Matern */
+    double knlWidth;             /* arg 15 */
+    double knlEps;               /* arg 16 */
+    const double *modVec;        /* arg 17: frame 3 (2D) / 4 (3D) moduli; isotropic 2 per layer */
+    int modNum;                  /* length of modVec */
+    double spdTanVal;            /* arg 18 */
+    double spdTsvVal;            /* arg 19 */
+    const double *ynkActPrmVec;  /* arg 20: tauMin, tauMax, actMax, dWidth */
+    const double *reaActPrmVec;  /* arg 21: tauMin, tauMax, actMax, dWidth, hat */
+    double btmWgt;               /* arg 22 */
+    double timeStp;              /* arg 23 */
+    int timeNum;                 /* arg 24 = floor(timeEnd / timeStp) + 1 */
+    int cgItrMax;                /* arg 25 */
+    double cgTolSqu;             /* arg 26 = cgTolVal^2 */
+    int precision;               /* SA_PREC_FP64 | SA_PREC_MIXED */
+    double rdTolVal;             /* relative residual of the sparse reaction-diffusion solve (0 -> 1e-13) */
+} sa_params;
+
+typedef struct sa_target {
+    int tgtElmNum;               /* rows of the target matrices */
+    const double *cenPosMat;     /* computeObjfcn arg 14, tgtElmNum x dim */
+    const double *uniDirMat;     /* arg 15, tgtElmNum x dim */
+    const double *elmVolVec;     /* arg 16, tgtElmNum */
+    char cenKnlType;             /* arg 17: 'G' Gaussian | 'C' Cauchy */
+    double cenKnlWidth;          /* arg 18 */
+    char dirKnlType;             /* arg 19: 'B' Binet | 'L' linear | 'O' | 'U' */
+    double dirKnlWidth;          /* arg 20 */
+} sa_target;
+
+/* per-solve counters */
+typedef struct sa_stats {
+    int status;                  /* 0 ok, 1 velocity solve failed (objective = DBL_MAX) */
+    int steps;                   /* time steps completed */
+    int pcgIters;                /* total velocity-solve iterations */
+    int rdIters;                 /* total reaction-diffusion iterations */
+    int refineSweeps;            /* outer FP64 refinement sweeps (mixed precision) */
+    double seconds;              /* wall time of the solve */
+} sa_stats;
+
+/* deformation_{2D,3D}_frame (deformation.cu:14-199).
+   tauVarMat: tauVarNum x candNum (arg 0); tauPrmVec (arg 1); lmkIniMat (arg 2).
+   Outputs: lmkStk (lmkNum*dim x timeNum) and tauMat (lmkNum x timeNum) of the
+   last candidate, objVec (candNum; 0, or DBL_MAX where the solve failed).
+   Any output pointer may be NULL. */
+int sa_deformation(const sa_mesh *mesh, const sa_params *prm, const double *tauVarMat, int tauVarNum,
+                   int candNum, const double *tauPrmVec, int tauPrmNum, const double *lmkIniMat,
+                   double *lmkStk, double *tauMat, double *objVec);
+
+/* computeObjfcn_{2D,3D}_frame (computeObjfcn.cu:14-212): the varifold
+   distance of the evolved surface to the target for each candidate.
+   dqObjMat (nullable, lmkNum x dim) receives the gradient of the last
+   candidate's objective with respect to its final landmark positions (the
+   compiled-but-unused gradient overload of varifold.cu:545-576|577-608). */
+int sa_compute_objfcn(const sa_mesh *mesh, const sa_target *tgt, const sa_params *prm, const double *tauVarMat,
+                      int tauVarNum, int candNum, const double *tauPrmVec, int tauPrmNum,
+                      const double *lmkIniMat, double *objVec, double *dqObjMat);
+
+/* Device-resident session: the mesh, target and per-step state stay in HBM
+   across candidates (the MEX re-allocates its arena on every call).  No
+   counterpart in the reference; it is the form the batch and serving layers
+   use. */
+typedef struct sa_session sa_session;
+int sa_session_create(const sa_mesh *mesh, const sa_target *tgt /* nullable */, const sa_params *prm,
+                      const double *lmkIniMat, sa_session **out);
+/* one candidate; any output may be NULL; returns 0 or an SA_E* code */
+int sa_session_run(sa_session *s, const double *tauVarVec, int tauVarNum, const double *tauPrmVec,
+                   int tauPrmNum, double *obj, double *lmkStk, double *tauMat, double *dqObjMat,
+                   sa_stats *stats);
+void sa_session_destroy(sa_session *s);
+/* Diagnostic probes (no counterpart in the reference): evaluate one operator
+   of the path at the initial state of a candidate (q0 and tau0 from
+   tauVarVec) so each kernel family can be checked against an oracle alone.
+     SA_PROBE_ELASTIC   out = E(q0) in                       (lmkNum x dim)
+     SA_PROBE_KMATVEC   out = (K(q0) + knlEps I) in          (lmkNum x dim)
+     SA_PROBE_YANK      out = [f(q0, tau0) (lmkNum x dim), tau0 (lmkNum)]
+     SA_PROBE_FEM       out = [M(q0) (nzrNum), M + dt K_D (nzrNum)] in femVtxMat order
+     SA_PROBE_GEOMETRY  out = per-element [Q^-1 (dim^2), det Q, frame 1 (dim), frame 2 (dim)] x elmNum
+     SA_PROBE_VARIFOLD  in = landmarks (lmkNum x dim); out = [objective, gradient (lmkNum x dim)] */
+#define SA_PROBE_ELASTIC 1
+#define SA_PROBE_KMATVEC 2
+#define SA_PROBE_YANK 3
+#define SA_PROBE_FEM 4
+#define SA_PROBE_GEOMETRY 5
+#define SA_PROBE_VARIFOLD 6
+int sa_session_probe(sa_session *s, int what, const double *tauVarVec, int tauVarNum, const double *tauPrmVec,
+                     int tauPrmNum, const double *in, double *out);
+/* bytes of device memory a session of this mesh holds */
+long long sa_session_footprint(const sa_mesh *mesh, const sa_target *tgt, const sa_params *prm);
+
+const char *sa_last_error(void);
+int sa_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SHAPE_ATROPHY_H */

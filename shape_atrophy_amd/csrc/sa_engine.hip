@@ -1,0 +1,720 @@
+// Host engine and C ABI of the gfx950 forward shape-evolution path.
+//
+// One Engine<D> holds a mesh, its derived tables and all per-step state in
+// device memory (a session).  run() performs one candidate solve: the time
+// loop of deform.cu:22-119 / objfcn.cu:23-123 with
+//   * the RKHS velocity system (ldmWgt K^{-1} + E + btmWgt P) v = f solved by
+//     the K-preconditioned CG of ipcg.cu:11-100 with K applied matrix-free;
+//     K^{-1} of the warm start is carried along (alpha = K^{-1} v), so no
+//     factorisation of K is ever formed;
+//   * the reaction-diffusion system (M + dt K_D) tau = M tau + dt M r solved
+//     by Jacobi-CG on the CSR matrix given by femVtxMat / femIfoMat instead
+//     of a dense Cholesky;
+//   * the varifold distance (and its gradient) by LDS-tiled pair sums.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "shape_atrophy.h"
+#include "sa_kernels.hpp"
+
+namespace sa {
+namespace {
+
+thread_local std::string g_err;
+
+struct SaError : std::runtime_error {
+    int code;
+    SaError(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define CK(x)                                                                                           \
+    do {                                                                                                \
+        hipError_t e_ = (x);                                                                            \
+        if (e_ != hipSuccess)                                                                           \
+            throw SaError(SA_E_DEVICE, std::string(#x) + ": " + hipGetErrorString(e_) + " (line " +   \
+                                           std::to_string(__LINE__) + ")");                            \
+    } while (0)
+
+#define CHECK_LAUNCH() CK(hipGetLastError())
+
+inline int gridOf(long n, int nt = NT) { return (int)((n + nt - 1) / nt); }
+
+template <typename T>
+struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    ~DBuf() { if (p) (void)hipFree(p); }
+    void alloc(size_t count, long long &total) {
+        if (p) { (void)hipFree(p); p = nullptr; }
+        n = count;
+        if (count == 0) return;
+        if (hipMalloc((void **)&p, sizeof(T) * count) != hipSuccess) {
+            p = nullptr;
+            throw SaError(SA_E_ALLOC, "Fail to allocate device memory (" + std::to_string(sizeof(T) * count) + " bytes).");
+        }
+        total += (long long)(sizeof(T) * count);
+    }
+    void upload(const T *h, size_t count, hipStream_t s) {
+        CK(hipMemcpyAsync(p, h, sizeof(T) * count, hipMemcpyHostToDevice, s));
+    }
+};
+
+// host copy of an input array that may live on the device
+template <typename T>
+std::vector<T> to_host(const T *src, size_t count, bool on_device) {
+    std::vector<T> h(count);
+    if (count == 0) return h;
+    if (!src) throw SaError(SA_E_ARG, "null input array");
+    if (on_device) {
+        if (hipMemcpy(h.data(), src, sizeof(T) * count, hipMemcpyDeviceToHost) != hipSuccess)
+            throw SaError(SA_E_ARG, "cannot read device input array");
+    } else {
+        std::memcpy(h.data(), src, sizeof(T) * count);
+    }
+    return h;
+}
+
+void require(bool ok, const std::string &msg) {
+    if (!ok) throw SaError(SA_E_ARG, msg);
+}
+
+// count-prefixed incidence table (vertex -> (facet, local) in ascending facet
+// order), the layout of vtxElmMat built by generateElementObject2D.m:32-41
+std::vector<int> incidence(const std::vector<int> &items, int nItems, int k, int num, int &rows) {
+    std::vector<int> cnt(num, 0);
+    for (int l = 0; l < k; ++l)
+        for (int f = 0; f < nItems; ++f) cnt[items[(size_t)l * nItems + f]]++;
+    int mx = 0;
+    for (int v = 0; v < num; ++v) mx = std::max(mx, cnt[v]);
+    rows = 1 + 2 * mx;
+    std::vector<int> tbl((size_t)rows * num, 0);
+    std::vector<int> fill(num, 0);
+    for (int f = 0; f < nItems; ++f)
+        for (int l = 0; l < k; ++l) {
+            int v = items[(size_t)l * nItems + f];
+            int r = fill[v]++;
+            tbl[(size_t)(1 + 2 * r) * num + v] = f;
+            tbl[(size_t)(2 + 2 * r) * num + v] = l;
+        }
+    for (int v = 0; v < num; ++v) tbl[v] = cnt[v];
+    return tbl;
+}
+
+struct Stats {
+    int status = 0, steps = 0, pcgIters = 0, rdIters = 0, refine = 0;
+};
+
+struct EngineBase {
+    long long bytes = 0;
+    virtual ~EngineBase() = default;
+    virtual void run(const double *tauVar, int nVar, const double *tauPrm, int nPrm, double *obj, double *lmkStk,
+                     double *tauMat, double *grad, Stats &st) = 0;
+    virtual void probe(int what, const double *tauVar, int nVar, const double *tauPrm, int nPrm, const double *in,
+                       double *out) = 0;
+};
+
+template <int D>
+struct Engine : EngineBase {
+    using G = GeoLayout<D>;
+    static constexpr int NS = G::NS;
+    // sizes
+    int V, E, Z, B, BL, F = 0, Ft = 0, T, NC = 0;
+    int vtxElmRows, vtxBtmRows, femIfoRows, vfdAdjRows = 0;
+    // parameters
+    int elaType, knlOrder, itrMax, precision;
+    double ldmWgt, knlWidth, knlEps, spdTan, spdTsv, btmWgt, dt, cgTolSqu, rdTolSqu;
+    double ynk[4], rea[5];
+    std::vector<double> mod;
+    bool hasTarget = false;
+    VfdPrm vp{};
+    hipStream_t s = nullptr;
+
+    // tables
+    DBuf<int> elmVtx, vtxElm, frmVtx, tsvVtx, btmVtx, vtxBtm, femIfo, rowPtr, colIdx, diagIdx, vfdVtx, vfdAdj, elmLyr,
+        ctlIdx;
+    DBuf<double> q0, dmod, tCen, tDir, tVol;
+    // state
+    DBuf<double> lmk, tau, geo[2], femElm, ppd[2], lhs, ya, ra, f, x, a, r, Kid, Kr, d, Ad, frc, pfrc, meas;
+    DBuf<double> rdRhs, rdR, rdZ, rdD, rdAd;
+    DBuf<double> P0, P1, P2, P3, hist, rdHist;
+    DBuf<double> dCen, dDir, dVol, rowVal, gC, gN, grad, scal;
+    DBuf<Ctl> ctl, rdCtl;
+    Ctl *hctl = nullptr;
+    double ttSum = 0.0;
+    bool ttReady = false;
+
+    ~Engine() override {
+        if (hctl) (void)hipHostFree(hctl);
+        if (s) (void)hipStreamDestroy(s);
+    }
+
+    Engine(const sa_mesh &m, const sa_target *tg, const sa_params &p, const double *lmkIni) {
+        const bool dev = m.on_device != 0;
+        V = m.lmkNum; E = m.elmNum; Z = m.nzrNum; B = m.btmElmNum; BL = m.btmLmkNum;
+        T = p.timeNum;
+        require(V > 0 && E > 0 && Z >= V, "empty mesh or inconsistent nzrNum");
+        require(T >= 1, "timeNum must be >= 1");
+        require(p.knlOrder >= -1 && p.knlOrder <= 4, "knlOrder must be in -1..4");
+        require(p.elaType == SA_ELA_FRAME || p.elaType == SA_ELA_ISOTROPIC, "unknown elasticity type");
+        vtxElmRows = m.vtxElmCols; vtxBtmRows = m.vtxBtmCols; femIfoRows = m.femIfoCols;
+        elaType = p.elaType; knlOrder = p.knlOrder; itrMax = p.cgItrMax; precision = p.precision;
+        ldmWgt = p.ldmWgt; knlWidth = p.knlWidth; knlEps = p.knlEps; spdTan = p.spdTanVal; spdTsv = p.spdTsvVal;
+        btmWgt = p.btmWgt; dt = p.timeStp; cgTolSqu = p.cgTolSqu;
+        double rdTol = p.rdTolVal > 0 ? p.rdTolVal : 1e-13;
+        rdTolSqu = rdTol * rdTol;
+        require(p.ynkActPrmVec && p.reaActPrmVec && p.modVec, "null parameter vector");
+        for (int i = 0; i < 4; ++i) ynk[i] = p.ynkActPrmVec[i];
+        for (int i = 0; i < 5; ++i) rea[i] = p.reaActPrmVec[i];
+        mod.assign(p.modVec, p.modVec + p.modNum);
+        if (elaType == SA_ELA_FRAME) require(p.modNum >= (D == 2 ? 3 : 4), "modVec too short for the frame model");
+        else require(p.modNum >= 2 && p.modNum % 2 == 0, "isotropic modVec needs (lambda, mu) pairs");
+        require(knlWidth > 0, "knlWidth must be positive");
+
+        // --- host copies, validation, derived tables
+        auto hElm = to_host(m.elmVtxMat, (size_t)E * (D + 1), dev);
+        auto hVtxElm = to_host(m.vtxElmMat, (size_t)V * vtxElmRows, dev);
+        auto hFrm = to_host(m.frmVtxMat, (size_t)E * (D == 2 ? 4 : 6), dev);
+        auto hTsv = to_host(m.tsvVtxMat, (size_t)E * (D == 2 ? 4 : 6), dev);
+        auto hFemVtx = to_host(m.femVtxMat, (size_t)Z * 2, dev);
+        auto hFemIfo = to_host(m.femIfoMat, (size_t)Z * femIfoRows, dev);
+        std::vector<int> hBtm, hVtxBtm;
+        if (B > 0) {
+            hBtm = to_host(m.btmVtxMat, (size_t)B * D, dev);
+            hVtxBtm = to_host(m.vtxBtmMat, (size_t)BL * vtxBtmRows, dev);
+        }
+        for (int v : hElm) require(v >= 0 && v < V, "elmVtxMat index out of range");
+        for (int v : hFrm) require(v >= 0 && v < V, "frame stencil index out of range");
+        for (int v : hTsv) require(v >= 0 && v < V, "tsvVtxMat index out of range");
+        for (int v : hBtm) require(v >= 0 && v < V, "btmVtxMat index out of range");
+        require(BL <= V, "btmLmkNum exceeds lmkNum");
+        for (int i = 0; i < V; ++i) {
+            int adj = hVtxElm[i];
+            require(adj >= 0 && 1 + 2 * adj <= vtxElmRows, "vtxElmMat count out of range");
+            for (int a = 0; a < adj; ++a) {
+                int e = hVtxElm[(size_t)(1 + 2 * a) * V + i], l = hVtxElm[(size_t)(2 + 2 * a) * V + i];
+                require(e >= 0 && e < E && l >= 0 && l <= D && hElm[(size_t)l * E + e] == i, "vtxElmMat inconsistent with elmVtxMat");
+            }
+        }
+        for (int i = 0; i < BL; ++i) {
+            int adj = hVtxBtm[i];
+            require(adj >= 0 && 1 + 2 * adj <= vtxBtmRows, "vtxBtmMat count out of range");
+            for (int a = 0; a < adj; ++a) {
+                int b = hVtxBtm[(size_t)(1 + 2 * a) * BL + i], l = hVtxBtm[(size_t)(2 + 2 * a) * BL + i];
+                require(b >= 0 && b < B && l >= 0 && l < D, "vtxBtmMat entry out of range");
+            }
+        }
+        // CSR from the sorted femVtxMat
+        std::vector<int> hRow(V + 1, 0), hCol(Z), hDiag(V, -1);
+        for (int z = 0; z < Z; ++z) {
+            int rr = hFemVtx[z], cc = hFemVtx[(size_t)Z + z];
+            require(rr >= 0 && rr < V && cc >= 0 && cc < V, "femVtxMat index out of range");
+            require(z == 0 || rr > hFemVtx[z - 1] || (rr == hFemVtx[z - 1] && cc > hFemVtx[(size_t)Z + z - 1]),
+                    "femVtxMat is not sorted by (row, col)");
+            hRow[rr + 1]++;
+            hCol[z] = cc;
+            if (rr == cc) hDiag[rr] = z;
+            int adj = hFemIfo[z];
+            require(adj >= 0 && 2 + 2 * adj <= femIfoRows, "femIfoMat count out of range");
+            for (int a = 0; a < adj; ++a) {
+                int e = hFemIfo[(size_t)(2 + 2 * a) * Z + z], gi = hFemIfo[(size_t)(3 + 2 * a) * Z + z];
+                require(e >= 0 && e < E && gi >= 0 && gi < NS * E, "femIfoMat entry out of range");
+            }
+        }
+        for (int i = 0; i < V; ++i) { hRow[i + 1] += hRow[i]; require(hDiag[i] >= 0, "femVtxMat lacks a diagonal entry"); }
+        std::vector<int> hLyr;
+        if (elaType == SA_ELA_ISOTROPIC && m.elmLyrVec) {
+            hLyr = to_host(m.elmLyrVec, (size_t)E, dev);
+            int nl = p.modNum / 2;
+            for (int l : hLyr) require(l >= 0 && l < nl, "elmLyrVec layer out of range");
+        }
+        if (m.ctlNum > 0) throw SaError(SA_E_ARG, "control-point form not available in this build");
+
+        CK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        CK(hipHostMalloc((void **)&hctl, sizeof(Ctl) * 2));
+        long long &tot = bytes;
+        elmVtx.alloc(hElm.size(), tot); elmVtx.upload(hElm.data(), hElm.size(), s);
+        vtxElm.alloc(hVtxElm.size(), tot); vtxElm.upload(hVtxElm.data(), hVtxElm.size(), s);
+        frmVtx.alloc(hFrm.size(), tot); frmVtx.upload(hFrm.data(), hFrm.size(), s);
+        tsvVtx.alloc(hTsv.size(), tot); tsvVtx.upload(hTsv.data(), hTsv.size(), s);
+        femIfo.alloc(hFemIfo.size(), tot); femIfo.upload(hFemIfo.data(), hFemIfo.size(), s);
+        rowPtr.alloc(V + 1, tot); rowPtr.upload(hRow.data(), V + 1, s);
+        colIdx.alloc(Z, tot); colIdx.upload(hCol.data(), Z, s);
+        diagIdx.alloc(V, tot); diagIdx.upload(hDiag.data(), V, s);
+        if (B > 0) {
+            btmVtx.alloc(hBtm.size(), tot); btmVtx.upload(hBtm.data(), hBtm.size(), s);
+            vtxBtm.alloc(hVtxBtm.size(), tot); vtxBtm.upload(hVtxBtm.data(), hVtxBtm.size(), s);
+        }
+        if (!hLyr.empty()) { elmLyr.alloc(E, tot); elmLyr.upload(hLyr.data(), E, s); }
+        dmod.alloc(mod.size(), tot); dmod.upload(mod.data(), mod.size(), s);
+        auto hq0 = to_host(lmkIni, (size_t)V * D, dev);
+        q0.alloc((size_t)V * D, tot); q0.upload(hq0.data(), hq0.size(), s);
+
+        if (tg) {
+            hasTarget = true;
+            F = m.vfdElmNum;
+            Ft = tg->tgtElmNum;
+            require(F > 0 && Ft > 0, "empty varifold facet set");
+            auto hVfd = to_host(m.vfdVtxMat, (size_t)F * D, dev);
+            for (int v : hVfd) require(v >= 0 && v < V, "vfdVtxMat index out of range");
+            auto adj = incidence(hVfd, F, D, V, vfdAdjRows);
+            vfdVtx.alloc(hVfd.size(), tot); vfdVtx.upload(hVfd.data(), hVfd.size(), s);
+            vfdAdj.alloc(adj.size(), tot); vfdAdj.upload(adj.data(), adj.size(), s);
+            auto hc = to_host(tg->cenPosMat, (size_t)Ft * D, dev);
+            auto hd = to_host(tg->uniDirMat, (size_t)Ft * D, dev);
+            auto hv = to_host(tg->elmVolVec, (size_t)Ft, dev);
+            tCen.alloc(hc.size(), tot); tCen.upload(hc.data(), hc.size(), s);
+            tDir.alloc(hd.size(), tot); tDir.upload(hd.data(), hd.size(), s);
+            tVol.alloc(hv.size(), tot); tVol.upload(hv.data(), hv.size(), s);
+            require(tg->cenKnlType == 'G' || tg->cenKnlType == 'C', "cenKnlType must be 'G' or 'C'");
+            require(tg->dirKnlType == 'B' || tg->dirKnlType == 'L' || tg->dirKnlType == 'O' || tg->dirKnlType == 'U',
+                    "dirKnlType must be 'B', 'L', 'O' or 'U'");
+            vp.cenType = tg->cenKnlType;
+            vp.dirType = tg->dirKnlType;
+            vp.cenInvW2 = 1.0 / (tg->cenKnlWidth * tg->cenKnlWidth);
+            vp.dirInvW2 = 1.0 / (tg->dirKnlWidth * tg->dirKnlWidth);
+            dCen.alloc((size_t)F * D, tot); dDir.alloc((size_t)F * D, tot); dVol.alloc(F, tot);
+            rowVal.alloc((size_t)F + Ft, tot);
+            gC.alloc((size_t)F * D, tot); gN.alloc((size_t)F * D, tot);
+            grad.alloc((size_t)V * D, tot);
+        }
+        size_t n = (size_t)V * D;
+        lmk.alloc(n * T, tot);
+        tau.alloc((size_t)V * T, tot);
+        for (int k = 0; k < 2; ++k) { geo[k].alloc((size_t)G::N * E, tot); ppd[k].alloc(Z, tot); }
+        femElm.alloc((size_t)(1 + NS) * E, tot);
+        lhs.alloc(Z, tot);
+        ya.alloc(V, tot); ra.alloc(V, tot);
+        f.alloc(n, tot); x.alloc(n, tot); a.alloc(n, tot); r.alloc(n, tot); Kid.alloc(n, tot);
+        Kr.alloc(n, tot); d.alloc(n, tot); Ad.alloc(n, tot);
+        frc.alloc((size_t)(D + 1) * D * E, tot);
+        if (B > 0) { pfrc.alloc((size_t)D * D * B, tot); meas.alloc(B, tot); }
+        rdRhs.alloc(V, tot); rdR.alloc(V, tot); rdZ.alloc(V, tot); rdD.alloc(V, tot); rdAd.alloc(V, tot);
+        P0.alloc(MAXP, tot); P1.alloc(MAXP, tot); P2.alloc(2 * MAXP, tot); P3.alloc(3 * MAXP, tot);
+        hist.alloc((size_t)std::max(itrMax, 1) + 2, tot);
+        rdHist.alloc(200002, tot);
+        scal.alloc(8, tot);
+        ctl.alloc(1, tot); rdCtl.alloc(1, tot);
+        require(gridOf(n) <= MAXP && gridOf(V, MV_ROWS) <= MAXP, "mesh too large for the partial-sum buffers");
+        CK(hipStreamSynchronize(s));
+    }
+
+    // ---------------- kernel-order dispatch
+    template <bool CHECK>
+    void kmatvec(const double *qq, const double *xin, double *yout, const double *Prr, int nPrr, int it) {
+        double invW = 1.0 / knlWidth;
+        int g = gridOf(V, MV_ROWS);
+        Ctl *c = ctl.p;
+#define SA_MV(O) kmatvec_k<D, O, CHECK><<<g, NT, 0, s>>>(V, qq, xin, yout, knlEps, invW, Prr, nPrr, P3.p, it, c)
+        switch (knlOrder) {
+            case -1: SA_MV(-1); break;
+            case 0: SA_MV(0); break;
+            case 1: SA_MV(1); break;
+            case 2: SA_MV(2); break;
+            case 3: SA_MV(3); break;
+            default: SA_MV(4); break;
+        }
+#undef SA_MV
+        CHECK_LAUNCH();
+    }
+
+    void elastic(const double *xin, const double *tauNow, const double *geoNow, const Ctl *c) {
+        int g = gridOf(E);
+        if (elaType == SA_ELA_FRAME)
+            elastic_elem_k<D, false, double><<<g, NT, 0, s>>>(E, V, elmVtx.p, geoNow, xin, tauNow, elmLyr.p, dmod.p, frc.p, c);
+        else
+            elastic_elem_k<D, true, double><<<g, NT, 0, s>>>(E, V, elmVtx.p, geoNow, xin, tauNow, elmLyr.p, dmod.p, frc.p, c);
+        CHECK_LAUNCH();
+        if (btmWgt != 0.0 && B > 0) {
+            pen_elem_k<D><<<gridOf(B), NT, 0, s>>>(B, V, btmVtx.p, meas.p, xin, pfrc.p, c);
+            CHECK_LAUNCH();
+        }
+    }
+
+    template <int MODE>
+    void gather_op(const double *u, const double *w, double *out, double *out2, double *P) {
+        const double *pf = (btmWgt != 0.0 && B > 0) ? pfrc.p : nullptr;
+        gather_op_k<D, MODE><<<gridOf(V), NT, 0, s>>>(V, E, vtxElm.p, frc.p, BL, B, vtxBtm.p, pf, btmWgt, ldmWgt, u, w,
+                                                     out, out2, P, ctl.p);
+        CHECK_LAUNCH();
+    }
+
+    void sync_ctl(Ctl *dev, Ctl *host) {
+        CK(hipMemcpyAsync(host, dev, sizeof(Ctl), hipMemcpyDeviceToHost, s));
+        CK(hipStreamSynchronize(s));
+    }
+
+    // velocity solve of one step; returns 0 ok, 1 failure
+    int velocity(const double *q, const double *tauNow, const double *geoNow, bool warm, int &iters) {
+        const int n = V * D;
+        const int gV = gridOf(V), gN = gridOf(n), gM = gridOf(V, MV_ROWS);
+        // f and |f|^2
+        yank_elem_k<D><<<gridOf(E), NT, 0, s>>>(E, elmVtx.p, geoNow, ya.p, frc.p);
+        CHECK_LAUNCH();
+        gather_k<D><<<gV, NT, 0, s>>>(V, E, vtxElm.p, frc.p, f.p, P0.p);
+        CHECK_LAUNCH();
+        pcg_start_k<<<1, NT, 0, s>>>(P0.p, gV, cgTolSqu, ctl.p);
+        CHECK_LAUNCH();
+        zero_if_empty_k<<<gN, NT, 0, s>>>(n, x.p, a.p, ctl.p);
+        CHECK_LAUNCH();
+        if (warm) kmatvec<false>(q, a.p, x.p, nullptr, 0, -1);   // x0 = K alpha_{t-1}
+        // r = b - A x0, Kid = r
+        elastic(x.p, tauNow, geoNow, ctl.p);
+        gather_op<1>(a.p, f.p, r.p, Kid.p, P2.p);
+        kmatvec<true>(q, r.p, Kr.p, P2.p, gV, -1);
+        pcg_dir_k<<<gN, NT, 0, s>>>(n, P3.p, gM, hist.p, -1, r.p, Kr.p, Kid.p, d.p, ctl.p);
+        CHECK_LAUNCH();
+        int it0 = 0, chunk = 4;
+        Ctl &h = hctl[0];
+        while (true) {
+            int itEnd = std::min(itrMax, it0 + chunk);
+            for (int it = it0; it < itEnd; ++it) {
+                elastic(d.p, tauNow, geoNow, ctl.p);
+                gather_op<0>(Kid.p, d.p, Ad.p, nullptr, P1.p);
+                pcg_update_k<<<gN, NT, 0, s>>>(n, P1.p, gV, hist.p, it, x.p, a.p, r.p, d.p, Kid.p, Ad.p, P2.p, ctl.p);
+                CHECK_LAUNCH();
+                kmatvec<true>(q, r.p, Kr.p, P2.p, gN, it);
+                pcg_dir_k<<<gN, NT, 0, s>>>(n, P3.p, gM, hist.p, it, r.p, Kr.p, Kid.p, d.p, ctl.p);
+                CHECK_LAUNCH();
+            }
+            sync_ctl(ctl.p, &h);
+            if (h.done) break;
+            if (itEnd >= itrMax) { iters = itrMax; return 1; }
+            it0 = itEnd;
+            chunk = std::min(chunk * 2, 64);
+        }
+        iters = h.iters;
+        return h.status;
+    }
+
+    int reaction_diffusion(const double *tauNow, double *tauNxt, const double *geoNow, const double *ppdNow,
+                           int &iters) {
+        const int gV = gridOf(V);
+        femvec_elem_k<D><<<gridOf(E), NT, 0, s>>>(E, elmVtx.p, geoNow, ra.p, frc.p);
+        CHECK_LAUNCH();
+        rd_start_k<<<gV, NT, 0, s>>>(V, E, vtxElm.p, frc.p, rowPtr.p, colIdx.p, diagIdx.p, ppdNow, lhs.p, tauNow, dt,
+                                    tauNxt, rdRhs.p, rdR.p, rdZ.p, rdD.p, P3.p);
+        CHECK_LAUNCH();
+        rd_fin_k<<<1, NT, 0, s>>>(P3.p, gV, rdTolSqu, rdHist.p, rdCtl.p);
+        CHECK_LAUNCH();
+        int it0 = 0, chunk = 8;
+        const int rdMax = 200000;
+        Ctl &h = hctl[1];
+        while (true) {
+            int itEnd = std::min(rdMax, it0 + chunk);
+            for (int it = it0; it < itEnd; ++it) {
+                rd_spmv_k<<<gV, NT, 0, s>>>(V, rowPtr.p, colIdx.p, lhs.p, rdD.p, rdAd.p, P1.p, rdCtl.p);
+                rd_update_k<<<gV, NT, 0, s>>>(V, P1.p, gV, rdHist.p, it, diagIdx.p, lhs.p, tauNxt, rdR.p, rdZ.p, rdD.p,
+                                             rdAd.p, P2.p, rdCtl.p);
+                rd_dir_k<<<gV, NT, 0, s>>>(V, P2.p, gV, rdHist.p, it, rdZ.p, rdD.p, rdCtl.p);
+                CHECK_LAUNCH();
+            }
+            sync_ctl(rdCtl.p, &h);
+            if (h.done) break;
+            if (itEnd >= rdMax) { iters = rdMax; return 1; }
+            it0 = itEnd;
+            chunk = std::min(chunk * 2, 128);
+        }
+        if (h.bb == 0.0) CK(hipMemsetAsync(tauNxt, 0, sizeof(double) * V, s));
+        iters = h.iters;
+        return 0;
+    }
+
+    double objective(const double *qT, double *gradOut) {
+        facets_k<D><<<gridOf(F), NT, 0, s>>>(F, V, vfdVtx.p, qT, dCen.p, dDir.p, dVol.p);
+        CHECK_LAUNCH();
+        int nbD = gridOf(F, MV_ROWS), nbT = ttReady ? 0 : gridOf(Ft, MV_ROWS);
+        if (gradOut)
+            vfd_rows_k<D, true><<<nbD + nbT, NT, 0, s>>>(F, Ft, nbD, dCen.p, dDir.p, dVol.p, tCen.p, tDir.p, tVol.p, vp,
+                                                        rowVal.p, gC.p, gN.p);
+        else
+            vfd_rows_k<D, false><<<nbD + nbT, NT, 0, s>>>(F, Ft, nbD, dCen.p, dDir.p, dVol.p, tCen.p, tDir.p, tVol.p, vp,
+                                                         rowVal.p, gC.p, gN.p);
+        CHECK_LAUNCH();
+        sum_k<<<1, NT, 0, s>>>(rowVal.p, F, scal.p);
+        CHECK_LAUNCH();
+        if (!ttReady) {
+            sum_k<<<1, NT, 0, s>>>(rowVal.p + F, Ft, scal.p + 1);
+            CHECK_LAUNCH();
+        }
+        double hs[2];
+        CK(hipMemcpyAsync(hs, scal.p, sizeof(double) * 2, hipMemcpyDeviceToHost, s));
+        if (gradOut) {
+            vfd_grad_k<D><<<gridOf(V), NT, 0, s>>>(V, F, vfdAdj.p, vfdVtx.p, qT, gC.p, gN.p, grad.p);
+            CHECK_LAUNCH();
+            CK(hipMemcpyAsync(gradOut, grad.p, sizeof(double) * V * D, hipMemcpyDeviceToHost, s));
+        }
+        CK(hipStreamSynchronize(s));
+        if (!ttReady) { ttSum = hs[1]; ttReady = true; }
+        return hs[0] + ttSum;
+    }
+
+    // initial state of a solve: lmkStk[0] = q0, tau[0] from the bump
+    // parameters (tauInitial.cu:52-63|53-64), geometry and FEM matrices at q0
+    void start(const double *tauVar, int nVar, const double *tauPrm, int nPrm) {
+        const size_t n = (size_t)V * D;
+        require(tauVar && tauPrm, "null tau arguments");
+        double c0, c1, c2 = 0.0, h, R;
+        int pw;
+        if (D == 2) {
+            require(nVar >= 4 && nPrm >= 1, "2D tauVar needs [cx, cy, h, R] and tauPrm [p]");
+            c0 = tauVar[0]; c1 = tauVar[1]; h = tauVar[2]; R = tauVar[3]; pw = (int)tauPrm[0];
+        } else {
+            require(nVar >= 4 && nPrm >= 2, "3D tauVar needs [cx, cy, cz, h] and tauPrm [R, p]");
+            c0 = tauVar[0]; c1 = tauVar[1]; c2 = tauVar[2]; h = tauVar[3]; R = tauPrm[0]; pw = (int)tauPrm[1];
+        }
+        CK(hipMemcpyAsync(lmk.p, q0.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+        tau_initial_k<D><<<gridOf(V), NT, 0, s>>>(V, q0.p, c0, c1, c2, h, R, pw, tau.p);
+        CHECK_LAUNCH();
+        CK(hipMemsetAsync(x.p, 0, sizeof(double) * n, s));
+        CK(hipMemsetAsync(a.p, 0, sizeof(double) * n, s));
+        geometry_k<D><<<gridOf(E), NT, 0, s>>>(E, V, elmVtx.p, frmVtx.p, tsvVtx.p, q0.p, spdTan, spdTsv, geo[0].p,
+                                               femElm.p);
+        CHECK_LAUNCH();
+        fem_gather_k<<<gridOf(Z), NT, 0, s>>>(Z, E, femIfo.p, femElm.p, dt, ppd[0].p, lhs.p);
+        CHECK_LAUNCH();
+    }
+
+    // single operators at the initial state, for the parity tests
+    void probe(int what, const double *tauVar, int nVar, const double *tauPrm, int nPrm, const double *in,
+               double *out) override {
+        const size_t n = (size_t)V * D;
+        start(tauVar, nVar, tauPrm, nPrm);
+        CK(hipMemsetAsync(ctl.p, 0, sizeof(Ctl), s));
+        switch (what) {
+            case SA_PROBE_ELASTIC:
+                CK(hipMemcpyAsync(x.p, in, sizeof(double) * n, hipMemcpyHostToDevice, s));
+                elastic(x.p, tau.p, geo[0].p, ctl.p);
+                gather_k<D><<<gridOf(V), NT, 0, s>>>(V, E, vtxElm.p, frc.p, f.p, P0.p);
+                CHECK_LAUNCH();
+                CK(hipMemcpyAsync(out, f.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+                break;
+            case SA_PROBE_KMATVEC:
+                CK(hipMemcpyAsync(r.p, in, sizeof(double) * n, hipMemcpyHostToDevice, s));
+                kmatvec<false>(q0.p, r.p, Kr.p, nullptr, 0, -1);
+                CK(hipMemcpyAsync(out, Kr.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+                break;
+            case SA_PROBE_YANK:
+                activity_k<<<gridOf(V), NT, 0, s>>>(V, tau.p, ynk[0], ynk[1], ynk[2], ynk[3], rea[0], rea[1], rea[2],
+                                                   rea[3], rea[4], ya.p, ra.p);
+                yank_elem_k<D><<<gridOf(E), NT, 0, s>>>(E, elmVtx.p, geo[0].p, ya.p, frc.p);
+                gather_k<D><<<gridOf(V), NT, 0, s>>>(V, E, vtxElm.p, frc.p, f.p, P0.p);
+                CHECK_LAUNCH();
+                CK(hipMemcpyAsync(out, f.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+                CK(hipMemcpyAsync(out + n, tau.p, sizeof(double) * V, hipMemcpyDeviceToHost, s));
+                break;
+            case SA_PROBE_FEM:
+                CK(hipMemcpyAsync(out, ppd[0].p, sizeof(double) * Z, hipMemcpyDeviceToHost, s));
+                CK(hipMemcpyAsync(out + Z, lhs.p, sizeof(double) * Z, hipMemcpyDeviceToHost, s));
+                break;
+            case SA_PROBE_GEOMETRY:
+                CK(hipMemcpyAsync(out, geo[0].p, sizeof(double) * G::N * E, hipMemcpyDeviceToHost, s));
+                break;
+            case SA_PROBE_VARIFOLD: {
+                require(hasTarget, "no target in this session");
+                double *qT = lmk.p + (size_t)(T - 1) * n;
+                CK(hipMemcpyAsync(qT, in, sizeof(double) * n, hipMemcpyHostToDevice, s));
+                out[0] = objective(qT, out + 1);
+                break;
+            }
+            default:
+                throw SaError(SA_E_ARG, "unknown probe");
+        }
+        CK(hipStreamSynchronize(s));
+    }
+
+    void run(const double *tauVar, int nVar, const double *tauPrm, int nPrm, double *obj, double *lmkOut,
+             double *tauOut, double *gradOut, Stats &st) override {
+        const size_t n = (size_t)V * D;
+        st = Stats{};
+        start(tauVar, nVar, tauPrm, nPrm);
+        int cur = 0;
+        int failed = 0;
+        for (int t = 0; t < T - 1; ++t) {
+            const double *q = lmk.p + (size_t)t * n;
+            double *qn = lmk.p + (size_t)(t + 1) * n;
+            const double *tn = tau.p + (size_t)t * V;
+            double *tx = tau.p + (size_t)(t + 1) * V;
+            activity_k<<<gridOf(V), NT, 0, s>>>(V, tn, ynk[0], ynk[1], ynk[2], ynk[3], rea[0], rea[1], rea[2], rea[3],
+                                               rea[4], ya.p, ra.p);
+            CHECK_LAUNCH();
+            if (btmWgt != 0.0 && B > 0) {
+                btm_meas_k<D><<<gridOf(B), NT, 0, s>>>(B, V, btmVtx.p, q, meas.p);
+                CHECK_LAUNCH();
+            }
+            int its = 0;
+            if (velocity(q, tn, geo[cur].p, t > 0, its) != 0) { st.pcgIters += its; failed = 1; break; }
+            st.pcgIters += its;
+            axpby_k<<<gridOf((long)n), NT, 0, s>>>((int)n, qn, 1.0, q, dt, x.p);
+            CHECK_LAUNCH();
+            int nx = cur ^ 1;
+            geometry_k<D><<<gridOf(E), NT, 0, s>>>(E, V, elmVtx.p, frmVtx.p, tsvVtx.p, qn, spdTan, spdTsv, geo[nx].p,
+                                                   femElm.p);
+            CHECK_LAUNCH();
+            fem_gather_k<<<gridOf(Z), NT, 0, s>>>(Z, E, femIfo.p, femElm.p, dt, ppd[nx].p, lhs.p);
+            CHECK_LAUNCH();
+            int rits = 0;
+            if (reaction_diffusion(tn, tx, geo[cur].p, ppd[cur].p, rits) != 0) { failed = 1; break; }
+            st.rdIters += rits;
+            st.steps = t + 1;
+            cur = nx;
+        }
+        st.status = failed;
+        if (obj) {
+            if (failed) *obj = DBL_MAX;
+            else if (hasTarget) *obj = objective(lmk.p + (size_t)(T - 1) * n, gradOut);
+            else *obj = 0.0;
+        }
+        if (lmkOut) CK(hipMemcpyAsync(lmkOut, lmk.p, sizeof(double) * n * T, hipMemcpyDeviceToHost, s));
+        if (tauOut) CK(hipMemcpyAsync(tauOut, tau.p, sizeof(double) * V * T, hipMemcpyDeviceToHost, s));
+        CK(hipStreamSynchronize(s));
+    }
+};
+
+EngineBase *make_engine(const sa_mesh *m, const sa_target *t, const sa_params *p, const double *lmkIni) {
+    if (!m || !p || !lmkIni) throw SaError(SA_E_ARG, "null mesh, params or landmarks");
+    if (m->dim == 2) return new Engine<2>(*m, t, *p, lmkIni);
+    if (m->dim == 3) return new Engine<3>(*m, t, *p, lmkIni);
+    throw SaError(SA_E_ARG, "Dimension " + std::to_string(m->dim) + " is not supported.");
+}
+
+template <typename Fn>
+int guarded(Fn &&fn) {
+    try {
+        fn();
+        return SA_OK;
+    } catch (const SaError &e) {
+        g_err = e.what();
+        return e.code;
+    } catch (const std::exception &e) {
+        g_err = e.what();
+        return SA_E_DEVICE;
+    }
+}
+
+}  // namespace
+}  // namespace sa
+
+struct sa_session {
+    std::unique_ptr<sa::EngineBase> eng;
+};
+
+extern "C" {
+
+const char *sa_last_error(void) { return sa::g_err.c_str(); }
+
+int sa_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int sa_session_create(const sa_mesh *mesh, const sa_target *tgt, const sa_params *prm, const double *lmkIniMat,
+                      sa_session **out) {
+    return sa::guarded([&] {
+        if (!out) throw sa::SaError(SA_E_ARG, "null session output");
+        *out = nullptr;
+        auto *ss = new sa_session;
+        try {
+            ss->eng.reset(sa::make_engine(mesh, tgt, prm, lmkIniMat));
+        } catch (...) {
+            delete ss;
+            throw;
+        }
+        *out = ss;
+    });
+}
+
+int sa_session_run(sa_session *ss, const double *tauVarVec, int tauVarNum, const double *tauPrmVec, int tauPrmNum,
+                   double *obj, double *lmkStk, double *tauMat, double *dqObjMat, sa_stats *stats) {
+    return sa::guarded([&] {
+        if (!ss || !ss->eng) throw sa::SaError(SA_E_NOTFOUND, "null session");
+        sa::Stats st;
+        auto t0 = std::chrono::steady_clock::now();
+        ss->eng->run(tauVarVec, tauVarNum, tauPrmVec, tauPrmNum, obj, lmkStk, tauMat, dqObjMat, st);
+        double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (stats) {
+            stats->status = st.status;
+            stats->steps = st.steps;
+            stats->pcgIters = st.pcgIters;
+            stats->rdIters = st.rdIters;
+            stats->refineSweeps = st.refine;
+            stats->seconds = sec;
+        }
+    });
+}
+
+void sa_session_destroy(sa_session *ss) { delete ss; }
+
+int sa_session_probe(sa_session *ss, int what, const double *tauVarVec, int tauVarNum, const double *tauPrmVec,
+                     int tauPrmNum, const double *in, double *out) {
+    return sa::guarded([&] {
+        if (!ss || !ss->eng) throw sa::SaError(SA_E_NOTFOUND, "null session");
+        if (!out) throw sa::SaError(SA_E_ARG, "null output");
+        ss->eng->probe(what, tauVarVec, tauVarNum, tauPrmVec, tauPrmNum, in, out);
+    });
+}
+
+long long sa_session_footprint(const sa_mesh *m, const sa_target *t, const sa_params *p) {
+    if (!m || !p) return -1;
+    long long D = m->dim, V = m->lmkNum, E = m->elmNum, Z = m->nzrNum, T = p->timeNum;
+    long long NS = (D + 1) * (D + 2) / 2, GN = D * D + 1 + 2 * D;
+    long long b = 0;
+    b += 4 * (E * (D + 1) + V * m->vtxElmCols + E * (D == 2 ? 8 : 12) + Z * m->femIfoCols + 2 * Z + 2 * V);
+    b += 8 * (V * D * T + V * T + 2 * GN * E + (1 + NS) * E + 3 * Z + 2 * V + 9 * V * D + (D + 1) * D * E + 5 * V);
+    b += 8 * (7 * sa::MAXP + p->cgItrMax + 200010);
+    if (t) {
+        long long F = m->vfdElmNum, Ft = t->tgtElmNum;
+        b += 8 * (Ft * (2 * D + 1) + F * (4 * D + 1) + F + Ft + V * D) + 4 * (F * D + V * 9);
+    }
+    return b;
+}
+
+int sa_deformation(const sa_mesh *mesh, const sa_params *prm, const double *tauVarMat, int tauVarNum, int candNum,
+                   const double *tauPrmVec, int tauPrmNum, const double *lmkIniMat, double *lmkStk, double *tauMat,
+                   double *objVec) {
+    return sa::guarded([&] {
+        if (candNum < 0 || (candNum > 0 && !tauVarMat)) throw sa::SaError(SA_E_ARG, "bad candidate matrix");
+        std::unique_ptr<sa::EngineBase> eng(sa::make_engine(mesh, nullptr, prm, lmkIniMat));
+        for (int c = 0; c < candNum; ++c) {
+            sa::Stats st;
+            double o = 0.0;
+            bool last = (c == candNum - 1);
+            eng->run(tauVarMat + (size_t)c * tauVarNum, tauVarNum, tauPrmVec, tauPrmNum, &o, last ? lmkStk : nullptr,
+                     last ? tauMat : nullptr, nullptr, st);
+            if (objVec) objVec[c] = st.status ? DBL_MAX : 0.0;
+        }
+    });
+}
+
+int sa_compute_objfcn(const sa_mesh *mesh, const sa_target *tgt, const sa_params *prm, const double *tauVarMat,
+                      int tauVarNum, int candNum, const double *tauPrmVec, int tauPrmNum, const double *lmkIniMat,
+                      double *objVec, double *dqObjMat) {
+    return sa::guarded([&] {
+        if (!tgt) throw sa::SaError(SA_E_ARG, "computeObjfcn needs a target");
+        if (candNum < 0 || (candNum > 0 && !tauVarMat)) throw sa::SaError(SA_E_ARG, "bad candidate matrix");
+        std::unique_ptr<sa::EngineBase> eng(sa::make_engine(mesh, tgt, prm, lmkIniMat));
+        for (int c = 0; c < candNum; ++c) {
+            sa::Stats st;
+            double o = 0.0;
+            bool last = (c == candNum - 1);
+            eng->run(tauVarMat + (size_t)c * tauVarNum, tauVarNum, tauPrmVec, tauPrmNum, &o, nullptr, nullptr,
+                     last ? dqObjMat : nullptr, st);
+            if (objVec) objVec[c] = o;
+        }
+    });
+}
+
+}  // extern "C"

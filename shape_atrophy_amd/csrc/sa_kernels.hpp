@@ -1,0 +1,934 @@
+// gfx950 kernels of the forward shape-evolution step.
+//
+// Element-parallel kernels write per-element contributions to SoA scratch
+// and vertex/nonzero-parallel kernels gather them through the reference's
+// count-prefixed adjacency tables in ascending element order, so every sum
+// has a fixed order and results are bitwise reproducible run to run (no
+// atomics anywhere).  Scalars of the Krylov solvers stay on the device: each
+// kernel reduces the partial sums of its predecessor itself (every block
+// computes the same value in the same order) and a done flag makes all later
+// kernels of a launched chunk return at once.
+#pragma once
+#include "sa_device.hpp"
+
+namespace sa {
+
+struct Ctl {            // solver control block, device resident
+    double bb, tol;     // |b|^2 and the stopping threshold on |r|^2
+    int done, status, iters, pad;
+};
+
+// ---------------------------------------------------------------------------
+// initial concentration, tauInitial.cu:4-64|4-66
+
+__device__ inline double intpow(double b, int e) {
+    double out = 1.0, pw = b;
+    do {
+        int quo = e >> 1, rem = e - 2 * quo;
+        if (rem == 1) out *= pw;
+        pw *= pw;
+        e = quo;
+    } while (e != 0);
+    return out;
+}
+
+template <int D>
+__global__ void tau_initial_k(int V, const double *__restrict__ q, double c0, double c1, double c2, double h,
+                              double R, int pw, double *__restrict__ tau) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= V) return;
+    double d0 = q[i] - c0, d1 = q[V + i] - c1, s = d0 * d0 + d1 * d1;
+    if constexpr (D == 3) { double d2 = q[2 * V + i] - c2; s += d2 * d2; }
+    double r2 = s / (R * R);
+    double t = (r2 - 1.0) * (r2 - 1.0);
+    tau[i] = h * (r2 < 1.0 ? 1.0 - intpow(1.0 - t, pw) : 0.0);
+}
+
+// ---------------------------------------------------------------------------
+// geometry of every element at positions q: Q^{-1}, det Q, frames
+// (vertexToEdge.cu, computeFrame.cu) and the element mass / anisotropic
+// stiffness entries (assembleFemMatrix.cu:5-58|5-118)
+
+template <int D>
+__global__ __launch_bounds__(NT) void geometry_k(int E, int V, const int *__restrict__ elmVtx,
+                                                 const int *__restrict__ frmVtx, const int *__restrict__ tsvVtx,
+                                                 const double *__restrict__ q, double spdTan, double spdTsv,
+                                                 double *__restrict__ geo, double *__restrict__ femElm) {
+    using G = GeoLayout<D>;
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    int v0 = elmVtx[e];
+    double p0[D];
+#pragma unroll
+    for (int c = 0; c < D; ++c) p0[c] = q[c * V + v0];
+    double ed[D][D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        int vk = elmVtx[(k + 1) * E + e];
+#pragma unroll
+        for (int c = 0; c < D; ++c) ed[k][c] = q[c * V + vk] - p0[c];
+    }
+    double inv[D][D], det;
+    mat_inv<D>(ed, inv, det);
+    // frames
+    double f1[D], f2[D];
+    {
+        double s1[D], s2[D];
+#pragma unroll
+        for (int c = 0; c < D; ++c) { s1[c] = 0.0; s2[c] = 0.0; }
+        if constexpr (D == 2) {
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                int a = frmVtx[(2 * k) * E + e], b = frmVtx[(2 * k + 1) * E + e];
+                double d0 = q[b] - q[a], d1 = q[V + b] - q[V + a];
+                double len = sqrt(d0 * d0 + d1 * d1);
+                s1[0] += d0 / len; s1[1] += d1 / len;
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                int a = frmVtx[(3 * k) * E + e], b = frmVtx[(3 * k + 1) * E + e], cc = frmVtx[(3 * k + 2) * E + e];
+                double u[3], w[3];
+#pragma unroll
+                for (int i = 0; i < 3; ++i) { u[i] = q[i * V + b] - q[i * V + a]; w[i] = q[i * V + cc] - q[i * V + a]; }
+                double cr0 = u[1] * w[2] - u[2] * w[1], cr1 = -u[0] * w[2] + u[2] * w[0], cr2 = u[0] * w[1] - u[1] * w[0];
+                double len = sqrt(cr0 * cr0 + cr1 * cr1 + cr2 * cr2);
+                s1[0] += cr0 / len; s1[1] += cr1 / len; s1[2] += cr2 / len;
+            }
+        }
+        constexpr int NTSV = (D == 2 ? 2 : 3);
+#pragma unroll
+        for (int k = 0; k < NTSV; ++k) {
+            int a = tsvVtx[(2 * k) * E + e], b = tsvVtx[(2 * k + 1) * E + e];
+            double dd[D], l2 = 0.0;
+#pragma unroll
+            for (int c = 0; c < D; ++c) { dd[c] = q[c * V + b] - q[c * V + a]; l2 += dd[c] * dd[c]; }
+            double len = sqrt(l2);
+#pragma unroll
+            for (int c = 0; c < D; ++c) s2[c] += dd[c] / len;
+        }
+        double l1 = 0.0, l2 = 0.0;
+#pragma unroll
+        for (int c = 0; c < D; ++c) { l1 += s1[c] * s1[c]; l2 += s2[c] * s2[c]; }
+        l1 = sqrt(l1); l2 = sqrt(l2);
+#pragma unroll
+        for (int c = 0; c < D; ++c) { f1[c] = s1[c] / l1; f2[c] = s2[c] / l2; }
+    }
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+#pragma unroll
+        for (int b = 0; b < D; ++b) geo[(G::INV + a * D + b) * (size_t)E + e] = inv[a][b];
+    geo[G::DET * (size_t)E + e] = det;
+#pragma unroll
+    for (int c = 0; c < D; ++c) { geo[(G::F1 + c) * (size_t)E + e] = f1[c]; geo[(G::F2 + c) * (size_t)E + e] = f2[c]; }
+
+    // FEM element matrices
+    double grad[D + 1][D];
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < D; ++k) { grad[k + 1][c] = inv[k][c]; s -= inv[k][c]; }
+        grad[0][c] = s;
+    }
+    double a1[D + 1], a2[D + 1];
+#pragma unroll
+    for (int k = 0; k <= D; ++k) {
+        double s1 = 0.0, s2 = 0.0;
+#pragma unroll
+        for (int c = 0; c < D; ++c) { s1 += grad[k][c] * f1[c]; s2 += grad[k][c] * f2[c]; }
+        a1[k] = s1; a2[k] = s2;
+    }
+    double ad = fabs(det);
+    femElm[e] = ad / (D == 2 ? 24.0 : 120.0);
+    int sl = 0;
+#pragma unroll
+    for (int a = 0; a <= D; ++a)
+#pragma unroll
+        for (int b = a; b <= D; ++b, ++sl) {
+            double v;
+            if constexpr (D == 2) {
+                v = 0.5 * ad * (spdTan * a1[a] * a1[b] + spdTsv * a2[a] * a2[b]);
+            } else {
+                double gg = grad[a][0] * grad[b][0] + grad[a][1] * grad[b][1] + grad[a][2] * grad[b][2];
+                v = ad / 6.0 * (spdTan * (gg - a1[a] * a1[b]) + spdTsv * a2[a] * a2[b]);
+            }
+            femElm[(1 + sl) * (size_t)E + e] = v;
+        }
+}
+
+// per-nonzero gather of the FEM matrices (femMatGatherKernel,
+// assembleFemMatrix.cu:61-100|120-160): mass M(z) and lhs(z) = M + dt K_D
+__global__ __launch_bounds__(NT) void fem_gather_k(int Z, int E, const int *__restrict__ femIfo,
+                                                   const double *__restrict__ femElm, double dt,
+                                                   double *__restrict__ ppd, double *__restrict__ lhs) {
+    int z = blockIdx.x * blockDim.x + threadIdx.x;
+    if (z >= Z) return;
+    int adj = femIfo[z];
+    double mul = femIfo[Z + z];
+    double pv = 0.0, gv = 0.0;
+    for (int a = 0; a < adj; ++a) {
+        int e = femIfo[(2 + 2 * a) * Z + z];
+        int gi = femIfo[(3 + 2 * a) * Z + z];
+        pv += mul * femElm[e];
+        gv += femElm[(size_t)E + gi];
+    }
+    ppd[z] = pv;
+    lhs[z] = 1.0 * pv + dt * gv;
+}
+
+__global__ void activity_k(int V, const double *__restrict__ tau, double y0, double y1, double y2, double y3,
+                           double r0, double r1, double r2, double r3, double r4, double *__restrict__ ya,
+                           double *__restrict__ ra) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= V) return;
+    double t = tau[i];
+    ya[i] = yank_activity(t, y0, y1, y2, y3);
+    ra[i] = reac_activity(t, r0, r1, r2, r3, r4);
+}
+
+// external yank per element, computeExternalYank.cu:5-47|5-56
+template <int D>
+__global__ __launch_bounds__(NT) void yank_elem_k(int E, const int *__restrict__ elmVtx, const double *__restrict__ geo,
+                                                  const double *__restrict__ act, double *__restrict__ frc) {
+    using G = GeoLayout<D>;
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    double fc = 0.0;
+#pragma unroll
+    for (int k = 0; k <= D; ++k) fc += act[elmVtx[k * E + e]];
+    fc /= (D + 1);
+    double vol = elem_vol<D>(geo[G::DET * (size_t)E + e]);
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+        double s0 = 0.0;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            double g = geo[(G::INV + k * D + c) * (size_t)E + e];
+            s0 -= g;
+            frc[((k + 1) * D + c) * (size_t)E + e] = -fc * g * vol;
+        }
+        frc[(size_t)c * E + e] = -fc * s0 * vol;
+    }
+}
+
+// vertex gather of element vectors (yankGatherKernel / frmElasticGatherKernel)
+// with a partial sum of |out|^2 per block
+template <int D>
+__global__ __launch_bounds__(NT) void gather_k(int V, int E, const int *__restrict__ vtxElm,
+                                               const double *__restrict__ frc, double *__restrict__ out,
+                                               double *__restrict__ P) {
+    __shared__ double red[NT / 64];
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double ss = 0.0;
+    if (i < V) {
+        double acc[D];
+#pragma unroll
+        for (int c = 0; c < D; ++c) acc[c] = 0.0;
+        int adj = vtxElm[i];
+        for (int a = 0; a < adj; ++a) {
+            int e = vtxElm[(1 + 2 * a) * V + i], l = vtxElm[(2 + 2 * a) * V + i];
+#pragma unroll
+            for (int c = 0; c < D; ++c) acc[c] += frc[(l * D + c) * (size_t)E + e];
+        }
+#pragma unroll
+        for (int c = 0; c < D; ++c) { out[c * V + i] = acc[c]; ss += acc[c] * acc[c]; }
+    }
+    double s = block_sum(ss, red);
+    if (P && threadIdx.x == 0) P[blockIdx.x] = s;
+}
+
+// elastic element forces for a velocity field x (strain, stress, |T| Q^{-1} sigma):
+// computeStrainTensor.cu:4-53|4-61 + frmElasticComputeKernel
+// (applyFrameElasticOperator.cu:41-85|113-164).  ISO selects the per-layer
+// isotropic law sigma = (1 - 0.5 u_bar)(lambda tr(eps) I + 2 mu eps).
+template <int D, bool ISO, typename T>
+__global__ __launch_bounds__(NT) void elastic_elem_k(int E, int V, const int *__restrict__ elmVtx,
+                                                     const double *__restrict__ geo, const T *__restrict__ x,
+                                                     const double *__restrict__ tau, const int *__restrict__ elmLyr,
+                                                     const double *__restrict__ mod, T *__restrict__ frc,
+                                                     const Ctl *ctl) {
+    using G = GeoLayout<D>;
+    if (ctl && ctl->done) return;
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    int vtx[D + 1];
+#pragma unroll
+    for (int k = 0; k <= D; ++k) vtx[k] = elmVtx[k * E + e];
+    double inv[D][D];
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+#pragma unroll
+        for (int b = 0; b < D; ++b) inv[a][b] = geo[(G::INV + a * D + b) * (size_t)E + e];
+    double vol = elem_vol<D>(geo[G::DET * (size_t)E + e]);
+    double x0[D];
+#pragma unroll
+    for (int c = 0; c < D; ++c) x0[c] = (double)x[c * V + vtx[0]];
+    double Vm[D][D];
+#pragma unroll
+    for (int k = 0; k < D; ++k)
+#pragma unroll
+        for (int c = 0; c < D; ++c) Vm[c][k] = (double)x[c * V + vtx[k + 1]] - x0[c];
+    double Dv[D][D];
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+#pragma unroll
+        for (int b = 0; b < D; ++b) {
+            double s = 0.0;
+#pragma unroll
+            for (int k = 0; k < D; ++k) s += Vm[a][k] * inv[k][b];
+            Dv[a][b] = s;
+        }
+    double eps[D][D];
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+#pragma unroll
+        for (int b = 0; b < D; ++b) eps[a][b] = (a == b ? Dv[a][a] : 0.5 * (Dv[a][b] + Dv[b][a]));
+    double sig[D][D];
+    if constexpr (ISO) {
+        int l = elmLyr ? elmLyr[e] : 0;
+        double ub = 0.0;
+#pragma unroll
+        for (int k = 0; k <= D; ++k) ub += tau[vtx[k]];
+        ub /= (D + 1);
+        double sc = 1.0 - 0.5 * ub;
+        double lam = mod[2 * l] * sc, mu = mod[2 * l + 1] * sc;
+        double tr = 0.0;
+#pragma unroll
+        for (int c = 0; c < D; ++c) tr += eps[c][c];
+#pragma unroll
+        for (int a = 0; a < D; ++a)
+#pragma unroll
+            for (int b = 0; b < D; ++b) sig[a][b] = (a == b ? lam * tr : 0.0) + 2.0 * mu * eps[a][b];
+    } else {
+        double f1[D], f2[D];
+#pragma unroll
+        for (int c = 0; c < D; ++c) { f1[c] = geo[(G::F1 + c) * (size_t)E + e]; f2[c] = geo[(G::F2 + c) * (size_t)E + e]; }
+        frame_stress<D>(eps, f1, f2, mod, sig);
+    }
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+        double s0 = 0.0;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            double s = 0.0;
+#pragma unroll
+            for (int j = 0; j < D; ++j) s += inv[k][j] * sig[j][c];
+            s0 -= s;
+            frc[((k + 1) * D + c) * (size_t)E + e] = (T)(s * vol);
+        }
+        frc[(size_t)c * E + e] = (T)(s0 * vol);
+    }
+}
+
+// bottom facet measure: 2D segment length, 3D xy-projected area
+// (applyPenaltyOperator.cu:13-14 | 3D :12-14)
+template <int D>
+__global__ void btm_meas_k(int B, int V, const int *__restrict__ btmVtx, const double *__restrict__ q,
+                           double *__restrict__ meas) {
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    int a0 = btmVtx[b], a1 = btmVtx[B + b];
+    if constexpr (D == 2) {
+        double d0 = q[a1] - q[a0], d1 = q[V + a1] - q[V + a0];
+        meas[b] = sqrt(d0 * d0 + d1 * d1);
+    } else {
+        int a2 = btmVtx[2 * B + b];
+        meas[b] = 0.5 * fabs((q[a1] - q[a0]) * (q[V + a2] - q[V + a0]) - (q[V + a1] - q[V + a0]) * (q[a2] - q[a0]));
+    }
+}
+
+// penalty element values v_k |F| / D (penaltyComputeKernel)
+template <int D>
+__global__ void pen_elem_k(int B, int V, const int *__restrict__ btmVtx, const double *__restrict__ meas,
+                           const double *__restrict__ x, double *__restrict__ pfrc, const Ctl *ctl) {
+    if (ctl && ctl->done) return;
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    double m = meas[b];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        int v = btmVtx[k * B + b];
+#pragma unroll
+        for (int c = 0; c < D; ++c) pfrc[(k * D + c) * (size_t)B + b] = x[c * V + v] * m / (double)D;
+    }
+}
+
+// Gather of E*x (+ btmWgt * P*x) and the PCG combinations around it:
+// MODE 0 (iteration): Ad = ldmWgt*Kid + Bd; partial d.Ad
+// MODE 1 (start):     r = b - (ldmWgt*a + Bx); Kid = r; partial r.r
+template <int D, int MODE>
+__global__ __launch_bounds__(NT) void gather_op_k(int V, int E, const int *__restrict__ vtxElm,
+                                                  const double *__restrict__ frc, int BL, int B,
+                                                  const int *__restrict__ vtxBtm, const double *__restrict__ pfrc,
+                                                  double btmWgt, double ldmWgt, const double *__restrict__ u,
+                                                  const double *__restrict__ w, double *__restrict__ out,
+                                                  double *__restrict__ out2, double *__restrict__ P, const Ctl *ctl) {
+    __shared__ double red[NT / 64];
+    if (ctl->done) return;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double ss = 0.0;
+    if (i < V) {
+        double acc[D];
+#pragma unroll
+        for (int c = 0; c < D; ++c) acc[c] = 0.0;
+        int adj = vtxElm[i];
+        for (int a = 0; a < adj; ++a) {
+            int e = vtxElm[(1 + 2 * a) * V + i], l = vtxElm[(2 + 2 * a) * V + i];
+#pragma unroll
+            for (int c = 0; c < D; ++c) acc[c] += frc[(l * D + c) * (size_t)E + e];
+        }
+        if (pfrc && i < BL) {
+            double pa[D];
+#pragma unroll
+            for (int c = 0; c < D; ++c) pa[c] = 0.0;
+            int adj2 = vtxBtm[i];
+            for (int a = 0; a < adj2; ++a) {
+                int b = vtxBtm[(1 + 2 * a) * BL + i], l = vtxBtm[(2 + 2 * a) * BL + i];
+#pragma unroll
+                for (int c = 0; c < D; ++c) pa[c] += pfrc[(l * D + c) * (size_t)B + b];
+            }
+#pragma unroll
+            for (int c = 0; c < D; ++c) acc[c] = 1.0 * acc[c] + btmWgt * pa[c];
+        }
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            int k = c * V + i;
+            if constexpr (MODE == 0) {
+                // u = Kid, w = d
+                double ad = ldmWgt * u[k] + 1.0 * acc[c];
+                out[k] = ad;
+                ss += w[k] * ad;
+            } else {
+                // u = a (alpha), w = b (rhs); out = r, out2 = Kid
+                double ax = ldmWgt * u[k] + acc[c];
+                double r = 1.0 * w[k] - 1.0 * ax;
+                out[k] = r;
+                out2[k] = r;
+                ss += r * r;
+            }
+        }
+    }
+    double s = block_sum(ss, red);
+    if (threadIdx.x == 0) P[blockIdx.x] = s;
+}
+
+// right-hand side |b|^2 and stopping threshold; bb == 0 -> x = 0 (ipcg.cu:25-30)
+__global__ void pcg_start_k(const double *__restrict__ P, int nP, double tolSqu, Ctl *ctl) {
+    __shared__ double red[NT / 64];
+    double bb = reduce_partials(P, nP, red);
+    if (threadIdx.x == 0) {
+        ctl->bb = bb;
+        ctl->tol = bb * tolSqu;
+        ctl->done = (bb == 0.0) ? 1 : 0;
+        ctl->status = 0;
+        ctl->iters = 0;
+    }
+}
+
+__global__ void zero_if_empty_k(int n, double *__restrict__ x, double *__restrict__ a, const Ctl *ctl) {
+    if (ctl->bb != 0.0) return;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) { x[i] = 0.0; if (a) a[i] = 0.0; }
+}
+
+// x += st d; a += st Kid; r -= st Ad with st = rKr / d.Ad; partial r.r
+__global__ __launch_bounds__(NT) void pcg_update_k(int n, const double *__restrict__ P, int nP,
+                                                   const double *__restrict__ hist, int it,
+                                                   double *__restrict__ x, double *__restrict__ a,
+                                                   double *__restrict__ r, const double *__restrict__ d,
+                                                   const double *__restrict__ Kid, const double *__restrict__ Ad,
+                                                   double *__restrict__ P2, Ctl *ctl) {
+    __shared__ double red[NT / 64];
+    if (ctl->done) return;
+    double dAd = reduce_partials(P, nP, red);
+    if (dAd == 0.0) {   // ipcg.cu:63-68
+        if (blockIdx.x == 0 && threadIdx.x == 0) { ctl->status = 1; ctl->done = 1; ctl->iters = it; }
+        return;
+    }
+    double st = hist[it] / dAd;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double ss = 0.0;
+    if (i < n) {
+        x[i] = 1.0 * x[i] + st * d[i];
+        if (a) a[i] = a[i] + st * Kid[i];
+        double ri = 1.0 * r[i] - st * Ad[i];
+        r[i] = ri;
+        ss = ri * ri;
+    }
+    double s = block_sum(ss, red);
+    if (threadIdx.x == 0) P2[blockIdx.x] = s;
+}
+
+// beta = r.Kr_new / r.Kr_old; Kid = r + beta Kid; d = Kr + beta d
+// it < 0: first direction d = Kr (ipcg.cu:41-53)
+__global__ __launch_bounds__(NT) void pcg_dir_k(int n, const double *__restrict__ P, int nP, double *__restrict__ hist,
+                                                int it, const double *__restrict__ r, const double *__restrict__ Kr,
+                                                double *__restrict__ Kid, double *__restrict__ d, Ctl *ctl) {
+    __shared__ double red[NT / 64];
+    if (ctl->done) return;
+    double rKr = reduce_partials(P, nP, red);
+    if (rKr == 0.0) {   // ipcg.cu:48-53, 83-88
+        if (blockIdx.x == 0 && threadIdx.x == 0) { ctl->status = 1; ctl->done = 1; ctl->iters = it + 1; }
+        return;
+    }
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (it < 0) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) hist[0] = rKr;
+        if (i < n) d[i] = Kr[i];
+        return;
+    }
+    double be = rKr / hist[it];
+    if (blockIdx.x == 0 && threadIdx.x == 0) hist[it + 1] = rKr;
+    if (i < n) {
+        Kid[i] = 1.0 * r[i] + be * Kid[i];
+        d[i] = 1.0 * Kr[i] + be * d[i];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// y = (K + eps I) x, K_ij = k(|q_i - q_j| / width), matrix-free.
+// Block = 64 rows x 4 column phases; columns are staged through LDS 256 at a
+// time and every lane of a wave reads the same column (LDS broadcast).  The 4
+// phase sums of a row are combined in a fixed order.
+// CHECK: first reduce |r|^2 of the predecessor, stop when converged, and
+// emit partials of x.y (r.Kr).
+constexpr int MV_ROWS = 64;
+constexpr int MV_PH = 4;
+
+template <int D, int ORD, bool CHECK>
+__global__ __launch_bounds__(NT) void kmatvec_k(int V, const double *__restrict__ q, const double *__restrict__ x,
+                                                double *__restrict__ y, double eps, double invW,
+                                                const double *__restrict__ Prr, int nPrr, double *__restrict__ P3,
+                                                int it, Ctl *ctl) {
+    __shared__ double sq[D][NT];
+    __shared__ double sx[D][NT];
+    __shared__ double racc[MV_PH][D][MV_ROWS];
+    __shared__ double red[NT / 64];
+    if (ctl && ctl->done) return;
+    if constexpr (CHECK) {
+        double rr = reduce_partials(Prr, nPrr, red);
+        if (rr < ctl->tol) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) { ctl->done = 1; ctl->iters = it + 1; }
+            return;
+        }
+    }
+    const int lr = threadIdx.x & (MV_ROWS - 1), ph = threadIdx.x / MV_ROWS;
+    const int i = blockIdx.x * MV_ROWS + lr;
+    double qi[D], acc[D];
+#pragma unroll
+    for (int c = 0; c < D; ++c) { qi[c] = (i < V) ? q[c * V + i] : 0.0; acc[c] = 0.0; }
+    for (int j0 = 0; j0 < V; j0 += NT) {
+        int j = j0 + threadIdx.x;
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            sq[c][threadIdx.x] = (j < V) ? q[c * V + j] : 0.0;
+            sx[c][threadIdx.x] = (j < V) ? x[c * V + j] : 0.0;
+        }
+        __syncthreads();
+        int jn = min(NT, V - j0);
+        for (int jj = ph; jj < jn; jj += MV_PH) {
+            double d2 = 0.0;
+#pragma unroll
+            for (int c = 0; c < D; ++c) { double dd = qi[c] - sq[c][jj]; d2 += dd * dd; }
+            double k = rkhs_kernel<D, ORD>(d2, invW);
+#pragma unroll
+            for (int c = 0; c < D; ++c) acc[c] += k * sx[c][jj];
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < D; ++c) racc[ph][c][lr] = acc[c];
+    __syncthreads();
+    double dotp = 0.0;
+    if (ph == 0 && i < V) {
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            double s = racc[0][c][lr];
+#pragma unroll
+            for (int p = 1; p < MV_PH; ++p) s += racc[p][c][lr];
+            double xi = x[c * V + i];
+            s += eps * xi;
+            y[c * V + i] = s;
+            dotp += xi * s;
+        }
+    }
+    if constexpr (CHECK) {
+        double s = block_sum(dotp, red);
+        if (threadIdx.x == 0) P3[blockIdx.x] = s;
+    }
+}
+
+// rectangular y(i) = sum_j k(|a_i - b_j|) x(j) for the control-point form
+template <int D, int ORD>
+__global__ __launch_bounds__(NT) void kcross_k(int na, const double *__restrict__ a, int nb,
+                                               const double *__restrict__ b, const double *__restrict__ x,
+                                               double *__restrict__ y, double invW, const Ctl *ctl) {
+    __shared__ double sq[D][NT];
+    __shared__ double sx[D][NT];
+    __shared__ double racc[MV_PH][D][MV_ROWS];
+    if (ctl && ctl->done) return;
+    const int lr = threadIdx.x & (MV_ROWS - 1), ph = threadIdx.x / MV_ROWS;
+    const int i = blockIdx.x * MV_ROWS + lr;
+    double qi[D], acc[D];
+#pragma unroll
+    for (int c = 0; c < D; ++c) { qi[c] = (i < na) ? a[c * na + i] : 0.0; acc[c] = 0.0; }
+    for (int j0 = 0; j0 < nb; j0 += NT) {
+        int j = j0 + threadIdx.x;
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            sq[c][threadIdx.x] = (j < nb) ? b[c * nb + j] : 0.0;
+            sx[c][threadIdx.x] = (j < nb) ? x[c * nb + j] : 0.0;
+        }
+        __syncthreads();
+        int jn = min(NT, nb - j0);
+        for (int jj = ph; jj < jn; jj += MV_PH) {
+            double d2 = 0.0;
+#pragma unroll
+            for (int c = 0; c < D; ++c) { double dd = qi[c] - sq[c][jj]; d2 += dd * dd; }
+            double k = rkhs_kernel<D, ORD>(d2, invW);
+#pragma unroll
+            for (int c = 0; c < D; ++c) acc[c] += k * sx[c][jj];
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < D; ++c) racc[ph][c][lr] = acc[c];
+    __syncthreads();
+    if (ph == 0 && i < na) {
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            double s = racc[0][c][lr];
+#pragma unroll
+            for (int p = 1; p < MV_PH; ++p) s += racc[p][c][lr];
+            y[c * na + i] = s;
+        }
+    }
+}
+
+__global__ void axpby_k(int n, double *__restrict__ out, double a, const double *__restrict__ x, double b,
+                        const double *__restrict__ y) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = a * x[i] + b * y[i];
+}
+
+// ---------------------------------------------------------------------------
+// reaction-diffusion: rhs = M_t tau_t + dt * (M_T r) gathered
+// (assembleFemVector.cu + the Dgemv of deform.cu:95-97)
+
+template <int D>
+__global__ __launch_bounds__(NT) void femvec_elem_k(int E, const int *__restrict__ elmVtx, const double *__restrict__ geo,
+                                                    const double *__restrict__ r, double *__restrict__ frc) {
+    using G = GeoLayout<D>;
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    double ad = fabs(geo[G::DET * (size_t)E + e]);
+    double rv[D + 1], s = 0.0;
+#pragma unroll
+    for (int k = 0; k <= D; ++k) { rv[k] = r[elmVtx[k * E + e]]; s += rv[k]; }
+#pragma unroll
+    for (int k = 0; k <= D; ++k) frc[(size_t)k * E + e] = (s + rv[k]) * ad / (D == 2 ? 24.0 : 120.0);
+}
+
+// rhs_i = sum_z M(z) tau(col z) + dt * F_i, and the Jacobi-CG start:
+// r = rhs - A x, z = r / diag, d = z; partials r.r, r.z, rhs.rhs
+__global__ __launch_bounds__(NT) void rd_start_k(int V, int E, const int *__restrict__ vtxElm,
+                                                 const double *__restrict__ frc, const int *__restrict__ rowPtr,
+                                                 const int *__restrict__ colIdx, const int *__restrict__ diagIdx,
+                                                 const double *__restrict__ ppd, const double *__restrict__ lhs,
+                                                 const double *__restrict__ tau, double dt, double *__restrict__ x,
+                                                 double *__restrict__ rhs, double *__restrict__ r,
+                                                 double *__restrict__ zv, double *__restrict__ d,
+                                                 double *__restrict__ P) {
+    __shared__ double red[NT / 64];
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double s_rr = 0.0, s_rz = 0.0, s_bb = 0.0;
+    if (i < V) {
+        double F = 0.0;
+        int adj = vtxElm[i];
+        for (int a = 0; a < adj; ++a) {
+            int e = vtxElm[(1 + 2 * a) * V + i], l = vtxElm[(2 + 2 * a) * V + i];
+            F += frc[(size_t)l * E + e];
+        }
+        double mt = 0.0, ax = 0.0;
+        int z0 = rowPtr[i], z1 = rowPtr[i + 1];
+        for (int z = z0; z < z1; ++z) {
+            double tc = tau[colIdx[z]];
+            mt += ppd[z] * tc;
+            ax += lhs[z] * tc;   // warm start x = tau_t
+        }
+        double b = 1.0 * mt + dt * F;
+        double xi = tau[i];
+        x[i] = xi;
+        rhs[i] = b;
+        double ri = b - ax;
+        double zi = ri / lhs[diagIdx[i]];
+        r[i] = ri;
+        zv[i] = zi;
+        d[i] = zi;
+        s_rr = ri * ri;
+        s_rz = ri * zi;
+        s_bb = b * b;
+    }
+    double a0 = block_sum(s_rr, red);
+    double a1 = block_sum(s_rz, red);
+    double a2 = block_sum(s_bb, red);
+    if (threadIdx.x == 0) {
+        P[blockIdx.x] = a0;
+        P[MAXP + blockIdx.x] = a1;
+        P[2 * MAXP + blockIdx.x] = a2;
+    }
+}
+
+__global__ void rd_fin_k(const double *__restrict__ P, int nP, double tolSqu, double *__restrict__ hist, Ctl *ctl) {
+    __shared__ double red[NT / 64];
+    double rr = reduce_partials(P, nP, red);
+    double rz = reduce_partials(P + MAXP, nP, red);
+    double bb = reduce_partials(P + 2 * MAXP, nP, red);
+    if (threadIdx.x == 0) {
+        ctl->bb = bb;
+        ctl->tol = bb * tolSqu;
+        ctl->status = 0;
+        ctl->iters = 0;
+        ctl->done = (bb == 0.0 || rr < bb * tolSqu) ? 1 : 0;
+        hist[0] = rz;
+    }
+}
+
+__global__ __launch_bounds__(NT) void rd_spmv_k(int V, const int *__restrict__ rowPtr, const int *__restrict__ colIdx,
+                                                const double *__restrict__ A, const double *__restrict__ d,
+                                                double *__restrict__ Ad, double *__restrict__ P, const Ctl *ctl) {
+    __shared__ double red[NT / 64];
+    if (ctl->done) return;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double ss = 0.0;
+    if (i < V) {
+        double s = 0.0;
+        for (int z = rowPtr[i]; z < rowPtr[i + 1]; ++z) s += A[z] * d[colIdx[z]];
+        Ad[i] = s;
+        ss = d[i] * s;
+    }
+    double t = block_sum(ss, red);
+    if (threadIdx.x == 0) P[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(NT) void rd_update_k(int V, const double *__restrict__ P, int nP,
+                                                  const double *__restrict__ hist, int it, const int *__restrict__ diagIdx,
+                                                  const double *__restrict__ A, double *__restrict__ x,
+                                                  double *__restrict__ r, double *__restrict__ zv,
+                                                  const double *__restrict__ d, const double *__restrict__ Ad,
+                                                  double *__restrict__ P2, Ctl *ctl) {
+    __shared__ double red[NT / 64];
+    if (ctl->done) return;
+    double dAd = reduce_partials(P, nP, red);
+    double al = hist[it] / dAd;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double s_rr = 0.0, s_rz = 0.0;
+    if (i < V) {
+        x[i] += al * d[i];
+        double ri = r[i] - al * Ad[i];
+        double zi = ri / A[diagIdx[i]];
+        r[i] = ri;
+        zv[i] = zi;
+        s_rr = ri * ri;
+        s_rz = ri * zi;
+    }
+    double a0 = block_sum(s_rr, red);
+    double a1 = block_sum(s_rz, red);
+    if (threadIdx.x == 0) { P2[blockIdx.x] = a0; P2[MAXP + blockIdx.x] = a1; }
+}
+
+__global__ __launch_bounds__(NT) void rd_dir_k(int V, const double *__restrict__ P, int nP, double *__restrict__ hist,
+                                               int it, const double *__restrict__ zv, double *__restrict__ d, Ctl *ctl) {
+    __shared__ double red[NT / 64];
+    if (ctl->done) return;
+    double rr = reduce_partials(P, nP, red);
+    if (rr < ctl->tol) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) { ctl->done = 1; ctl->iters = it + 1; }
+        return;
+    }
+    double rz = reduce_partials(P + MAXP, nP, red);
+    double be = rz / hist[it];
+    if (blockIdx.x == 0 && threadIdx.x == 0) hist[it + 1] = rz;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < V) d[i] = zv[i] + be * d[i];
+}
+
+// ---------------------------------------------------------------------------
+// varifold (2D varifold.cu:9-357, 3D :9-370)
+
+template <int D>
+__global__ void facets_k(int F, int V, const int *__restrict__ vfdVtx, const double *__restrict__ q,
+                         double *__restrict__ cen, double *__restrict__ dir, double *__restrict__ vol) {
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= F) return;
+    if constexpr (D == 2) {
+        int a = vfdVtx[f], b = vfdVtx[F + f];
+        double d0 = q[b] - q[a], d1 = q[V + b] - q[V + a];
+        cen[f] = 0.5 * (q[a] + q[b]);
+        cen[F + f] = 0.5 * (q[V + a] + q[V + b]);
+        double len = sqrt(d0 * d0 + d1 * d1);
+        dir[f] = d0 / len; dir[F + f] = d1 / len;
+        vol[f] = len;
+    } else {
+        int a = vfdVtx[f], b = vfdVtx[F + f], c = vfdVtx[2 * F + f];
+        double u[3], w[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            cen[i * F + f] = (q[i * V + a] + q[i * V + b] + q[i * V + c]) / 3.0;
+            u[i] = q[i * V + b] - q[i * V + a];
+            w[i] = q[i * V + c] - q[i * V + a];
+        }
+        double cr0 = u[1] * w[2] - u[2] * w[1], cr1 = -u[0] * w[2] + u[2] * w[0], cr2 = u[0] * w[1] - u[1] * w[0];
+        double len = sqrt(cr0 * cr0 + cr1 * cr1 + cr2 * cr2);
+        dir[f] = cr0 / len; dir[F + f] = cr1 / len; dir[2 * F + f] = cr2 / len;
+        vol[f] = 0.5 * len;
+    }
+}
+
+struct VfdPrm {
+    int cenType, dirType;
+    double cenInvW2, dirInvW2;
+};
+
+// Row sums of the varifold pair kernel.  Blocks [0, nbD) take rows of the
+// evolved set (columns: evolved set, then target set with factor -2); blocks
+// [nbD, nbD + nbT) take target rows against the target set.  GRAD also
+// accumulates d/dcentre and d/d(unnormalised direction) of each evolved row.
+template <int D, bool GRAD>
+__global__ __launch_bounds__(NT) void vfd_rows_k(int F, int Ft, int nbD, const double *__restrict__ dc_,
+                                                 const double *__restrict__ du_, const double *__restrict__ dw_,
+                                                 const double *__restrict__ tc_, const double *__restrict__ tu_,
+                                                 const double *__restrict__ tw_, VfdPrm vp,
+                                                 double *__restrict__ rowVal, double *__restrict__ gC,
+                                                 double *__restrict__ gN) {
+    __shared__ double sc[D][NT], su[D][NT], sw[NT];
+    __shared__ double racc[MV_PH][1 + 2 * D][MV_ROWS];
+    const int lr = threadIdx.x & (MV_ROWS - 1), ph = threadIdx.x / MV_ROWS;
+    const bool drow = blockIdx.x < nbD;
+    const int i = (drow ? blockIdx.x : blockIdx.x - nbD) * MV_ROWS + lr;
+    const int nrow = drow ? F : Ft;
+    const double *rc = drow ? dc_ : tc_, *ru = drow ? du_ : tu_, *rw = drow ? dw_ : tw_;
+    double ci[D], ui[D], wi = 0.0;
+#pragma unroll
+    for (int c = 0; c < D; ++c) { ci[c] = i < nrow ? rc[c * nrow + i] : 0.0; ui[c] = i < nrow ? ru[c * nrow + i] : 0.0; }
+    if (i < nrow) wi = rw[i];
+    const double nlen = (D == 2 ? wi : 2.0 * wi);   // |T| (2D) or |N| = 2 area (3D)
+    const double half = (D == 2 ? 1.0 : 0.5);
+    double val = 0.0, gc[D], gn[D];
+#pragma unroll
+    for (int c = 0; c < D; ++c) { gc[c] = 0.0; gn[c] = 0.0; }
+    const int npass = drow ? 2 : 1;
+    for (int pass = 0; pass < npass; ++pass) {
+        const bool tgt = (!drow) || pass == 1;
+        const double *pc = tgt ? tc_ : dc_, *pu = tgt ? tu_ : du_, *pw = tgt ? tw_ : dw_;
+        const int n = tgt ? Ft : F;
+        const double sg = (drow && pass == 1) ? -2.0 : 1.0;
+        double pv = 0.0, pgc[D], pgn[D];
+#pragma unroll
+        for (int c = 0; c < D; ++c) { pgc[c] = 0.0; pgn[c] = 0.0; }
+        for (int j0 = 0; j0 < n; j0 += NT) {
+            int j = j0 + threadIdx.x;
+            __syncthreads();
+#pragma unroll
+            for (int c = 0; c < D; ++c) {
+                sc[c][threadIdx.x] = j < n ? pc[c * n + j] : 0.0;
+                su[c][threadIdx.x] = j < n ? pu[c * n + j] : 0.0;
+            }
+            sw[threadIdx.x] = j < n ? pw[j] : 0.0;
+            __syncthreads();
+            int jn = min(NT, n - j0);
+            for (int jj = ph; jj < jn; jj += MV_PH) {
+                double d2 = 0.0, ang = 0.0, dv[D];
+#pragma unroll
+                for (int c = 0; c < D; ++c) { dv[c] = ci[c] - sc[c][jj]; d2 += dv[c] * dv[c]; ang += ui[c] * su[c][jj]; }
+                double kc = cen_kernel(vp.cenType, d2, vp.cenInvW2);
+                double ku = dir_kernel(vp.dirType, ang, vp.dirInvW2);
+                double wj = sw[jj];
+                pv += kc * ku * wj;
+                if constexpr (GRAD) {
+                    double kcd = cen_kernel_d(vp.cenType, kc, vp.cenInvW2);
+                    double kud = dir_kernel_d(vp.dirType, ang, ku, vp.dirInvW2);
+#pragma unroll
+                    for (int c = 0; c < D; ++c) {
+                        double dku = kud / nlen * (-ang * ui[c] + su[c][jj]);
+                        pgc[c] += kcd * dv[c] * ku * wj;
+                        pgn[c] += kc * (dku * wi + ku * half * ui[c]) * wj;
+                    }
+                }
+            }
+        }
+        val += sg * pv * wi;
+        if constexpr (GRAD) {
+            // d/dc_i of the DD double sum and of -2 DT both carry factor 2
+            const double gs = (drow && pass == 1) ? -2.0 : 2.0;
+#pragma unroll
+            for (int c = 0; c < D; ++c) { gc[c] += gs * pgc[c] * wi; gn[c] += gs * pgn[c]; }
+        }
+    }
+    racc[ph][0][lr] = val;
+    if constexpr (GRAD) {
+#pragma unroll
+        for (int c = 0; c < D; ++c) { racc[ph][1 + c][lr] = gc[c]; racc[ph][1 + D + c][lr] = gn[c]; }
+    }
+    __syncthreads();
+    if (ph == 0 && i < nrow) {
+        double s = racc[0][0][lr];
+#pragma unroll
+        for (int p = 1; p < MV_PH; ++p) s += racc[p][0][lr];
+        rowVal[(drow ? 0 : F) + i] = s;
+        if constexpr (GRAD) {
+            if (drow) {
+#pragma unroll
+                for (int c = 0; c < D; ++c) {
+                    double a = racc[0][1 + c][lr], b = racc[0][1 + D + c][lr];
+#pragma unroll
+                    for (int p = 1; p < MV_PH; ++p) { a += racc[p][1 + c][lr]; b += racc[p][1 + D + c][lr]; }
+                    gC[c * F + i] = a;
+                    gN[c * F + i] = b;
+                }
+            }
+        }
+    }
+}
+
+// facet -> vertex gradient (dqVfdGatherKernel, varifold.cu:472-498|493-530)
+template <int D>
+__global__ void vfd_grad_k(int V, int F, const int *__restrict__ adjTbl, const int *__restrict__ vfdVtx,
+                           const double *__restrict__ q, const double *__restrict__ gC, const double *__restrict__ gN,
+                           double *__restrict__ grad) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= V) return;
+    double g[D];
+#pragma unroll
+    for (int c = 0; c < D; ++c) g[c] = 0.0;
+    int adj = adjTbl[i];
+    for (int a = 0; a < adj; ++a) {
+        int f = adjTbl[(1 + 2 * a) * V + i], l = adjTbl[(2 + 2 * a) * V + i];
+        if constexpr (D == 2) {
+            double sgn = (l == 0 ? -1.0 : 1.0);
+            g[0] += 0.5 * gC[f] + sgn * gN[f];
+            g[1] += 0.5 * gC[F + f] + sgn * gN[F + f];
+        } else {
+            int qa = vfdVtx[((l + 2) % 3) * F + f], qb = vfdVtx[((l + 1) % 3) * F + f];
+            double dd0 = q[qa] - q[qb], dd1 = q[V + qa] - q[V + qb], dd2 = q[2 * V + qa] - q[2 * V + qb];
+            double n0 = gN[f], n1 = gN[F + f], n2 = gN[2 * F + f];
+            g[0] += gC[f] / 3.0 + (n1 * dd2 - n2 * dd1);
+            g[1] += gC[F + f] / 3.0 + (-n0 * dd2 + n2 * dd0);
+            g[2] += gC[2 * F + f] / 3.0 + (n0 * dd1 - n1 * dd0);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < D; ++c) grad[c * V + i] = g[c];
+}
+
+// single-block fixed-order sum of n values into out[0]
+__global__ void sum_k(const double *__restrict__ v, int n, double *__restrict__ out) {
+    __shared__ double red[NT / 64];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) s += v[i];
+    s = block_sum(s, red);
+    if (threadIdx.x == 0) out[0] = s;
+}
+
+}  // namespace sa

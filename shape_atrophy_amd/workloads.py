@@ -102,8 +102,10 @@ def strip_problem(vfd_layers=(1, 31)) -> dict:
     """The demo strip: 101 x 31 landmarks on [-3,3] x [-1,1]
     (demo/getInitialShape.m:1-15), run_demo.m parameters and tau bump."""
     L, C = 101, 31
-    x = np.linspace(-3.0, 3.0, L)
-    y = np.linspace(-1.0, 1.0, C)
+    # MATLAB linspace(d1, d2, n) = d1 + (0:n-1) * (d2 - d1) / (n - 1), which
+    # rounds differently from numpy.linspace in the last bit
+    x = -3.0 + np.arange(L) * 6.0 / (L - 1)
+    y = -1.0 + np.arange(C) * 2.0 / (C - 1)
     pos = np.stack([np.tile(x, C), np.repeat(y, L)], axis=1)
     elm = _mesh.generate_element_object_2d(L, C, list(vfd_layers))
     return dict(dim=2, elm=elm, q0=pos.T.copy(), params=demo_params(),

@@ -1,0 +1,61 @@
+"""The C ABI library loads without a GPU, exports every function that
+include/shape_atrophy.h declares, and rejects malformed inputs with SA_E_ARG
+before touching the device."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from shape_atrophy_amd import api, workloads as W
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    txt = open(os.path.join(ROOT, "include", "shape_atrophy.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(sa_[a-z_0-9]+)\s*\(", txt)))
+
+
+def test_header_symbols_exported():
+    lib = api.lib()
+    names = declared_functions()
+    assert {"sa_deformation", "sa_compute_objfcn", "sa_session_create", "sa_session_run"} <= set(names)
+    for n in names:
+        assert hasattr(lib, n), n
+
+
+def test_invalid_tables_rejected_without_device():
+    p = W.ring_problem(n_rings=4, n_per_ring=12)
+    elm = dict(p["elm"])
+    bad = elm["elmVtxMat"].copy()
+    bad[0, 3] = 10_000
+    elm["elmVtxMat"] = bad
+    args = api.problem_args(dict(p, elm=elm))
+    with pytest.raises(api.SaError) as ei:
+        api.computeObjfcn(*args)
+    assert ei.value.code == -1 and "out of range" in str(ei.value)
+
+
+def test_unknown_elasticity_and_dimension():
+    p = W.ring_problem(n_rings=4, n_per_ring=12)
+    args = api.problem_args(p)
+    args[2] = "neohookean"
+    with pytest.raises(ValueError, match="Unknown elasticity type"):
+        api.computeObjfcn(*args)
+    args = api.problem_args(p)
+    args[3] = np.zeros((4, p["q0"].shape[1]))
+    with pytest.raises(ValueError, match="Dimension 4 is not supported"):
+        api.computeObjfcn(*args)
+
+
+def test_unsorted_fem_table_rejected():
+    p = W.ring_problem(n_rings=4, n_per_ring=12)
+    elm = dict(p["elm"])
+    fv = elm["femVtxMat"].copy()
+    fv[:, [3, 4]] = fv[:, [4, 3]]
+    elm["femVtxMat"] = fv
+    with pytest.raises(api.SaError, match="not sorted"):
+        api.computeObjfcn(*api.problem_args(dict(p, elm=elm)))

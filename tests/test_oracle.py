@@ -1,0 +1,392 @@
+"""Pins of the FP64 oracle.  No golden vectors exist in the reference beyond
+demo/initialShape.mat, so each oracle function is checked against what the
+mathematics fixes: library routines (scipy Bessel functions, dense LAPACK
+solves), energies the operators must reproduce, conservation identities,
+brute-force sums and finite differences.  A dropped term, a sign or index
+error or a transposed operand in the oracle breaks one of these."""
+import numpy as np
+import pytest
+import scipy.linalg
+import scipy.special as sp
+
+from shape_atrophy_amd import workloads as W
+
+
+def small_ring(**kw):
+    return W.ring_problem(n_rings=6, n_per_ring=40, **kw)
+
+
+def small_shell(**kw):
+    return W.shell_problem(level=1, n_layers=3, time_steps=10, **kw)
+
+
+# --------------------------------------------------------------------------
+# kernels
+
+def test_bessel_k01_against_scipy(oracle):
+    x = np.concatenate([np.geomspace(1e-6, 2.0, 400), np.linspace(2.0, 60.0, 400), [100.0, 300.0, 650.0]])
+    k0, k1 = oracle.bessel_k01(x)
+    np.testing.assert_allclose(k0, sp.k0(x), rtol=4e-15)
+    np.testing.assert_allclose(k1, sp.k1(x), rtol=4e-15)
+
+
+def matern(nu, rho):
+    """Matern kernel without sqrt(2 nu) rescaling, 2^{1-nu}/Gamma(nu) rho^nu K_nu(rho)."""
+    rho = np.maximum(rho, 1e-300)
+    with np.errstate(invalid="ignore", over="ignore"):
+        return 2.0 ** (1.0 - nu) / sp.gamma(nu) * rho ** nu * sp.kv(nu, rho)
+
+
+@pytest.mark.parametrize("dim,order", [(3, 0), (3, 1), (3, 2), (3, 3), (3, 4), (2, 0), (2, 1), (2, 2), (2, 3), (2, 4)])
+def test_matern_kernels_against_bessel_kv(oracle, dim, order):
+    w = 0.2
+    r = np.linspace(1e-3, 8.0, 500)
+    got = oracle.kernel_values(dim, order, w, (r * w) ** 2)
+    nu = order + (0.5 if dim == 3 else 1.0)
+    np.testing.assert_allclose(got, matern(nu, r), rtol=1e-13)
+    # unit value at zero distance
+    assert oracle.kernel_values(dim, order, w, np.zeros(1))[0] == pytest.approx(1.0, abs=1e-15)
+
+
+def test_gaussian_kernel(oracle):
+    d2 = np.linspace(0, 1, 50)
+    np.testing.assert_allclose(oracle.kernel_values(3, -1, 0.3, d2), np.exp(-d2 / 0.09), rtol=1e-15)
+
+
+def kernel_matrix(q, order, w, eps):
+    d = q.shape[0]
+    D = np.sqrt(((q[:, :, None] - q[:, None, :]) ** 2).sum(0)) / w
+    nu = order + (0.5 if d == 3 else 1.0)
+    K = np.exp(-D ** 2) if order == -1 else matern(nu, D)
+    np.fill_diagonal(K, 1.0 + eps)
+    return K
+
+
+@pytest.mark.parametrize("maker", [small_ring, small_shell])
+def test_kernel_matvec_dense(oracle, maker):
+    p = maker()
+    q = p["q0"]
+    x = np.random.default_rng(1).standard_normal(q.shape)
+    for order in (-1, 1, 3):
+        K = kernel_matrix(q, order, 0.2, 1e-10)
+        got = oracle.kernel_matvec(p["dim"], order, 0.2, 1e-10, q, x)
+        np.testing.assert_allclose(got, x @ K.T, rtol=1e-12, atol=1e-12 * np.abs(x @ K.T).max())
+
+
+# --------------------------------------------------------------------------
+# activities and initial bump
+
+def test_activities_breakpoints_and_smoothness(oracle):
+    ynk = [0.01, 1.0, 4.0, 0.5]
+    rea = [0.01, 1.0, 4.0, 0.5, 0.3]
+    tau = np.linspace(-0.2, 1.2, 14001)
+    ya, ra = oracle.activities(tau, ynk, rea)
+    x = -1 + 2 * (tau - 0.01) / 0.99
+    assert np.all(ya[(x < -1) | (x >= 1)] == 0) and np.all(ra[(x < -1) | (x >= 1)] == 0)
+    # middle pieces: actMax * (1 - c x^2) and actMax * (1 - a x^2)
+    dw = 0.5
+    c = 6.0 / (6.0 + dw * (-6.0 + dw))
+    mid = (x >= -1 + dw) & (x < 1 - dw)
+    np.testing.assert_allclose(ya[mid], 4.0 * (1 - c * x[mid] ** 2), rtol=1e-14)
+    np.testing.assert_allclose(ra[mid], 4.0 * (1 - 0.3 * x[mid] ** 2), rtol=1e-14)
+    # C2: first and second differences stay bounded across the breakpoints
+    h = tau[1] - tau[0]
+    for f in (ya, ra):
+        d1 = np.diff(f) / h
+        d2 = np.diff(f, 2) / h ** 2
+        assert np.abs(np.diff(d1)).max() < 50 * h * np.abs(d2).max() + 1e-9
+        assert np.abs(np.diff(d2)).max() < 0.2 * np.abs(d2).max()
+
+
+def test_reaction_is_fisher_kpp_on_interior(oracle):
+    r = 5.0
+    tau = np.linspace(0.06, 0.94, 200)
+    _, ra = oracle.activities(tau, [0.01, 1.0, 4.0, 0.5], [0.0, 1.0, r / 4, 0.1, 1.0])
+    np.testing.assert_allclose(ra, r * tau * (1 - tau), rtol=1e-13)
+
+
+def test_tau_initial_closed_form(oracle):
+    p = small_shell()
+    for pw in (1, 2, 3):
+        p["tauPrm"] = np.array([0.4, float(pw)])
+        tau = oracle.tau_initial(p)
+        c = p["tauVar"][:3, 0]
+        rho2 = ((p["q0"] - c[:, None]) ** 2).sum(0) / 0.16
+        ref = np.where(rho2 < 1, 1 - (1 - (rho2 - 1) ** 2) ** pw, 0.0) * p["tauVar"][3, 0]
+        np.testing.assert_allclose(tau, ref, rtol=1e-13, atol=1e-15)
+
+
+# --------------------------------------------------------------------------
+# elastic operator through the strain energy it must derive from
+
+def elem_grads(q, elm):
+    """P1 gradients (E, D+1, D) and volumes via numpy.linalg.inv."""
+    d = q.shape[0]
+    X = q.T[elm.T]                         # (E, D+1, D)
+    Q = np.transpose(X[:, 1:] - X[:, :1], (0, 2, 1))   # columns q_k - q_0
+    Qi = np.linalg.inv(Q)
+    g = np.concatenate([-Qi.sum(1, keepdims=True), Qi], axis=1)
+    vol = np.abs(np.linalg.det(Q)) / (2.0 if d == 2 else 6.0)
+    return g, vol
+
+
+def strain_energy_density(eps, f1, f2, prm, tau_bar=None, lyr=None):
+    """W(eps) whose symmetric derivative is the stress law; frame model
+    (2D: W = mt/2 (t.eps.t)^2 + ms/2 (s.eps.s)^2 + ma (t.eps.s)^2;
+     3D: W = lt/2 (tr eps - n.eps.n)^2 + mt/2 |P eps P|^2 + ms/2 (s.eps.s)^2
+         + ma |P eps s|^2, P = I - n n^T) or isotropic."""
+    d = eps.shape[1]
+    if prm["elaType"] == "isotropic":
+        mm = np.asarray(prm["modMat"])[lyr]
+        sc = 1 - 0.5 * tau_bar
+        tr = np.trace(eps, axis1=1, axis2=2)
+        return sc * (0.5 * mm[:, 0] * tr ** 2 + mm[:, 1] * (eps * eps).sum((1, 2)))
+    m = prm["modVec"]
+    if d == 2:
+        tet = np.einsum("ei,eij,ej->e", f1, eps, f1)
+        ses = np.einsum("ei,eij,ej->e", f2, eps, f2)
+        tes = np.einsum("ei,eij,ej->e", f1, eps, f2)
+        return 0.5 * m[0] * tet ** 2 + 0.5 * m[1] * ses ** 2 + m[2] * tes ** 2
+    P = np.eye(3)[None] - f1[:, :, None] * f1[:, None, :]
+    PeP = P @ eps @ P
+    nen = np.einsum("ei,eij,ej->e", f1, eps, f1)
+    tr = np.trace(eps, axis1=1, axis2=2)
+    ses = np.einsum("ei,eij,ej->e", f2, eps, f2)
+    Pes = np.einsum("eij,ejk,ek->ei", P, eps, f2)
+    return (0.5 * m[0] * (tr - nen) ** 2 + 0.5 * m[1] * (PeP * PeP).sum((1, 2)) + 0.5 * m[2] * ses ** 2
+            + m[3] * (Pes * Pes).sum(1))
+
+
+def energy(p, oracle, v, tau):
+    q = p["q0"]
+    elm = p["elm"]["elmVtxMat"]
+    g, vol = elem_grads(q, elm)
+    V = v.T[elm.T]                          # (E, D+1, D) nodal velocities
+    grad_v = np.einsum("eka,ekb->eab", V, g)      # (dv_a / dx_b)
+    eps = 0.5 * (grad_v + np.transpose(grad_v, (0, 2, 1)))
+    f1, f2 = oracle.frames(p, q)
+    tb = tau[elm].mean(0)
+    W_ = strain_energy_density(eps, f1.T, f2.T, p["params"], tb, p["elm"]["elmLyrVec"])
+    return 2.0 * (vol * W_).sum()           # v^T E v = sum |T| sigma : eps = sum |T| 2 W
+
+
+@pytest.mark.parametrize("maker,ela", [(small_ring, "frame"), (small_ring, "isotropic"),
+                                       (small_shell, "frame"), (small_shell, "isotropic")])
+def test_elastic_operator_is_the_energy_hessian(oracle, maker, ela):
+    p = maker(elaType=ela)
+    rng = np.random.default_rng(3)
+    u = rng.standard_normal(p["q0"].shape)
+    v = rng.standard_normal(p["q0"].shape)
+    tau = oracle.tau_initial(p)
+    Eu = oracle.elastic_apply(p, p["q0"], tau, u)
+    Ev = oracle.elastic_apply(p, p["q0"], tau, v)
+    # quadratic form from the strain energy
+    assert (v * Ev).sum() == pytest.approx(energy(p, oracle, v, tau), rel=1e-12)
+    # bilinear form by polarisation, hence symmetry
+    pol = 0.25 * (energy(p, oracle, u + v, tau) - energy(p, oracle, u - v, tau))
+    assert (v * Eu).sum() == pytest.approx(pol, rel=1e-10, abs=1e-12)
+    assert (v * Eu).sum() == pytest.approx((u * Ev).sum(), rel=1e-12)
+    # rigid translations are in the null space
+    t = np.ones_like(p["q0"]) * rng.standard_normal((p["dim"], 1))
+    Et = oracle.elastic_apply(p, p["q0"], tau, t)
+    assert np.abs(Et).max() < 1e-10 * np.abs(Ev).max()
+
+
+# --------------------------------------------------------------------------
+# yank, FEM matrices, penalty
+
+@pytest.mark.parametrize("maker", [small_ring, small_shell])
+def test_yank_work_identity(oracle, maker):
+    p = maker()
+    q = p["q0"]
+    act = np.random.default_rng(4).uniform(0, 2, q.shape[1])
+    f = oracle.external_yank(p, q, act)
+    # translation invariance: sum of forces is zero
+    assert np.abs(f.sum(1)).max() < 1e-12 * np.abs(f).max() * q.shape[1]
+    # for a linear field v = A x: v.f = -tr(A) sum_T abar_T |T|
+    A = np.random.default_rng(5).standard_normal((p["dim"], p["dim"]))
+    v = A @ q
+    _, vol = elem_grads(q, p["elm"]["elmVtxMat"])
+    abar = act[p["elm"]["elmVtxMat"]].mean(0)
+    assert (v * f).sum() == pytest.approx(-np.trace(A) * (abar * vol).sum(), rel=1e-12)
+
+
+@pytest.mark.parametrize("maker", [small_ring, small_shell])
+def test_fem_matrices(oracle, maker):
+    p = maker()
+    q = p["q0"]
+    d, V = q.shape
+    ppd, ggd = oracle.fem_assemble(p, q)
+    rows, cols = p["elm"]["femVtxMat"]
+    M = np.zeros((V, V)); M[rows, cols] = ppd
+    K = np.zeros((V, V)); K[rows, cols] = ggd
+    elm = p["elm"]["elmVtxMat"]
+    g, vol = elem_grads(q, elm)
+    assert M.sum() == pytest.approx(vol.sum(), rel=1e-13)                 # sum_ij M_ij = |Omega|
+    np.testing.assert_allclose(M, M.T, atol=1e-16)
+    assert np.abs(K.sum(1)).max() < 1e-12 * np.abs(K).max()              # K_D 1 = 0
+    np.testing.assert_allclose(K, K.T, atol=1e-13 * np.abs(K).max())
+    # stiffness energy u^T K_D u = sum |T| grad(u)^T Dten grad(u)
+    u = np.random.default_rng(6).standard_normal(V)
+    gu = np.einsum("ek,eka->ea", u[elm.T], g)
+    f1, f2 = oracle.frames(p, q)
+    f1, f2 = f1.T, f2.T
+    prm = p["params"]
+    if d == 2:
+        dens = prm["spdTanVal"] * (gu * f1).sum(1) ** 2 + prm["spdTsvVal"] * (gu * f2).sum(1) ** 2
+    else:
+        dens = prm["spdTanVal"] * ((gu * gu).sum(1) - (gu * f1).sum(1) ** 2) + prm["spdTsvVal"] * (gu * f2).sum(1) ** 2
+    assert u @ K @ u == pytest.approx((vol * dens).sum(), rel=1e-12)
+    # consistent P1 mass: u^T M u = sum |T| (sum u_k^2 + (sum u_k)^2) / ((D+1)(D+2))
+    uk = u[elm.T]
+    mref = vol * ((uk ** 2).sum(1) + uk.sum(1) ** 2) / ((d + 1) * (d + 2))
+    assert u @ M @ u == pytest.approx(mref.sum(), rel=1e-12)
+    # reaction load vector = M r
+    r = np.random.default_rng(7).uniform(size=V)
+    np.testing.assert_allclose(oracle.fem_vector(p, q, r), M @ r, rtol=1e-12, atol=1e-15)
+
+
+def test_penalty_operator(oracle):
+    for p in (small_ring(), W.strip_problem()):
+        q = p["q0"]
+        v = np.random.default_rng(8).standard_normal(q.shape)
+        out = oracle.penalty_apply(p, q, v)
+        btm = p["elm"]["btmVtxMat"]
+        seg = np.sqrt(((q[:, btm[1]] - q[:, btm[0]]) ** 2).sum(0))
+        lump = np.zeros(q.shape[1])
+        np.add.at(lump, btm[0], seg / 2)
+        np.add.at(lump, btm[1], seg / 2)
+        np.testing.assert_allclose(out, lump * v, rtol=1e-13, atol=1e-15)
+
+
+# --------------------------------------------------------------------------
+# solvers
+
+@pytest.mark.parametrize("maker", [small_ring, small_shell])
+def test_first_step_matches_dense_solves(oracle, maker):
+    """q1 = q0 + dt v0 with (ldmWgt K^{-1} + E) v0 = f, and tau1 from the
+    reaction-diffusion step, against dense LAPACK solves."""
+    p = maker()
+    p["timeNum"] = 2
+    o = oracle.objfcn(p, pcg_init=0, rd_solve=1, with_target=False)
+    prm = p["params"]
+    q0 = p["q0"]
+    d, V = q0.shape
+    tau0 = oracle.tau_initial(p)
+    ya, ra = oracle.activities(tau0, prm["ynkActPrmVec"], prm["reaActPrmVec"])
+    f = oracle.external_yank(p, q0, ya).reshape(-1)
+    Emat = np.zeros((d * V, d * V))
+    for j in range(d * V):
+        e = np.zeros(d * V); e[j] = 1.0
+        Emat[:, j] = oracle.elastic_apply(p, q0, tau0, e.reshape(d, V)).reshape(-1)
+    K = kernel_matrix(q0, prm["knlOrder"], prm["knlWidth"], prm["knlEps"])
+    # K (ldmWgt K^{-1} + E) v = K f, i.e. (ldmWgt I + K E) v = K f: the same
+    # solution without inverting the ill-conditioned K
+    KK = np.kron(np.eye(d), K)
+    v0 = np.linalg.solve(prm["ldmWgt"] * np.eye(d * V) + KK @ Emat, KK @ f).reshape(d, V)
+    v_or = (o["lmkStk"][1] - o["lmkStk"][0]) / prm["timeStp"]
+    assert np.linalg.norm(v_or - v0) / np.linalg.norm(v0) < 1e-7
+    q1 = o["lmkStk"][1]
+    M0, _ = oracle.fem_assemble(p, q0)
+    M1, K1 = oracle.fem_assemble(p, q1)
+    rows, cols = p["elm"]["femVtxMat"]
+    Mn = np.zeros((V, V)); Mn[rows, cols] = M0
+    L = np.zeros((V, V)); L[rows, cols] = M1 + prm["timeStp"] * K1
+    rhs = Mn @ tau0 + prm["timeStp"] * oracle.fem_vector(p, q0, ra)
+    tau1 = scipy.linalg.cho_solve(scipy.linalg.cho_factor(L), rhs)
+    np.testing.assert_allclose(o["tauMat"][1], tau1, rtol=1e-11, atol=1e-12 * np.abs(tau1).max())
+
+
+@pytest.mark.parametrize("maker", [small_ring, small_shell])
+def test_solver_modes_agree(oracle, maker):
+    """alpha-carried warm start + sparse CG against the reference-exact
+    dense Cholesky warm start + dense reaction-diffusion solve."""
+    p = maker()
+    a = oracle.objfcn(p, pcg_init=1, rd_solve=1)
+    b = oracle.objfcn(p, pcg_init=0, rd_solve=0)
+    disp = np.linalg.norm(b["lmkStk"][-1] - b["lmkStk"][0])
+    assert np.linalg.norm(a["lmkStk"][-1] - b["lmkStk"][-1]) / disp < 1e-9
+    assert np.linalg.norm(a["tauMat"] - b["tauMat"]) / np.linalg.norm(b["tauMat"]) < 1e-10
+    assert abs(a["obj"] - b["obj"]) < 1e-10 * abs(b["obj"])
+
+
+def test_solver_failure_sets_dbl_max(oracle):
+    p = small_ring()
+    p["params"] = dict(p["params"], cgItrMax=2, cgTolVal=1e-14)
+    o = oracle.objfcn(p)
+    assert o["status"] == 1 and o["obj"] == np.finfo(float).max
+
+
+def test_zero_force_gives_zero_velocity(oracle):
+    p = small_ring()
+    p["params"] = dict(p["params"], ynkActPrmVec=[0.01, 1.0, 0.0, 0.5])   # actMax = 0 -> b = 0
+    p["timeNum"] = 3
+    o = oracle.objfcn(p, with_target=False)
+    assert o["status"] == 0
+    np.testing.assert_array_equal(o["lmkStk"][-1], p["q0"])
+
+
+# --------------------------------------------------------------------------
+# varifold
+
+def brute_varifold(c1, u1, w1, c2, u2, w2, vfd):
+    d2 = ((c1[:, :, None] - c2[:, None, :]) ** 2).sum(0)
+    ang = u1.T @ u2
+    kc = np.exp(-d2 / vfd["cenKnlWidth"] ** 2) if vfd["cenKnlType"] == "G" else 1 / (1 + d2 / vfd["cenKnlWidth"] ** 2)
+    t = vfd["dirKnlType"]
+    s2 = vfd["dirKnlWidth"] ** 2
+    ku = {"B": ang ** 2, "L": ang, "O": np.exp(2 * (ang - 1) / s2), "U": np.exp(2 * (ang ** 2 - 1) / s2)}[t]
+    return (kc * ku * w1[:, None] * w2[None, :]).sum()
+
+
+@pytest.mark.parametrize("maker", [small_ring, small_shell])
+@pytest.mark.parametrize("ck,dk", [("G", "B"), ("C", "L"), ("G", "O"), ("C", "U")])
+def test_varifold_value_and_gradient(oracle, maker, ck, dk):
+    p = maker()
+    p["vfd"] = dict(p["vfd"], cenKnlType=ck, dirKnlType=dk, dirKnlWidth=0.7)
+    rng = np.random.default_rng(11)
+    q = p["q0"] + 0.02 * rng.standard_normal(p["q0"].shape)
+    val, grad = oracle.varifold(p, q, with_grad=True)
+    c, u, w = oracle.facets(p, q)
+    t = p["target"]
+    ref = (brute_varifold(c, u, w, c, u, w, p["vfd"]) - 2 * brute_varifold(c, u, w, t["cen"], t["dir"], t["vol"], p["vfd"])
+           + brute_varifold(t["cen"], t["dir"], t["vol"], t["cen"], t["dir"], t["vol"], p["vfd"]))
+    assert val == pytest.approx(ref, rel=1e-12)
+    # distance of the target to itself is zero
+    # central finite differences of the gradient on a few coordinates
+    idx = np.unique(p["elm"]["vfdVtxMat"])[:6]
+    h = 1e-6
+    for i in idx:
+        for k in range(p["dim"]):
+            qp = q.copy(); qp[k, i] += h
+            qm = q.copy(); qm[k, i] -= h
+            fd = (oracle.varifold(p, qp) - oracle.varifold(p, qm)) / (2 * h)
+            assert grad[k, i] == pytest.approx(fd, rel=1e-5, abs=1e-7 * np.abs(grad).max())
+
+
+def test_varifold_self_distance_zero(oracle):
+    p = small_shell()
+    c, u, w = oracle.facets(p, p["q0"])
+    p["target"] = dict(cen=c, dir=u, vol=w)
+    assert abs(oracle.varifold(p, p["q0"])) < 1e-12 * (w.sum() ** 2)
+
+
+def test_frames_closed_forms(oracle):
+    # demo strip: tangent along x, transverse along y everywhere
+    p = W.strip_problem()
+    t, s = oracle.frames(p, p["q0"])
+    np.testing.assert_allclose(t, np.array([[1.0], [0.0]]) * np.ones_like(t), atol=1e-15)
+    np.testing.assert_allclose(s, np.array([[0.0], [1.0]]) * np.ones_like(s), atol=1e-15)
+    # sphere shell of scaled copies: n is the (outward) facet normal and s the
+    # normalised sum of the three radial directions
+    sph, tri = W.icosphere(1)
+    p = W.shell_problem(level=1, n_layers=3, axes=(1.0, 1.0, 1.0), time_steps=2)
+    n, s = oracle.frames(p, p["q0"])
+    v = sph[tri]
+    fn = np.cross(v[:, 1] - v[:, 0], v[:, 2] - v[:, 0])
+    fn /= np.linalg.norm(fn, axis=1, keepdims=True)
+    rs = (v / np.linalg.norm(v, axis=2, keepdims=True)).sum(1)
+    rs /= np.linalg.norm(rs, axis=1, keepdims=True)
+    E1 = tri.shape[0] * 3
+    np.testing.assert_allclose(n[:, :E1].T, np.repeat(fn, 3, axis=0), atol=1e-14)
+    np.testing.assert_allclose(s[:, :E1].T, np.repeat(rs, 3, axis=0), atol=1e-14)
