@@ -1,0 +1,170 @@
+"""Benchmark: 3D evolution steps/s of the forward shape-evolution solve on one
+MI355X, with the roofline of its dominant kernel and the FP64 CPU oracle
+timed beside it.
+
+    python bench.py [--steps K] [--warmup W] [--repeats R] [--config c2|c2-all|c1]
+                    [--precision fp64|mixed] [--no-cpu]
+
+A "step" is one evolution time step (velocity solve, landmark update,
+reaction-diffusion update) of the workload; the timed region is R solves of
+K steps each, every solve ending with the varifold objective, with the mesh
+and parameters already resident in HBM (a device-resident session).
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "3D evolution steps/s at ≤1e-6 rel err vs FP64; p50/p99 solve latency"
+
+# algorithmic work (counted per unit, see the measurement notes of the design)
+FLOP_PER_PAIR = {-1: 14, 0: 18, 1: 19, 2: 21, 3: 23, 4: 25}   # d2 (8) + kernel + 3 multiply-adds (6)
+PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+
+
+def workload(name: str, steps: int):
+    from shape_atrophy_amd import workloads as W
+    if name == "c2":
+        p = W.shell_problem(level=5, n_layers=5, time_steps=50, n_ctl=5000)
+        desc = "ellipsoidal shell, icosphere-5 x 5 layers, Matern nu=3/2 on 5000 control points"
+    elif name == "c2-all":
+        p = W.shell_problem(level=5, n_layers=5, time_steps=50)
+        desc = "ellipsoidal shell, icosphere-5 x 5 layers, Matern nu=3/2 on all landmarks"
+    elif name == "c1":
+        p = W.ring_problem()
+        desc = "2D concentric-ring disk, 10 rings x 200 nodes"
+    else:
+        raise SystemExit(f"unknown config {name}")
+    p["timeNum"] = steps + 1
+    return p, desc
+
+
+def percentile(xs, q):
+    return float(np.percentile(np.asarray(xs), q)) if xs else None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--repeats", type=int, default=1)
+    ap.add_argument("--config", default="c2-all")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-iters", type=int, default=1)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: the solve runs on one MI355X (asked for --gpus {args.gpus}); "
+              "sharding across GPUs is not part of this build", file=sys.stderr)
+        sys.exit(2)
+
+    from shape_atrophy_amd import api
+    steps = args.steps if args.steps is not None else 5
+    prob, desc = workload(args.config, steps)
+    d, V = prob["q0"].shape
+    E = prob["elm"]["elmVtxMat"].shape[1]
+    F = prob["elm"]["vfdVtxMat"].shape[1]
+    print(f"[bench] {args.config}: V={V} E={E} F={F} steps={steps} precision={args.precision}", file=sys.stderr,
+          flush=True)
+
+    sess = api.Session(prob, precision=args.precision)
+    tv, tp = prob["tauVar"][:, 0], prob["tauPrm"]
+    if args.warmup > 0:
+        sess.set_time_num(min(args.warmup, steps) + 1)
+        w = sess.run(tv, tp)
+        print(f"[bench] warmup: {w['steps']} steps in {w['seconds']:.3f}s status={w['status']}", file=sys.stderr,
+              flush=True)
+    sess.set_time_num(steps + 1)
+    sess.set_profiling(True)
+    lat, runs = [], []
+    t0 = time.perf_counter()
+    for r in range(args.repeats):
+        out = sess.run(tv, tp)
+        lat.append(out["seconds"])
+        runs.append(out)
+        print(f"[bench] solve {r}: {out['steps']} steps {out['seconds']:.3f}s obj={out['obj']:.12e} "
+              f"pcg={out['pcg_iters']} rd={out['rd_iters']} status={out['status']}", file=sys.stderr, flush=True)
+    wall = time.perf_counter() - t0
+    total_steps = sum(o["steps"] for o in runs)
+    value = total_steps / wall
+
+    # roofline of the dominant kernel: the kernel-matrix product
+    kmv_ms = sum(o["kmatvec_ms"] for o in runs)
+    kmv_n = sum(o["kmatvec_count"] for o in runs)
+    op_ms = sum(o["operator_ms"] for o in runs)
+    op_n = sum(o["operator_count"] for o in runs)
+    order = int(prob["params"]["knlOrder"])
+    nc = len(prob.get("ctlIdx", [])) or 0
+    pairs = (2 * V * nc) if nc else V * V
+    flops = FLOP_PER_PAIR[order] * pairs
+    roof = None
+    if kmv_n:
+        avg = kmv_ms / kmv_n
+        ach = flops / (avg * 1e-3) / 1e12
+        roof = {"bound": "mfma", "engine": "fp64 VALU (fp64 vector peak = fp64 matrix peak on MI355X)",
+                "kernel": "kmatvec_k" if not nc else "kcross_k", "achieved": round(ach, 3),
+                "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": round(ach / PEAK_FP64_TFLOPS, 4),
+                "traffic": None, "avg_ms": round(avg, 4), "launches": kmv_n,
+                "work": f"{FLOP_PER_PAIR[order]} flop/pair x {pairs} pairs"}
+    op = None
+    if op_n:
+        avg = op_ms / op_n
+        B = (64 * E + 72 * V) if d == 3 else (40 * E + 48 * V)
+        op = {"bound": "hbm", "kernel": "elastic_elem_k + gather_op_k", "achieved": round(B / (avg * 1e-3) / 1e9, 1),
+              "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(B / (avg * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
+              "avg_ms": round(avg, 4), "algorithmic_bytes": B}
+
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        O.build()
+        threads = min(os.cpu_count() or 1, 16)
+        smp = O.pcg_sample(prob, args.cpu_iters, threads=threads)
+        t_it = smp["cg_s"] / (smp["iters"] + 1)
+        its_per_step = sum(o["pcg_iters"] for o in runs) / max(total_steps, 1)
+        cpu_step = smp["setup_s"] + (its_per_step + 1) * t_it
+        cpu = {"value": round(1.0 / cpu_step, 6), "unit": "steps/s", "cores": threads, "kind": "port",
+               "sample": (f"FP64 oracle, first step of the same workload: set-up {smp['setup_s']:.2f}s + "
+                          f"{smp['iters']} CG iterations in {smp['cg_s']:.2f}s, scaled to the GPU-measured "
+                          f"{its_per_step:.1f} CG iterations per step")}
+
+    res = {
+        "metric": METRIC,
+        "value": round(value, 4),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": total_steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * wall / max(total_steps, 1), 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f64+f32",
+        "data": "synthetic",
+        "config": {"workload": desc, "config": args.config, "landmarks": V, "tets": E, "varifold_facets": F,
+                   "control_points": nc, "time_steps_per_solve": steps, "solves": args.repeats,
+                   "precision": args.precision},
+        "latency_s": {"p50": percentile(lat, 50), "p99": percentile(lat, 99)},
+        "pcg_iters_per_step": round(sum(o["pcg_iters"] for o in runs) / max(total_steps, 1), 1),
+        "objective": runs[-1]["obj"],
+        "roofline": roof,
+        "operator_roofline": op,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

@@ -123,6 +123,10 @@ typedef struct sa_stats {
     int rdIters;                 /* total reaction-diffusion iterations */
     int refineSweeps;            /* outer FP64 refinement sweeps (mixed precision) */
     double seconds;              /* wall time of the solve */
+    double kmatvecMs;            /* with profiling on: summed hipEvent time of the kernel-matrix products */
+    int kmatvecCount;            /*   ... and their number (CG iterations that ran one) */
+    double operatorMs;           /* summed time of the elastic operator applies (element + gather kernels) */
+    int operatorCount;
 } sa_stats;
 
 /* deformation_{2D,3D}_frame (deformation.cu:14-199).
@@ -155,6 +159,11 @@ int sa_session_run(sa_session *s, const double *tauVarVec, int tauVarNum, const 
                    int tauPrmNum, double *obj, double *lmkStk, double *tauMat, double *dqObjMat,
                    sa_stats *stats);
 void sa_session_destroy(sa_session *s);
+/* time the kernel-matrix product and the elastic operator of every CG
+   iteration with hipEvents on the session's stream (reported in sa_stats) */
+int sa_session_set_profiling(sa_session *s, int on);
+/* run only the first timeNum time levels (<= the session's timeNum) from now on */
+int sa_session_set_time_num(sa_session *s, int timeNum);
 /* Diagnostic probes (no counterpart in the reference): evaluate one operator
    of the path at the initial state of a candidate (q0 and tau0 from
    tauVarVec) so each kernel family can be checked against an oracle alone.

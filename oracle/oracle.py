@@ -317,3 +317,20 @@ def facets(prob: dict, q):
     vol = np.empty(m.vfdElmNum)
     lib().or_facets(C.byref(m), _dp(q), _dp(cen), _dp(dr), _dp(vol))
     return cen, dr, vol
+
+
+def pcg_sample(prob: dict, n_iter: int, cand: int = 0, threads: int | None = None) -> dict:
+    """Time the set-up of the first step and n_iter velocity-CG iterations
+    (the CPU-baseline sample)."""
+    L = lib()
+    if threads:
+        L.or_set_threads(int(threads))
+    keep = _Keep()
+    m = make_mesh(prob, keep)
+    p = make_params(prob, keep)
+    q0 = np.ascontiguousarray(prob["q0"], dtype=np.float64)
+    tv = np.ascontiguousarray(np.asarray(prob["tauVar"], dtype=np.float64)[:, cand])
+    tp = np.ascontiguousarray(prob["tauPrm"], dtype=np.float64)
+    out = np.zeros(3)
+    L.or_pcg_sample(C.byref(m), C.byref(p), _dp(q0), _dp(tv), _dp(tp), int(n_iter), _dp(out))
+    return dict(setup_s=float(out[0]), cg_s=float(out[1]), iters=int(out[2]))

@@ -20,6 +20,7 @@
 //   rdSolve = 1  Jacobi-preconditioned CG on the same sparse matrix.
 #include <algorithm>
 #include <cfloat>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -1105,6 +1106,42 @@ struct Solver {
     }
 };
 
+// CPU-baseline sample: the set-up of the first time step (frames, FEM
+// matrices, yank, kernel factorisation in the control-point form) and nIter
+// velocity-CG iterations, timed separately
+template <int D>
+void pcg_sample_impl(const OMesh *m, const OParams *p, const double *q0, const double *tauVar, const double *tauPrm,
+                     int nIter, double *out) {
+    OParams pp = *p;
+    pp.cgItrMax = nIter;
+    pp.cgTolSqu = 0.0;
+    Solver<D> s(*m, pp);
+    int V = m->lmkNum;
+    size_t nv = (size_t)D * V;
+    auto t0 = std::chrono::steady_clock::now();
+    std::vector<double> tau(V), ya(V), ra(V), f(nv), v(nv, 0.0);
+    std::vector<double> alpha(m->ctlNum > 0 ? (size_t)D * m->ctlNum : nv, 0.0);
+    tau_initial<D>(q0, V, tauVar, tauPrm, tau.data());
+    compute_frames<D>(*m, q0, s.ws.f1, s.ws.f2);
+    std::vector<double> ppd, ggd;
+    fem_assemble<D>(*m, pp, q0, s.ws.f1, s.ws.f2, ppd, ggd);
+    for (int i = 0; i < V; ++i) { ya[i] = yank_activity(tau[i], pp.ynk); ra[i] = reac_activity(tau[i], pp.rea); }
+    external_yank<D>(*m, q0, ya.data(), s.frc, f.data());
+    auto t1 = std::chrono::steady_clock::now();
+    int its = 0;
+    if (m->ctlNum > 0) {
+        s.ctl_pcg(q0, tau.data(), f.data(), v.data(), alpha.data(), &its);
+    } else {
+        std::vector<double> L;
+        if (V <= 8000) s.K_build(q0);
+        s.ipcg(q0, tau.data(), f.data(), v.data(), alpha.data(), &L, &its);
+    }
+    auto t2 = std::chrono::steady_clock::now();
+    out[0] = std::chrono::duration<double>(t1 - t0).count();
+    out[1] = std::chrono::duration<double>(t2 - t1).count();
+    out[2] = its;
+}
+
 template <int D>
 int objfcn_impl(const OMesh *m, const OParams *p, const OTarget *t, const double *q0, const double *tauVar,
                 const double *tauPrm, double *lmkStk, double *tauMat, double *obj, double *grad, int *pcgIters,
@@ -1150,6 +1187,12 @@ int or_objfcn(const OMesh *m, const OParams *p, const OTarget *t, const double *
               int *rdIters) {
     if (m->dim == 2) return objfcn_impl<2>(m, p, t, q0, tauVar, tauPrm, lmkStk, tauMat, obj, grad, pcgIters, rdIters);
     return objfcn_impl<3>(m, p, t, q0, tauVar, tauPrm, lmkStk, tauMat, obj, grad, pcgIters, rdIters);
+}
+
+void or_pcg_sample(const OMesh *m, const OParams *p, const double *q0, const double *tauVar, const double *tauPrm,
+                   int nIter, double *out) {
+    if (m->dim == 2) pcg_sample_impl<2>(m, p, q0, tauVar, tauPrm, nIter, out);
+    else pcg_sample_impl<3>(m, p, q0, tauVar, tauPrm, nIter, out);
 }
 
 // --- unit-level entry points for the tests ---------------------------------

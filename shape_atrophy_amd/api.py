@@ -63,7 +63,8 @@ class SaTarget(C.Structure):
 
 class SaStats(C.Structure):
     _fields_ = [("status", C.c_int), ("steps", C.c_int), ("pcgIters", C.c_int), ("rdIters", C.c_int),
-                ("refineSweeps", C.c_int), ("seconds", C.c_double)]
+                ("refineSweeps", C.c_int), ("seconds", C.c_double), ("kmatvecMs", C.c_double),
+                ("kmatvecCount", C.c_int), ("operatorMs", C.c_double), ("operatorCount", C.c_int)]
 
 
 class SaError(RuntimeError):
@@ -85,6 +86,8 @@ def lib():
         L.sa_session_run.argtypes = [C.c_void_p, _PD, C.c_int, _PD, C.c_int, _PD, _PD, _PD, _PD, _P(SaStats)]
         L.sa_session_destroy.argtypes = [C.c_void_p]
         L.sa_session_probe.argtypes = [C.c_void_p, C.c_int, _PD, C.c_int, _PD, C.c_int, _PD, _PD]
+        L.sa_session_set_profiling.argtypes = [C.c_void_p, C.c_int]
+        L.sa_session_set_time_num.argtypes = [C.c_void_p, C.c_int]
         L.sa_deformation.argtypes = [_P(SaMesh), _P(SaParams), _PD, C.c_int, C.c_int, _PD, C.c_int, _PD, _PD, _PD, _PD]
         L.sa_compute_objfcn.argtypes = [_P(SaMesh), _P(SaTarget), _P(SaParams), _PD, C.c_int, C.c_int, _PD, C.c_int,
                                         _PD, _PD, _PD]
@@ -321,7 +324,16 @@ class Session:
         _check(lib().sa_session_run(self._h, _dp(tv), tv.size, _dp(tp), tp.size, _dp(obj), _dp(lmk), _dp(tau),
                                     _dp(grad), C.byref(st)))
         return dict(obj=float(obj[0]), lmkStk=lmk, tauMat=tau, grad=grad, status=st.status, steps=st.steps,
-                    pcg_iters=st.pcgIters, rd_iters=st.rdIters, refine=st.refineSweeps, seconds=st.seconds)
+                    pcg_iters=st.pcgIters, rd_iters=st.rdIters, refine=st.refineSweeps, seconds=st.seconds,
+                    kmatvec_ms=st.kmatvecMs, kmatvec_count=st.kmatvecCount, operator_ms=st.operatorMs,
+                    operator_count=st.operatorCount)
+
+    def set_time_num(self, timeNum: int):
+        _check(lib().sa_session_set_time_num(self._h, int(timeNum)))
+        self.T = int(timeNum)
+
+    def set_profiling(self, on: bool = True):
+        _check(lib().sa_session_set_profiling(self._h, 1 if on else 0))
 
     PROBES = {"elastic": 1, "kmatvec": 2, "yank": 3, "fem": 4, "geometry": 5, "varifold": 6}
 

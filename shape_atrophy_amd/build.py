@@ -31,7 +31,8 @@ def sources():
 
 def build(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, *FLAGS, "-o", tmp, SRC]
+    cmd = [HIPCC, *FLAGS, "-o", tmp, SRC, "-L/opt/rocm/lib", "-lrocsolver", "-lrocblas",
+           "-Wl,-rpath,/opt/rocm/lib"]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)

@@ -210,14 +210,56 @@ __device__ inline void bessel_f01(double x, double &f0, double &f1) {
     f1 = x * k1;
 }
 
-// kernel value from the squared distance; invW = 1 / width
+// exp(-r) for r >= 0 to about 1 ulp: r / (ln2 / 64) = -k with k integer,
+// exp(-r) = 2^(k >> 6) * 2^((k & 63) / 64) * exp(s), |s| <= ln2 / 128, the
+// 64 powers 2^(j/64) held in LDS (filled by init_exp_table) and exp(s) by its
+// degree-5 Taylor polynomial (truncation < 4e-17).  The remainder s is formed
+// with two FMAs against ln2/64 split into a double and its residual.
+__device__ inline void init_exp_table(double *tab) {
+    if (threadIdx.x < 64) tab[threadIdx.x] = exp2((double)threadIdx.x / 64.0);
+}
+
+__device__ inline double exp_neg(double r, const double *__restrict__ tab) {
+    const double kInvStep = 92.332482616893656877;        // 64 / ln 2
+    const double kStepHi = 0.010830424696249145;          // ln 2 / 64 (double)
+    const double kStepLo = 3.6235106466348434e-19;        // ln 2 / 64 - kStepHi
+    r = fmin(r, 1000.0);
+    double kd = rint(-r * kInvStep);
+    double s = fma(-kd, kStepHi, -r);
+    s = fma(-kd, kStepLo, s);
+    int k = (int)kd;
+    double p = fma(s, 1.0 / 120.0, 1.0 / 24.0);
+    p = fma(s, p, 1.0 / 6.0);
+    p = fma(s, p, 0.5);
+    p = fma(s, p, 1.0);
+    p = fma(s, p, 1.0);
+    return ldexp(tab[k & 63] * p, k >> 6);
+}
+
+// sqrt(x) for x >= 0 (x = 0 gives ~1e-150): hardware reciprocal square root
+// refined by Goldschmidt iterations and two Newton corrections
+__device__ inline double sqrt_pos(double x) {
+    x = fmax(x, 1e-300);
+    double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = 0.5 * y;
+    double e = fma(-h, g, 0.5);
+    g = fma(g, e, g);
+    h = fma(h, e, h);
+    double d = fma(-g, g, x);
+    g = fma(d, h, g);
+    d = fma(-g, g, x);
+    return fma(d, h, g);
+}
+
+// kernel value from the squared distance; invW = 1 / width; tab = LDS table
+// of init_exp_table (3D and Gaussian kernels)
 template <int D, int ORD>
-__device__ inline double rkhs_kernel(double d2, double invW) {
+__device__ inline double rkhs_kernel(double d2, double invW, const double *__restrict__ tab) {
     if constexpr (ORD == -1) {
-        return exp(-d2 * (invW * invW));
+        return exp_neg(d2 * (invW * invW), tab);
     } else if constexpr (D == 3) {
-        double r = sqrt(d2) * invW;
-        double e = exp(-r);
+        double r = sqrt_pos(d2) * invW;
+        double e = exp_neg(r, tab);
         if constexpr (ORD == 0) return e;
         if constexpr (ORD == 1) return (1.0 + r) * e;
         if constexpr (ORD == 2) return (3.0 + r * (3.0 + r)) / 3.0 * e;
@@ -250,7 +292,7 @@ __device__ inline float rkhs_kernel_f(float d2, float invW) {
         if constexpr (ORD == 3) return (15.0f + r * (15.0f + r * (6.0f + r))) * (1.0f / 15.0f) * e;
         return (105.0f + r * (105.0f + r * (45.0f + r * (10.0f + r)))) * (1.0f / 105.0f) * e;
     } else {
-        return (float)rkhs_kernel<D, ORD>((double)d2, (double)invW);
+        return (float)rkhs_kernel<D, ORD>((double)d2, (double)invW, nullptr);
     }
 }
 

@@ -24,6 +24,9 @@
 #include <string>
 #include <vector>
 
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
 #include "shape_atrophy.h"
 #include "sa_kernels.hpp"
 
@@ -115,6 +118,23 @@ std::vector<int> incidence(const std::vector<int> &items, int nItems, int k, int
 
 struct Stats {
     int status = 0, steps = 0, pcgIters = 0, rdIters = 0, refine = 0;
+    double kmvMs = 0.0, opMs = 0.0;   // summed durations of the timed kernels (profiling on)
+    int kmvCount = 0, opCount = 0;
+};
+
+// hipEvent pairs around the kernel-matrix product and the elastic operator
+// of every velocity-CG iteration; only iterations that did work are summed
+struct KernelTimer {
+    bool on = false;
+    std::vector<hipEvent_t> ev;   // per chunk slot: [kmv start, kmv end, op start, op end, kmv2 start, kmv2 end]
+    void ensure(int slots) {
+        while ((int)ev.size() < 6 * slots) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) throw SaError(SA_E_DEVICE, "hipEventCreate failed");
+            ev.push_back(e);
+        }
+    }
+    ~KernelTimer() { for (auto e : ev) (void)hipEventDestroy(e); }
 };
 
 struct EngineBase {
@@ -122,8 +142,11 @@ struct EngineBase {
     virtual ~EngineBase() = default;
     virtual void run(const double *tauVar, int nVar, const double *tauPrm, int nPrm, double *obj, double *lmkStk,
                      double *tauMat, double *grad, Stats &st) = 0;
+    int Trun = 0;                      // time levels of the next run (<= allocated)
+    virtual int timeLevels() const = 0;
     virtual void probe(int what, const double *tauVar, int nVar, const double *tauPrm, int nPrm, const double *in,
                        double *out) = 0;
+    KernelTimer timer;
 };
 
 template <int D>
@@ -152,11 +175,21 @@ struct Engine : EngineBase {
     DBuf<double> P0, P1, P2, P3, hist, rdHist;
     DBuf<double> dCen, dDir, dVol, rowVal, gC, gN, grad, scal;
     DBuf<Ctl> ctl, rdCtl;
+    // control-point form
+    DBuf<double> qc, Kcc, cA, cG, cR, cZ, cD, cS, cY, cK, part;
+    DBuf<int> potrfInfo;
+    rocblas_handle blas = nullptr;
+    int slicesC = 1;
     Ctl *hctl = nullptr;
+    Stats *prof = nullptr;       // where the kernel timer accumulates during run()
+    int lastIters = 0;           // CG iterations of the previous step (sizes the first launch chunk)
     double ttSum = 0.0;
     bool ttReady = false;
 
+    int timeLevels() const override { return T; }
+
     ~Engine() override {
+        if (blas) (void)rocblas_destroy_handle(blas);
         if (hctl) (void)hipHostFree(hctl);
         if (s) (void)hipStreamDestroy(s);
     }
@@ -240,7 +273,12 @@ struct Engine : EngineBase {
             int nl = p.modNum / 2;
             for (int l : hLyr) require(l >= 0 && l < nl, "elmLyrVec layer out of range");
         }
-        if (m.ctlNum > 0) throw SaError(SA_E_ARG, "control-point form not available in this build");
+        std::vector<int> hCtl;
+        if (m.ctlNum > 0) {
+            NC = m.ctlNum;
+            hCtl = to_host(m.ctlIdxVec, (size_t)NC, dev);
+            for (int v : hCtl) require(v >= 0 && v < V, "ctlIdxVec index out of range");
+        }
 
         CK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
         CK(hipHostMalloc((void **)&hctl, sizeof(Ctl) * 2));
@@ -307,6 +345,20 @@ struct Engine : EngineBase {
         rdHist.alloc(200002, tot);
         scal.alloc(8, tot);
         ctl.alloc(1, tot); rdCtl.alloc(1, tot);
+        if (NC > 0) {
+            size_t nc = (size_t)NC * D;
+            ctlIdx.alloc(NC, tot); ctlIdx.upload(hCtl.data(), NC, s);
+            qc.alloc(nc, tot); Kcc.alloc((size_t)NC * NC, tot);
+            cA.alloc(nc, tot); cG.alloc(nc, tot); cR.alloc(nc, tot); cZ.alloc(nc, tot); cD.alloc(nc, tot);
+            cS.alloc(nc, tot); cY.alloc(nc, tot); cK.alloc(nc, tot);
+            int rb = gridOf(NC, MV_ROWS);
+            slicesC = std::max(1, std::min(32, (1024 + rb - 1) / rb));
+            part.alloc((size_t)slicesC * nc, tot);
+            potrfInfo.alloc(1, tot);
+            if (rocblas_create_handle(&blas) != rocblas_status_success) throw SaError(SA_E_DEVICE, "rocblas_create_handle failed");
+            if (rocblas_set_stream(blas, s) != rocblas_status_success) throw SaError(SA_E_DEVICE, "rocblas_set_stream failed");
+            require(gridOf((long)nc) <= MAXP, "too many control points for the partial-sum buffers");
+        }
         require(gridOf(n) <= MAXP && gridOf(V, MV_ROWS) <= MAXP, "mesh too large for the partial-sum buffers");
         CK(hipStreamSynchronize(s));
     }
@@ -380,22 +432,175 @@ struct Engine : EngineBase {
         Ctl &h = hctl[0];
         while (true) {
             int itEnd = std::min(itrMax, it0 + chunk);
+            if (timer.on) timer.ensure(itEnd - it0);
             for (int it = it0; it < itEnd; ++it) {
+                hipEvent_t *ev = timer.on ? &timer.ev[6 * (it - it0)] : nullptr;
+                if (ev) CK(hipEventRecord(ev[2], s));
                 elastic(d.p, tauNow, geoNow, ctl.p);
                 gather_op<0>(Kid.p, d.p, Ad.p, nullptr, P1.p);
+                if (ev) CK(hipEventRecord(ev[3], s));
                 pcg_update_k<<<gN, NT, 0, s>>>(n, P1.p, gV, hist.p, it, x.p, a.p, r.p, d.p, Kid.p, Ad.p, P2.p, ctl.p);
                 CHECK_LAUNCH();
+                if (ev) CK(hipEventRecord(ev[0], s));
                 kmatvec<true>(q, r.p, Kr.p, P2.p, gN, it);
+                if (ev) CK(hipEventRecord(ev[1], s));
                 pcg_dir_k<<<gN, NT, 0, s>>>(n, P3.p, gM, hist.p, it, r.p, Kr.p, Kid.p, d.p, ctl.p);
                 CHECK_LAUNCH();
             }
             sync_ctl(ctl.p, &h);
+            if (timer.on) {
+                // iteration it did the operator apply iff it < iters (or not yet done);
+                // its product ran iff it + 1 < iters
+                int lastOp = h.done ? h.iters : itEnd;
+                for (int it = it0; it < itEnd; ++it) {
+                    hipEvent_t *ev = &timer.ev[6 * (it - it0)];
+                    float ms;
+                    if (it < lastOp) {
+                        CK(hipEventElapsedTime(&ms, ev[2], ev[3]));
+                        prof->opMs += ms; prof->opCount++;
+                    }
+                    if (!h.done || it + 1 < h.iters) {
+                        CK(hipEventElapsedTime(&ms, ev[0], ev[1]));
+                        prof->kmvMs += ms; prof->kmvCount++;
+                    }
+                }
+            }
             if (h.done) break;
             if (itEnd >= itrMax) { iters = itrMax; return 1; }
             it0 = itEnd;
             chunk = std::min(chunk * 2, 64);
         }
         iters = h.iters;
+        return h.status;
+    }
+
+    // ---------------- control-point form
+    // y = K(a_i, b_j) x over nb columns, split into `slices` column slices
+    void kcross(int na, const double *A, int nb, const double *Bp, const double *xin, double *yout, int slices,
+                double *P, const Ctl *c) {
+        double invW = 1.0 / knlWidth;
+        dim3 g(gridOf(na, MV_ROWS), slices);
+        double *dst = slices == 1 ? yout : part.p;
+#define SA_KC(O) kcross_k<D, O><<<g, NT, 0, s>>>(na, A, nb, Bp, xin, dst, invW, c)
+        switch (knlOrder) {
+            case -1: SA_KC(-1); break;
+            case 0: SA_KC(0); break;
+            case 1: SA_KC(1); break;
+            case 2: SA_KC(2); break;
+            case 3: SA_KC(3); break;
+            default: SA_KC(4); break;
+        }
+#undef SA_KC
+        CHECK_LAUNCH();
+        if (slices > 1) {
+            slice_sum_k<<<gridOf((long)D * na), NT, 0, s>>>(D * na, slices, part.p, yout, P, c);
+            CHECK_LAUNCH();
+        }
+    }
+
+    void kcc_build() {
+        double invW = 1.0 / knlWidth;
+        long nn = (long)NC * NC;
+        int g = gridOf(nn);
+#define SA_KB(O) kcc_build_k<D, O><<<g, NT, 0, s>>>(NC, qc.p, invW, knlEps, Kcc.p)
+        switch (knlOrder) {
+            case -1: SA_KB(-1); break;
+            case 0: SA_KB(0); break;
+            case 1: SA_KB(1); break;
+            case 2: SA_KB(2); break;
+            case 3: SA_KB(3); break;
+            default: SA_KB(4); break;
+        }
+#undef SA_KB
+        CHECK_LAUNCH();
+        if (rocsolver_dpotrf(blas, rocblas_fill_lower, NC, Kcc.p, NC, potrfInfo.p) != rocblas_status_success)
+            throw SaError(SA_E_DEVICE, "rocsolver_dpotrf failed");
+    }
+
+    void prec_solve(double *z) {   // z <- K_cc^{-1} z, one right-hand side per dimension
+        if (rocsolver_dpotrs(blas, rocblas_fill_lower, NC, D, Kcc.p, NC, z, NC) != rocblas_status_success)
+            throw SaError(SA_E_DEVICE, "rocsolver_dpotrs failed");
+    }
+
+    // S u = K_cx B K_xc u (-> cY) and K_cc u (-> cK)
+    void S_apply(const double *q, const double *tauNow, const double *geoNow, const double *u, hipEvent_t *ev) {
+        if (ev) CK(hipEventRecord(ev[0], s));
+        kcross(V, q, NC, qc.p, u, Kr.p, 1, nullptr, ctl.p);                 // w = K_xc u
+        if (ev) { CK(hipEventRecord(ev[1], s)); CK(hipEventRecord(ev[2], s)); }
+        elastic(Kr.p, tauNow, geoNow, ctl.p);
+        gather_op<2>(nullptr, nullptr, Ad.p, nullptr, nullptr);                 // B w
+        if (ev) { CK(hipEventRecord(ev[3], s)); CK(hipEventRecord(ev[4], s)); }
+        kcross(NC, qc.p, V, q, Ad.p, cY.p, slicesC, nullptr, ctl.p);         // K_cx B w
+        kcross(NC, qc.p, NC, qc.p, u, cK.p, slicesC, nullptr, ctl.p);        // K_cc u
+        if (ev) CK(hipEventRecord(ev[5], s));
+    }
+
+    int velocity_ctl(const double *q, const double *tauNow, const double *geoNow, int &iters) {
+        const int nc = NC * D, gC = gridOf(nc);
+        yank_elem_k<D><<<gridOf(E), NT, 0, s>>>(E, elmVtx.p, geoNow, ya.p, frc.p);
+        CHECK_LAUNCH();
+        gather_k<D><<<gridOf(V), NT, 0, s>>>(V, E, vtxElm.p, frc.p, f.p, P0.p);
+        CHECK_LAUNCH();
+        gather_ctl_k<D><<<gridOf(NC), NT, 0, s>>>(NC, V, ctlIdx.p, q, qc.p);
+        CHECK_LAUNCH();
+        kcc_build();
+        CK(hipMemsetAsync(ctl.p, 0, sizeof(Ctl), s));
+        kcross(NC, qc.p, V, q, f.p, cG.p, slicesC, P0.p, nullptr);            // g = K_cx f, |g|^2
+        pcg_start_k<<<1, NT, 0, s>>>(P0.p, gC, cgTolSqu, ctl.p);
+        CHECK_LAUNCH();
+        zero_if_empty_k<<<gC, NT, 0, s>>>(nc, cA.p, nullptr, ctl.p);
+        CHECK_LAUNCH();
+        S_apply(q, tauNow, geoNow, cA.p, nullptr);
+        ctl_combine_k<1><<<gC, NT, 0, s>>>(nc, cY.p, cK.p, cA.p, cG.p, ldmWgt, knlEps, cR.p, P2.p, ctl.p);
+        CHECK_LAUNCH();
+        CK(hipMemcpyAsync(cZ.p, cR.p, sizeof(double) * nc, hipMemcpyDeviceToDevice, s));
+        prec_solve(cZ.p);
+        ctl_dot_k<<<gC, NT, 0, s>>>(nc, P2.p, gC, -1, cR.p, cZ.p, P3.p, ctl.p);
+        ctl_dir_k<<<gC, NT, 0, s>>>(nc, P3.p, gC, hist.p, -1, cZ.p, cD.p, ctl.p);
+        CHECK_LAUNCH();
+        int it0 = 0;
+        int chunk = lastIters > 8 ? lastIters - 4 : 4;
+        Ctl &h = hctl[0];
+        while (true) {
+            int itEnd = std::min(itrMax, it0 + chunk);
+            if (timer.on) timer.ensure(itEnd - it0);
+            for (int it = it0; it < itEnd; ++it) {
+                hipEvent_t *ev = timer.on ? &timer.ev[6 * (it - it0)] : nullptr;
+                S_apply(q, tauNow, geoNow, cD.p, ev);
+                ctl_combine_k<0><<<gC, NT, 0, s>>>(nc, cY.p, cK.p, cD.p, nullptr, ldmWgt, knlEps, cS.p, P1.p, ctl.p);
+                ctl_update_k<<<gC, NT, 0, s>>>(nc, P1.p, gC, hist.p, it, cA.p, cR.p, cZ.p, cD.p, cS.p, P2.p, ctl.p);
+                CHECK_LAUNCH();
+                prec_solve(cZ.p);
+                ctl_dot_k<<<gC, NT, 0, s>>>(nc, P2.p, gC, it, cR.p, cZ.p, P3.p, ctl.p);
+                ctl_dir_k<<<gC, NT, 0, s>>>(nc, P3.p, gC, hist.p, it, cZ.p, cD.p, ctl.p);
+                CHECK_LAUNCH();
+            }
+            sync_ctl(ctl.p, &h);
+            if (timer.on) {
+                int last = h.done ? h.iters : itEnd;   // iterations it < last applied S
+                for (int it = it0; it < std::min(itEnd, last); ++it) {
+                    hipEvent_t *ev = &timer.ev[6 * (it - it0)];
+                    float a0, a1, a2;
+                    CK(hipEventElapsedTime(&a0, ev[0], ev[1]));
+                    CK(hipEventElapsedTime(&a1, ev[2], ev[3]));
+                    CK(hipEventElapsedTime(&a2, ev[4], ev[5]));
+                    prof->kmvMs += a0 + a2; prof->kmvCount++;
+                    prof->opMs += a1; prof->opCount++;
+                }
+            }
+            if (h.done) break;
+            if (itEnd >= itrMax) { iters = itrMax; return 1; }
+            it0 = itEnd;
+            chunk = 8;
+        }
+        iters = h.iters;
+        if (h.status == 0) {
+            kcross(V, q, NC, qc.p, cA.p, x.p, 1, nullptr, nullptr);             // v = K_xc alpha
+            int info = 0;
+            CK(hipMemcpyAsync(&info, potrfInfo.p, sizeof(int), hipMemcpyDeviceToHost, s));
+            CK(hipStreamSynchronize(s));
+            if (info != 0) return 1;
+        }
         return h.status;
     }
 
@@ -480,6 +685,8 @@ struct Engine : EngineBase {
         CHECK_LAUNCH();
         CK(hipMemsetAsync(x.p, 0, sizeof(double) * n, s));
         CK(hipMemsetAsync(a.p, 0, sizeof(double) * n, s));
+        if (NC > 0) CK(hipMemsetAsync(cA.p, 0, sizeof(double) * NC * D, s));
+        lastIters = 0;
         geometry_k<D><<<gridOf(E), NT, 0, s>>>(E, V, elmVtx.p, frmVtx.p, tsvVtx.p, q0.p, spdTan, spdTsv, geo[0].p,
                                                femElm.p);
         CHECK_LAUNCH();
@@ -539,10 +746,12 @@ struct Engine : EngineBase {
              double *tauOut, double *gradOut, Stats &st) override {
         const size_t n = (size_t)V * D;
         st = Stats{};
+        prof = &st;
+        const int TT = (Trun > 0 && Trun <= T) ? Trun : T;
         start(tauVar, nVar, tauPrm, nPrm);
         int cur = 0;
         int failed = 0;
-        for (int t = 0; t < T - 1; ++t) {
+        for (int t = 0; t < TT - 1; ++t) {
             const double *q = lmk.p + (size_t)t * n;
             double *qn = lmk.p + (size_t)(t + 1) * n;
             const double *tn = tau.p + (size_t)t * V;
@@ -555,8 +764,10 @@ struct Engine : EngineBase {
                 CHECK_LAUNCH();
             }
             int its = 0;
-            if (velocity(q, tn, geo[cur].p, t > 0, its) != 0) { st.pcgIters += its; failed = 1; break; }
+            int vs = NC > 0 ? velocity_ctl(q, tn, geo[cur].p, its) : velocity(q, tn, geo[cur].p, t > 0, its);
             st.pcgIters += its;
+            lastIters = its;
+            if (vs != 0) { failed = 1; break; }
             axpby_k<<<gridOf((long)n), NT, 0, s>>>((int)n, qn, 1.0, q, dt, x.p);
             CHECK_LAUNCH();
             int nx = cur ^ 1;
@@ -574,11 +785,11 @@ struct Engine : EngineBase {
         st.status = failed;
         if (obj) {
             if (failed) *obj = DBL_MAX;
-            else if (hasTarget) *obj = objective(lmk.p + (size_t)(T - 1) * n, gradOut);
+            else if (hasTarget) *obj = objective(lmk.p + (size_t)(TT - 1) * n, gradOut);
             else *obj = 0.0;
         }
-        if (lmkOut) CK(hipMemcpyAsync(lmkOut, lmk.p, sizeof(double) * n * T, hipMemcpyDeviceToHost, s));
-        if (tauOut) CK(hipMemcpyAsync(tauOut, tau.p, sizeof(double) * V * T, hipMemcpyDeviceToHost, s));
+        if (lmkOut) CK(hipMemcpyAsync(lmkOut, lmk.p, sizeof(double) * n * TT, hipMemcpyDeviceToHost, s));
+        if (tauOut) CK(hipMemcpyAsync(tauOut, tau.p, sizeof(double) * V * TT, hipMemcpyDeviceToHost, s));
         CK(hipStreamSynchronize(s));
     }
 };
@@ -652,11 +863,31 @@ int sa_session_run(sa_session *ss, const double *tauVarVec, int tauVarNum, const
             stats->rdIters = st.rdIters;
             stats->refineSweeps = st.refine;
             stats->seconds = sec;
+            stats->kmatvecMs = st.kmvMs;
+            stats->kmatvecCount = st.kmvCount;
+            stats->operatorMs = st.opMs;
+            stats->operatorCount = st.opCount;
         }
     });
 }
 
 void sa_session_destroy(sa_session *ss) { delete ss; }
+
+int sa_session_set_time_num(sa_session *ss, int timeNum) {
+    return sa::guarded([&] {
+        if (!ss || !ss->eng) throw sa::SaError(SA_E_NOTFOUND, "null session");
+        if (timeNum < 1 || timeNum > ss->eng->timeLevels())
+            throw sa::SaError(SA_E_ARG, "timeNum must be in 1..the session's timeNum");
+        ss->eng->Trun = timeNum;
+    });
+}
+
+int sa_session_set_profiling(sa_session *ss, int on) {
+    return sa::guarded([&] {
+        if (!ss || !ss->eng) throw sa::SaError(SA_E_NOTFOUND, "null session");
+        ss->eng->timer.on = on != 0;
+    });
+}
 
 int sa_session_probe(sa_session *ss, int what, const double *tauVarVec, int tauVarNum, const double *tauPrmVec,
                      int tauPrmNum, const double *in, double *out) {

@@ -399,6 +399,8 @@ __global__ __launch_bounds__(NT) void gather_op_k(int V, int E, const int *__res
                 double ad = ldmWgt * u[k] + 1.0 * acc[c];
                 out[k] = ad;
                 ss += w[k] * ad;
+            } else if constexpr (MODE == 2) {
+                out[k] = acc[c];   // B x alone (control-point form)
             } else {
                 // u = a (alpha), w = b (rhs); out = r, out2 = Kid
                 double ax = ldmWgt * u[k] + acc[c];
@@ -409,8 +411,10 @@ __global__ __launch_bounds__(NT) void gather_op_k(int V, int E, const int *__res
             }
         }
     }
-    double s = block_sum(ss, red);
-    if (threadIdx.x == 0) P[blockIdx.x] = s;
+    if constexpr (MODE != 2) {
+        double s = block_sum(ss, red);
+        if (threadIdx.x == 0) P[blockIdx.x] = s;
+    }
 }
 
 // right-hand side |b|^2 and stopping threshold; bb == 0 -> x = 0 (ipcg.cu:25-30)
@@ -505,7 +509,9 @@ __global__ __launch_bounds__(NT) void kmatvec_k(int V, const double *__restrict_
     __shared__ double sx[D][NT];
     __shared__ double racc[MV_PH][D][MV_ROWS];
     __shared__ double red[NT / 64];
+    __shared__ double etab[64];
     if (ctl && ctl->done) return;
+    init_exp_table(etab);
     if constexpr (CHECK) {
         double rr = reduce_partials(Prr, nPrr, red);
         if (rr < ctl->tol) {
@@ -532,7 +538,7 @@ __global__ __launch_bounds__(NT) void kmatvec_k(int V, const double *__restrict_
             double d2 = 0.0;
 #pragma unroll
             for (int c = 0; c < D; ++c) { double dd = qi[c] - sq[c][jj]; d2 += dd * dd; }
-            double k = rkhs_kernel<D, ORD>(d2, invW);
+            double k = rkhs_kernel<D, ORD>(d2, invW, etab);
 #pragma unroll
             for (int c = 0; c < D; ++c) acc[c] += k * sx[c][jj];
         }
@@ -559,7 +565,10 @@ __global__ __launch_bounds__(NT) void kmatvec_k(int V, const double *__restrict_
     }
 }
 
-// rectangular y(i) = sum_j k(|a_i - b_j|) x(j) for the control-point form
+// rectangular y(i) = sum_j k(|a_i - b_j|) x(j) for the control-point form.
+// blockIdx.y splits the column range into gridDim.y slices; slice s writes
+// its partial sums to y + s * D * na (reduced afterwards in slice order) so
+// short row ranges still fill the chip.
 template <int D, int ORD>
 __global__ __launch_bounds__(NT) void kcross_k(int na, const double *__restrict__ a, int nb,
                                                const double *__restrict__ b, const double *__restrict__ x,
@@ -567,27 +576,31 @@ __global__ __launch_bounds__(NT) void kcross_k(int na, const double *__restrict_
     __shared__ double sq[D][NT];
     __shared__ double sx[D][NT];
     __shared__ double racc[MV_PH][D][MV_ROWS];
+    __shared__ double etab[64];
     if (ctl && ctl->done) return;
+    init_exp_table(etab);
     const int lr = threadIdx.x & (MV_ROWS - 1), ph = threadIdx.x / MV_ROWS;
     const int i = blockIdx.x * MV_ROWS + lr;
+    const int per = (nb + gridDim.y - 1) / gridDim.y;
+    const int jb = blockIdx.y * per, je = min(nb, jb + per);
     double qi[D], acc[D];
 #pragma unroll
     for (int c = 0; c < D; ++c) { qi[c] = (i < na) ? a[c * na + i] : 0.0; acc[c] = 0.0; }
-    for (int j0 = 0; j0 < nb; j0 += NT) {
+    for (int j0 = jb; j0 < je; j0 += NT) {
         int j = j0 + threadIdx.x;
         __syncthreads();
 #pragma unroll
         for (int c = 0; c < D; ++c) {
-            sq[c][threadIdx.x] = (j < nb) ? b[c * nb + j] : 0.0;
-            sx[c][threadIdx.x] = (j < nb) ? x[c * nb + j] : 0.0;
+            sq[c][threadIdx.x] = (j < je) ? b[c * nb + j] : 0.0;
+            sx[c][threadIdx.x] = (j < je) ? x[c * nb + j] : 0.0;
         }
         __syncthreads();
-        int jn = min(NT, nb - j0);
+        int jn = min(NT, je - j0);
         for (int jj = ph; jj < jn; jj += MV_PH) {
             double d2 = 0.0;
 #pragma unroll
             for (int c = 0; c < D; ++c) { double dd = qi[c] - sq[c][jj]; d2 += dd * dd; }
-            double k = rkhs_kernel<D, ORD>(d2, invW);
+            double k = rkhs_kernel<D, ORD>(d2, invW, etab);
 #pragma unroll
             for (int c = 0; c < D; ++c) acc[c] += k * sx[c][jj];
         }
@@ -596,14 +609,146 @@ __global__ __launch_bounds__(NT) void kcross_k(int na, const double *__restrict_
     for (int c = 0; c < D; ++c) racc[ph][c][lr] = acc[c];
     __syncthreads();
     if (ph == 0 && i < na) {
+        double *yo = y + (size_t)blockIdx.y * D * na;
 #pragma unroll
         for (int c = 0; c < D; ++c) {
             double s = racc[0][c][lr];
 #pragma unroll
             for (int p = 1; p < MV_PH; ++p) s += racc[p][c][lr];
-            y[c * na + i] = s;
+            yo[c * na + i] = s;
         }
     }
+}
+
+// y = sum over slices of the kcross partials (slice order); optional |y|^2 partials
+__global__ __launch_bounds__(NT) void slice_sum_k(int n, int slices, const double *__restrict__ part,
+                                                  double *__restrict__ y, double *__restrict__ P, const Ctl *ctl) {
+    __shared__ double red[NT / 64];
+    if (ctl && ctl->done) return;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double ss = 0.0;
+    if (i < n) {
+        double s = part[i];
+        for (int k = 1; k < slices; ++k) s += part[(size_t)k * n + i];
+        y[i] = s;
+        ss = s * s;
+    }
+    if (P) {
+        double t = block_sum(ss, red);
+        if (threadIdx.x == 0) P[blockIdx.x] = t;
+    }
+}
+
+template <int D>
+__global__ void gather_ctl_k(int C, int V, const int *__restrict__ idx, const double *__restrict__ q,
+                             double *__restrict__ qc) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= C) return;
+    int v = idx[k];
+#pragma unroll
+    for (int c = 0; c < D; ++c) qc[c * C + k] = q[c * V + v];
+}
+
+// dense K_cc + eps I, column-major, for its Cholesky factorisation
+template <int D, int ORD>
+__global__ void kcc_build_k(int C, const double *__restrict__ qc, double invW, double eps, double *__restrict__ K) {
+    __shared__ double etab[64];
+    init_exp_table(etab);
+    __syncthreads();
+    long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)C * C) return;
+    int j = (int)(idx / C), i = (int)(idx - (long)j * C);
+    double d2 = 0.0;
+#pragma unroll
+    for (int c = 0; c < D; ++c) { double dd = qc[c * C + i] - qc[c * C + j]; d2 += dd * dd; }
+    K[idx] = rkhs_kernel<D, ORD>(d2, invW, etab) + (i == j ? eps : 0.0);
+}
+
+// S d = K_cx B K_xc d + ldmWgt (K_cc d + eps d).
+// MODE 0: out = S d, partial d.(S d).  MODE 1: out = g - S a, partial |out|^2.
+template <int MODE>
+__global__ __launch_bounds__(NT) void ctl_combine_k(int n, const double *__restrict__ y, const double *__restrict__ kd,
+                                                    const double *__restrict__ d, const double *__restrict__ g,
+                                                    double ldmWgt, double eps, double *__restrict__ out,
+                                                    double *__restrict__ P, const Ctl *ctl) {
+    __shared__ double red[NT / 64];
+    if (ctl->done) return;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double ss = 0.0;
+    if (i < n) {
+        double sd = y[i] + ldmWgt * (kd[i] + eps * d[i]);
+        if constexpr (MODE == 0) { out[i] = sd; ss = d[i] * sd; }
+        else { double r = g[i] - sd; out[i] = r; ss = r * r; }
+    }
+    double t = block_sum(ss, red);
+    if (threadIdx.x == 0) P[blockIdx.x] = t;
+}
+
+// a += st d; r -= st Sd (st = rz / d.Sd); partial |r|^2; z = r (copy for the
+// preconditioner solve)
+__global__ __launch_bounds__(NT) void ctl_update_k(int n, const double *__restrict__ P, int nP,
+                                                   const double *__restrict__ hist, int it, double *__restrict__ a,
+                                                   double *__restrict__ r, double *__restrict__ z,
+                                                   const double *__restrict__ d, const double *__restrict__ Sd,
+                                                   double *__restrict__ P2, Ctl *ctl) {
+    __shared__ double red[NT / 64];
+    if (ctl->done) return;
+    double dSd = reduce_partials(P, nP, red);
+    if (dSd == 0.0) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) { ctl->status = 1; ctl->done = 1; ctl->iters = it; }
+        return;
+    }
+    double st = hist[it] / dSd;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double ss = 0.0;
+    if (i < n) {
+        a[i] += st * d[i];
+        double ri = r[i] - st * Sd[i];
+        r[i] = ri;
+        z[i] = ri;
+        ss = ri * ri;
+    }
+    double t = block_sum(ss, red);
+    if (threadIdx.x == 0) P2[blockIdx.x] = t;
+}
+
+// after z = M^{-1} r: stop on |r|^2 < tol, else partial r.z
+__global__ __launch_bounds__(NT) void ctl_dot_k(int n, const double *__restrict__ Prr, int nP, int it,
+                                                const double *__restrict__ r, const double *__restrict__ z,
+                                                double *__restrict__ P3, Ctl *ctl) {
+    __shared__ double red[NT / 64];
+    if (ctl->done) return;
+    double rr = reduce_partials(Prr, nP, red);
+    if (rr < ctl->tol) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) { ctl->done = 1; ctl->iters = it + 1; }
+        return;
+    }
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double ss = (i < n) ? r[i] * z[i] : 0.0;
+    double t = block_sum(ss, red);
+    if (threadIdx.x == 0) P3[blockIdx.x] = t;
+}
+
+// beta = rz_new / rz_old; d = z + beta d (it < 0: d = z)
+__global__ __launch_bounds__(NT) void ctl_dir_k(int n, const double *__restrict__ P, int nP, double *__restrict__ hist,
+                                                int it, const double *__restrict__ z, double *__restrict__ d,
+                                                Ctl *ctl) {
+    __shared__ double red[NT / 64];
+    if (ctl->done) return;
+    double rz = reduce_partials(P, nP, red);
+    if (rz == 0.0) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) { ctl->status = 1; ctl->done = 1; ctl->iters = it + 1; }
+        return;
+    }
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (it < 0) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) hist[0] = rz;
+        if (i < n) d[i] = z[i];
+        return;
+    }
+    double be = rz / hist[it];
+    if (blockIdx.x == 0 && threadIdx.x == 0) hist[it + 1] = rz;
+    if (i < n) d[i] = z[i] + be * d[i];
 }
 
 __global__ void axpby_k(int n, double *__restrict__ out, double a, const double *__restrict__ x, double b,

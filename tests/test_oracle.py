@@ -390,3 +390,48 @@ def test_frames_closed_forms(oracle):
     E1 = tri.shape[0] * 3
     np.testing.assert_allclose(n[:, :E1].T, np.repeat(fn, 3, axis=0), atol=1e-14)
     np.testing.assert_allclose(s[:, :E1].T, np.repeat(rs, 3, axis=0), atol=1e-14)
+
+
+def test_control_points_all_landmarks_is_the_reference_system(oracle):
+    """With every landmark a control point, v = K alpha solves
+    (ldmWgt K + K B K) alpha = K f, i.e. the reference system multiplied by K."""
+    p = small_shell()
+    p["params"] = dict(p["params"], cgTolVal=1e-13)
+    ref = oracle.objfcn(p)
+    p2 = dict(p, ctlIdx=np.arange(p["q0"].shape[1], dtype=np.int32))
+    ctl = oracle.objfcn(p2)
+    assert ctl["status"] == 0
+    disp = np.linalg.norm(ref["lmkStk"][-1] - ref["lmkStk"][0])
+    assert np.linalg.norm(ctl["lmkStk"][-1] - ref["lmkStk"][-1]) / disp < 1e-6
+    assert abs(ctl["obj"] - ref["obj"]) < 1e-6 * abs(ref["obj"])
+
+
+def test_control_point_first_step_dense(oracle):
+    """First velocity of the control-point form against a dense solve of
+    (ldmWgt K_cc + K_cx E K_xc) alpha = K_cx f, v = K_xc alpha."""
+    p = small_shell()
+    V = p["q0"].shape[1]
+    ctl = np.sort(np.random.default_rng(2).choice(V, 40, replace=False)).astype(np.int32)
+    p = dict(p, ctlIdx=ctl, timeNum=2)
+    o = oracle.objfcn(p, with_target=False)
+    prm = p["params"]
+    q0 = p["q0"]
+    d = p["dim"]
+    tau0 = oracle.tau_initial(p)
+    ya, _ = oracle.activities(tau0, prm["ynkActPrmVec"], prm["reaActPrmVec"])
+    f = oracle.external_yank(p, q0, ya).reshape(-1)
+    Emat = np.zeros((d * V, d * V))
+    for j in range(d * V):
+        e = np.zeros(d * V); e[j] = 1.0
+        Emat[:, j] = oracle.elastic_apply(p, q0, tau0, e.reshape(d, V)).reshape(-1)
+    qc = q0[:, ctl]
+    Dxc = np.sqrt(((q0[:, :, None] - qc[:, None, :]) ** 2).sum(0)) / prm["knlWidth"]
+    Kxc = matern(1.5, Dxc)
+    Kxc[Dxc == 0] = 1.0
+    Kcc = kernel_matrix(qc, prm["knlOrder"], prm["knlWidth"], prm["knlEps"])
+    KX = np.kron(np.eye(d), Kxc)
+    S = prm["ldmWgt"] * np.kron(np.eye(d), Kcc) + KX.T @ Emat @ KX
+    alpha = np.linalg.solve(S, KX.T @ f)
+    v = (KX @ alpha).reshape(d, V)
+    v_or = (o["lmkStk"][1] - o["lmkStk"][0]) / prm["timeStp"]
+    assert np.linalg.norm(v_or - v) / np.linalg.norm(v) < 1e-7
