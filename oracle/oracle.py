@@ -333,4 +333,4 @@ def pcg_sample(prob: dict, n_iter: int, cand: int = 0, threads: int | None = Non
     tp = np.ascontiguousarray(prob["tauPrm"], dtype=np.float64)
     out = np.zeros(3)
     L.or_pcg_sample(C.byref(m), C.byref(p), _dp(q0), _dp(tv), _dp(tp), int(n_iter), _dp(out))
-    return dict(setup_s=float(out[0]), cg_s=float(out[1]), iters=int(out[2]))
+    return dict(setup_s=float(out[0]), cg_start_s=float(out[1]), per_iter_s=float(out[2]), iters=int(n_iter))

@@ -176,13 +176,17 @@ struct Engine : EngineBase {
     DBuf<double> dCen, dDir, dVol, rowVal, gC, gN, grad, scal;
     DBuf<Ctl> ctl, rdCtl;
     // control-point form
-    DBuf<double> qc, Kcc, cA, cG, cR, cZ, cD, cS, cY, cK, part;
+    DBuf<double> qc, Kcc, cA, cG, cR, cZ, cZ2, cD, cS, cY, cK, cRout, cDelta, cZero, part, partS;
+    DBuf<double> A64, At64;      // stored K_xc / K_cx, FP64
+    DBuf<float> A32, At32;       // stored K_xc / K_cx, FP32 (mixed precision)
+    int ldA = 0, ldT = 0, slicesA = 1, slicesT = 1;
     DBuf<int> potrfInfo;
     rocblas_handle blas = nullptr;
     int slicesC = 1;
     Ctl *hctl = nullptr;
     Stats *prof = nullptr;       // where the kernel timer accumulates during run()
     int lastIters = 0;           // CG iterations of the previous step (sizes the first launch chunk)
+    int chunkHint = 4;           // first launch chunk of the next control-point CG
     double ttSum = 0.0;
     bool ttReady = false;
 
@@ -350,11 +354,30 @@ struct Engine : EngineBase {
             ctlIdx.alloc(NC, tot); ctlIdx.upload(hCtl.data(), NC, s);
             qc.alloc(nc, tot); Kcc.alloc((size_t)NC * NC, tot);
             cA.alloc(nc, tot); cG.alloc(nc, tot); cR.alloc(nc, tot); cZ.alloc(nc, tot); cD.alloc(nc, tot);
-            cS.alloc(nc, tot); cY.alloc(nc, tot); cK.alloc(nc, tot);
+            cS.alloc(nc, tot); cY.alloc(nc, tot); cK.alloc(nc, tot); cZ2.alloc(nc, tot);
+            cRout.alloc(nc, tot); cDelta.alloc(nc, tot); cZero.alloc(nc, tot);
+            CK(hipMemsetAsync(cZero.p, 0, sizeof(double) * nc, s));
+            // stored kernel matrices: FP64 for the plain form, FP32 for the mixed one
+            ldA = (V + 1) / 2 * 2;
+            ldT = (NC + 1) / 2 * 2;
+            size_t el = (size_t)ldA * NC + (size_t)ldT * V;
+            if (precision == SA_PREC_MIXED) {
+                A32.alloc((size_t)ldA * NC, tot); At32.alloc((size_t)ldT * V, tot);
+                CK(hipMemsetAsync(A32.p, 0, sizeof(float) * ldA * (size_t)NC, s));
+                CK(hipMemsetAsync(At32.p, 0, sizeof(float) * ldT * (size_t)V, s));
+            } else if (el * sizeof(double) <= (size_t)48 << 30) {
+                A64.alloc((size_t)ldA * NC, tot); At64.alloc((size_t)ldT * V, tot);
+                CK(hipMemsetAsync(A64.p, 0, sizeof(double) * ldA * (size_t)NC, s));
+                CK(hipMemsetAsync(At64.p, 0, sizeof(double) * ldT * (size_t)V, s));
+            }
+            int rbA = gridOf(V, 2 * NT), rbT = gridOf(NC, 2 * NT);
+            slicesA = std::max(1, std::min(64, (1536 + rbA - 1) / rbA));
+            slicesT = std::max(1, std::min(256, (1536 + rbT - 1) / rbT));
+            partS.alloc(std::max((size_t)slicesA * D * V, (size_t)slicesT * nc), tot);
             int rb = gridOf(NC, MV_ROWS);
             slicesC = std::max(1, std::min(32, (1024 + rb - 1) / rb));
             part.alloc((size_t)slicesC * nc, tot);
-            potrfInfo.alloc(1, tot);
+            potrfInfo.alloc(2, tot);
             if (rocblas_create_handle(&blas) != rocblas_status_success) throw SaError(SA_E_DEVICE, "rocblas_create_handle failed");
             if (rocblas_set_stream(blas, s) != rocblas_status_success) throw SaError(SA_E_DEVICE, "rocblas_set_stream failed");
             require(gridOf((long)nc) <= MAXP, "too many control points for the partial-sum buffers");
@@ -480,7 +503,8 @@ struct Engine : EngineBase {
                 double *P, const Ctl *c) {
         double invW = 1.0 / knlWidth;
         dim3 g(gridOf(na, MV_ROWS), slices);
-        double *dst = slices == 1 ? yout : part.p;
+        const bool viaPart = slices > 1 || P != nullptr;
+        double *dst = viaPart ? part.p : yout;
 #define SA_KC(O) kcross_k<D, O><<<g, NT, 0, s>>>(na, A, nb, Bp, xin, dst, invW, c)
         switch (knlOrder) {
             case -1: SA_KC(-1); break;
@@ -492,7 +516,7 @@ struct Engine : EngineBase {
         }
 #undef SA_KC
         CHECK_LAUNCH();
-        if (slices > 1) {
+        if (viaPart) {
             slice_sum_k<<<gridOf((long)D * na), NT, 0, s>>>(D * na, slices, part.p, yout, P, c);
             CHECK_LAUNCH();
         }
@@ -515,43 +539,106 @@ struct Engine : EngineBase {
         CHECK_LAUNCH();
         if (rocsolver_dpotrf(blas, rocblas_fill_lower, NC, Kcc.p, NC, potrfInfo.p) != rocblas_status_success)
             throw SaError(SA_E_DEVICE, "rocsolver_dpotrf failed");
+        // X = L^{-1} in place (lower triangle)
+        if (rocsolver_dtrtri(blas, rocblas_fill_lower, rocblas_diagonal_non_unit, NC, Kcc.p, NC, potrfInfo.p + 1) !=
+            rocblas_status_success)
+            throw SaError(SA_E_DEVICE, "rocsolver_dtrtri failed");
     }
 
-    void prec_solve(double *z) {   // z <- K_cc^{-1} z, one right-hand side per dimension
-        if (rocsolver_dpotrs(blas, rocblas_fill_lower, NC, D, Kcc.p, NC, z, NC) != rocblas_status_success)
-            throw SaError(SA_E_DEVICE, "rocsolver_dpotrs failed");
+    // z <- X^T X z ~ K_cc^{-1} z (cZ2 holds X z)
+    void prec_solve(double *z) {
+        dim3 g(gridOf(NC, MV_ROWS), slicesC);
+        trmv_lower_k<D><<<g, NT, 0, s>>>(NC, Kcc.p, z, part.p, ctl.p);
+        CHECK_LAUNCH();
+        slice_sum_k<<<gridOf((long)D * NC), NT, 0, s>>>(D * NC, slicesC, part.p, cZ2.p, nullptr, ctl.p);
+        CHECK_LAUNCH();
+        trmv_lower_t_k<D><<<gridOf(NC, NT / 64), NT, 0, s>>>(NC, Kcc.p, cZ2.p, z, ctl.p);
+        CHECK_LAUNCH();
+    }
+
+    // stored kernel matrices of the control-point form (see kstore_build_k)
+    enum class KMode { MatrixFree64, Stored64, Stored32 };
+
+    void kstore_build(const double *q) {
+        double invW = 1.0 / knlWidth;
+        dim3 g(gridOf(V, 64), gridOf(NC, 64));
+        if (A64.p) {
+#define SA_KS(O) kstore_build_k<D, O, double><<<g, NT, 0, s>>>(V, q, NC, qc.p, invW, A64.p, ldA, At64.p, ldT)
+            switch (knlOrder) {
+                case -1: SA_KS(-1); break;
+                case 0: SA_KS(0); break;
+                case 1: SA_KS(1); break;
+                case 2: SA_KS(2); break;
+                case 3: SA_KS(3); break;
+                default: SA_KS(4); break;
+            }
+#undef SA_KS
+            CHECK_LAUNCH();
+        }
+        if (A32.p) {
+#define SA_KS(O) kstore_build_k<D, O, float><<<g, NT, 0, s>>>(V, q, NC, qc.p, invW, A32.p, ldA, At32.p, ldT)
+            switch (knlOrder) {
+                case -1: SA_KS(-1); break;
+                case 0: SA_KS(0); break;
+                case 1: SA_KS(1); break;
+                case 2: SA_KS(2); break;
+                case 3: SA_KS(3); break;
+                default: SA_KS(4); break;
+            }
+#undef SA_KS
+            CHECK_LAUNCH();
+        }
+    }
+
+    // y = K_xc u (xc = true, V rows) or y = K_cx u (C rows) in the given mode
+    void kx_product(KMode md, bool xc, const double *q, const double *u, double *y, double *P, const Ctl *c) {
+        if (md == KMode::MatrixFree64) {
+            if (xc) kcross(V, q, NC, qc.p, u, y, 1, P, c);
+            else kcross(NC, qc.p, V, q, u, y, slicesC, P, c);
+            return;
+        }
+        int nr = xc ? V : NC, ncol = xc ? NC : V, ld = xc ? ldA : ldT;
+        int sl = xc ? slicesA : slicesT;
+        dim3 g(gridOf(nr, 2 * NT), sl);
+        if (md == KMode::Stored64)
+            kstore_mv_k<D, double><<<g, NT, 0, s>>>(nr, ncol, xc ? A64.p : At64.p, ld, u, partS.p, c);
+        else
+            kstore_mv_k<D, float><<<g, NT, 0, s>>>(nr, ncol, xc ? A32.p : At32.p, ld, u, partS.p, c);
+        CHECK_LAUNCH();
+        slice_sum_k<<<gridOf((long)D * nr), NT, 0, s>>>(D * nr, sl, partS.p, y, P, c);
+        CHECK_LAUNCH();
     }
 
     // S u = K_cx B K_xc u (-> cY) and K_cc u (-> cK)
-    void S_apply(const double *q, const double *tauNow, const double *geoNow, const double *u, hipEvent_t *ev) {
+    void S_apply(KMode md, const double *q, const double *tauNow, const double *geoNow, const double *u,
+                 hipEvent_t *ev) {
         if (ev) CK(hipEventRecord(ev[0], s));
-        kcross(V, q, NC, qc.p, u, Kr.p, 1, nullptr, ctl.p);                 // w = K_xc u
+        kx_product(md, true, q, u, Kr.p, nullptr, ctl.p);                     // w = K_xc u
         if (ev) { CK(hipEventRecord(ev[1], s)); CK(hipEventRecord(ev[2], s)); }
         elastic(Kr.p, tauNow, geoNow, ctl.p);
         gather_op<2>(nullptr, nullptr, Ad.p, nullptr, nullptr);                 // B w
         if (ev) { CK(hipEventRecord(ev[3], s)); CK(hipEventRecord(ev[4], s)); }
-        kcross(NC, qc.p, V, q, Ad.p, cY.p, slicesC, nullptr, ctl.p);         // K_cx B w
-        kcross(NC, qc.p, NC, qc.p, u, cK.p, slicesC, nullptr, ctl.p);        // K_cc u
+        kx_product(md, false, q, Ad.p, cY.p, nullptr, ctl.p);                  // K_cx B w
         if (ev) CK(hipEventRecord(ev[5], s));
+        kcross(NC, qc.p, NC, qc.p, u, cK.p, slicesC, nullptr, ctl.p);        // K_cc u
     }
 
-    int velocity_ctl(const double *q, const double *tauNow, const double *geoNow, int &iters) {
+    // PCG on S x = rhs (|rhs|^2 partials in P0, gC of them), preconditioned by
+    // X^T X; x in/out (x0 = 0 when zeroStart); stop when |r|^2 < tolSqu |rhs|^2
+    int ctl_pcg(KMode md, const double *q, const double *tauNow, const double *geoNow, const double *rhs, double *xs,
+                bool zeroStart, double tolSqu, int maxIt, int &iters) {
         const int nc = NC * D, gC = gridOf(nc);
-        yank_elem_k<D><<<gridOf(E), NT, 0, s>>>(E, elmVtx.p, geoNow, ya.p, frc.p);
+        pcg_start_k<<<1, NT, 0, s>>>(P0.p, gC, tolSqu, ctl.p);
         CHECK_LAUNCH();
-        gather_k<D><<<gridOf(V), NT, 0, s>>>(V, E, vtxElm.p, frc.p, f.p, P0.p);
+        zero_if_empty_k<<<gC, NT, 0, s>>>(nc, xs, nullptr, ctl.p);
         CHECK_LAUNCH();
-        gather_ctl_k<D><<<gridOf(NC), NT, 0, s>>>(NC, V, ctlIdx.p, q, qc.p);
-        CHECK_LAUNCH();
-        kcc_build();
-        CK(hipMemsetAsync(ctl.p, 0, sizeof(Ctl), s));
-        kcross(NC, qc.p, V, q, f.p, cG.p, slicesC, P0.p, nullptr);            // g = K_cx f, |g|^2
-        pcg_start_k<<<1, NT, 0, s>>>(P0.p, gC, cgTolSqu, ctl.p);
-        CHECK_LAUNCH();
-        zero_if_empty_k<<<gC, NT, 0, s>>>(nc, cA.p, nullptr, ctl.p);
-        CHECK_LAUNCH();
-        S_apply(q, tauNow, geoNow, cA.p, nullptr);
-        ctl_combine_k<1><<<gC, NT, 0, s>>>(nc, cY.p, cK.p, cA.p, cG.p, ldmWgt, knlEps, cR.p, P2.p, ctl.p);
+        if (zeroStart) {
+            CK(hipMemsetAsync(xs, 0, sizeof(double) * nc, s));
+            ctl_combine_k<1><<<gC, NT, 0, s>>>(nc, cZero.p, cZero.p, cZero.p, rhs, 0.0, 0.0, cR.p, P2.p, ctl.p);
+        } else {
+            S_apply(md, q, tauNow, geoNow, xs, nullptr);
+            ctl_combine_k<1><<<gC, NT, 0, s>>>(nc, cY.p, cK.p, xs, rhs, ldmWgt, knlEps, cR.p, P2.p, ctl.p);
+        }
         CHECK_LAUNCH();
         CK(hipMemcpyAsync(cZ.p, cR.p, sizeof(double) * nc, hipMemcpyDeviceToDevice, s));
         prec_solve(cZ.p);
@@ -559,16 +646,16 @@ struct Engine : EngineBase {
         ctl_dir_k<<<gC, NT, 0, s>>>(nc, P3.p, gC, hist.p, -1, cZ.p, cD.p, ctl.p);
         CHECK_LAUNCH();
         int it0 = 0;
-        int chunk = lastIters > 8 ? lastIters - 4 : 4;
+        int chunk = std::max(4, std::min(64, chunkHint));
         Ctl &h = hctl[0];
         while (true) {
-            int itEnd = std::min(itrMax, it0 + chunk);
+            int itEnd = std::min(maxIt, it0 + chunk);
             if (timer.on) timer.ensure(itEnd - it0);
             for (int it = it0; it < itEnd; ++it) {
                 hipEvent_t *ev = timer.on ? &timer.ev[6 * (it - it0)] : nullptr;
-                S_apply(q, tauNow, geoNow, cD.p, ev);
+                S_apply(md, q, tauNow, geoNow, cD.p, ev);
                 ctl_combine_k<0><<<gC, NT, 0, s>>>(nc, cY.p, cK.p, cD.p, nullptr, ldmWgt, knlEps, cS.p, P1.p, ctl.p);
-                ctl_update_k<<<gC, NT, 0, s>>>(nc, P1.p, gC, hist.p, it, cA.p, cR.p, cZ.p, cD.p, cS.p, P2.p, ctl.p);
+                ctl_update_k<<<gC, NT, 0, s>>>(nc, P1.p, gC, hist.p, it, xs, cR.p, cZ.p, cD.p, cS.p, P2.p, ctl.p);
                 CHECK_LAUNCH();
                 prec_solve(cZ.p);
                 ctl_dot_k<<<gC, NT, 0, s>>>(nc, P2.p, gC, it, cR.p, cZ.p, P3.p, ctl.p);
@@ -589,19 +676,82 @@ struct Engine : EngineBase {
                 }
             }
             if (h.done) break;
-            if (itEnd >= itrMax) { iters = itrMax; return 1; }
+            if (itEnd >= maxIt) { iters = maxIt; return 2; }   // iteration budget spent
             it0 = itEnd;
-            chunk = 8;
+            chunk = 16;
         }
         iters = h.iters;
-        if (h.status == 0) {
-            kcross(V, q, NC, qc.p, cA.p, x.p, 1, nullptr, nullptr);             // v = K_xc alpha
-            int info = 0;
-            CK(hipMemcpyAsync(&info, potrfInfo.p, sizeof(int), hipMemcpyDeviceToHost, s));
-            CK(hipStreamSynchronize(s));
-            if (info != 0) return 1;
-        }
         return h.status;
+    }
+
+    int velocity_ctl(const double *q, const double *tauNow, const double *geoNow, int &iters, int &sweeps) {
+        const int nc = NC * D, gC = gridOf(nc);
+        yank_elem_k<D><<<gridOf(E), NT, 0, s>>>(E, elmVtx.p, geoNow, ya.p, frc.p);
+        CHECK_LAUNCH();
+        gather_k<D><<<gridOf(V), NT, 0, s>>>(V, E, vtxElm.p, frc.p, f.p, P0.p);
+        CHECK_LAUNCH();
+        gather_ctl_k<D><<<gridOf(NC), NT, 0, s>>>(NC, V, ctlIdx.p, q, qc.p);
+        CHECK_LAUNCH();
+        kcc_build();
+        if (A64.p || A32.p) kstore_build(q);
+        CK(hipMemsetAsync(ctl.p, 0, sizeof(Ctl), s));
+        // g = K_cx f in FP64 (stored FP64 matrix when present), |g|^2 partials
+        const KMode exact = A64.p ? KMode::Stored64 : KMode::MatrixFree64;
+        kx_product(exact, false, q, f.p, cG.p, P0.p, nullptr);
+        int status;
+        sweeps = 0;
+        if (precision != SA_PREC_MIXED || !A32.p) {
+            chunkHint = lastIters - 4;
+            status = ctl_pcg(exact, q, tauNow, geoNow, cG.p, cA.p, false, cgTolSqu, itrMax, iters);
+        } else {
+            // iterative refinement: FP64 residual r = g - S alpha, inner
+            // solve of S~ delta = r with the FP32 stored K (relative 1e-5),
+            // alpha += delta, until |r|^2 < cgTolSqu |g|^2
+            double gg = 0.0;
+            {
+                pcg_start_k<<<1, NT, 0, s>>>(P0.p, gC, cgTolSqu, ctl.p);
+                CHECK_LAUNCH();
+                sync_ctl(ctl.p, &hctl[0]);
+                gg = hctl[0].bb;
+            }
+            iters = 0;
+            status = 1;
+            if (gg == 0.0) {
+                CK(hipMemsetAsync(cA.p, 0, sizeof(double) * nc, s));
+                status = 0;
+            }
+            for (int sw = 0; sw < 12 && gg != 0.0; ++sw) {
+                CK(hipMemsetAsync(ctl.p, 0, sizeof(Ctl), s));
+                S_apply(KMode::MatrixFree64, q, tauNow, geoNow, cA.p, nullptr);
+                ctl_combine_k<1><<<gC, NT, 0, s>>>(nc, cY.p, cK.p, cA.p, cG.p, ldmWgt, knlEps, cRout.p, P0.p, ctl.p);
+                CHECK_LAUNCH();
+                pcg_start_k<<<1, NT, 0, s>>>(P0.p, gC, 1e-10, ctl.p);
+                CHECK_LAUNCH();
+                sync_ctl(ctl.p, &hctl[0]);
+                double rr = hctl[0].bb;
+                if (rr < cgTolSqu * gg) { status = 0; break; }
+                ++sweeps;
+                int its = 0;
+                // inner tolerance: relative 1e-5 of this residual, never below the outer target
+                double inner = std::max(1e-10, cgTolSqu * gg / rr);
+                chunkHint = 16;
+                int st = ctl_pcg(KMode::Stored32, q, tauNow, geoNow, cRout.p, cDelta.p, true, inner,
+                                 std::max(1, itrMax - iters), its);
+                iters += its;
+                if (st == 1) break;   // breakdown
+                axpby_k<<<gC, NT, 0, s>>>(nc, cA.p, 1.0, cA.p, 1.0, cDelta.p);
+                CHECK_LAUNCH();
+                if (iters >= itrMax) break;
+            }
+        }
+        if (status == 0) {
+            kx_product(exact, true, q, cA.p, x.p, nullptr, nullptr);            // v = K_xc alpha
+            int info[2] = {0, 0};
+            CK(hipMemcpyAsync(info, potrfInfo.p, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
+            CK(hipStreamSynchronize(s));
+            if (info[0] != 0 || info[1] != 0) return 1;
+        }
+        return status == 0 ? 0 : 1;
     }
 
     int reaction_diffusion(const double *tauNow, double *tauNxt, const double *geoNow, const double *ppdNow,
@@ -764,7 +914,9 @@ struct Engine : EngineBase {
                 CHECK_LAUNCH();
             }
             int its = 0;
-            int vs = NC > 0 ? velocity_ctl(q, tn, geo[cur].p, its) : velocity(q, tn, geo[cur].p, t > 0, its);
+            int sweeps = 0;
+            int vs = NC > 0 ? velocity_ctl(q, tn, geo[cur].p, its, sweeps) : velocity(q, tn, geo[cur].p, t > 0, its);
+            st.refine += sweeps;
             st.pcgIters += its;
             lastIters = its;
             if (vs != 0) { failed = 1; break; }

@@ -664,6 +664,162 @@ __global__ void kcc_build_k(int C, const double *__restrict__ qc, double invW, d
     K[idx] = rkhs_kernel<D, ORD>(d2, invW, etab) + (i == j ? eps : 0.0);
 }
 
+// Preconditioner of the control-point CG: K_cc^{-1} ~ X^T X with X = L^{-1}
+// the inverted Cholesky factor (lower triangle of a column-major C x C
+// array).  Two triangular products replace the two sequential triangular
+// solves; X^T X is symmetric positive definite whatever the rounding in X.
+//
+// z1 = X r: 64 rows x 4 column phases per block, blockIdx.y = column slice
+// (partials per slice, summed by slice_sum_k in slice order)
+template <int D>
+__global__ __launch_bounds__(NT) void trmv_lower_k(int C, const double *__restrict__ X, const double *__restrict__ r,
+                                                   double *__restrict__ part, const Ctl *ctl) {
+    __shared__ double racc[MV_PH][D][MV_ROWS];
+    if (ctl && ctl->done) return;
+    const int lr = threadIdx.x & (MV_ROWS - 1), ph = threadIdx.x / MV_ROWS;
+    const int i = blockIdx.x * MV_ROWS + lr;
+    const int per = (C + gridDim.y - 1) / gridDim.y;
+    const int jb = blockIdx.y * per;
+    const int je = min(min(C, jb + per), blockIdx.x * MV_ROWS + MV_ROWS);   // j <= i for some row of the block
+    double acc[D];
+#pragma unroll
+    for (int c = 0; c < D; ++c) acc[c] = 0.0;
+    if (i < C) {
+        for (int j = jb + ph; j < je; j += MV_PH) {
+            if (j > i) break;
+            double xij = X[(size_t)j * C + i];
+#pragma unroll
+            for (int c = 0; c < D; ++c) acc[c] += xij * r[c * C + j];
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < D; ++c) racc[ph][c][lr] = acc[c];
+    __syncthreads();
+    if (ph == 0 && i < C) {
+        double *po = part + (size_t)blockIdx.y * D * C;
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            double s = racc[0][c][lr];
+#pragma unroll
+            for (int p = 1; p < MV_PH; ++p) s += racc[p][c][lr];
+            po[c * C + i] = s;
+        }
+    }
+}
+
+// z = X^T z1: one wavefront per column j, lanes stride the rows i >= j
+template <int D>
+__global__ __launch_bounds__(NT) void trmv_lower_t_k(int C, const double *__restrict__ X, const double *__restrict__ z1,
+                                                     double *__restrict__ z, const Ctl *ctl) {
+    if (ctl && ctl->done) return;
+    const int lane = threadIdx.x & 63;
+    const int j = blockIdx.x * (NT / 64) + (threadIdx.x >> 6);
+    if (j >= C) return;   // whole wavefront leaves together
+    double acc[D];
+#pragma unroll
+    for (int c = 0; c < D; ++c) acc[c] = 0.0;
+    const double *col = X + (size_t)j * C;
+    for (int i = j + lane; i < C; i += 64) {
+        double xij = col[i];
+#pragma unroll
+        for (int c = 0; c < D; ++c) acc[c] += xij * z1[c * C + i];
+    }
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+        double s = wave_sum(acc[c]);
+        if (lane == 0) z[c * C + j] = s;
+    }
+}
+
+// Stored landmark x control-point kernel matrix, rebuilt once per time step:
+// A (V x C, column-major, leading dimension lda) and its transpose At (C x V,
+// leading dimension ldt) hold the same rounded values, so the two products
+// of a CG iteration use exactly K and K^T.  T = double (plain FP64 form) or
+// float (mixed form).  A 64 x 64 tile per block; At is written through LDS so
+// both stores are coalesced.
+template <int D, int ORD, typename T>
+__global__ __launch_bounds__(NT) void kstore_build_k(int V, const double *__restrict__ q, int C,
+                                                     const double *__restrict__ qc, double invW, T *__restrict__ A,
+                                                     int lda, T *__restrict__ At, int ldt) {
+    __shared__ double etab[64];
+    __shared__ T tile[64][65];
+    init_exp_table(etab);
+    __syncthreads();
+    const int i0 = blockIdx.x * 64, j0 = blockIdx.y * 64;
+    const int li = threadIdx.x & 63, lj0 = threadIdx.x >> 6;   // 4 column lanes
+    const int i = i0 + li;
+    double qi[D];
+#pragma unroll
+    for (int c = 0; c < D; ++c) qi[c] = i < V ? q[c * V + i] : 0.0;
+    for (int lj = lj0; lj < 64; lj += 4) {
+        int j = j0 + lj;
+        T v = 0;
+        if (i < V && j < C) {
+            double d2 = 0.0;
+#pragma unroll
+            for (int c = 0; c < D; ++c) { double dd = qi[c] - qc[c * C + j]; d2 += dd * dd; }
+            v = (T)rkhs_kernel<D, ORD>(d2, invW, etab);
+            A[(size_t)j * lda + i] = v;
+        }
+        tile[lj][li] = v;
+    }
+    __syncthreads();
+    // transposed store: lanes run along j
+    const int lj = threadIdx.x & 63, li0 = threadIdx.x >> 6;
+    for (int l = li0; l < 64; l += 4) {
+        int ii = i0 + l, jj = j0 + lj;
+        if (ii < V && jj < C) At[(size_t)ii * ldt + jj] = tile[lj][l];
+    }
+}
+
+// y = M x for a stored column-major M (nr x nc, leading dimension ld), x and
+// y with D columns (stride nc and nr).  Each thread owns 2 consecutive rows;
+// columns are split over blockIdx.y slices whose partials (slice s at
+// part + s * D * nr) are summed in slice order by slice_sum_k; x is staged
+// through LDS and read as a broadcast.
+template <int D, typename T>
+__global__ __launch_bounds__(NT) void kstore_mv_k(int nr, int nc, const T *__restrict__ M, int ld,
+                                                  const double *__restrict__ x, double *__restrict__ part,
+                                                  const Ctl *ctl) {
+    __shared__ double sx[D][NT];
+    if (ctl && ctl->done) return;
+    const int r0 = 2 * (blockIdx.x * NT + threadIdx.x);
+    const int per = (nc + gridDim.y - 1) / gridDim.y;
+    const int jb = blockIdx.y * per, je = min(nc, jb + per);
+    double a0[D], a1[D];
+#pragma unroll
+    for (int c = 0; c < D; ++c) { a0[c] = 0.0; a1[c] = 0.0; }
+    for (int j0 = jb; j0 < je; j0 += NT) {
+        int j = j0 + threadIdx.x;
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < D; ++c) sx[c][threadIdx.x] = j < je ? x[c * nc + j] : 0.0;
+        __syncthreads();
+        int jn = min(NT, je - j0);
+        if (r0 + 1 < ld) {
+            const T *col = M + (size_t)j0 * ld + r0;
+            for (int jj = 0; jj < jn; ++jj, col += ld) {
+                double m0, m1;
+                if constexpr (sizeof(T) == 4) {
+                    float2 v = *reinterpret_cast<const float2 *>(col);
+                    m0 = v.x; m1 = v.y;
+                } else {
+                    double2 v = *reinterpret_cast<const double2 *>(col);
+                    m0 = v.x; m1 = v.y;
+                }
+#pragma unroll
+                for (int c = 0; c < D; ++c) { a0[c] += m0 * sx[c][jj]; a1[c] += m1 * sx[c][jj]; }
+            }
+        }
+    }
+    double *po = part + (size_t)blockIdx.y * D * nr;
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+        if (r0 < nr) po[c * nr + r0] = a0[c];
+        if (r0 + 1 < nr) po[c * nr + r0 + 1] = a1[c];
+    }
+}
+
 // S d = K_cx B K_xc d + ldmWgt (K_cc d + eps d).
 // MODE 0: out = S d, partial d.(S d).  MODE 1: out = g - S a, partial |out|^2.
 template <int MODE>
