@@ -164,6 +164,10 @@ void sa_session_destroy(sa_session *s);
 int sa_session_set_profiling(sa_session *s, int on);
 /* run only the first timeNum time levels (<= the session's timeNum) from now on */
 int sa_session_set_time_num(sa_session *s, int timeNum);
+/* new scalar parameters (moduli, diffusion speeds, activities, weights,
+   kernel width, time step, CG controls) for the following runs; sizes, the
+   elasticity model, kernel order and precision stay those of the session */
+int sa_session_set_params(sa_session *s, const sa_params *prm);
 /* Diagnostic probes (no counterpart in the reference): evaluate one operator
    of the path at the initial state of a candidate (q0 and tau0 from
    tauVarVec) so each kernel family can be checked against an oracle alone.
@@ -183,6 +187,25 @@ int sa_session_probe(sa_session *s, int what, const double *tauVarVec, int tauVa
                      int tauPrmNum, const double *in, double *out);
 /* bytes of device memory a session of this mesh holds */
 long long sa_session_footprint(const sa_mesh *mesh, const sa_target *tgt, const sa_params *prm);
+
+/* Request server (no counterpart in the reference): solves of registered
+   meshes run on worker threads, each request on a device-resident session of
+   its mesh with its own HIP stream, so requests overlap on the device.
+   Requests are admitted in arrival order as workers and device memory free
+   up (memBudgetBytes <= 0: 80% of the free device memory); idle sessions of
+   other meshes are released, least recently used first, to make room.  A
+   request may carry its own scalar parameters (NULL: the mesh's). */
+typedef struct sa_server sa_server;
+int sa_server_create(int workers, long long memBudgetBytes, sa_server **out);
+void sa_server_destroy(sa_server *sv);
+int sa_server_add_mesh(sa_server *sv, const sa_mesh *mesh, const sa_target *tgt /* nullable */,
+                       const sa_params *prm, const double *lmkIniMat, int *meshId);
+int sa_server_submit(sa_server *sv, int meshId, const double *tauVarVec, int tauVarNum, const double *tauPrmVec,
+                     int tauPrmNum, const sa_params *prm /* nullable */, long long *reqId);
+/* blocks until the request finished; returns its status (SA_OK or SA_E_*);
+   times = [submitted, started, finished] in seconds since the server started */
+int sa_server_wait(sa_server *sv, long long reqId, double *obj, sa_stats *stats, double *times);
+int sa_server_info(sa_server *sv, long long *memUsed, int *sessions, int *queued, int *running);
 
 const char *sa_last_error(void);
 int sa_device_count(void);

@@ -88,6 +88,14 @@ def lib():
         L.sa_session_probe.argtypes = [C.c_void_p, C.c_int, _PD, C.c_int, _PD, C.c_int, _PD, _PD]
         L.sa_session_set_profiling.argtypes = [C.c_void_p, C.c_int]
         L.sa_session_set_time_num.argtypes = [C.c_void_p, C.c_int]
+        L.sa_session_set_params.argtypes = [C.c_void_p, _P(SaParams)]
+        L.sa_server_create.argtypes = [C.c_int, C.c_longlong, _P(C.c_void_p)]
+        L.sa_server_destroy.argtypes = [C.c_void_p]
+        L.sa_server_add_mesh.argtypes = [C.c_void_p, _P(SaMesh), _P(SaTarget), _P(SaParams), _PD, _PI]
+        L.sa_server_submit.argtypes = [C.c_void_p, C.c_int, _PD, C.c_int, _PD, C.c_int, _P(SaParams),
+                                       _P(C.c_longlong)]
+        L.sa_server_wait.argtypes = [C.c_void_p, C.c_longlong, _PD, _P(SaStats), _PD]
+        L.sa_server_info.argtypes = [C.c_void_p, _P(C.c_longlong), _PI, _PI, _PI]
         L.sa_deformation.argtypes = [_P(SaMesh), _P(SaParams), _PD, C.c_int, C.c_int, _PD, C.c_int, _PD, _PD, _PD, _PD]
         L.sa_compute_objfcn.argtypes = [_P(SaMesh), _P(SaTarget), _P(SaParams), _PD, C.c_int, C.c_int, _PD, C.c_int,
                                         _PD, _PD, _PD]
@@ -328,6 +336,12 @@ class Session:
                     kmatvec_ms=st.kmatvecMs, kmatvec_count=st.kmatvecCount, operator_ms=st.operatorMs,
                     operator_count=st.operatorCount)
 
+    def set_params(self, params: dict):
+        """New scalar parameters for the following runs (sa_session_set_params)."""
+        keep = _Keep()
+        prm = params_struct(params, keep, "mixed" if self.prm.precision == SA_PREC_MIXED else "fp64", self.T)
+        _check(lib().sa_session_set_params(self._h, C.byref(prm)))
+
     def set_time_num(self, timeNum: int):
         _check(lib().sa_session_set_time_num(self._h, int(timeNum)))
         self.T = int(timeNum)
@@ -354,6 +368,85 @@ class Session:
     def close(self):
         if getattr(self, "_h", None):
             lib().sa_session_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def params_struct(p: dict, keep, precision: str = "fp64", timeNum: int | None = None) -> SaParams:
+    """SaParams of a workload parameter dict."""
+    prm = make_params(p["elaType"], p["ldmWgt"], p["knlOrder"], p["knlWidth"], p["knlEps"],
+                      p["modMat"] if p["elaType"] == "isotropic" else p["modVec"], p["ynkActPrmVec"],
+                      p["spdTanVal"], p["spdTsvVal"], p["reaActPrmVec"], p["btmWgt"], p["timeStp"], p["timeEnd"],
+                      p["cgItrMax"], p["cgTolVal"], keep, precision, p.get("rdTolVal", 1e-13))
+    if timeNum:
+        prm.timeNum = int(timeNum)
+    return prm
+
+
+class Server:
+    """Request server over device-resident sessions (sa_server_* in
+    shape_atrophy.h): worker threads, per-session HIP streams, admission by
+    device-memory footprint, per-request parameters."""
+
+    def __init__(self, workers: int = 4, mem_budget: int = 0):
+        h = C.c_void_p()
+        _check(lib().sa_server_create(int(workers), int(mem_budget), C.byref(h)))
+        self._h = h
+        self._keep = []
+        self._prec = {}
+
+    def add_mesh(self, prob: dict, precision: str = "fp64") -> int:
+        keep = _Keep()
+        lmk = _f64(prob["q0"], keep)
+        elm = dict(prob["elm"])
+        if prob.get("ctlIdx") is not None:
+            elm["ctlIdxVec"] = prob["ctlIdx"]
+        prm = params_struct(prob["params"], keep, precision, prob.get("timeNum"))
+        with_t = "target" in prob
+        mesh = make_mesh(elm, lmk, keep, with_vfd=with_t)
+        tgt = make_target(prob["target"], prob["vfd"], keep) if with_t else None
+        mid = C.c_int()
+        _check(lib().sa_server_add_mesh(self._h, C.byref(mesh), C.byref(tgt) if tgt is not None else None,
+                                        C.byref(prm), _dp(lmk), C.byref(mid)))
+        self._prec[mid.value] = (precision, prob.get("timeNum"))
+        return mid.value
+
+    def submit(self, mesh_id: int, tauVar, tauPrm, params: dict | None = None) -> int:
+        keep = _Keep()
+        tv = _f64(np.asarray(tauVar, dtype=np.float64).reshape(-1), keep)
+        tp = _f64(np.asarray(tauPrm, dtype=np.float64).reshape(-1), keep)
+        prm = None
+        if params is not None:
+            prec, tn = self._prec[mesh_id]
+            prm = params_struct(params, keep, prec, tn)
+        rid = C.c_longlong()
+        _check(lib().sa_server_submit(self._h, int(mesh_id), _dp(tv), tv.size, _dp(tp), tp.size,
+                                      C.byref(prm) if prm is not None else None, C.byref(rid)))
+        return rid.value
+
+    def wait(self, req_id: int) -> dict:
+        obj = np.zeros(1)
+        st = SaStats()
+        times = np.zeros(3)
+        rc = lib().sa_server_wait(self._h, C.c_longlong(req_id), _dp(obj), C.byref(st), _dp(times))
+        return dict(rc=rc, error=lib().sa_last_error().decode() if rc else "", obj=float(obj[0]),
+                    status=st.status, steps=st.steps, pcg_iters=st.pcgIters, refine=st.refineSweeps,
+                    seconds=st.seconds, t_submit=times[0], t_start=times[1], t_end=times[2])
+
+    def info(self) -> dict:
+        mem = C.c_longlong()
+        ns, nq, nr = C.c_int(), C.c_int(), C.c_int()
+        _check(lib().sa_server_info(self._h, C.byref(mem), C.byref(ns), C.byref(nq), C.byref(nr)))
+        return dict(mem_used=mem.value, sessions=ns.value, queued=nq.value, running=nr.value)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sa_server_destroy(self._h)
             self._h = None
 
     def __del__(self):

@@ -26,6 +26,7 @@
 
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
+#include <rocblas/rocblas.h>
 
 #include "shape_atrophy.h"
 #include "sa_kernels.hpp"
@@ -146,6 +147,7 @@ struct EngineBase {
     virtual int timeLevels() const = 0;
     virtual void probe(int what, const double *tauVar, int nVar, const double *tauPrm, int nPrm, const double *in,
                        double *out) = 0;
+    virtual void set_params(const sa_params &p) = 0;
     KernelTimer timer;
 };
 
@@ -157,7 +159,7 @@ struct Engine : EngineBase {
     int V, E, Z, B, BL, F = 0, Ft = 0, T, NC = 0;
     int vtxElmRows, vtxBtmRows, femIfoRows, vfdAdjRows = 0;
     // parameters
-    int elaType, knlOrder, itrMax, precision;
+    int elaType, knlOrder, itrMax, itrMax0, precision;
     double ldmWgt, knlWidth, knlEps, spdTan, spdTsv, btmWgt, dt, cgTolSqu, rdTolSqu;
     double ynk[4], rea[5];
     std::vector<double> mod;
@@ -176,10 +178,10 @@ struct Engine : EngineBase {
     DBuf<double> dCen, dDir, dVol, rowVal, gC, gN, grad, scal;
     DBuf<Ctl> ctl, rdCtl;
     // control-point form
-    DBuf<double> qc, Kcc, cA, cG, cR, cZ, cZ2, cD, cS, cY, cK, cRout, cDelta, cZero, part, partS;
+    DBuf<double> qc, Kcc, Kc, Mpre, cA, cG, cR, cZ, cZ2, cD, cS, cY, cK, cRout, cDelta, cZero, part, partS;
     DBuf<double> A64, At64;      // stored K_xc / K_cx, FP64
     DBuf<float> A32, At32;       // stored K_xc / K_cx, FP32 (mixed precision)
-    int ldA = 0, ldT = 0, slicesA = 1, slicesT = 1;
+    int ldA = 0, ldT = 0, slicesA = 1, slicesT = 1, rowsPT = 2;
     DBuf<int> potrfInfo;
     rocblas_handle blas = nullptr;
     int slicesC = 1;
@@ -191,6 +193,29 @@ struct Engine : EngineBase {
     bool ttReady = false;
 
     int timeLevels() const override { return T; }
+
+    // scalar model parameters for the following runs (sizes, the elasticity
+    // model, precision and the allocated time levels stay)
+    void set_params(const sa_params &p) override {
+        require(p.elaType == elaType, "set_params cannot change the elasticity model");
+        require(p.modVec && p.modNum == (int)mod.size(), "set_params needs a modVec of the session's length");
+        require(p.ynkActPrmVec && p.reaActPrmVec, "null activity parameters");
+        require(p.knlOrder == knlOrder, "set_params cannot change the kernel order");
+        require(p.knlWidth > 0, "knlWidth must be positive");
+        require(p.timeNum >= 1 && p.timeNum <= T, "timeNum exceeds the session's allocation");
+        require(p.cgItrMax >= 1 && p.cgItrMax <= itrMax0, "cgItrMax exceeds the session's allocation");
+        mod.assign(p.modVec, p.modVec + p.modNum);
+        CK(hipMemcpyAsync(dmod.p, mod.data(), sizeof(double) * mod.size(), hipMemcpyHostToDevice, s));
+        CK(hipStreamSynchronize(s));
+        ldmWgt = p.ldmWgt; knlWidth = p.knlWidth; knlEps = p.knlEps;
+        spdTan = p.spdTanVal; spdTsv = p.spdTsvVal; btmWgt = p.btmWgt; dt = p.timeStp;
+        itrMax = p.cgItrMax; cgTolSqu = p.cgTolSqu;
+        double rdTol = p.rdTolVal > 0 ? p.rdTolVal : 1e-13;
+        rdTolSqu = rdTol * rdTol;
+        for (int i = 0; i < 4; ++i) ynk[i] = p.ynkActPrmVec[i];
+        for (int i = 0; i < 5; ++i) rea[i] = p.reaActPrmVec[i];
+        Trun = p.timeNum;
+    }
 
     ~Engine() override {
         if (blas) (void)rocblas_destroy_handle(blas);
@@ -207,7 +232,7 @@ struct Engine : EngineBase {
         require(p.knlOrder >= -1 && p.knlOrder <= 4, "knlOrder must be in -1..4");
         require(p.elaType == SA_ELA_FRAME || p.elaType == SA_ELA_ISOTROPIC, "unknown elasticity type");
         vtxElmRows = m.vtxElmCols; vtxBtmRows = m.vtxBtmCols; femIfoRows = m.femIfoCols;
-        elaType = p.elaType; knlOrder = p.knlOrder; itrMax = p.cgItrMax; precision = p.precision;
+        elaType = p.elaType; knlOrder = p.knlOrder; itrMax = p.cgItrMax; itrMax0 = p.cgItrMax; precision = p.precision;
         ldmWgt = p.ldmWgt; knlWidth = p.knlWidth; knlEps = p.knlEps; spdTan = p.spdTanVal; spdTsv = p.spdTsvVal;
         btmWgt = p.btmWgt; dt = p.timeStp; cgTolSqu = p.cgTolSqu;
         double rdTol = p.rdTolVal > 0 ? p.rdTolVal : 1e-13;
@@ -352,14 +377,15 @@ struct Engine : EngineBase {
         if (NC > 0) {
             size_t nc = (size_t)NC * D;
             ctlIdx.alloc(NC, tot); ctlIdx.upload(hCtl.data(), NC, s);
-            qc.alloc(nc, tot); Kcc.alloc((size_t)NC * NC, tot);
+            qc.alloc(nc, tot); Kcc.alloc((size_t)NC * NC, tot); Kc.alloc((size_t)NC * NC, tot);
+            Mpre.alloc((size_t)NC * NC, tot);
             cA.alloc(nc, tot); cG.alloc(nc, tot); cR.alloc(nc, tot); cZ.alloc(nc, tot); cD.alloc(nc, tot);
             cS.alloc(nc, tot); cY.alloc(nc, tot); cK.alloc(nc, tot); cZ2.alloc(nc, tot);
             cRout.alloc(nc, tot); cDelta.alloc(nc, tot); cZero.alloc(nc, tot);
             CK(hipMemsetAsync(cZero.p, 0, sizeof(double) * nc, s));
             // stored kernel matrices: FP64 for the plain form, FP32 for the mixed one
-            ldA = (V + 1) / 2 * 2;
-            ldT = (NC + 1) / 2 * 2;
+            ldA = (V + 3) / 4 * 4;
+            ldT = (NC + 3) / 4 * 4;
             size_t el = (size_t)ldA * NC + (size_t)ldT * V;
             if (precision == SA_PREC_MIXED) {
                 A32.alloc((size_t)ldA * NC, tot); At32.alloc((size_t)ldT * V, tot);
@@ -370,9 +396,11 @@ struct Engine : EngineBase {
                 CK(hipMemsetAsync(A64.p, 0, sizeof(double) * ldA * (size_t)NC, s));
                 CK(hipMemsetAsync(At64.p, 0, sizeof(double) * ldT * (size_t)V, s));
             }
-            int rbA = gridOf(V, 2 * NT), rbT = gridOf(NC, 2 * NT);
-            slicesA = std::max(1, std::min(64, (1536 + rbA - 1) / rbA));
-            slicesT = std::max(1, std::min(256, (1536 + rbT - 1) / rbT));
+            const int rowsPerThread = precision == SA_PREC_MIXED ? 4 : 2;
+            rowsPT = rowsPerThread;
+            int rbA = gridOf(V, rowsPerThread * NT), rbT = gridOf(NC, rowsPerThread * NT);
+            slicesA = std::max(1, std::min(64, (1024 + rbA - 1) / rbA));
+            slicesT = std::max(1, std::min(256, (1024 + rbT - 1) / rbT));
             partS.alloc(std::max((size_t)slicesA * D * V, (size_t)slicesT * nc), tot);
             int rb = gridOf(NC, MV_ROWS);
             slicesC = std::max(1, std::min(32, (1024 + rb - 1) / rb));
@@ -522,11 +550,14 @@ struct Engine : EngineBase {
         }
     }
 
+    // K_cc (+ eps I) and its preconditioner M = X^T X, X = L^{-1} the inverted
+    // Cholesky factor (rocSOLVER potrf + trtri, rocBLAS dgemm), once per step;
+    // Kc keeps K_cc without eps for the products K_cc u
     void kcc_build() {
         double invW = 1.0 / knlWidth;
         long nn = (long)NC * NC;
         int g = gridOf(nn);
-#define SA_KB(O) kcc_build_k<D, O><<<g, NT, 0, s>>>(NC, qc.p, invW, knlEps, Kcc.p)
+#define SA_KB(O) kcc_build_k<D, O><<<g, NT, 0, s>>>(NC, qc.p, invW, knlEps, Kcc.p, Kc.p)
         switch (knlOrder) {
             case -1: SA_KB(-1); break;
             case 0: SA_KB(0); break;
@@ -539,20 +570,21 @@ struct Engine : EngineBase {
         CHECK_LAUNCH();
         if (rocsolver_dpotrf(blas, rocblas_fill_lower, NC, Kcc.p, NC, potrfInfo.p) != rocblas_status_success)
             throw SaError(SA_E_DEVICE, "rocsolver_dpotrf failed");
-        // X = L^{-1} in place (lower triangle)
         if (rocsolver_dtrtri(blas, rocblas_fill_lower, rocblas_diagonal_non_unit, NC, Kcc.p, NC, potrfInfo.p + 1) !=
             rocblas_status_success)
             throw SaError(SA_E_DEVICE, "rocsolver_dtrtri failed");
+        zero_upper_k<<<g, NT, 0, s>>>(NC, Kcc.p);
+        CHECK_LAUNCH();
+        const double one = 1.0, zero = 0.0;
+        if (rocblas_dgemm(blas, rocblas_operation_transpose, rocblas_operation_none, NC, NC, NC, &one, Kcc.p, NC,
+                          Kcc.p, NC, &zero, Mpre.p, NC) != rocblas_status_success)
+            throw SaError(SA_E_DEVICE, "rocblas_dgemm failed");
     }
 
-    // z <- X^T X z ~ K_cc^{-1} z (cZ2 holds X z)
+    // z <- M z, M = X^T X ~ K_cc^{-1}
     void prec_solve(double *z) {
-        dim3 g(gridOf(NC, MV_ROWS), slicesC);
-        trmv_lower_k<D><<<g, NT, 0, s>>>(NC, Kcc.p, z, part.p, ctl.p);
-        CHECK_LAUNCH();
-        slice_sum_k<<<gridOf((long)D * NC), NT, 0, s>>>(D * NC, slicesC, part.p, cZ2.p, nullptr, ctl.p);
-        CHECK_LAUNCH();
-        trmv_lower_t_k<D><<<gridOf(NC, NT / 64), NT, 0, s>>>(NC, Kcc.p, cZ2.p, z, ctl.p);
+        CK(hipMemcpyAsync(cZ2.p, z, sizeof(double) * NC * D, hipMemcpyDeviceToDevice, s));
+        symv_k<D><<<gridOf(NC, NT / 64), NT, 0, s>>>(NC, Mpre.p, cZ2.p, z, ctl.p);
         CHECK_LAUNCH();
     }
 
@@ -599,7 +631,7 @@ struct Engine : EngineBase {
         }
         int nr = xc ? V : NC, ncol = xc ? NC : V, ld = xc ? ldA : ldT;
         int sl = xc ? slicesA : slicesT;
-        dim3 g(gridOf(nr, 2 * NT), sl);
+        dim3 g(gridOf(nr, (md == KMode::Stored32 ? 4 : 2) * NT), sl);
         if (md == KMode::Stored64)
             kstore_mv_k<D, double><<<g, NT, 0, s>>>(nr, ncol, xc ? A64.p : At64.p, ld, u, partS.p, c);
         else
@@ -620,7 +652,8 @@ struct Engine : EngineBase {
         if (ev) { CK(hipEventRecord(ev[3], s)); CK(hipEventRecord(ev[4], s)); }
         kx_product(md, false, q, Ad.p, cY.p, nullptr, ctl.p);                  // K_cx B w
         if (ev) CK(hipEventRecord(ev[5], s));
-        kcross(NC, qc.p, NC, qc.p, u, cK.p, slicesC, nullptr, ctl.p);        // K_cc u
+        symv_k<D><<<gridOf(NC, NT / 64), NT, 0, s>>>(NC, Kc.p, u, cK.p, ctl.p);   // K_cc u
+        CHECK_LAUNCH();
     }
 
     // PCG on S x = rhs (|rhs|^2 partials in P0, gC of them), preconditioned by
@@ -759,7 +792,7 @@ struct Engine : EngineBase {
         const int gV = gridOf(V);
         femvec_elem_k<D><<<gridOf(E), NT, 0, s>>>(E, elmVtx.p, geoNow, ra.p, frc.p);
         CHECK_LAUNCH();
-        rd_start_k<<<gV, NT, 0, s>>>(V, E, vtxElm.p, frc.p, rowPtr.p, colIdx.p, diagIdx.p, ppdNow, lhs.p, tauNow, dt,
+        rd_start_k<<<gV, NT, 0, s>>>(V, E, D + 1, vtxElm.p, frc.p, rowPtr.p, colIdx.p, diagIdx.p, ppdNow, lhs.p, tauNow, dt,
                                     tauNxt, rdRhs.p, rdR.p, rdZ.p, rdD.p, P3.p);
         CHECK_LAUNCH();
         rd_fin_k<<<1, NT, 0, s>>>(P3.p, gV, rdTolSqu, rdHist.p, rdCtl.p);
@@ -974,6 +1007,12 @@ struct sa_session {
     std::unique_ptr<sa::EngineBase> eng;
 };
 
+#include "sa_server.hpp"
+
+struct sa_server {
+    std::unique_ptr<sa::Server> srv;
+};
+
 extern "C" {
 
 const char *sa_last_error(void) { return sa::g_err.c_str(); }
@@ -1025,6 +1064,14 @@ int sa_session_run(sa_session *ss, const double *tauVarVec, int tauVarNum, const
 
 void sa_session_destroy(sa_session *ss) { delete ss; }
 
+int sa_session_set_params(sa_session *ss, const sa_params *prm) {
+    return sa::guarded([&] {
+        if (!ss || !ss->eng) throw sa::SaError(SA_E_NOTFOUND, "null session");
+        if (!prm) throw sa::SaError(SA_E_ARG, "null params");
+        ss->eng->set_params(*prm);
+    });
+}
+
 int sa_session_set_time_num(sa_session *ss, int timeNum) {
     return sa::guarded([&] {
         if (!ss || !ss->eng) throw sa::SaError(SA_E_NOTFOUND, "null session");
@@ -1062,6 +1109,13 @@ long long sa_session_footprint(const sa_mesh *m, const sa_target *t, const sa_pa
         long long F = m->vfdElmNum, Ft = t->tgtElmNum;
         b += 8 * (Ft * (2 * D + 1) + F * (4 * D + 1) + F + Ft + V * D) + 4 * (F * D + V * 9);
     }
+    if (m->ctlNum > 0) {
+        long long C = m->ctlNum;
+        b += 8 * (C * C + 14 * C * D + 64 * C * D + 64 * D * V);
+        long long el = ((V + 1) / 2 * 2) * C + ((C + 1) / 2 * 2) * V;
+        if (p->precision == SA_PREC_MIXED) b += 4 * el;
+        else if (el * 8 <= (48LL << 30)) b += 8 * el;
+    }
     return b;
 }
 
@@ -1097,6 +1151,59 @@ int sa_compute_objfcn(const sa_mesh *mesh, const sa_target *tgt, const sa_params
                      last ? dqObjMat : nullptr, st);
             if (objVec) objVec[c] = o;
         }
+    });
+}
+
+int sa_server_create(int workers, long long memBudgetBytes, sa_server **out) {
+    return sa::guarded([&] {
+        if (!out) throw sa::SaError(SA_E_ARG, "null server output");
+        if (memBudgetBytes <= 0) {
+            size_t fr = 0, tot = 0;
+            if (hipMemGetInfo(&fr, &tot) != hipSuccess) throw sa::SaError(SA_E_DEVICE, "hipMemGetInfo failed");
+            memBudgetBytes = (long long)(0.8 * (double)fr);
+        }
+        auto *sv = new sa_server;
+        sv->srv.reset(new sa::Server(workers, memBudgetBytes));
+        *out = sv;
+    });
+}
+
+void sa_server_destroy(sa_server *sv) { delete sv; }
+
+int sa_server_add_mesh(sa_server *sv, const sa_mesh *mesh, const sa_target *tgt, const sa_params *prm,
+                       const double *lmkIniMat, int *meshId) {
+    return sa::guarded([&] {
+        if (!sv || !mesh || !prm || !lmkIniMat || !meshId) throw sa::SaError(SA_E_ARG, "null argument");
+        *meshId = sv->srv->add_mesh(*mesh, tgt, *prm, lmkIniMat);
+    });
+}
+
+int sa_server_submit(sa_server *sv, int meshId, const double *tauVarVec, int tauVarNum, const double *tauPrmVec,
+                     int tauPrmNum, const sa_params *prm, long long *reqId) {
+    return sa::guarded([&] {
+        if (!sv || !reqId) throw sa::SaError(SA_E_ARG, "null argument");
+        *reqId = sv->srv->submit(meshId, tauVarVec, tauVarNum, tauPrmVec, tauPrmNum, prm);
+    });
+}
+
+int sa_server_wait(sa_server *sv, long long reqId, double *obj, sa_stats *stats, double *times) {
+    int rc = SA_OK;
+    int g = sa::guarded([&] {
+        if (!sv) throw sa::SaError(SA_E_ARG, "null server");
+        sa::Request r = sv->srv->wait(reqId);
+        if (obj) *obj = r.obj;
+        if (stats) *stats = r.st;
+        if (times) { times[0] = r.tSubmit; times[1] = r.tStart; times[2] = r.tEnd; }
+        rc = r.rc;
+        if (rc != SA_OK) sa::g_err = r.err;
+    });
+    return g != SA_OK ? g : rc;
+}
+
+int sa_server_info(sa_server *sv, long long *memUsed, int *sessions, int *queued, int *running) {
+    return sa::guarded([&] {
+        if (!sv) throw sa::SaError(SA_E_ARG, "null server");
+        sv->srv->info(memUsed, sessions, queued, running);
     });
 }
 

@@ -206,9 +206,9 @@ __global__ __launch_bounds__(NT) void yank_elem_k(int E, const int *__restrict__
         for (int k = 0; k < D; ++k) {
             double g = geo[(G::INV + k * D + c) * (size_t)E + e];
             s0 -= g;
-            frc[((k + 1) * D + c) * (size_t)E + e] = -fc * g * vol;
+            frc[((size_t)e * (D + 1) + k + 1) * D + c] = -fc * g * vol;
         }
-        frc[(size_t)c * E + e] = -fc * s0 * vol;
+        frc[(size_t)e * (D + 1) * D + c] = -fc * s0 * vol;
     }
 }
 
@@ -229,7 +229,7 @@ __global__ __launch_bounds__(NT) void gather_k(int V, int E, const int *__restri
         for (int a = 0; a < adj; ++a) {
             int e = vtxElm[(1 + 2 * a) * V + i], l = vtxElm[(2 + 2 * a) * V + i];
 #pragma unroll
-            for (int c = 0; c < D; ++c) acc[c] += frc[(l * D + c) * (size_t)E + e];
+            for (int c = 0; c < D; ++c) acc[c] += frc[((size_t)e * (D + 1) + l) * D + c];
         }
 #pragma unroll
         for (int c = 0; c < D; ++c) { out[c * V + i] = acc[c]; ss += acc[c] * acc[c]; }
@@ -315,9 +315,9 @@ __global__ __launch_bounds__(NT) void elastic_elem_k(int E, int V, const int *__
 #pragma unroll
             for (int j = 0; j < D; ++j) s += inv[k][j] * sig[j][c];
             s0 -= s;
-            frc[((k + 1) * D + c) * (size_t)E + e] = (T)(s * vol);
+            frc[((size_t)e * (D + 1) + k + 1) * D + c] = (T)(s * vol);
         }
-        frc[(size_t)c * E + e] = (T)(s0 * vol);
+        frc[(size_t)e * (D + 1) * D + c] = (T)(s0 * vol);
     }
 }
 
@@ -376,7 +376,7 @@ __global__ __launch_bounds__(NT) void gather_op_k(int V, int E, const int *__res
         for (int a = 0; a < adj; ++a) {
             int e = vtxElm[(1 + 2 * a) * V + i], l = vtxElm[(2 + 2 * a) * V + i];
 #pragma unroll
-            for (int c = 0; c < D; ++c) acc[c] += frc[(l * D + c) * (size_t)E + e];
+            for (int c = 0; c < D; ++c) acc[c] += frc[((size_t)e * (D + 1) + l) * D + c];
         }
         if (pfrc && i < BL) {
             double pa[D];
@@ -651,7 +651,8 @@ __global__ void gather_ctl_k(int C, int V, const int *__restrict__ idx, const do
 
 // dense K_cc + eps I, column-major, for its Cholesky factorisation
 template <int D, int ORD>
-__global__ void kcc_build_k(int C, const double *__restrict__ qc, double invW, double eps, double *__restrict__ K) {
+__global__ void kcc_build_k(int C, const double *__restrict__ qc, double invW, double eps, double *__restrict__ K,
+                            double *__restrict__ K2) {
     __shared__ double etab[64];
     init_exp_table(etab);
     __syncthreads();
@@ -661,7 +662,9 @@ __global__ void kcc_build_k(int C, const double *__restrict__ qc, double invW, d
     double d2 = 0.0;
 #pragma unroll
     for (int c = 0; c < D; ++c) { double dd = qc[c * C + i] - qc[c * C + j]; d2 += dd * dd; }
-    K[idx] = rkhs_kernel<D, ORD>(d2, invW, etab) + (i == j ? eps : 0.0);
+    double kv = rkhs_kernel<D, ORD>(d2, invW, etab);
+    K[idx] = kv + (i == j ? eps : 0.0);
+    if (K2) K2[idx] = kv;   // K_cc without eps, for the products K_cc u
 }
 
 // Preconditioner of the control-point CG: K_cc^{-1} ~ X^T X with X = L^{-1}
@@ -781,14 +784,18 @@ template <int D, typename T>
 __global__ __launch_bounds__(NT) void kstore_mv_k(int nr, int nc, const T *__restrict__ M, int ld,
                                                   const double *__restrict__ x, double *__restrict__ part,
                                                   const Ctl *ctl) {
+    // R consecutive rows per thread: 16-byte loads for both element types
+    constexpr int R = sizeof(T) == 4 ? 4 : 2;
     __shared__ double sx[D][NT];
     if (ctl && ctl->done) return;
-    const int r0 = 2 * (blockIdx.x * NT + threadIdx.x);
+    const int r0 = R * (blockIdx.x * NT + threadIdx.x);
     const int per = (nc + gridDim.y - 1) / gridDim.y;
     const int jb = blockIdx.y * per, je = min(nc, jb + per);
-    double a0[D], a1[D];
+    double a[R][D];
 #pragma unroll
-    for (int c = 0; c < D; ++c) { a0[c] = 0.0; a1[c] = 0.0; }
+    for (int k = 0; k < R; ++k)
+#pragma unroll
+        for (int c = 0; c < D; ++c) a[k][c] = 0.0;
     for (int j0 = jb; j0 < je; j0 += NT) {
         int j = j0 + threadIdx.x;
         __syncthreads();
@@ -796,28 +803,66 @@ __global__ __launch_bounds__(NT) void kstore_mv_k(int nr, int nc, const T *__res
         for (int c = 0; c < D; ++c) sx[c][threadIdx.x] = j < je ? x[c * nc + j] : 0.0;
         __syncthreads();
         int jn = min(NT, je - j0);
-        if (r0 + 1 < ld) {
+        if (r0 + R - 1 < ld) {
             const T *col = M + (size_t)j0 * ld + r0;
             for (int jj = 0; jj < jn; ++jj, col += ld) {
-                double m0, m1;
+                double m[R];
                 if constexpr (sizeof(T) == 4) {
-                    float2 v = *reinterpret_cast<const float2 *>(col);
-                    m0 = v.x; m1 = v.y;
+                    float4 v = *reinterpret_cast<const float4 *>(col);
+                    m[0] = v.x; m[1] = v.y; m[2] = v.z; m[3] = v.w;
                 } else {
                     double2 v = *reinterpret_cast<const double2 *>(col);
-                    m0 = v.x; m1 = v.y;
+                    m[0] = v.x; m[1] = v.y;
                 }
 #pragma unroll
-                for (int c = 0; c < D; ++c) { a0[c] += m0 * sx[c][jj]; a1[c] += m1 * sx[c][jj]; }
+                for (int c = 0; c < D; ++c) {
+                    double xv = sx[c][jj];
+#pragma unroll
+                    for (int k = 0; k < R; ++k) a[k][c] += m[k] * xv;
+                }
             }
         }
     }
     double *po = part + (size_t)blockIdx.y * D * nr;
 #pragma unroll
-    for (int c = 0; c < D; ++c) {
-        if (r0 < nr) po[c * nr + r0] = a0[c];
-        if (r0 + 1 < nr) po[c * nr + r0 + 1] = a1[c];
+    for (int c = 0; c < D; ++c)
+#pragma unroll
+        for (int k = 0; k < R; ++k)
+            if (r0 + k < nr) po[c * nr + r0 + k] = a[k][c];
+}
+
+// y = M x for a dense symmetric C x C matrix (column-major), D right-hand
+// sides: one wavefront per column j of M (= row j), lanes stride the rows.
+// Used for K_cc u and for the preconditioner M = X^T X.
+template <int D>
+__global__ __launch_bounds__(NT) void symv_k(int C, const double *__restrict__ M, const double *__restrict__ x,
+                                             double *__restrict__ y, const Ctl *ctl) {
+    if (ctl && ctl->done) return;
+    const int lane = threadIdx.x & 63;
+    const int j = blockIdx.x * (NT / 64) + (threadIdx.x >> 6);
+    if (j >= C) return;   // the whole wavefront leaves together
+    double acc[D];
+#pragma unroll
+    for (int c = 0; c < D; ++c) acc[c] = 0.0;
+    const double *col = M + (size_t)j * C;
+    for (int i = lane; i < C; i += 64) {
+        double m = col[i];
+#pragma unroll
+        for (int c = 0; c < D; ++c) acc[c] += m * x[c * C + i];
     }
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+        double s = wave_sum(acc[c]);
+        if (lane == 0) y[c * C + j] = s;
+    }
+}
+
+// zero the strict upper triangle of a column-major C x C matrix
+__global__ void zero_upper_k(int C, double *__restrict__ M) {
+    long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)C * C) return;
+    int j = (int)(idx / C), i = (int)(idx - (long)j * C);
+    if (i < j) M[idx] = 0.0;
 }
 
 // S d = K_cx B K_xc d + ldmWgt (K_cc d + eps d).
@@ -928,12 +973,12 @@ __global__ __launch_bounds__(NT) void femvec_elem_k(int E, const int *__restrict
 #pragma unroll
     for (int k = 0; k <= D; ++k) { rv[k] = r[elmVtx[k * E + e]]; s += rv[k]; }
 #pragma unroll
-    for (int k = 0; k <= D; ++k) frc[(size_t)k * E + e] = (s + rv[k]) * ad / (D == 2 ? 24.0 : 120.0);
+    for (int k = 0; k <= D; ++k) frc[(size_t)e * (D + 1) + k] = (s + rv[k]) * ad / (D == 2 ? 24.0 : 120.0);
 }
 
 // rhs_i = sum_z M(z) tau(col z) + dt * F_i, and the Jacobi-CG start:
 // r = rhs - A x, z = r / diag, d = z; partials r.r, r.z, rhs.rhs
-__global__ __launch_bounds__(NT) void rd_start_k(int V, int E, const int *__restrict__ vtxElm,
+__global__ __launch_bounds__(NT) void rd_start_k(int V, int E, int NV1, const int *__restrict__ vtxElm,
                                                  const double *__restrict__ frc, const int *__restrict__ rowPtr,
                                                  const int *__restrict__ colIdx, const int *__restrict__ diagIdx,
                                                  const double *__restrict__ ppd, const double *__restrict__ lhs,
@@ -949,7 +994,7 @@ __global__ __launch_bounds__(NT) void rd_start_k(int V, int E, const int *__rest
         int adj = vtxElm[i];
         for (int a = 0; a < adj; ++a) {
             int e = vtxElm[(1 + 2 * a) * V + i], l = vtxElm[(2 + 2 * a) * V + i];
-            F += frc[(size_t)l * E + e];
+            F += frc[(size_t)e * NV1 + l];
         }
         double mt = 0.0, ax = 0.0;
         int z0 = rowPtr[i], z1 = rowPtr[i + 1];

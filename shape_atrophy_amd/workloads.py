@@ -205,3 +205,59 @@ def shell_problem(level: int = 5, n_layers: int = 5, scales=None, axes=(1.0, 0.8
         crng = np.random.default_rng(ctl_seed)
         out["ctlIdx"] = np.sort(crng.choice(pos.shape[0], size=n_ctl, replace=False)).astype(np.int32)
     return out
+
+
+def perturb_params(params: dict, rng, frac: float = 0.3) -> dict:
+    """Seeded parameter candidate: elastic moduli, diffusion speeds and the
+    reaction rate each scaled by an independent factor in [1-frac, 1+frac]."""
+    p = dict(params)
+    u = lambda: float(rng.uniform(1.0 - frac, 1.0 + frac))
+    p["modVec"] = [m * u() for m in params["modVec"]]
+    if "modMat" in params:
+        p["modMat"] = [[lam * u(), mu * u()] for lam, mu in params["modMat"]]
+    s = u()
+    p["spdTanVal"] = params["spdTanVal"] * s
+    p["spdTsvVal"] = params["spdTsvVal"] * s
+    rea = list(params["reaActPrmVec"])
+    rea[2] *= u()
+    p["reaActPrmVec"] = rea
+    return p
+
+
+def batch_problems(n: int = 64, seed: int = 4, time_steps: int = 20, ctl_frac: float = 0.1,
+                   max_ctl: int = 2000, v2d=(1000, 20000), v3d=(5000, 60000)):
+    """Independent layered meshes of uneven size: half 2D rings with
+    log-uniform vertex counts in v2d, half 3D shells with counts in v3d, each
+    with seeded parameters (+-30%), its own blob and its own control points
+    (ctl_frac of the landmarks, at most max_ctl)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        two_d = k < n // 2
+        lo, hi = v2d if two_d else v3d
+        Vt = float(np.exp(rng.uniform(np.log(lo), np.log(hi))))
+        if two_d:
+            rings = int(np.clip(round(np.sqrt(Vt / 20.0)), 5, 40))
+            per = max(20, int(round(Vt / rings)))
+            p = ring_problem(n_rings=rings, n_per_ring=per, seed=100 + k, target_seed=200 + k)
+        else:
+            # icosphere level and layer count closest to the requested size
+            best = None
+            for lvl in (2, 3, 4, 5):
+                L = 10 * 4 ** lvl + 2
+                layers = int(round(Vt / L))
+                if 3 <= layers <= 12:
+                    err = abs(layers * L - Vt)
+                    if best is None or err < best[0]:
+                        best = (err, lvl, layers)
+            if best is None:
+                best = (0, 2, 3)
+            p = shell_problem(level=best[1], n_layers=best[2], target_seed=300 + k, blob_seed=400 + k)
+        p["params"] = perturb_params(p["params"], rng)
+        p["params"]["timeStp"] = 1.0 / time_steps
+        p["timeNum"] = time_steps + 1
+        V = p["q0"].shape[1]
+        nc = int(min(max_ctl, max(50, ctl_frac * V)))
+        p["ctlIdx"] = np.sort(rng.choice(V, nc, replace=False)).astype(np.int32)
+        out.append(p)
+    return out

@@ -59,3 +59,20 @@ def test_unsorted_fem_table_rejected():
     elm["femVtxMat"] = fv
     with pytest.raises(api.SaError, match="not sorted"):
         api.computeObjfcn(*api.problem_args(dict(p, elm=elm)))
+
+
+def test_server_failure_path_without_device():
+    """A request whose session cannot be created reports its error instead
+    of hanging (here: no GPU in this container, or an invalid mesh)."""
+    p = W.ring_problem(n_rings=4, n_per_ring=12)
+    elm = dict(p["elm"])
+    bad = elm["elmVtxMat"].copy()
+    bad[0, 0] = 99999
+    elm["elmVtxMat"] = bad
+    srv = api.Server(workers=1, mem_budget=1 << 30)
+    mid = srv.add_mesh(dict(p, elm=elm))
+    g = srv.wait(srv.submit(mid, p["tauVar"][:, 0], p["tauPrm"]))
+    assert g["rc"] in (-1, -3) and g["error"], g
+    if g["rc"] == -1:
+        assert "range" in g["error"]
+    srv.close()

@@ -1,0 +1,85 @@
+"""Request server: every result is the same alone and in the mix (the
+solves share no state and reduce in a fixed order, so they are bitwise
+equal), admission by device-memory footprint evicts idle sessions, and an
+oversized request is refused."""
+import numpy as np
+import pytest
+
+from shape_atrophy_amd import api, workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def meshes():
+    a = W.shell_problem(level=2, n_layers=3, time_steps=6)
+    a["ctlIdx"] = np.sort(np.random.default_rng(1).choice(a["q0"].shape[1], 150, replace=False)).astype(np.int32)
+    b = W.ring_problem(n_rings=6, n_per_ring=60)
+    b["params"] = dict(b["params"], timeStp=1.0 / 6)
+    b["timeNum"] = 7
+    b["ctlIdx"] = np.sort(np.random.default_rng(2).choice(b["q0"].shape[1], 80, replace=False)).astype(np.int32)
+    return [a, b]
+
+
+def requests(ms, n=8, seed=9):
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        m = k % len(ms)
+        prm = W.perturb_params(ms[m]["params"], rng)
+        tv = ms[m]["tauVar"][:, 0] * (1.0 + 0.05 * rng.standard_normal(ms[m]["tauVar"].shape[0]))
+        out.append((m, tv, prm))
+    return out
+
+
+def alone(ms, reqs, precision):
+    res = []
+    for m, tv, prm in reqs:
+        s = api.Session(ms[m], precision=precision)
+        s.set_params(prm)
+        r = s.run(tv, ms[m]["tauPrm"])
+        s.close()
+        assert r["status"] == 0
+        res.append(r["obj"])
+    return res
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_alone_equals_mix(precision):
+    ms = meshes()
+    reqs = requests(ms)
+    ref = alone(ms, reqs, precision)
+    srv = api.Server(workers=3)
+    ids = [srv.add_mesh(p, precision) for p in ms]
+    rids = [srv.submit(ids[m], tv, ms[m]["tauPrm"], prm) for m, tv, prm in reqs]
+    got = [srv.wait(r) for r in rids]
+    srv.close()
+    for g, r in zip(got, ref):
+        assert g["rc"] == 0 and g["status"] == 0, g
+        assert g["obj"] == r, (g["obj"], r)
+    # candidates differ, so the objectives differ
+    assert len(set(ref)) == len(ref)
+
+
+def test_admission_by_footprint_and_refusal():
+    ms = meshes()
+    reqs = requests(ms, n=6, seed=5)
+    ref = alone(ms, reqs, "fp64")
+    s0 = api.Session(ms[0])
+    s1 = api.Session(ms[1])
+    big = max(s0.footprint, s1.footprint)
+    s0.close()
+    s1.close()
+    # room for one session at a time: alternating meshes force evictions
+    srv = api.Server(workers=2, mem_budget=int(1.05 * big))
+    ids = [srv.add_mesh(p) for p in ms]
+    rids = [srv.submit(ids[m], tv, ms[m]["tauPrm"], prm) for m, tv, prm in reqs]
+    got = [srv.wait(r) for r in rids]
+    for g, r in zip(got, ref):
+        assert g["rc"] == 0 and g["obj"] == r
+    assert srv.info()["mem_used"] <= int(1.05 * big)
+    srv.close()
+    small = api.Server(workers=1, mem_budget=1024)
+    mid = small.add_mesh(ms[0])
+    g = small.wait(small.submit(mid, ms[0]["tauVar"][:, 0], ms[0]["tauPrm"]))
+    assert g["rc"] == -2 and "budget" in g["error"]
+    small.close()
