@@ -31,6 +31,13 @@ FLOP_PER_PAIR = {-1: 14, 0: 18, 1: 19, 2: 21, 3: 23, 4: 25}   # d2 (8) + kernel 
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 
+# HBM bytes per launch of the stored product, from rocprofv3 --pmc FETCH_SIZE
+# and WRITE_SIZE runs of this script (KiB per dispatch, averaged over the
+# kstore_mv_k dispatches), each scaled by what the same counter reads for a
+# plain device copy of known size (--copy-calibrate): FETCH_SIZE x 2.0,
+# WRITE_SIZE x 1.0 on MI355X.  Keyed by (config, precision).
+TRAFFIC: dict = {}
+
 
 def workload(name: str, steps: int):
     from shape_atrophy_amd import workloads as W
@@ -63,7 +70,19 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=1)
+    ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MIB",
+                    help="only run 8 device-to-device copies of MIB MiB (scale for the FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
+    if args.copy_calibrate:
+        import torch
+        n = args.copy_calibrate << 20
+        a = torch.ones(n, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(8):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": n, "copies": 8}))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the solve runs on one MI355X (asked for --gpus {args.gpus}); "
               "sharding across GPUs is not part of this build", file=sys.stderr)
@@ -110,15 +129,17 @@ def main():
     if kmv_n:
         avg = kmv_ms / kmv_n
         if nc:
-            # per CG iteration: K_xc u and K_cx y from the stored matrices
-            # (FP64 plain form / FP32 mixed form), each read once
+            # each launch (K_xc u or K_cx y) reads one stored V x C kernel
+            # matrix (FP64 plain form / FP32 mixed form) once, plus its
+            # D-column input and output vectors
             esz = 8 if args.precision == "fp64" else 4
-            B = esz * 2 * V * nc
+            B = esz * V * nc + 8 * d * (V + nc)
             ach = B / (avg * 1e-3) / 1e9
-            roof = {"bound": "hbm", "kernel": f"kstore_mv_k<{'double' if esz == 8 else 'float'}> x2",
+            roof = {"bound": "hbm", "kernel": f"kstore_mv_k<{d}, {'double' if esz == 8 else 'float'}>",
                     "achieved": round(ach, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                    "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": None, "avg_ms": round(avg, 4),
-                    "launches": kmv_n, "work": f"{esz} B x 2 x {V} x {nc} stored kernel entries per iteration"}
+                    "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": TRAFFIC.get((args.config, args.precision)),
+                    "avg_ms": round(avg, 4), "launches": kmv_n,
+                    "work": f"{esz} B x {V} x {nc} stored kernel entries + 8 B x {d} x ({V} + {nc}) vectors per launch"}
         else:
             pairs = V * V
             flops = FLOP_PER_PAIR[order] * pairs

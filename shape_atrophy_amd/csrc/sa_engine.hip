@@ -178,10 +178,10 @@ struct Engine : EngineBase {
     DBuf<double> dCen, dDir, dVol, rowVal, gC, gN, grad, scal;
     DBuf<Ctl> ctl, rdCtl;
     // control-point form
-    DBuf<double> qc, Kcc, Kc, Mpre, cA, cG, cR, cZ, cZ2, cD, cS, cY, cK, cRout, cDelta, cZero, part, partS;
+    DBuf<double> qc, Kcc, Kc, Mpre, cA, cG, cR, cZ, cD, cS, cY, cK, cRout, cDelta, cZero, part, partS;
     DBuf<double> A64, At64;      // stored K_xc / K_cx, FP64
     DBuf<float> A32, At32;       // stored K_xc / K_cx, FP32 (mixed precision)
-    int ldA = 0, ldT = 0, slicesA = 1, slicesT = 1, rowsPT = 2;
+    int ldA = 0, ldT = 0, slicesA = 1, slicesT = 1, slicesM = 1;
     DBuf<int> potrfInfo;
     rocblas_handle blas = nullptr;
     int slicesC = 1;
@@ -377,15 +377,24 @@ struct Engine : EngineBase {
         if (NC > 0) {
             size_t nc = (size_t)NC * D;
             ctlIdx.alloc(NC, tot); ctlIdx.upload(hCtl.data(), NC, s);
-            qc.alloc(nc, tot); Kcc.alloc((size_t)NC * NC, tot); Kc.alloc((size_t)NC * NC, tot);
-            Mpre.alloc((size_t)NC * NC, tot);
+            qc.alloc(nc, tot); Kcc.alloc((size_t)NC * NC, tot);
             cA.alloc(nc, tot); cG.alloc(nc, tot); cR.alloc(nc, tot); cZ.alloc(nc, tot); cD.alloc(nc, tot);
-            cS.alloc(nc, tot); cY.alloc(nc, tot); cK.alloc(nc, tot); cZ2.alloc(nc, tot);
+            cS.alloc(nc, tot); cY.alloc(nc, tot); cK.alloc(nc, tot);
             cRout.alloc(nc, tot); cDelta.alloc(nc, tot); cZero.alloc(nc, tot);
             CK(hipMemsetAsync(cZero.p, 0, sizeof(double) * nc, s));
             // stored kernel matrices: FP64 for the plain form, FP32 for the mixed one
             ldA = (V + 3) / 4 * 4;
             ldT = (NC + 3) / 4 * 4;
+            // preconditioner with the padded leading dimension of the stored
+            // products (padding rows stay zero)
+            Mpre.alloc((size_t)ldT * NC, tot);
+            CK(hipMemsetAsync(Mpre.p, 0, sizeof(double) * ldT * (size_t)NC, s));
+            if (precision == SA_PREC_MIXED) {
+                // FP64 K_cc for the FP32 inner solve: its FP32 rounding need
+                // not be positive definite when K_cc is ill conditioned
+                Kc.alloc((size_t)ldT * NC, tot);
+                CK(hipMemsetAsync(Kc.p, 0, sizeof(double) * ldT * (size_t)NC, s));
+            }
             size_t el = (size_t)ldA * NC + (size_t)ldT * V;
             if (precision == SA_PREC_MIXED) {
                 A32.alloc((size_t)ldA * NC, tot); At32.alloc((size_t)ldT * V, tot);
@@ -396,12 +405,14 @@ struct Engine : EngineBase {
                 CK(hipMemsetAsync(A64.p, 0, sizeof(double) * ldA * (size_t)NC, s));
                 CK(hipMemsetAsync(At64.p, 0, sizeof(double) * ldT * (size_t)V, s));
             }
-            const int rowsPerThread = precision == SA_PREC_MIXED ? 4 : 2;
-            rowsPT = rowsPerThread;
-            int rbA = gridOf(V, rowsPerThread * NT), rbT = gridOf(NC, rowsPerThread * NT);
+            // about 1024 blocks per stored product (row blocks x column slices)
+            const int rbA = gridOf(V, rows_per_thread(precision == SA_PREC_MIXED) * 64);
+            const int rbT = gridOf(NC, rows_per_thread(precision == SA_PREC_MIXED) * 64);
+            const int rbM = gridOf(NC, rows_per_thread(false) * 64);
             slicesA = std::max(1, std::min(64, (1024 + rbA - 1) / rbA));
             slicesT = std::max(1, std::min(256, (1024 + rbT - 1) / rbT));
-            partS.alloc(std::max((size_t)slicesA * D * V, (size_t)slicesT * nc), tot);
+            slicesM = std::max(1, std::min(256, (1024 + rbM - 1) / rbM));
+            partS.alloc(std::max({(size_t)slicesA * D * V, (size_t)slicesT * nc, (size_t)slicesM * nc}), tot);
             int rb = gridOf(NC, MV_ROWS);
             slicesC = std::max(1, std::min(32, (1024 + rb - 1) / rb));
             part.alloc((size_t)slicesC * nc, tot);
@@ -557,7 +568,7 @@ struct Engine : EngineBase {
         double invW = 1.0 / knlWidth;
         long nn = (long)NC * NC;
         int g = gridOf(nn);
-#define SA_KB(O) kcc_build_k<D, O><<<g, NT, 0, s>>>(NC, qc.p, invW, knlEps, Kcc.p, Kc.p)
+#define SA_KB(O) kcc_build_k<D, O><<<g, NT, 0, s>>>(NC, qc.p, invW, knlEps, Kcc.p, Kc.p, ldT)
         switch (knlOrder) {
             case -1: SA_KB(-1); break;
             case 0: SA_KB(0); break;
@@ -577,16 +588,24 @@ struct Engine : EngineBase {
         CHECK_LAUNCH();
         const double one = 1.0, zero = 0.0;
         if (rocblas_dgemm(blas, rocblas_operation_transpose, rocblas_operation_none, NC, NC, NC, &one, Kcc.p, NC,
-                          Kcc.p, NC, &zero, Mpre.p, NC) != rocblas_status_success)
+                          Kcc.p, NC, &zero, Mpre.p, ldT) != rocblas_status_success)
             throw SaError(SA_E_DEVICE, "rocblas_dgemm failed");
     }
 
-    // z <- M z, M = X^T X ~ K_cc^{-1}
-    void prec_solve(double *z) {
-        CK(hipMemcpyAsync(cZ2.p, z, sizeof(double) * NC * D, hipMemcpyDeviceToDevice, s));
-        symv_k<D><<<gridOf(NC, NT / 64), NT, 0, s>>>(NC, Mpre.p, cZ2.p, z, ctl.p);
+    static int rows_per_thread(bool fp32) { return fp32 ? 4 : 2; }
+
+    // y = M x for a dense FP64 C x C matrix stored with leading dimension ldT
+    void cc_product(const double *M, const double *x, double *y) {
+        dim3 g(gridOf(NC, rows_per_thread(false) * 64), slicesM);
+        kstore_mv_k<D, double><<<g, NT, 0, s>>>(NC, NC, M, ldT, x, partS.p, ctl.p);
+        CHECK_LAUNCH();
+        slice_rows_k<<<gridOf(D * NC, 64), NT, 0, s>>>(D * NC, slicesM, partS.p, y, ctl.p);
         CHECK_LAUNCH();
     }
+
+    // z = M r, M = X^T X ~ K_cc^{-1}, in FP64 in every form: an FP32 rounding
+    // of M need not be positive definite
+    void prec_apply(const double *r, double *z) { cc_product(Mpre.p, r, z); }
 
     // stored kernel matrices of the control-point form (see kstore_build_k)
     enum class KMode { MatrixFree64, Stored64, Stored32 };
@@ -622,38 +641,45 @@ struct Engine : EngineBase {
         }
     }
 
-    // y = K_xc u (xc = true, V rows) or y = K_cx u (C rows) in the given mode
-    void kx_product(KMode md, bool xc, const double *q, const double *u, double *y, double *P, const Ctl *c) {
+    // y = K_xc u (xc = true, V rows) or y = K_cx u (C rows) in the given mode;
+    // |y|^2 partials in P when given.  ev (optional): events recorded around
+    // the stored-matrix product kernel alone.
+    void kx_product(KMode md, bool xc, const double *q, const double *u, double *y, double *P, const Ctl *c,
+                    hipEvent_t *ev = nullptr) {
         if (md == KMode::MatrixFree64) {
+            if (ev) CK(hipEventRecord(ev[0], s));
             if (xc) kcross(V, q, NC, qc.p, u, y, 1, P, c);
             else kcross(NC, qc.p, V, q, u, y, slicesC, P, c);
+            if (ev) CK(hipEventRecord(ev[1], s));
             return;
         }
-        int nr = xc ? V : NC, ncol = xc ? NC : V, ld = xc ? ldA : ldT;
-        int sl = xc ? slicesA : slicesT;
-        dim3 g(gridOf(nr, (md == KMode::Stored32 ? 4 : 2) * NT), sl);
+        const int nr = xc ? V : NC, ncol = xc ? NC : V, ld = xc ? ldA : ldT;
+        const int sl = xc ? slicesA : slicesT;
+        dim3 g(gridOf(nr, rows_per_thread(md == KMode::Stored32) * 64), sl);
+        if (ev) CK(hipEventRecord(ev[0], s));
         if (md == KMode::Stored64)
             kstore_mv_k<D, double><<<g, NT, 0, s>>>(nr, ncol, xc ? A64.p : At64.p, ld, u, partS.p, c);
         else
             kstore_mv_k<D, float><<<g, NT, 0, s>>>(nr, ncol, xc ? A32.p : At32.p, ld, u, partS.p, c);
         CHECK_LAUNCH();
-        slice_sum_k<<<gridOf((long)D * nr), NT, 0, s>>>(D * nr, sl, partS.p, y, P, c);
+        if (ev) CK(hipEventRecord(ev[1], s));
+        if (P) slice_sum_k<<<gridOf((long)D * nr), NT, 0, s>>>(D * nr, sl, partS.p, y, P, c);   // + |y|^2 partials
+        else slice_rows_k<<<gridOf(D * nr, 64), NT, 0, s>>>(D * nr, sl, partS.p, y, c);
         CHECK_LAUNCH();
     }
 
-    // S u = K_cx B K_xc u (-> cY) and K_cc u (-> cK)
+    // S u = K_cx B K_xc u (-> cY); K_xc u stays in Kr.  Its control-point
+    // rows are K_cc u in the FP64 forms; the FP32 form takes K_cc u from the
+    // FP64 K_cc (-> cK).  ev: 6 events (product, operator, product).
     void S_apply(KMode md, const double *q, const double *tauNow, const double *geoNow, const double *u,
                  hipEvent_t *ev) {
-        if (ev) CK(hipEventRecord(ev[0], s));
-        kx_product(md, true, q, u, Kr.p, nullptr, ctl.p);                     // w = K_xc u
-        if (ev) { CK(hipEventRecord(ev[1], s)); CK(hipEventRecord(ev[2], s)); }
+        kx_product(md, true, q, u, Kr.p, nullptr, ctl.p, ev);                  // w = K_xc u
+        if (ev) CK(hipEventRecord(ev[2], s));
         elastic(Kr.p, tauNow, geoNow, ctl.p);
         gather_op<2>(nullptr, nullptr, Ad.p, nullptr, nullptr);                 // B w
-        if (ev) { CK(hipEventRecord(ev[3], s)); CK(hipEventRecord(ev[4], s)); }
-        kx_product(md, false, q, Ad.p, cY.p, nullptr, ctl.p);                  // K_cx B w
-        if (ev) CK(hipEventRecord(ev[5], s));
-        symv_k<D><<<gridOf(NC, NT / 64), NT, 0, s>>>(NC, Kc.p, u, cK.p, ctl.p);   // K_cc u
-        CHECK_LAUNCH();
+        if (ev) CK(hipEventRecord(ev[3], s));
+        kx_product(md, false, q, Ad.p, cY.p, nullptr, ctl.p, ev ? ev + 4 : nullptr);   // K_cx B w
+        if (md == KMode::Stored32) cc_product(Kc.p, u, cK.p);
     }
 
     // PCG on S x = rhs (|rhs|^2 partials in P0, gC of them), preconditioned by
@@ -661,20 +687,22 @@ struct Engine : EngineBase {
     int ctl_pcg(KMode md, const double *q, const double *tauNow, const double *geoNow, const double *rhs, double *xs,
                 bool zeroStart, double tolSqu, int maxIt, int &iters) {
         const int nc = NC * D, gC = gridOf(nc);
+        const double *kdv = md == KMode::Stored32 ? cK.p : nullptr;   // K_cc d (else gathered from K_xc d)
         pcg_start_k<<<1, NT, 0, s>>>(P0.p, gC, tolSqu, ctl.p);
         CHECK_LAUNCH();
         zero_if_empty_k<<<gC, NT, 0, s>>>(nc, xs, nullptr, ctl.p);
         CHECK_LAUNCH();
         if (zeroStart) {
             CK(hipMemsetAsync(xs, 0, sizeof(double) * nc, s));
-            ctl_combine_k<1><<<gC, NT, 0, s>>>(nc, cZero.p, cZero.p, cZero.p, rhs, 0.0, 0.0, cR.p, P2.p, ctl.p);
+            ctl_combine_k<D, 1><<<gC, NT, 0, s>>>(NC, V, cZero.p, nullptr, nullptr, ctlIdx.p, cZero.p, rhs, 0.0, 0.0, cR.p, P2.p,
+                                                 ctl.p);
         } else {
             S_apply(md, q, tauNow, geoNow, xs, nullptr);
-            ctl_combine_k<1><<<gC, NT, 0, s>>>(nc, cY.p, cK.p, xs, rhs, ldmWgt, knlEps, cR.p, P2.p, ctl.p);
+            ctl_combine_k<D, 1><<<gC, NT, 0, s>>>(NC, V, cY.p, kdv, Kr.p, ctlIdx.p, xs, rhs, ldmWgt, knlEps, cR.p, P2.p,
+                                                 ctl.p);
         }
         CHECK_LAUNCH();
-        CK(hipMemcpyAsync(cZ.p, cR.p, sizeof(double) * nc, hipMemcpyDeviceToDevice, s));
-        prec_solve(cZ.p);
+        prec_apply(cR.p, cZ.p);
         ctl_dot_k<<<gC, NT, 0, s>>>(nc, P2.p, gC, -1, cR.p, cZ.p, P3.p, ctl.p);
         ctl_dir_k<<<gC, NT, 0, s>>>(nc, P3.p, gC, hist.p, -1, cZ.p, cD.p, ctl.p);
         CHECK_LAUNCH();
@@ -687,10 +715,11 @@ struct Engine : EngineBase {
             for (int it = it0; it < itEnd; ++it) {
                 hipEvent_t *ev = timer.on ? &timer.ev[6 * (it - it0)] : nullptr;
                 S_apply(md, q, tauNow, geoNow, cD.p, ev);
-                ctl_combine_k<0><<<gC, NT, 0, s>>>(nc, cY.p, cK.p, cD.p, nullptr, ldmWgt, knlEps, cS.p, P1.p, ctl.p);
-                ctl_update_k<<<gC, NT, 0, s>>>(nc, P1.p, gC, hist.p, it, xs, cR.p, cZ.p, cD.p, cS.p, P2.p, ctl.p);
+                ctl_combine_k<D, 0><<<gC, NT, 0, s>>>(NC, V, cY.p, kdv, Kr.p, ctlIdx.p, cD.p, nullptr, ldmWgt, knlEps, cS.p,
+                                                     P1.p, ctl.p);
+                ctl_update_k<<<gC, NT, 0, s>>>(nc, P1.p, gC, hist.p, it, xs, cR.p, cD.p, cS.p, P2.p, ctl.p);
                 CHECK_LAUNCH();
-                prec_solve(cZ.p);
+                prec_apply(cR.p, cZ.p);
                 ctl_dot_k<<<gC, NT, 0, s>>>(nc, P2.p, gC, it, cR.p, cZ.p, P3.p, ctl.p);
                 ctl_dir_k<<<gC, NT, 0, s>>>(nc, P3.p, gC, hist.p, it, cZ.p, cD.p, ctl.p);
                 CHECK_LAUNCH();
@@ -704,7 +733,7 @@ struct Engine : EngineBase {
                     CK(hipEventElapsedTime(&a0, ev[0], ev[1]));
                     CK(hipEventElapsedTime(&a1, ev[2], ev[3]));
                     CK(hipEventElapsedTime(&a2, ev[4], ev[5]));
-                    prof->kmvMs += a0 + a2; prof->kmvCount++;
+                    prof->kmvMs += a0 + a2; prof->kmvCount += 2;   // two product launches
                     prof->opMs += a1; prof->opCount++;
                 }
             }
@@ -756,7 +785,8 @@ struct Engine : EngineBase {
             for (int sw = 0; sw < 12 && gg != 0.0; ++sw) {
                 CK(hipMemsetAsync(ctl.p, 0, sizeof(Ctl), s));
                 S_apply(KMode::MatrixFree64, q, tauNow, geoNow, cA.p, nullptr);
-                ctl_combine_k<1><<<gC, NT, 0, s>>>(nc, cY.p, cK.p, cA.p, cG.p, ldmWgt, knlEps, cRout.p, P0.p, ctl.p);
+                ctl_combine_k<D, 1><<<gC, NT, 0, s>>>(NC, V, cY.p, nullptr, Kr.p, ctlIdx.p, cA.p, cG.p, ldmWgt, knlEps, cRout.p,
+                                                     P0.p, ctl.p);
                 CHECK_LAUNCH();
                 pcg_start_k<<<1, NT, 0, s>>>(P0.p, gC, 1e-10, ctl.p);
                 CHECK_LAUNCH();
@@ -1111,9 +1141,11 @@ long long sa_session_footprint(const sa_mesh *m, const sa_target *t, const sa_pa
     }
     if (m->ctlNum > 0) {
         long long C = m->ctlNum;
-        b += 8 * (C * C + 14 * C * D + 64 * C * D + 64 * D * V);
-        long long el = ((V + 1) / 2 * 2) * C + ((C + 1) / 2 * 2) * V;
-        if (p->precision == SA_PREC_MIXED) b += 4 * el;
+        const long long ldT = (C + 3) / 4 * 4, ldA = (V + 3) / 4 * 4;
+        // K_cc, preconditioner, vectors, slice partials, cross partials
+        b += 8 * (C * C + ldT * C + 12 * C * D + 256 * C * D + 64 * D * V + 32 * C * D) + 4 * 2 * sa::MAXP;
+        long long el = ldA * C + ldT * V;
+        if (p->precision == SA_PREC_MIXED) b += 4 * el + 8 * ldT * C;
         else if (el * 8 <= (48LL << 30)) b += 8 * el;
     }
     return b;

@@ -652,7 +652,7 @@ __global__ void gather_ctl_k(int C, int V, const int *__restrict__ idx, const do
 // dense K_cc + eps I, column-major, for its Cholesky factorisation
 template <int D, int ORD>
 __global__ void kcc_build_k(int C, const double *__restrict__ qc, double invW, double eps, double *__restrict__ K,
-                            double *__restrict__ K2) {
+                            double *__restrict__ K2, int ld2) {
     __shared__ double etab[64];
     init_exp_table(etab);
     __syncthreads();
@@ -664,74 +664,7 @@ __global__ void kcc_build_k(int C, const double *__restrict__ qc, double invW, d
     for (int c = 0; c < D; ++c) { double dd = qc[c * C + i] - qc[c * C + j]; d2 += dd * dd; }
     double kv = rkhs_kernel<D, ORD>(d2, invW, etab);
     K[idx] = kv + (i == j ? eps : 0.0);
-    if (K2) K2[idx] = kv;   // K_cc without eps, for the products K_cc u
-}
-
-// Preconditioner of the control-point CG: K_cc^{-1} ~ X^T X with X = L^{-1}
-// the inverted Cholesky factor (lower triangle of a column-major C x C
-// array).  Two triangular products replace the two sequential triangular
-// solves; X^T X is symmetric positive definite whatever the rounding in X.
-//
-// z1 = X r: 64 rows x 4 column phases per block, blockIdx.y = column slice
-// (partials per slice, summed by slice_sum_k in slice order)
-template <int D>
-__global__ __launch_bounds__(NT) void trmv_lower_k(int C, const double *__restrict__ X, const double *__restrict__ r,
-                                                   double *__restrict__ part, const Ctl *ctl) {
-    __shared__ double racc[MV_PH][D][MV_ROWS];
-    if (ctl && ctl->done) return;
-    const int lr = threadIdx.x & (MV_ROWS - 1), ph = threadIdx.x / MV_ROWS;
-    const int i = blockIdx.x * MV_ROWS + lr;
-    const int per = (C + gridDim.y - 1) / gridDim.y;
-    const int jb = blockIdx.y * per;
-    const int je = min(min(C, jb + per), blockIdx.x * MV_ROWS + MV_ROWS);   // j <= i for some row of the block
-    double acc[D];
-#pragma unroll
-    for (int c = 0; c < D; ++c) acc[c] = 0.0;
-    if (i < C) {
-        for (int j = jb + ph; j < je; j += MV_PH) {
-            if (j > i) break;
-            double xij = X[(size_t)j * C + i];
-#pragma unroll
-            for (int c = 0; c < D; ++c) acc[c] += xij * r[c * C + j];
-        }
-    }
-#pragma unroll
-    for (int c = 0; c < D; ++c) racc[ph][c][lr] = acc[c];
-    __syncthreads();
-    if (ph == 0 && i < C) {
-        double *po = part + (size_t)blockIdx.y * D * C;
-#pragma unroll
-        for (int c = 0; c < D; ++c) {
-            double s = racc[0][c][lr];
-#pragma unroll
-            for (int p = 1; p < MV_PH; ++p) s += racc[p][c][lr];
-            po[c * C + i] = s;
-        }
-    }
-}
-
-// z = X^T z1: one wavefront per column j, lanes stride the rows i >= j
-template <int D>
-__global__ __launch_bounds__(NT) void trmv_lower_t_k(int C, const double *__restrict__ X, const double *__restrict__ z1,
-                                                     double *__restrict__ z, const Ctl *ctl) {
-    if (ctl && ctl->done) return;
-    const int lane = threadIdx.x & 63;
-    const int j = blockIdx.x * (NT / 64) + (threadIdx.x >> 6);
-    if (j >= C) return;   // whole wavefront leaves together
-    double acc[D];
-#pragma unroll
-    for (int c = 0; c < D; ++c) acc[c] = 0.0;
-    const double *col = X + (size_t)j * C;
-    for (int i = j + lane; i < C; i += 64) {
-        double xij = col[i];
-#pragma unroll
-        for (int c = 0; c < D; ++c) acc[c] += xij * z1[c * C + i];
-    }
-#pragma unroll
-    for (int c = 0; c < D; ++c) {
-        double s = wave_sum(acc[c]);
-        if (lane == 0) z[c * C + j] = s;
-    }
+    if (K2) K2[(size_t)j * ld2 + i] = kv;   // K_cc without eps (leading dimension ld2)
 }
 
 // Stored landmark x control-point kernel matrix, rebuilt once per time step:
@@ -776,19 +709,24 @@ __global__ __launch_bounds__(NT) void kstore_build_k(int V, const double *__rest
 }
 
 // y = M x for a stored column-major M (nr x nc, leading dimension ld), x and
-// y with D columns (stride nc and nr).  Each thread owns 2 consecutive rows;
-// columns are split over blockIdx.y slices whose partials (slice s at
-// part + s * D * nr) are summed in slice order by slice_sum_k; x is staged
-// through LDS and read as a broadcast.
+// y with D columns (stride nc and nr).  A block owns 64 R consecutive rows
+// (R per lane, one 16-byte load per column) and one slice of the columns
+// (blockIdx.y); its four wavefronts take consecutive quarters of every
+// 256-column chunk and are summed in wavefront order through LDS.  The
+// block's partial goes to part + s * D * nr; slice_rows_k sums the slices in
+// slice order.  x is staged through LDS and read as a broadcast.
 template <int D, typename T>
 __global__ __launch_bounds__(NT) void kstore_mv_k(int nr, int nc, const T *__restrict__ M, int ld,
                                                   const double *__restrict__ x, double *__restrict__ part,
                                                   const Ctl *ctl) {
-    // R consecutive rows per thread: 16-byte loads for both element types
     constexpr int R = sizeof(T) == 4 ? 4 : 2;
+    constexpr int W = NT / 64;
+    constexpr int U = 4;   // columns whose loads are issued together
     __shared__ double sx[D][NT];
+    __shared__ double red[W - 1][D][64 * R];
     if (ctl && ctl->done) return;
-    const int r0 = R * (blockIdx.x * NT + threadIdx.x);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int r0 = R * (blockIdx.x * 64 + lane);
     const int per = (nc + gridDim.y - 1) / gridDim.y;
     const int jb = blockIdx.y * per, je = min(nc, jb + per);
     double a[R][D];
@@ -796,64 +734,97 @@ __global__ __launch_bounds__(NT) void kstore_mv_k(int nr, int nc, const T *__res
     for (int k = 0; k < R; ++k)
 #pragma unroll
         for (int c = 0; c < D; ++c) a[k][c] = 0.0;
+    const bool in = r0 + R - 1 < ld;
     for (int j0 = jb; j0 < je; j0 += NT) {
         int j = j0 + threadIdx.x;
         __syncthreads();
 #pragma unroll
         for (int c = 0; c < D; ++c) sx[c][threadIdx.x] = j < je ? x[c * nc + j] : 0.0;
         __syncthreads();
-        int jn = min(NT, je - j0);
-        if (r0 + R - 1 < ld) {
-            const T *col = M + (size_t)j0 * ld + r0;
-            for (int jj = 0; jj < jn; ++jj, col += ld) {
-                double m[R];
+        // this wavefront's columns of the chunk: [q0, q1)
+        const int q0 = wv * 64, q1 = min(q0 + 64, je - j0);
+        if (!in || q0 >= q1) continue;
+        const T *col = M + (size_t)(j0 + q0) * ld + r0;
+        int jj = q0;
+        for (; jj + U <= q1; jj += U, col += (size_t)U * ld) {
+            double m[U][R];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
                 if constexpr (sizeof(T) == 4) {
-                    float4 v = *reinterpret_cast<const float4 *>(col);
-                    m[0] = v.x; m[1] = v.y; m[2] = v.z; m[3] = v.w;
+                    float4 v = *reinterpret_cast<const float4 *>(col + (size_t)u * ld);
+                    m[u][0] = v.x; m[u][1] = v.y; m[u][2] = v.z; m[u][3] = v.w;
                 } else {
-                    double2 v = *reinterpret_cast<const double2 *>(col);
-                    m[0] = v.x; m[1] = v.y;
+                    double2 v = *reinterpret_cast<const double2 *>(col + (size_t)u * ld);
+                    m[u][0] = v.x; m[u][1] = v.y;
                 }
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u)
 #pragma unroll
                 for (int c = 0; c < D; ++c) {
-                    double xv = sx[c][jj];
+                    double xv = sx[c][jj + u];
 #pragma unroll
-                    for (int k = 0; k < R; ++k) a[k][c] += m[k] * xv;
+                    for (int k = 0; k < R; ++k) a[k][c] += m[u][k] * xv;
                 }
+        }
+        for (; jj < q1; ++jj, col += ld) {
+            double m[R];
+            if constexpr (sizeof(T) == 4) {
+                float4 v = *reinterpret_cast<const float4 *>(col);
+                m[0] = v.x; m[1] = v.y; m[2] = v.z; m[3] = v.w;
+            } else {
+                double2 v = *reinterpret_cast<const double2 *>(col);
+                m[0] = v.x; m[1] = v.y;
+            }
+#pragma unroll
+            for (int c = 0; c < D; ++c) {
+                double xv = sx[c][jj];
+#pragma unroll
+                for (int k = 0; k < R; ++k) a[k][c] += m[k] * xv;
             }
         }
     }
-    double *po = part + (size_t)blockIdx.y * D * nr;
+    // wavefront order: ((w0 + w1) + w2) + w3
+    if (wv > 0) {
 #pragma unroll
-    for (int c = 0; c < D; ++c)
+        for (int c = 0; c < D; ++c)
 #pragma unroll
-        for (int k = 0; k < R; ++k)
-            if (r0 + k < nr) po[c * nr + r0 + k] = a[k][c];
+            for (int k = 0; k < R; ++k) red[wv - 1][c][lane * R + k] = a[k][c];
+    }
+    __syncthreads();
+    if (wv == 0) {
+        double *po = part + (size_t)blockIdx.y * D * nr;
+#pragma unroll
+        for (int c = 0; c < D; ++c)
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+                double sum = a[k][c];
+#pragma unroll
+                for (int w = 0; w < W - 1; ++w) sum += red[w][c][lane * R + k];
+                if (r0 + k < nr) po[c * nr + r0 + k] = sum;
+            }
+    }
 }
 
-// y = M x for a dense symmetric C x C matrix (column-major), D right-hand
-// sides: one wavefront per column j of M (= row j), lanes stride the rows.
-// Used for K_cc u and for the preconditioner M = X^T X.
-template <int D>
-__global__ __launch_bounds__(NT) void symv_k(int C, const double *__restrict__ M, const double *__restrict__ x,
-                                             double *__restrict__ y, const Ctl *ctl) {
+// y[i] = sum over the slices s (in slice order) of part[s * n + i]: 64
+// outputs per block, wavefront w adding slices w, w + 4, ..., combined in
+// wavefront order
+__global__ __launch_bounds__(NT) void slice_rows_k(int n, int slices, const double *__restrict__ part,
+                                                   double *__restrict__ y, const Ctl *ctl) {
+    __shared__ double red[NT / 64][64];
     if (ctl && ctl->done) return;
-    const int lane = threadIdx.x & 63;
-    const int j = blockIdx.x * (NT / 64) + (threadIdx.x >> 6);
-    if (j >= C) return;   // the whole wavefront leaves together
-    double acc[D];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int i = blockIdx.x * 64 + lane;
+    double s = 0.0;
+    if (i < n)
+        for (int k = wv; k < slices; k += NT / 64) s += part[(size_t)k * n + i];
+    red[wv][lane] = s;
+    __syncthreads();
+    if (wv == 0 && i < n) {
+        double t = red[0][lane];
 #pragma unroll
-    for (int c = 0; c < D; ++c) acc[c] = 0.0;
-    const double *col = M + (size_t)j * C;
-    for (int i = lane; i < C; i += 64) {
-        double m = col[i];
-#pragma unroll
-        for (int c = 0; c < D; ++c) acc[c] += m * x[c * C + i];
-    }
-#pragma unroll
-    for (int c = 0; c < D; ++c) {
-        double s = wave_sum(acc[c]);
-        if (lane == 0) y[c * C + j] = s;
+        for (int w = 1; w < NT / 64; ++w) t += red[w][lane];
+        y[i] = t;
     }
 }
 
@@ -865,19 +836,25 @@ __global__ void zero_upper_k(int C, double *__restrict__ M) {
     if (i < j) M[idx] = 0.0;
 }
 
-// S d = K_cx B K_xc d + ldmWgt (K_cc d + eps d).
+// S d = K_cx B K_xc d + ldmWgt (K_cc d + eps d), with K_cc d given (kd) or,
+// since the control points are landmarks, read from w = K_xc d (V rows) at
+// the control-point rows; both null: K_cc d taken as 0.
 // MODE 0: out = S d, partial d.(S d).  MODE 1: out = g - S a, partial |out|^2.
-template <int MODE>
-__global__ __launch_bounds__(NT) void ctl_combine_k(int n, const double *__restrict__ y, const double *__restrict__ kd,
-                                                    const double *__restrict__ d, const double *__restrict__ g,
+template <int D, int MODE>
+__global__ __launch_bounds__(NT) void ctl_combine_k(int C, int V, const double *__restrict__ y,
+                                                    const double *__restrict__ kdv, const double *__restrict__ w,
+                                                    const int *__restrict__ idx, const double *__restrict__ d,
+                                                    const double *__restrict__ g,
                                                     double ldmWgt, double eps, double *__restrict__ out,
                                                     double *__restrict__ P, const Ctl *ctl) {
     __shared__ double red[NT / 64];
     if (ctl->done) return;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     double ss = 0.0;
-    if (i < n) {
-        double sd = y[i] + ldmWgt * (kd[i] + eps * d[i]);
+    if (i < C * D) {
+        const int c = i / C;
+        const double kd = kdv ? kdv[i] : w ? w[(size_t)c * V + idx[i - c * C]] : 0.0;
+        double sd = y[i] + ldmWgt * (kd + eps * d[i]);
         if constexpr (MODE == 0) { out[i] = sd; ss = d[i] * sd; }
         else { double r = g[i] - sd; out[i] = r; ss = r * r; }
     }
@@ -885,12 +862,10 @@ __global__ __launch_bounds__(NT) void ctl_combine_k(int n, const double *__restr
     if (threadIdx.x == 0) P[blockIdx.x] = t;
 }
 
-// a += st d; r -= st Sd (st = rz / d.Sd); partial |r|^2; z = r (copy for the
-// preconditioner solve)
+// a += st d; r -= st Sd (st = rz / d.Sd); partial |r|^2
 __global__ __launch_bounds__(NT) void ctl_update_k(int n, const double *__restrict__ P, int nP,
                                                    const double *__restrict__ hist, int it, double *__restrict__ a,
-                                                   double *__restrict__ r, double *__restrict__ z,
-                                                   const double *__restrict__ d, const double *__restrict__ Sd,
+                                                   double *__restrict__ r, const double *__restrict__ d, const double *__restrict__ Sd,
                                                    double *__restrict__ P2, Ctl *ctl) {
     __shared__ double red[NT / 64];
     if (ctl->done) return;
@@ -906,7 +881,6 @@ __global__ __launch_bounds__(NT) void ctl_update_k(int n, const double *__restri
         a[i] += st * d[i];
         double ri = r[i] - st * Sd[i];
         r[i] = ri;
-        z[i] = ri;
         ss = ri * ri;
     }
     double t = block_sum(ss, red);
