@@ -178,16 +178,23 @@ struct Engine : EngineBase {
     DBuf<double> dCen, dDir, dVol, rowVal, gC, gN, grad, scal;
     DBuf<Ctl> ctl, rdCtl;
     // control-point form
-    DBuf<double> qc, Kcc, Kc, Mpre, cA, cG, cR, cZ, cD, cS, cY, cK, cRout, cDelta, cZero, part, partS;
+    DBuf<double> qc, Kcc, Kc, Mpre, Mcol, cA, cG, cR, cZ, cD, cS, cY, cK, cRout, cDelta, cZero, part, partS;
     DBuf<double> A64, At64;      // stored K_xc / K_cx, FP64
     DBuf<float> A32, At32;       // stored K_xc / K_cx, FP32 (mixed precision)
-    int ldA = 0, ldT = 0, slicesA = 1, slicesT = 1, slicesM = 1;
+    int slicesA = 1, slicesT = 1, slicesM = 1;
     DBuf<int> potrfInfo;
     rocblas_handle blas = nullptr;
     int slicesC = 1;
     Ctl *hctl = nullptr;
     Stats *prof = nullptr;       // where the kernel timer accumulates during run()
     int lastIters = 0;           // CG iterations of the previous step (sizes the first launch chunk)
+    // control-point preconditioner X^T X from the K_cc of an earlier step:
+    // rebuilt on a run's first step, after PREC_MAX_AGE steps, or when a
+    // step needs PREC_GROWTH times the CG iterations of the first step that
+    // used it (any SPD preconditioner leaves the solution unchanged)
+    static constexpr int PREC_MAX_AGE = 8;
+    static constexpr double PREC_GROWTH = 1.2;
+    int precAge = -1, precIters = 0;
     int chunkHint = 4;           // first launch chunk of the next control-point CG
     double ttSum = 0.0;
     bool ttReady = false;
@@ -383,32 +390,35 @@ struct Engine : EngineBase {
             cRout.alloc(nc, tot); cDelta.alloc(nc, tot); cZero.alloc(nc, tot);
             CK(hipMemsetAsync(cZero.p, 0, sizeof(double) * nc, s));
             // stored kernel matrices: FP64 for the plain form, FP32 for the mixed one
-            ldA = (V + 3) / 4 * 4;
-            ldT = (NC + 3) / 4 * 4;
-            // preconditioner with the padded leading dimension of the stored
-            // products (padding rows stay zero)
-            Mpre.alloc((size_t)ldT * NC, tot);
-            CK(hipMemsetAsync(Mpre.p, 0, sizeof(double) * ldT * (size_t)NC, s));
+            // dense products are panel-major with rows padded to whole panels
+            // (zero padding rows); the preconditioner leaves rocBLAS
+            // column-major (Mcol) and is repacked
+            const size_t pc = padded_rows<double>(NC) * (size_t)NC;
+            Mcol.alloc((size_t)NC * NC, tot);
+            Mpre.alloc(pc, tot);
+            CK(hipMemsetAsync(Mpre.p, 0, sizeof(double) * pc, s));
             if (precision == SA_PREC_MIXED) {
                 // FP64 K_cc for the FP32 inner solve: its FP32 rounding need
                 // not be positive definite when K_cc is ill conditioned
-                Kc.alloc((size_t)ldT * NC, tot);
-                CK(hipMemsetAsync(Kc.p, 0, sizeof(double) * ldT * (size_t)NC, s));
+                Kc.alloc(pc, tot);
+                CK(hipMemsetAsync(Kc.p, 0, sizeof(double) * pc, s));
             }
-            size_t el = (size_t)ldA * NC + (size_t)ldT * V;
             if (precision == SA_PREC_MIXED) {
-                A32.alloc((size_t)ldA * NC, tot); At32.alloc((size_t)ldT * V, tot);
-                CK(hipMemsetAsync(A32.p, 0, sizeof(float) * ldA * (size_t)NC, s));
-                CK(hipMemsetAsync(At32.p, 0, sizeof(float) * ldT * (size_t)V, s));
-            } else if (el * sizeof(double) <= (size_t)48 << 30) {
-                A64.alloc((size_t)ldA * NC, tot); At64.alloc((size_t)ldT * V, tot);
-                CK(hipMemsetAsync(A64.p, 0, sizeof(double) * ldA * (size_t)NC, s));
-                CK(hipMemsetAsync(At64.p, 0, sizeof(double) * ldT * (size_t)V, s));
+                const size_t na = padded_rows<float>(V) * (size_t)NC, nt = padded_rows<float>(NC) * (size_t)V;
+                A32.alloc(na, tot); At32.alloc(nt, tot);
+                CK(hipMemsetAsync(A32.p, 0, sizeof(float) * na, s));
+                CK(hipMemsetAsync(At32.p, 0, sizeof(float) * nt, s));
+            } else {
+                const size_t na = padded_rows<double>(V) * (size_t)NC, nt = padded_rows<double>(NC) * (size_t)V;
+                if ((na + nt) * sizeof(double) <= (size_t)48 << 30) {
+                    A64.alloc(na, tot); At64.alloc(nt, tot);
+                    CK(hipMemsetAsync(A64.p, 0, sizeof(double) * na, s));
+                    CK(hipMemsetAsync(At64.p, 0, sizeof(double) * nt, s));
+                }
             }
             // about 1024 blocks per stored product (row blocks x column slices)
-            const int rbA = gridOf(V, rows_per_thread(precision == SA_PREC_MIXED) * 64);
-            const int rbT = gridOf(NC, rows_per_thread(precision == SA_PREC_MIXED) * 64);
-            const int rbM = gridOf(NC, rows_per_thread(false) * 64);
+            const int prS = precision == SA_PREC_MIXED ? panel_rows<float>() : panel_rows<double>();
+            const int rbA = gridOf(V, prS), rbT = gridOf(NC, prS), rbM = gridOf(NC, panel_rows<double>());
             slicesA = std::max(1, std::min(64, (1024 + rbA - 1) / rbA));
             slicesT = std::max(1, std::min(256, (1024 + rbT - 1) / rbT));
             slicesM = std::max(1, std::min(256, (1024 + rbM - 1) / rbM));
@@ -561,14 +571,17 @@ struct Engine : EngineBase {
         }
     }
 
-    // K_cc (+ eps I) and its preconditioner M = X^T X, X = L^{-1} the inverted
-    // Cholesky factor (rocSOLVER potrf + trtri, rocBLAS dgemm), once per step;
-    // Kc keeps K_cc without eps for the products K_cc u
-    void kcc_build() {
+    // K_cc (+ eps I) and, when refresh is set, its preconditioner M = X^T X,
+    // X = L^{-1} the inverted Cholesky factor (rocSOLVER potrf + trtri,
+    // rocBLAS dgemm); Kc (FP32 inner solve only) keeps K_cc without eps for
+    // the products K_cc u and is rebuilt every step
+    void kcc_build(bool refresh) {
+        if (!refresh && !Kc.p) return;
         double invW = 1.0 / knlWidth;
         long nn = (long)NC * NC;
         int g = gridOf(nn);
-#define SA_KB(O) kcc_build_k<D, O><<<g, NT, 0, s>>>(NC, qc.p, invW, knlEps, Kcc.p, Kc.p, ldT)
+        double *K = refresh ? Kcc.p : nullptr;
+#define SA_KB(O) kcc_build_k<D, O><<<g, NT, 0, s>>>(NC, qc.p, invW, knlEps, K, Kc.p)
         switch (knlOrder) {
             case -1: SA_KB(-1); break;
             case 0: SA_KB(0); break;
@@ -579,6 +592,7 @@ struct Engine : EngineBase {
         }
 #undef SA_KB
         CHECK_LAUNCH();
+        if (!refresh) return;
         if (rocsolver_dpotrf(blas, rocblas_fill_lower, NC, Kcc.p, NC, potrfInfo.p) != rocblas_status_success)
             throw SaError(SA_E_DEVICE, "rocsolver_dpotrf failed");
         if (rocsolver_dtrtri(blas, rocblas_fill_lower, rocblas_diagonal_non_unit, NC, Kcc.p, NC, potrfInfo.p + 1) !=
@@ -588,16 +602,19 @@ struct Engine : EngineBase {
         CHECK_LAUNCH();
         const double one = 1.0, zero = 0.0;
         if (rocblas_dgemm(blas, rocblas_operation_transpose, rocblas_operation_none, NC, NC, NC, &one, Kcc.p, NC,
-                          Kcc.p, NC, &zero, Mpre.p, ldT) != rocblas_status_success)
+                          Kcc.p, NC, &zero, Mcol.p, NC) != rocblas_status_success)
             throw SaError(SA_E_DEVICE, "rocblas_dgemm failed");
+        panelize_k<<<g, NT, 0, s>>>(NC, Mcol.p, NC, Mpre.p);
+        CHECK_LAUNCH();
     }
 
-    static int rows_per_thread(bool fp32) { return fp32 ? 4 : 2; }
+    template <typename T>
+    static size_t padded_rows(int n) { return (size_t)gridOf(n, panel_rows<T>()) * panel_rows<T>(); }
 
-    // y = M x for a dense FP64 C x C matrix stored with leading dimension ldT
+    // y = M x for a dense FP64 C x C panel-major matrix
     void cc_product(const double *M, const double *x, double *y) {
-        dim3 g(gridOf(NC, rows_per_thread(false) * 64), slicesM);
-        kstore_mv_k<D, double><<<g, NT, 0, s>>>(NC, NC, M, ldT, x, partS.p, ctl.p);
+        dim3 g(gridOf(NC, panel_rows<double>()), slicesM);
+        kstore_mv_k<D, double><<<g, NT, 0, s>>>(NC, NC, M, x, partS.p, ctl.p);
         CHECK_LAUNCH();
         slice_rows_k<<<gridOf(D * NC, 64), NT, 0, s>>>(D * NC, slicesM, partS.p, y, ctl.p);
         CHECK_LAUNCH();
@@ -614,7 +631,7 @@ struct Engine : EngineBase {
         double invW = 1.0 / knlWidth;
         dim3 g(gridOf(V, 64), gridOf(NC, 64));
         if (A64.p) {
-#define SA_KS(O) kstore_build_k<D, O, double><<<g, NT, 0, s>>>(V, q, NC, qc.p, invW, A64.p, ldA, At64.p, ldT)
+#define SA_KS(O) kstore_build_k<D, O, double><<<g, NT, 0, s>>>(V, q, NC, qc.p, invW, A64.p, At64.p)
             switch (knlOrder) {
                 case -1: SA_KS(-1); break;
                 case 0: SA_KS(0); break;
@@ -627,7 +644,7 @@ struct Engine : EngineBase {
             CHECK_LAUNCH();
         }
         if (A32.p) {
-#define SA_KS(O) kstore_build_k<D, O, float><<<g, NT, 0, s>>>(V, q, NC, qc.p, invW, A32.p, ldA, At32.p, ldT)
+#define SA_KS(O) kstore_build_k<D, O, float><<<g, NT, 0, s>>>(V, q, NC, qc.p, invW, A32.p, At32.p)
             switch (knlOrder) {
                 case -1: SA_KS(-1); break;
                 case 0: SA_KS(0); break;
@@ -653,14 +670,16 @@ struct Engine : EngineBase {
             if (ev) CK(hipEventRecord(ev[1], s));
             return;
         }
-        const int nr = xc ? V : NC, ncol = xc ? NC : V, ld = xc ? ldA : ldT;
+        const int nr = xc ? V : NC, ncol = xc ? NC : V;
         const int sl = xc ? slicesA : slicesT;
-        dim3 g(gridOf(nr, rows_per_thread(md == KMode::Stored32) * 64), sl);
         if (ev) CK(hipEventRecord(ev[0], s));
-        if (md == KMode::Stored64)
-            kstore_mv_k<D, double><<<g, NT, 0, s>>>(nr, ncol, xc ? A64.p : At64.p, ld, u, partS.p, c);
-        else
-            kstore_mv_k<D, float><<<g, NT, 0, s>>>(nr, ncol, xc ? A32.p : At32.p, ld, u, partS.p, c);
+        if (md == KMode::Stored64) {
+            dim3 g(gridOf(nr, panel_rows<double>()), sl);
+            kstore_mv_k<D, double><<<g, NT, 0, s>>>(nr, ncol, xc ? A64.p : At64.p, u, partS.p, c);
+        } else {
+            dim3 g(gridOf(nr, panel_rows<float>()), sl);
+            kstore_mv_k<D, float><<<g, NT, 0, s>>>(nr, ncol, xc ? A32.p : At32.p, u, partS.p, c);
+        }
         CHECK_LAUNCH();
         if (ev) CK(hipEventRecord(ev[1], s));
         if (P) slice_sum_k<<<gridOf((long)D * nr), NT, 0, s>>>(D * nr, sl, partS.p, y, P, c);   // + |y|^2 partials
@@ -754,7 +773,9 @@ struct Engine : EngineBase {
         CHECK_LAUNCH();
         gather_ctl_k<D><<<gridOf(NC), NT, 0, s>>>(NC, V, ctlIdx.p, q, qc.p);
         CHECK_LAUNCH();
-        kcc_build();
+        const bool refresh = precAge < 0 || precAge >= PREC_MAX_AGE || lastIters > PREC_GROWTH * precIters + 4;
+        kcc_build(refresh);
+        if (refresh) precAge = 0;
         if (A64.p || A32.p) kstore_build(q);
         CK(hipMemsetAsync(ctl.p, 0, sizeof(Ctl), s));
         // g = K_cx f in FP64 (stored FP64 matrix when present), |g|^2 partials
@@ -900,6 +921,7 @@ struct Engine : EngineBase {
         CK(hipMemsetAsync(a.p, 0, sizeof(double) * n, s));
         if (NC > 0) CK(hipMemsetAsync(cA.p, 0, sizeof(double) * NC * D, s));
         lastIters = 0;
+        precAge = -1;
         geometry_k<D><<<gridOf(E), NT, 0, s>>>(E, V, elmVtx.p, frmVtx.p, tsvVtx.p, q0.p, spdTan, spdTsv, geo[0].p,
                                                femElm.p);
         CHECK_LAUNCH();
@@ -982,6 +1004,10 @@ struct Engine : EngineBase {
             st.refine += sweeps;
             st.pcgIters += its;
             lastIters = its;
+            if (NC > 0) {
+                if (precAge == 0) precIters = its;
+                ++precAge;
+            }
             if (vs != 0) { failed = 1; break; }
             axpby_k<<<gridOf((long)n), NT, 0, s>>>((int)n, qn, 1.0, q, dt, x.p);
             CHECK_LAUNCH();
@@ -1141,12 +1167,12 @@ long long sa_session_footprint(const sa_mesh *m, const sa_target *t, const sa_pa
     }
     if (m->ctlNum > 0) {
         long long C = m->ctlNum;
-        const long long ldT = (C + 3) / 4 * 4, ldA = (V + 3) / 4 * 4;
-        // K_cc, preconditioner, vectors, slice partials, cross partials
-        b += 8 * (C * C + ldT * C + 12 * C * D + 256 * C * D + 64 * D * V + 32 * C * D) + 4 * 2 * sa::MAXP;
-        long long el = ldA * C + ldT * V;
-        if (p->precision == SA_PREC_MIXED) b += 4 * el + 8 * ldT * C;
-        else if (el * 8 <= (48LL << 30)) b += 8 * el;
+        // rows padded to whole panels of 256 (FP32) / 128 (FP64) rows
+        auto pad = [](long long n, long long pr) { return (n + pr - 1) / pr * pr; };
+        // K_cc, rocBLAS output, preconditioner, vectors, slice partials, cross partials
+        b += 8 * (2 * C * C + pad(C, 128) * C + 12 * C * D + 256 * C * D + 64 * D * V + 32 * C * D);
+        if (p->precision == SA_PREC_MIXED) b += 4 * (pad(V, 256) * C + pad(C, 256) * V) + 8 * pad(C, 128) * C;
+        else if ((pad(V, 128) * C + pad(C, 128) * V) * 8 <= (48LL << 30)) b += 8 * (pad(V, 128) * C + pad(C, 128) * V);
     }
     return b;
 }

@@ -9,6 +9,7 @@
 // computes the same value in the same order) and a done flag makes all later
 // kernels of a launched chunk return at once.
 #pragma once
+#include <type_traits>
 #include "sa_device.hpp"
 
 namespace sa {
@@ -639,6 +640,18 @@ __global__ __launch_bounds__(NT) void slice_sum_k(int n, int slices, const doubl
     }
 }
 
+// Panel-major storage of the dense products: rows are grouped in panels of
+// panel_rows<T>() = 1024 / sizeof(T) rows (64 lanes x one 16-byte load), and
+// a panel's columns follow one another, each holding the panel's rows, so a
+// wavefront streaming consecutive columns reads one contiguous range.  The
+// row count is padded to whole panels (padding rows hold zeros).
+template <typename T>
+__host__ __device__ constexpr int panel_rows() { return 1024 / (int)sizeof(T); }
+
+__host__ __device__ inline size_t panel_at(int i, int j, int PR, int ncols) {
+    return (size_t)(i / PR) * PR * ncols + (size_t)j * PR + (i % PR);
+}
+
 template <int D>
 __global__ void gather_ctl_k(int C, int V, const int *__restrict__ idx, const double *__restrict__ q,
                              double *__restrict__ qc) {
@@ -652,7 +665,7 @@ __global__ void gather_ctl_k(int C, int V, const int *__restrict__ idx, const do
 // dense K_cc + eps I, column-major, for its Cholesky factorisation
 template <int D, int ORD>
 __global__ void kcc_build_k(int C, const double *__restrict__ qc, double invW, double eps, double *__restrict__ K,
-                            double *__restrict__ K2, int ld2) {
+                            double *__restrict__ K2) {
     __shared__ double etab[64];
     init_exp_table(etab);
     __syncthreads();
@@ -663,20 +676,21 @@ __global__ void kcc_build_k(int C, const double *__restrict__ qc, double invW, d
 #pragma unroll
     for (int c = 0; c < D; ++c) { double dd = qc[c * C + i] - qc[c * C + j]; d2 += dd * dd; }
     double kv = rkhs_kernel<D, ORD>(d2, invW, etab);
-    K[idx] = kv + (i == j ? eps : 0.0);
-    if (K2) K2[(size_t)j * ld2 + i] = kv;   // K_cc without eps (leading dimension ld2)
+    if (K) K[idx] = kv + (i == j ? eps : 0.0);
+    if (K2) K2[panel_at(i, j, panel_rows<double>(), C)] = kv;   // K_cc without eps, panel-major
 }
 
 // Stored landmark x control-point kernel matrix, rebuilt once per time step:
-// A (V x C, column-major, leading dimension lda) and its transpose At (C x V,
-// leading dimension ldt) hold the same rounded values, so the two products
-// of a CG iteration use exactly K and K^T.  T = double (plain FP64 form) or
+// A (V x C) and its transpose At (C x V), both panel-major (panel_at), hold
+// the same rounded values, so the two products of a CG iteration use exactly
+// K and K^T.  T = double (plain FP64 form) or
 // float (mixed form).  A 64 x 64 tile per block; At is written through LDS so
 // both stores are coalesced.
 template <int D, int ORD, typename T>
 __global__ __launch_bounds__(NT) void kstore_build_k(int V, const double *__restrict__ q, int C,
                                                      const double *__restrict__ qc, double invW, T *__restrict__ A,
-                                                     int lda, T *__restrict__ At, int ldt) {
+                                                     T *__restrict__ At) {
+    constexpr int PR = panel_rows<T>();
     __shared__ double etab[64];
     __shared__ T tile[64][65];
     init_exp_table(etab);
@@ -695,7 +709,7 @@ __global__ __launch_bounds__(NT) void kstore_build_k(int V, const double *__rest
 #pragma unroll
             for (int c = 0; c < D; ++c) { double dd = qi[c] - qc[c * C + j]; d2 += dd * dd; }
             v = (T)rkhs_kernel<D, ORD>(d2, invW, etab);
-            A[(size_t)j * lda + i] = v;
+            A[panel_at(i, j, PR, C)] = v;
         }
         tile[lj][li] = v;
     }
@@ -704,83 +718,70 @@ __global__ __launch_bounds__(NT) void kstore_build_k(int V, const double *__rest
     const int lj = threadIdx.x & 63, li0 = threadIdx.x >> 6;
     for (int l = li0; l < 64; l += 4) {
         int ii = i0 + l, jj = j0 + lj;
-        if (ii < V && jj < C) At[(size_t)ii * ldt + jj] = tile[lj][l];
+        if (ii < V && jj < C) At[panel_at(jj, ii, PR, V)] = tile[lj][l];
     }
 }
 
-// y = M x for a stored column-major M (nr x nc, leading dimension ld), x and
-// y with D columns (stride nc and nr).  A block owns 64 R consecutive rows
-// (R per lane, one 16-byte load per column) and one slice of the columns
-// (blockIdx.y); its four wavefronts take consecutive quarters of every
-// 256-column chunk and are summed in wavefront order through LDS.  The
-// block's partial goes to part + s * D * nr; slice_rows_k sums the slices in
-// slice order.  x is staged through LDS and read as a broadcast.
+// y = M x for a stored panel-major M (nr x nc, see panel_at), x and y with D
+// columns (stride nc and nr).  A block owns one panel of 64 R rows (R per
+// lane, one 16-byte load per column) and one slice of the columns
+// (blockIdx.y); its wavefronts each stream a contiguous quarter of the slice
+// (consecutive columns of a panel are adjacent in memory) with no barrier,
+// reading x at wavefront-uniform addresses, and are summed in wavefront order
+// through LDS at the end.  The block's partial goes to part + s * D * nr;
+// slice_rows_k sums the slices in slice order.
 template <int D, typename T>
-__global__ __launch_bounds__(NT) void kstore_mv_k(int nr, int nc, const T *__restrict__ M, int ld,
+__global__ __launch_bounds__(NT) void kstore_mv_k(int nr, int nc, const T *__restrict__ M,
                                                   const double *__restrict__ x, double *__restrict__ part,
                                                   const Ctl *ctl) {
-    constexpr int R = sizeof(T) == 4 ? 4 : 2;
+    constexpr int PR = panel_rows<T>();
+    constexpr int R = PR / 64;
     constexpr int W = NT / 64;
-    constexpr int U = 4;   // columns whose loads are issued together
-    __shared__ double sx[D][NT];
-    __shared__ double red[W - 1][D][64 * R];
+    constexpr int U = 8;   // columns whose loads are issued together
+    using V4 = typename std::conditional<sizeof(T) == 4, float4, double2>::type;
+    __shared__ double red[W - 1][D][PR];
     if (ctl && ctl->done) return;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int r0 = R * (blockIdx.x * 64 + lane);
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int r0 = blockIdx.x * PR + R * lane;
     const int per = (nc + gridDim.y - 1) / gridDim.y;
     const int jb = blockIdx.y * per, je = min(nc, jb + per);
+    const int pw = (je - jb + W - 1) / W;
+    const int wb = min(je, jb + wv * pw), we = min(je, wb + pw);
     double a[R][D];
 #pragma unroll
     for (int k = 0; k < R; ++k)
 #pragma unroll
         for (int c = 0; c < D; ++c) a[k][c] = 0.0;
-    const bool in = r0 + R - 1 < ld;
-    for (int j0 = jb; j0 < je; j0 += NT) {
-        int j = j0 + threadIdx.x;
-        __syncthreads();
+    const T *col = M + (size_t)blockIdx.x * PR * nc + (size_t)wb * PR + R * lane;
+    int j = wb;
+    for (; j + U <= we; j += U, col += U * PR) {
+        V4 m[U];
 #pragma unroll
-        for (int c = 0; c < D; ++c) sx[c][threadIdx.x] = j < je ? x[c * nc + j] : 0.0;
-        __syncthreads();
-        // this wavefront's columns of the chunk: [q0, q1)
-        const int q0 = wv * 64, q1 = min(q0 + 64, je - j0);
-        if (!in || q0 >= q1) continue;
-        const T *col = M + (size_t)(j0 + q0) * ld + r0;
-        int jj = q0;
-        for (; jj + U <= q1; jj += U, col += (size_t)U * ld) {
-            double m[U][R];
+        for (int u = 0; u < U; ++u) m[u] = *reinterpret_cast<const V4 *>(col + u * PR);
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                if constexpr (sizeof(T) == 4) {
-                    float4 v = *reinterpret_cast<const float4 *>(col + (size_t)u * ld);
-                    m[u][0] = v.x; m[u][1] = v.y; m[u][2] = v.z; m[u][3] = v.w;
-                } else {
-                    double2 v = *reinterpret_cast<const double2 *>(col + (size_t)u * ld);
-                    m[u][0] = v.x; m[u][1] = v.y;
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < U; ++u)
-#pragma unroll
-                for (int c = 0; c < D; ++c) {
-                    double xv = sx[c][jj + u];
-#pragma unroll
-                    for (int k = 0; k < R; ++k) a[k][c] += m[u][k] * xv;
-                }
-        }
-        for (; jj < q1; ++jj, col += ld) {
-            double m[R];
-            if constexpr (sizeof(T) == 4) {
-                float4 v = *reinterpret_cast<const float4 *>(col);
-                m[0] = v.x; m[1] = v.y; m[2] = v.z; m[3] = v.w;
-            } else {
-                double2 v = *reinterpret_cast<const double2 *>(col);
-                m[0] = v.x; m[1] = v.y;
-            }
+        for (int u = 0; u < U; ++u)
 #pragma unroll
             for (int c = 0; c < D; ++c) {
-                double xv = sx[c][jj];
+                const double xv = x[c * nc + j + u];
+                if constexpr (sizeof(T) == 4) {
+                    a[0][c] += (double)m[u].x * xv; a[1][c] += (double)m[u].y * xv;
+                    a[2][c] += (double)m[u].z * xv; a[3][c] += (double)m[u].w * xv;
+                } else {
+                    a[0][c] += m[u].x * xv; a[1][c] += m[u].y * xv;
+                }
+            }
+    }
+    for (; j < we; ++j, col += PR) {
+        V4 m = *reinterpret_cast<const V4 *>(col);
 #pragma unroll
-                for (int k = 0; k < R; ++k) a[k][c] += m[k] * xv;
+        for (int c = 0; c < D; ++c) {
+            const double xv = x[c * nc + j];
+            if constexpr (sizeof(T) == 4) {
+                a[0][c] += (double)m.x * xv; a[1][c] += (double)m.y * xv;
+                a[2][c] += (double)m.z * xv; a[3][c] += (double)m.w * xv;
+            } else {
+                a[0][c] += m.x * xv; a[1][c] += m.y * xv;
             }
         }
     }
@@ -804,6 +805,14 @@ __global__ __launch_bounds__(NT) void kstore_mv_k(int nr, int nc, const T *__res
                 if (r0 + k < nr) po[c * nr + r0 + k] = sum;
             }
     }
+}
+
+// column-major (leading dimension ld) n x n matrix -> panel-major copy
+__global__ void panelize_k(int n, const double *__restrict__ src, int ld, double *__restrict__ dst) {
+    long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)n * n) return;
+    int j = (int)(idx / n), i = (int)(idx - (long)j * n);
+    dst[panel_at(i, j, panel_rows<double>(), n)] = src[(size_t)j * ld + i];
 }
 
 // y[i] = sum over the slices s (in slice order) of part[s * n + i]: 64
