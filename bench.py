@@ -60,6 +60,123 @@ def percentile(xs, q):
     return float(np.percentile(np.asarray(xs), q)) if xs else None
 
 
+def run_requests(api, meshes, reqs, precision, workers, mem_budget=0, rate=None, seed=5):
+    """Submit requests (mesh index, tauVar, params) to a server holding the
+    meshes; all at once (rate None) or as a Poisson stream of the given rate
+    (requests/s).  Returns the results and the wall time from the first
+    submission to the last completion."""
+    srv = api.Server(workers=workers, mem_budget=mem_budget)
+    ids = [srv.add_mesh(p, precision) for p in meshes]
+    rng = np.random.default_rng(seed)
+    t0 = time.perf_counter()
+    t_next = t0
+    rids = []
+    for m, tv, prm in reqs:
+        if rate:
+            t_next += rng.exponential(1.0 / rate)
+            dt = t_next - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+        rids.append(srv.submit(ids[m], tv, meshes[m]["tauPrm"], prm))
+    out = []
+    for k, r in enumerate(rids):
+        g = srv.wait(r)
+        out.append(g)
+        m = reqs[k][0]
+        print(f"[bench]   {precision} req {k}: V={meshes[m]['q0'].shape[1]} d={meshes[m]['q0'].shape[0]} "
+              f"steps={g['steps']} iters={g['pcg_iters']} sweeps={g['refine']} fp64_steps={g['fp64_steps']} "
+              f"run={g['seconds']:.2f}s done_at={time.perf_counter() - t0:.1f}s rc={g['rc']} status={g['status']}",
+              file=sys.stderr, flush=True)
+    wall = time.perf_counter() - t0
+    srv.close()
+    return out, wall
+
+
+def mode_batch(args, api, W):
+    """64 independent meshes of uneven size (half 2D rings, half 3D shells),
+    each with its own parameters, 20 steps, run concurrently through the
+    server in the mixed form, against the same solves run one at a time in
+    the plain FP64 form."""
+    steps = args.steps if args.steps is not None else 20
+    meshes = W.batch_problems(n=args.batch_n, time_steps=steps)
+    reqs = [(k, p["tauVar"][:, 0], p["params"]) for k, p in enumerate(meshes)]
+    sizes = [p["q0"].shape[1] for p in meshes]
+    print(f"[bench] batch: {len(meshes)} meshes, V {min(sizes)}..{max(sizes)}, {steps} steps", file=sys.stderr,
+          flush=True)
+    got, wall = run_requests(api, meshes, reqs, args.precision, args.workers)
+    bad = [g for g in got if g["rc"] != 0 or g["status"] != 0]
+    if bad:
+        raise SystemExit(f"batch: {len(bad)} failed requests: {bad[0]}")
+    tot_steps = sum(g["steps"] for g in got)
+    print(f"[bench] batched {args.precision}: {tot_steps} steps in {wall:.2f}s", file=sys.stderr, flush=True)
+    ref, wall64 = run_requests(api, meshes, reqs, "fp64", 1)
+    print(f"[bench] serial fp64: {sum(g['steps'] for g in ref)} steps in {wall64:.2f}s", file=sys.stderr, flush=True)
+    err = max(abs(g["obj"] - r["obj"]) / abs(r["obj"]) for g, r in zip(got, ref))
+    lat = [g["t_end"] - g["t_submit"] for g in got]
+    return {
+        "metric": "batched evolution steps/s (64 meshes, mixed form) vs serial plain FP64",
+        "value": round(tot_steps / wall, 3), "unit": "steps/s", "n_gpus": 1, "steps": tot_steps,
+        "warmup": 0, "ms_per_step": round(1e3 * wall / tot_steps, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64+f32" if args.precision == "mixed" else "f64", "data": "synthetic",
+        "config": {"workload": f"{len(meshes)} layered meshes, half 2D rings V 1k-20k, half 3D shells V 5k-60k, "
+                               f"log-uniform sizes, per-mesh parameters +-30%, control points 10% (max 2000)",
+                   "time_steps": steps, "workers": args.workers, "precision": args.precision,
+                   "landmarks_total": int(sum(sizes))},
+        "serial_fp64": {"value": round(sum(g["steps"] for g in ref) / wall64, 3), "unit": "steps/s"},
+        "speedup_vs_serial_fp64": round(wall64 / wall, 3),
+        "max_obj_rel_err_vs_fp64": err,
+        "latency_s": {"p50": percentile(lat, 50), "p99": percentile(lat, 99)},
+    }
+
+
+def mode_serve(args, api, W):
+    """Poisson stream of forward-evolution + objective requests over a pool of
+    meshes (2D/3D, V log-uniform 1k-200k), each request with its own
+    parameter vector; rate 0.7 x the saturated throughput measured first.
+    p50/p99 latency and requests/s, against serial plain FP64 solves."""
+    steps = args.steps if args.steps is not None else 10
+    pool = W.batch_problems(n=args.pool, seed=5, time_steps=steps, v2d=(1000, 200000), v3d=(1000, 200000),
+                            mixed_dims=True)
+    rng = np.random.default_rng(55)
+    reqs = []
+    for k in range(args.requests):
+        m = int(rng.integers(len(pool)))
+        prm = W.perturb_params(pool[m]["params"], rng)
+        tv = pool[m]["tauVar"][:, 0] * (1.0 + 0.05 * rng.standard_normal(pool[m]["tauVar"].shape[0]))
+        reqs.append((m, tv, prm))
+    sizes = [p["q0"].shape[1] for p in pool]
+    print(f"[bench] serve: pool of {len(pool)} meshes V {min(sizes)}..{max(sizes)}, {len(reqs)} requests",
+          file=sys.stderr, flush=True)
+    sat, wsat = run_requests(api, pool, reqs, args.precision, args.workers, mem_budget=args.mem_budget)
+    x_sat = len(reqs) / wsat
+    rate = 0.7 * x_sat
+    print(f"[bench] saturated: {x_sat:.3f} req/s; Poisson rate {rate:.3f} req/s", file=sys.stderr, flush=True)
+    got, wall = run_requests(api, pool, reqs, args.precision, args.workers, mem_budget=args.mem_budget, rate=rate)
+    ref, w64 = run_requests(api, pool, reqs, "fp64", 1)
+    for g in got + ref:
+        if g["rc"] != 0 or g["status"] != 0:
+            raise SystemExit(f"serve: failed request {g}")
+    lat = [g["t_end"] - g["t_submit"] for g in got]
+    qlat = [g["t_start"] - g["t_submit"] for g in got]
+    err = max(abs(g["obj"] - r["obj"]) / abs(r["obj"]) for g, r in zip(got, ref))
+    same = max(abs(g["obj"] - s["obj"]) / abs(s["obj"]) for g, s in zip(got, sat))
+    return {
+        "metric": "served requests/s and p50/p99 latency (Poisson arrivals, mixed form) vs serial plain FP64",
+        "value": round(len(reqs) / wall, 4), "unit": "requests/s", "n_gpus": 1, "steps": int(sum(g["steps"] for g in got)),
+        "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64+f32" if args.precision == "mixed" else "f64", "data": "synthetic",
+        "config": {"workload": f"pool of {len(pool)} layered meshes (2D rings / 3D shells) V 1k-200k log-uniform, "
+                               f"{len(reqs)} requests with per-request parameters, {steps} steps each",
+                   "workers": args.workers, "precision": args.precision, "arrival_rate": round(rate, 4)},
+        "saturated": {"value": round(x_sat, 4), "unit": "requests/s"},
+        "serial_fp64": {"value": round(len(reqs) / w64, 4), "unit": "requests/s"},
+        "latency_s": {"p50": percentile(lat, 50), "p99": percentile(lat, 99)},
+        "queue_s": {"p50": percentile(qlat, 50), "p99": percentile(qlat, 99)},
+        "max_obj_rel_err_vs_fp64": err,
+        "max_rel_diff_stream_vs_burst": same,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -67,9 +184,18 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--repeats", type=int, default=1)
     ap.add_argument("--config", default="c2")
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-profile", action="store_true",
+                    help="time without the per-kernel events (graph-captured CG iterations); no roofline")
     ap.add_argument("--cpu-iters", type=int, default=1)
+    ap.add_argument("--mode", default="single", choices=["single", "batch", "serve"],
+                    help="single: one solve of --config; batch: 64 uneven meshes concurrently; serve: Poisson stream")
+    ap.add_argument("--workers", type=int, default=8)
+    ap.add_argument("--batch-n", type=int, default=64)
+    ap.add_argument("--pool", type=int, default=12)
+    ap.add_argument("--requests", type=int, default=48)
+    ap.add_argument("--mem-budget", type=int, default=0)
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MIB",
                     help="only run 8 device-to-device copies of MIB MiB (scale for the FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -89,6 +215,11 @@ def main():
         sys.exit(2)
 
     from shape_atrophy_amd import api
+    if args.mode != "single":
+        from shape_atrophy_amd import workloads as W
+        res = (mode_batch if args.mode == "batch" else mode_serve)(args, api, W)
+        print(json.dumps(res))
+        return
     steps = args.steps if args.steps is not None else 5
     prob, desc = workload(args.config, steps)
     d, V = prob["q0"].shape
@@ -105,7 +236,7 @@ def main():
         print(f"[bench] warmup: {w['steps']} steps in {w['seconds']:.3f}s status={w['status']}", file=sys.stderr,
               flush=True)
     sess.set_time_num(steps + 1)
-    sess.set_profiling(True)
+    sess.set_profiling(not args.no_profile)
     lat, runs = [], []
     t0 = time.perf_counter()
     for r in range(args.repeats):

@@ -127,6 +127,7 @@ typedef struct sa_stats {
     int kmatvecCount;            /*   ... and their number (CG iterations that ran one) */
     double operatorMs;           /* summed time of the elastic operator applies (element + gather kernels) */
     int operatorCount;
+    int fp64Steps;               /* mixed precision: steps whose refinement stalled and were finished in FP64 */
 } sa_stats;
 
 /* deformation_{2D,3D}_frame (deformation.cu:14-199).

@@ -118,7 +118,7 @@ std::vector<int> incidence(const std::vector<int> &items, int nItems, int k, int
 }
 
 struct Stats {
-    int status = 0, steps = 0, pcgIters = 0, rdIters = 0, refine = 0;
+    int status = 0, steps = 0, pcgIters = 0, rdIters = 0, refine = 0, fp64Steps = 0;
     double kmvMs = 0.0, opMs = 0.0;   // summed durations of the timed kernels (profiling on)
     int kmvCount = 0, opCount = 0;
 };
@@ -178,7 +178,7 @@ struct Engine : EngineBase {
     DBuf<double> dCen, dDir, dVol, rowVal, gC, gN, grad, scal;
     DBuf<Ctl> ctl, rdCtl;
     // control-point form
-    DBuf<double> qc, Kcc, Kc, Mpre, Mcol, cA, cG, cR, cZ, cD, cS, cY, cK, cRout, cDelta, cZero, part, partS;
+    DBuf<double> qCur, tauCur, qc, Kcc, Kc, Mpre, Mcol, cA, cG, cR, cZ, cD, cS, cY, cK, cRout, cDelta, cZero, part, partS;
     DBuf<double> A64, At64;      // stored K_xc / K_cx, FP64
     DBuf<float> A32, At32;       // stored K_xc / K_cx, FP32 (mixed precision)
     int slicesA = 1, slicesT = 1, slicesM = 1;
@@ -195,6 +195,29 @@ struct Engine : EngineBase {
     static constexpr int PREC_MAX_AGE = 8;
     static constexpr double PREC_GROWTH = 1.2;
     int precAge = -1, precIters = 0;
+    int fp64Steps = 0;           // mixed-form steps finished by the FP64 form (this run)
+    bool mixedOff = false;       // a step of this run stalled: the rest run in FP64
+    bool no64Store = false;      // the FP64 stored matrices of the fallback did not fit
+
+    // mixed form: FP64 stored matrices for the steps that fall back to FP64,
+    // allocated on the first such step (matrix-free FP64 products if they do
+    // not fit in device memory)
+    void ensure_fp64_store() {
+        if (A64.p || no64Store) return;
+        const size_t na = padded_rows<double>(V) * (size_t)NC, nt = padded_rows<double>(NC) * (size_t)V;
+        long long tot = 0;
+        try {
+            A64.alloc(na, tot);
+            At64.alloc(nt, tot);
+            CK(hipMemsetAsync(A64.p, 0, sizeof(double) * na, s));
+            CK(hipMemsetAsync(At64.p, 0, sizeof(double) * nt, s));
+        } catch (const SaError &e) {
+            if (e.code != SA_E_ALLOC) throw;
+            A64.alloc(0, tot);
+            At64.alloc(0, tot);
+            no64Store = true;
+        }
+    }
     int chunkHint = 4;           // first launch chunk of the next control-point CG
     double ttSum = 0.0;
     bool ttReady = false;
@@ -225,6 +248,7 @@ struct Engine : EngineBase {
     }
 
     ~Engine() override {
+        drop_graphs();
         if (blas) (void)rocblas_destroy_handle(blas);
         if (hctl) (void)hipHostFree(hctl);
         if (s) (void)hipStreamDestroy(s);
@@ -385,6 +409,7 @@ struct Engine : EngineBase {
             size_t nc = (size_t)NC * D;
             ctlIdx.alloc(NC, tot); ctlIdx.upload(hCtl.data(), NC, s);
             qc.alloc(nc, tot); Kcc.alloc((size_t)NC * NC, tot);
+            qCur.alloc((size_t)V * D, tot); tauCur.alloc(V, tot);
             cA.alloc(nc, tot); cG.alloc(nc, tot); cR.alloc(nc, tot); cZ.alloc(nc, tot); cD.alloc(nc, tot);
             cS.alloc(nc, tot); cY.alloc(nc, tot); cK.alloc(nc, tot);
             cRout.alloc(nc, tot); cDelta.alloc(nc, tot); cZero.alloc(nc, tot);
@@ -429,6 +454,9 @@ struct Engine : EngineBase {
             potrfInfo.alloc(2, tot);
             if (rocblas_create_handle(&blas) != rocblas_status_success) throw SaError(SA_E_DEVICE, "rocblas_create_handle failed");
             if (rocblas_set_stream(blas, s) != rocblas_status_success) throw SaError(SA_E_DEVICE, "rocblas_set_stream failed");
+            // deterministic factorisation and products: no atomics in rocBLAS/rocSOLVER kernels
+            if (rocblas_set_atomics_mode(blas, rocblas_atomics_not_allowed) != rocblas_status_success)
+                throw SaError(SA_E_DEVICE, "rocblas_set_atomics_mode failed");
             require(gridOf((long)nc) <= MAXP, "too many control points for the partial-sum buffers");
         }
         require(gridOf(n) <= MAXP && gridOf(V, MV_ROWS) <= MAXP, "mesh too large for the partial-sum buffers");
@@ -489,7 +517,7 @@ struct Engine : EngineBase {
         CHECK_LAUNCH();
         gather_k<D><<<gV, NT, 0, s>>>(V, E, vtxElm.p, frc.p, f.p, P0.p);
         CHECK_LAUNCH();
-        pcg_start_k<<<1, NT, 0, s>>>(P0.p, gV, cgTolSqu, ctl.p);
+        pcg_start_k<<<1, NT, 0, s>>>(P0.p, gV, cgTolSqu, itrMax, ctl.p);
         CHECK_LAUNCH();
         zero_if_empty_k<<<gN, NT, 0, s>>>(n, x.p, a.p, ctl.p);
         CHECK_LAUNCH();
@@ -627,10 +655,10 @@ struct Engine : EngineBase {
     // stored kernel matrices of the control-point form (see kstore_build_k)
     enum class KMode { MatrixFree64, Stored64, Stored32 };
 
-    void kstore_build(const double *q) {
+    void kstore_build(const double *q, bool fp64, bool fp32) {
         double invW = 1.0 / knlWidth;
         dim3 g(gridOf(V, 64), gridOf(NC, 64));
-        if (A64.p) {
+        if (fp64 && A64.p) {
 #define SA_KS(O) kstore_build_k<D, O, double><<<g, NT, 0, s>>>(V, q, NC, qc.p, invW, A64.p, At64.p)
             switch (knlOrder) {
                 case -1: SA_KS(-1); break;
@@ -643,7 +671,7 @@ struct Engine : EngineBase {
 #undef SA_KS
             CHECK_LAUNCH();
         }
-        if (A32.p) {
+        if (fp32 && A32.p) {
 #define SA_KS(O) kstore_build_k<D, O, float><<<g, NT, 0, s>>>(V, q, NC, qc.p, invW, A32.p, At32.p)
             switch (knlOrder) {
                 case -1: SA_KS(-1); break;
@@ -701,48 +729,114 @@ struct Engine : EngineBase {
         if (md == KMode::Stored32) cc_product(Kc.p, u, cK.p);
     }
 
+    // one control-point CG iteration on S x = rhs (x = xs): S d, step,
+    // preconditioned residual, new direction, counter; every kernel leaves at
+    // once when the solve is done, so iterations queued past the end are
+    // no-ops.  ev: optional per-iteration events (6).
+    void cg_iter(KMode md, const double *q, const double *tauNow, const double *geoNow, double *xs, hipEvent_t *ev) {
+        const int nc = NC * D, gC = gridOf(nc);
+        const double *kdv = md == KMode::Stored32 ? cK.p : nullptr;
+        S_apply(md, q, tauNow, geoNow, cD.p, ev);
+        ctl_combine_k<D, 0><<<gC, NT, 0, s>>>(NC, V, cY.p, kdv, Kr.p, ctlIdx.p, cD.p, nullptr, ldmWgt, knlEps, cS.p,
+                                             P1.p, ctl.p);
+        ctl_update_k<<<gC, NT, 0, s>>>(nc, P1.p, gC, hist.p, xs, cR.p, cD.p, cS.p, P2.p, ctl.p);
+        CHECK_LAUNCH();
+        prec_apply(cR.p, cZ.p);
+        ctl_dot_k<<<gC, NT, 0, s>>>(nc, P2.p, gC, cR.p, cZ.p, P3.p, ctl.p);
+        ctl_dir_k<<<gC, NT, 0, s>>>(nc, P3.p, gC, hist.p, cZ.p, cD.p, ctl.p);
+        ctl_next_k<<<1, 1, 0, s>>>(ctl.p);
+        CHECK_LAUNCH();
+    }
+
+    // HIP graphs of GRAPH_ITERS CG iterations, one per operand set (captured
+    // on first use, kept for the session: the buffers never move)
+    static constexpr int GRAPH_ITERS = 16;
+    struct CgGraph {
+        KMode md;
+        const double *q, *tau, *geo;
+        double *xs;
+        hipGraphExec_t ex;
+    };
+    std::vector<CgGraph> cgGraphs;
+    bool useGraphs = true;
+
+    hipGraphExec_t cg_graph(KMode md, const double *q, const double *tauNow, const double *geoNow, double *xs) {
+        for (auto &g : cgGraphs)
+            if (g.md == md && g.q == q && g.tau == tauNow && g.geo == geoNow && g.xs == xs) return g.ex;
+        if (cgGraphs.size() >= 16) drop_graphs();
+        hipGraph_t gr = nullptr;
+        CK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        try {
+            for (int k = 0; k < GRAPH_ITERS; ++k) cg_iter(md, q, tauNow, geoNow, xs, nullptr);
+        } catch (...) {
+            (void)hipStreamEndCapture(s, &gr);
+            if (gr) (void)hipGraphDestroy(gr);
+            throw;
+        }
+        CK(hipStreamEndCapture(s, &gr));
+        hipGraphExec_t ex = nullptr;
+        hipError_t e = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gr);
+        CK(e);
+        cgGraphs.push_back({md, q, tauNow, geoNow, xs, ex});
+        return ex;
+    }
+
+    void drop_graphs() {
+        for (auto &g : cgGraphs) (void)hipGraphExecDestroy(g.ex);
+        cgGraphs.clear();
+    }
+
     // PCG on S x = rhs (|rhs|^2 partials in P0, gC of them), preconditioned by
-    // X^T X; x in/out (x0 = 0 when zeroStart); stop when |r|^2 < tolSqu |rhs|^2
+    // X^T X; x in/out (x0 = 0 when zeroStart); stop when |r|^2 < tolSqu |rhs|^2.
+    // Returns 0 converged, 1 breakdown, 2 iteration budget spent.  Without
+    // the kernel timer the iterations run as replays of a captured graph of
+    // GRAPH_ITERS iterations, the host checking the device control block
+    // between launches.
     int ctl_pcg(KMode md, const double *q, const double *tauNow, const double *geoNow, const double *rhs, double *xs,
                 bool zeroStart, double tolSqu, int maxIt, int &iters) {
         const int nc = NC * D, gC = gridOf(nc);
         const double *kdv = md == KMode::Stored32 ? cK.p : nullptr;   // K_cc d (else gathered from K_xc d)
-        pcg_start_k<<<1, NT, 0, s>>>(P0.p, gC, tolSqu, ctl.p);
+        pcg_start_k<<<1, NT, 0, s>>>(P0.p, gC, tolSqu, maxIt, ctl.p);
         CHECK_LAUNCH();
         zero_if_empty_k<<<gC, NT, 0, s>>>(nc, xs, nullptr, ctl.p);
         CHECK_LAUNCH();
         if (zeroStart) {
             CK(hipMemsetAsync(xs, 0, sizeof(double) * nc, s));
-            ctl_combine_k<D, 1><<<gC, NT, 0, s>>>(NC, V, cZero.p, nullptr, nullptr, ctlIdx.p, cZero.p, rhs, 0.0, 0.0, cR.p, P2.p,
-                                                 ctl.p);
+            ctl_combine_k<D, 1><<<gC, NT, 0, s>>>(NC, V, cZero.p, nullptr, nullptr, ctlIdx.p, cZero.p, rhs, 0.0, 0.0,
+                                                 cR.p, P2.p, ctl.p);
         } else {
             S_apply(md, q, tauNow, geoNow, xs, nullptr);
-            ctl_combine_k<D, 1><<<gC, NT, 0, s>>>(NC, V, cY.p, kdv, Kr.p, ctlIdx.p, xs, rhs, ldmWgt, knlEps, cR.p, P2.p,
-                                                 ctl.p);
+            ctl_combine_k<D, 1><<<gC, NT, 0, s>>>(NC, V, cY.p, kdv, Kr.p, ctlIdx.p, xs, rhs, ldmWgt, knlEps, cR.p,
+                                                 P2.p, ctl.p);
         }
         CHECK_LAUNCH();
         prec_apply(cR.p, cZ.p);
-        ctl_dot_k<<<gC, NT, 0, s>>>(nc, P2.p, gC, -1, cR.p, cZ.p, P3.p, ctl.p);
-        ctl_dir_k<<<gC, NT, 0, s>>>(nc, P3.p, gC, hist.p, -1, cZ.p, cD.p, ctl.p);
+        ctl_dot_k<<<gC, NT, 0, s>>>(nc, P2.p, gC, cR.p, cZ.p, P3.p, ctl.p);
+        ctl_dir_k<<<gC, NT, 0, s>>>(nc, P3.p, gC, hist.p, cZ.p, cD.p, ctl.p);
+        ctl_next_k<<<1, 1, 0, s>>>(ctl.p);
         CHECK_LAUNCH();
+        Ctl &h = hctl[0];
+        if (!timer.on && useGraphs) {
+            hipGraphExec_t ex = cg_graph(md, q, tauNow, geoNow, xs);
+            // first sync after about the previous solve's iteration count
+            int launches = std::max(1, std::min(8, chunkHint / GRAPH_ITERS));
+            while (true) {
+                for (int k = 0; k < launches; ++k) CK(hipGraphLaunch(ex, s));
+                sync_ctl(ctl.p, &h);
+                if (h.done) break;
+                launches = 1;
+            }
+            iters = h.iters;
+            return h.status;
+        }
         int it0 = 0;
         int chunk = std::max(4, std::min(64, chunkHint));
-        Ctl &h = hctl[0];
         while (true) {
             int itEnd = std::min(maxIt, it0 + chunk);
             if (timer.on) timer.ensure(itEnd - it0);
-            for (int it = it0; it < itEnd; ++it) {
-                hipEvent_t *ev = timer.on ? &timer.ev[6 * (it - it0)] : nullptr;
-                S_apply(md, q, tauNow, geoNow, cD.p, ev);
-                ctl_combine_k<D, 0><<<gC, NT, 0, s>>>(NC, V, cY.p, kdv, Kr.p, ctlIdx.p, cD.p, nullptr, ldmWgt, knlEps, cS.p,
-                                                     P1.p, ctl.p);
-                ctl_update_k<<<gC, NT, 0, s>>>(nc, P1.p, gC, hist.p, it, xs, cR.p, cD.p, cS.p, P2.p, ctl.p);
-                CHECK_LAUNCH();
-                prec_apply(cR.p, cZ.p);
-                ctl_dot_k<<<gC, NT, 0, s>>>(nc, P2.p, gC, it, cR.p, cZ.p, P3.p, ctl.p);
-                ctl_dir_k<<<gC, NT, 0, s>>>(nc, P3.p, gC, hist.p, it, cZ.p, cD.p, ctl.p);
-                CHECK_LAUNCH();
-            }
+            for (int it = it0; it < itEnd; ++it)
+                cg_iter(md, q, tauNow, geoNow, xs, timer.on ? &timer.ev[6 * (it - it0)] : nullptr);
             sync_ctl(ctl.p, &h);
             if (timer.on) {
                 int last = h.done ? h.iters : itEnd;   // iterations it < last applied S
@@ -757,7 +851,6 @@ struct Engine : EngineBase {
                 }
             }
             if (h.done) break;
-            if (itEnd >= maxIt) { iters = maxIt; return 2; }   // iteration budget spent
             it0 = itEnd;
             chunk = 16;
         }
@@ -765,8 +858,32 @@ struct Engine : EngineBase {
         return h.status;
     }
 
-    int velocity_ctl(const double *q, const double *tauNow, const double *geoNow, int &iters, int &sweeps) {
+    // ctl_pcg guarded against a stale preconditioner: a solve that runs past
+    // 4 x the iterations of the first step after the last refresh (+100)
+    // rebuilds the preconditioner from this step's K_cc and continues from
+    // its iterate with the rest of the budget
+    int ctl_solve(KMode md, const double *q, const double *tauNow, const double *geoNow, const double *rhs, double *xs,
+                  bool zeroStart, double tolSqu, int maxIt, int &iters) {
+        const int cap = precAge > 0 ? std::min(maxIt, 4 * precIters + 100) : maxIt;
+        int st = ctl_pcg(md, q, tauNow, geoNow, rhs, xs, zeroStart, tolSqu, cap, iters);
+        if (st == 2 && cap < maxIt) {
+            kcc_build(true);
+            precAge = 0;
+            int more = 0;
+            chunkHint = 16;
+            st = ctl_pcg(md, q, tauNow, geoNow, rhs, xs, false, tolSqu, maxIt - iters, more);
+            iters += more;
+        }
+        return st;
+    }
+
+    int velocity_ctl(const double *qStep, const double *tauStep, const double *geoNow, int &iters, int &sweeps) {
         const int nc = NC * D, gC = gridOf(nc);
+        // this step's positions and concentration in fixed buffers, so the
+        // captured CG graphs stay valid from step to step
+        CK(hipMemcpyAsync(qCur.p, qStep, sizeof(double) * V * D, hipMemcpyDeviceToDevice, s));
+        CK(hipMemcpyAsync(tauCur.p, tauStep, sizeof(double) * V, hipMemcpyDeviceToDevice, s));
+        const double *q = qCur.p, *tauNow = tauCur.p;
         yank_elem_k<D><<<gridOf(E), NT, 0, s>>>(E, elmVtx.p, geoNow, ya.p, frc.p);
         CHECK_LAUNCH();
         gather_k<D><<<gridOf(V), NT, 0, s>>>(V, E, vtxElm.p, frc.p, f.p, P0.p);
@@ -776,23 +893,28 @@ struct Engine : EngineBase {
         const bool refresh = precAge < 0 || precAge >= PREC_MAX_AGE || lastIters > PREC_GROWTH * precIters + 4;
         kcc_build(refresh);
         if (refresh) precAge = 0;
-        if (A64.p || A32.p) kstore_build(q);
+        // this step in the mixed form (FP32 stored K inside FP64 refinement)
+        // or in the FP64 form (stored FP64 K when it fits)
+        const bool mixedStep = precision == SA_PREC_MIXED && A32.p && !mixedOff;
+        if (!mixedStep && precision == SA_PREC_MIXED) ensure_fp64_store();
+        kstore_build(q, !mixedStep, mixedStep);
         CK(hipMemsetAsync(ctl.p, 0, sizeof(Ctl), s));
         // g = K_cx f in FP64 (stored FP64 matrix when present), |g|^2 partials
-        const KMode exact = A64.p ? KMode::Stored64 : KMode::MatrixFree64;
+        KMode exact = !mixedStep && A64.p ? KMode::Stored64 : KMode::MatrixFree64;
         kx_product(exact, false, q, f.p, cG.p, P0.p, nullptr);
         int status;
         sweeps = 0;
-        if (precision != SA_PREC_MIXED || !A32.p) {
+        if (!mixedStep) {
+            if (precision == SA_PREC_MIXED) ++fp64Steps;
             chunkHint = lastIters - 4;
-            status = ctl_pcg(exact, q, tauNow, geoNow, cG.p, cA.p, false, cgTolSqu, itrMax, iters);
+            status = ctl_solve(exact, q, tauNow, geoNow, cG.p, cA.p, false, cgTolSqu, itrMax, iters);
         } else {
             // iterative refinement: FP64 residual r = g - S alpha, inner
             // solve of S~ delta = r with the FP32 stored K (relative 1e-5),
             // alpha += delta, until |r|^2 < cgTolSqu |g|^2
             double gg = 0.0;
             {
-                pcg_start_k<<<1, NT, 0, s>>>(P0.p, gC, cgTolSqu, ctl.p);
+                pcg_start_k<<<1, NT, 0, s>>>(P0.p, gC, cgTolSqu, itrMax, ctl.p);
                 CHECK_LAUNCH();
                 sync_ctl(ctl.p, &hctl[0]);
                 gg = hctl[0].bb;
@@ -803,29 +925,52 @@ struct Engine : EngineBase {
                 CK(hipMemsetAsync(cA.p, 0, sizeof(double) * nc, s));
                 status = 0;
             }
+            double rrPrev = 0.0;
+            bool stalled = false;
             for (int sw = 0; sw < 12 && gg != 0.0; ++sw) {
                 CK(hipMemsetAsync(ctl.p, 0, sizeof(Ctl), s));
                 S_apply(KMode::MatrixFree64, q, tauNow, geoNow, cA.p, nullptr);
                 ctl_combine_k<D, 1><<<gC, NT, 0, s>>>(NC, V, cY.p, nullptr, Kr.p, ctlIdx.p, cA.p, cG.p, ldmWgt, knlEps, cRout.p,
                                                      P0.p, ctl.p);
                 CHECK_LAUNCH();
-                pcg_start_k<<<1, NT, 0, s>>>(P0.p, gC, 1e-10, ctl.p);
+                pcg_start_k<<<1, NT, 0, s>>>(P0.p, gC, 1e-10, itrMax, ctl.p);
                 CHECK_LAUNCH();
                 sync_ctl(ctl.p, &hctl[0]);
                 double rr = hctl[0].bb;
                 if (rr < cgTolSqu * gg) { status = 0; break; }
+                // a sweep that does not cut the FP64 residual tenfold: the
+                // FP32 operator is too far from the FP64 one (ill-conditioned
+                // K_cc); the step is finished by the FP64 form
+                if (sw > 0 && rr > 1e-2 * rrPrev) { stalled = true; break; }
+                rrPrev = rr;
                 ++sweeps;
                 int its = 0;
                 // inner tolerance: relative 1e-5 of this residual, never below the outer target
                 double inner = std::max(1e-10, cgTolSqu * gg / rr);
                 chunkHint = 16;
-                int st = ctl_pcg(KMode::Stored32, q, tauNow, geoNow, cRout.p, cDelta.p, true, inner,
+                int st = ctl_solve(KMode::Stored32, q, tauNow, geoNow, cRout.p, cDelta.p, true, inner,
                                  std::max(1, itrMax - iters), its);
                 iters += its;
-                if (st == 1) break;   // breakdown
+                if (st == 1) { stalled = true; break; }   // breakdown of the FP32 solve
                 axpby_k<<<gC, NT, 0, s>>>(nc, cA.p, 1.0, cA.p, 1.0, cDelta.p);
                 CHECK_LAUNCH();
                 if (iters >= itrMax) break;
+            }
+            if (status != 0 && (stalled || iters < itrMax)) {
+                // FP64 PCG from the current alpha, for this step and the rest
+                // of the run (stored FP64 kernel matrices when they fit)
+                ++fp64Steps;
+                mixedOff = true;
+                ensure_fp64_store();
+                if (A64.p) {
+                    kstore_build(q, true, false);
+                    exact = KMode::Stored64;
+                }
+                chunkHint = 16;
+                int its = 0;
+                status = ctl_solve(exact, q, tauNow, geoNow, cG.p, cA.p, false, cgTolSqu,
+                                 std::max(1, itrMax - iters), its);
+                iters += its;
             }
         }
         if (status == 0) {
@@ -922,6 +1067,8 @@ struct Engine : EngineBase {
         if (NC > 0) CK(hipMemsetAsync(cA.p, 0, sizeof(double) * NC * D, s));
         lastIters = 0;
         precAge = -1;
+        fp64Steps = 0;
+        mixedOff = false;
         geometry_k<D><<<gridOf(E), NT, 0, s>>>(E, V, elmVtx.p, frmVtx.p, tsvVtx.p, q0.p, spdTan, spdTsv, geo[0].p,
                                                femElm.p);
         CHECK_LAUNCH();
@@ -1002,6 +1149,7 @@ struct Engine : EngineBase {
             int sweeps = 0;
             int vs = NC > 0 ? velocity_ctl(q, tn, geo[cur].p, its, sweeps) : velocity(q, tn, geo[cur].p, t > 0, its);
             st.refine += sweeps;
+            st.fp64Steps = fp64Steps;
             st.pcgIters += its;
             lastIters = its;
             if (NC > 0) {
@@ -1109,6 +1257,7 @@ int sa_session_run(sa_session *ss, const double *tauVarVec, int tauVarNum, const
             stats->pcgIters = st.pcgIters;
             stats->rdIters = st.rdIters;
             stats->refineSweeps = st.refine;
+            stats->fp64Steps = st.fp64Steps;
             stats->seconds = sec;
             stats->kmatvecMs = st.kmvMs;
             stats->kmatvecCount = st.kmvCount;

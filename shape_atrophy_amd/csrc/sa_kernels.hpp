@@ -16,7 +16,9 @@ namespace sa {
 
 struct Ctl {            // solver control block, device resident
     double bb, tol;     // |b|^2 and the stopping threshold on |r|^2
-    int done, status, iters, pad;
+    int done, status, iters;
+    int it, maxIt;      // control-point CG: current iteration (-1 before the first) and budget
+    int pad;
 };
 
 // ---------------------------------------------------------------------------
@@ -419,7 +421,7 @@ __global__ __launch_bounds__(NT) void gather_op_k(int V, int E, const int *__res
 }
 
 // right-hand side |b|^2 and stopping threshold; bb == 0 -> x = 0 (ipcg.cu:25-30)
-__global__ void pcg_start_k(const double *__restrict__ P, int nP, double tolSqu, Ctl *ctl) {
+__global__ void pcg_start_k(const double *__restrict__ P, int nP, double tolSqu, int maxIt, Ctl *ctl) {
     __shared__ double red[NT / 64];
     double bb = reduce_partials(P, nP, red);
     if (threadIdx.x == 0) {
@@ -428,7 +430,18 @@ __global__ void pcg_start_k(const double *__restrict__ P, int nP, double tolSqu,
         ctl->done = (bb == 0.0) ? 1 : 0;
         ctl->status = 0;
         ctl->iters = 0;
+        ctl->it = -1;
+        ctl->maxIt = maxIt;
     }
+}
+
+// end of a control-point CG iteration: advance the iteration counter; the
+// budget spent stops the solve with status 2
+__global__ void ctl_next_k(Ctl *ctl) {
+    if (ctl->done) return;
+    const int it = ctl->it + 1;
+    ctl->it = it;
+    if (it >= ctl->maxIt) { ctl->done = 1; ctl->status = 2; ctl->iters = it; }
 }
 
 __global__ void zero_if_empty_k(int n, double *__restrict__ x, double *__restrict__ a, const Ctl *ctl) {
@@ -873,11 +886,12 @@ __global__ __launch_bounds__(NT) void ctl_combine_k(int C, int V, const double *
 
 // a += st d; r -= st Sd (st = rz / d.Sd); partial |r|^2
 __global__ __launch_bounds__(NT) void ctl_update_k(int n, const double *__restrict__ P, int nP,
-                                                   const double *__restrict__ hist, int it, double *__restrict__ a,
+                                                   const double *__restrict__ hist, double *__restrict__ a,
                                                    double *__restrict__ r, const double *__restrict__ d, const double *__restrict__ Sd,
                                                    double *__restrict__ P2, Ctl *ctl) {
     __shared__ double red[NT / 64];
     if (ctl->done) return;
+    const int it = ctl->it;
     double dSd = reduce_partials(P, nP, red);
     if (dSd == 0.0) {
         if (blockIdx.x == 0 && threadIdx.x == 0) { ctl->status = 1; ctl->done = 1; ctl->iters = it; }
@@ -897,11 +911,12 @@ __global__ __launch_bounds__(NT) void ctl_update_k(int n, const double *__restri
 }
 
 // after z = M^{-1} r: stop on |r|^2 < tol, else partial r.z
-__global__ __launch_bounds__(NT) void ctl_dot_k(int n, const double *__restrict__ Prr, int nP, int it,
+__global__ __launch_bounds__(NT) void ctl_dot_k(int n, const double *__restrict__ Prr, int nP,
                                                 const double *__restrict__ r, const double *__restrict__ z,
                                                 double *__restrict__ P3, Ctl *ctl) {
     __shared__ double red[NT / 64];
     if (ctl->done) return;
+    const int it = ctl->it;
     double rr = reduce_partials(Prr, nP, red);
     if (rr < ctl->tol) {
         if (blockIdx.x == 0 && threadIdx.x == 0) { ctl->done = 1; ctl->iters = it + 1; }
@@ -915,10 +930,10 @@ __global__ __launch_bounds__(NT) void ctl_dot_k(int n, const double *__restrict_
 
 // beta = rz_new / rz_old; d = z + beta d (it < 0: d = z)
 __global__ __launch_bounds__(NT) void ctl_dir_k(int n, const double *__restrict__ P, int nP, double *__restrict__ hist,
-                                                int it, const double *__restrict__ z, double *__restrict__ d,
-                                                Ctl *ctl) {
+                                                const double *__restrict__ z, double *__restrict__ d, Ctl *ctl) {
     __shared__ double red[NT / 64];
     if (ctl->done) return;
+    const int it = ctl->it;
     double rz = reduce_partials(P, nP, red);
     if (rz == 0.0) {
         if (blockIdx.x == 0 && threadIdx.x == 0) { ctl->status = 1; ctl->done = 1; ctl->iters = it + 1; }

@@ -225,25 +225,26 @@ def perturb_params(params: dict, rng, frac: float = 0.3) -> dict:
 
 
 def batch_problems(n: int = 64, seed: int = 4, time_steps: int = 20, ctl_frac: float = 0.1,
-                   max_ctl: int = 2000, v2d=(1000, 20000), v3d=(5000, 60000)):
+                   max_ctl: int = 2000, v2d=(1000, 20000), v3d=(5000, 60000), mixed_dims: bool = False):
     """Independent layered meshes of uneven size: half 2D rings with
-    log-uniform vertex counts in v2d, half 3D shells with counts in v3d, each
-    with seeded parameters (+-30%), its own blob and its own control points
-    (ctl_frac of the landmarks, at most max_ctl)."""
+    log-uniform vertex counts in v2d, half 3D shells with counts in v3d (the
+    first half 2D, or alternating when mixed_dims), each with seeded
+    parameters (+-30%), its own blob and its own control points (ctl_frac of
+    the landmarks, at least 50, at most max_ctl)."""
     rng = np.random.default_rng(seed)
     out = []
     for k in range(n):
-        two_d = k < n // 2
+        two_d = (k % 2 == 0) if mixed_dims else k < n // 2
         lo, hi = v2d if two_d else v3d
         Vt = float(np.exp(rng.uniform(np.log(lo), np.log(hi))))
         if two_d:
-            rings = int(np.clip(round(np.sqrt(Vt / 20.0)), 5, 40))
+            rings = int(np.clip(round(np.sqrt(Vt / 20.0)), 5, 120))
             per = max(20, int(round(Vt / rings)))
             p = ring_problem(n_rings=rings, n_per_ring=per, seed=100 + k, target_seed=200 + k)
         else:
             # icosphere level and layer count closest to the requested size
             best = None
-            for lvl in (2, 3, 4, 5):
+            for lvl in (2, 3, 4, 5, 6):
                 L = 10 * 4 ** lvl + 2
                 layers = int(round(Vt / L))
                 if 3 <= layers <= 12:
