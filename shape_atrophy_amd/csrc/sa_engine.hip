@@ -198,6 +198,7 @@ struct Engine : EngineBase {
     int fp64Steps = 0;           // mixed-form steps finished by the FP64 form (this run)
     bool mixedOff = false;       // a step of this run stalled: the rest run in FP64
     bool no64Store = false;      // the FP64 stored matrices of the fallback did not fit
+    bool precEveryStep = false;  // a stale preconditioner failed in this run: refresh every step
 
     // mixed form: FP64 stored matrices for the steps that fall back to FP64,
     // allocated on the first such step (matrix-free FP64 products if they do
@@ -869,6 +870,7 @@ struct Engine : EngineBase {
         if (st == 2 && cap < maxIt) {
             kcc_build(true);
             precAge = 0;
+            precEveryStep = true;   // K_cc too sensitive to reuse: fresh preconditioner each step
             int more = 0;
             chunkHint = 16;
             st = ctl_pcg(md, q, tauNow, geoNow, rhs, xs, false, tolSqu, maxIt - iters, more);
@@ -890,7 +892,8 @@ struct Engine : EngineBase {
         CHECK_LAUNCH();
         gather_ctl_k<D><<<gridOf(NC), NT, 0, s>>>(NC, V, ctlIdx.p, q, qc.p);
         CHECK_LAUNCH();
-        const bool refresh = precAge < 0 || precAge >= PREC_MAX_AGE || lastIters > PREC_GROWTH * precIters + 4;
+        const bool refresh = precAge < 0 || precAge >= PREC_MAX_AGE || lastIters > PREC_GROWTH * precIters + 4 ||
+                             precEveryStep;
         kcc_build(refresh);
         if (refresh) precAge = 0;
         // this step in the mixed form (FP32 stored K inside FP64 refinement)
@@ -1069,6 +1072,7 @@ struct Engine : EngineBase {
         precAge = -1;
         fp64Steps = 0;
         mixedOff = false;
+        precEveryStep = false;
         geometry_k<D><<<gridOf(E), NT, 0, s>>>(E, V, elmVtx.p, frmVtx.p, tsvVtx.p, q0.p, spdTan, spdTsv, geo[0].p,
                                                femElm.p);
         CHECK_LAUNCH();
