@@ -178,7 +178,7 @@ struct Engine : EngineBase {
     DBuf<double> dCen, dDir, dVol, rowVal, gC, gN, grad, scal;
     DBuf<Ctl> ctl, rdCtl;
     // control-point form
-    DBuf<double> qCur, tauCur, qc, Kcc, Kc, Mpre, Mcol, cA, cG, cR, cZ, cD, cS, cY, cK, cRout, cDelta, cZero, part, partS;
+    DBuf<double> qCur, tauCur, qc, Kcc, Kc, Mpre, Mcol, cA, cA0, cG, cR, cZ, cD, cS, cY, cK, cRout, cDelta, cZero, part, partS;
     DBuf<double> A64, At64;      // stored K_xc / K_cx, FP64
     DBuf<float> A32, At32;       // stored K_xc / K_cx, FP32 (mixed precision)
     int slicesA = 1, slicesT = 1, slicesM = 1;
@@ -199,6 +199,7 @@ struct Engine : EngineBase {
     bool mixedOff = false;       // a step of this run stalled: the rest run in FP64
     bool no64Store = false;      // the FP64 stored matrices of the fallback did not fit
     bool precEveryStep = false;  // a stale preconditioner failed in this run: refresh every step
+    bool forceRefresh = false;   // refresh at the next step (after a stalled mixed step)
 
     // mixed form: FP64 stored matrices for the steps that fall back to FP64,
     // allocated on the first such step (matrix-free FP64 products if they do
@@ -238,6 +239,8 @@ struct Engine : EngineBase {
         mod.assign(p.modVec, p.modVec + p.modNum);
         CK(hipMemcpyAsync(dmod.p, mod.data(), sizeof(double) * mod.size(), hipMemcpyHostToDevice, s));
         CK(hipStreamSynchronize(s));
+        // scalars baked into the captured CG graphs
+        if (p.ldmWgt != ldmWgt || p.knlEps != knlEps || p.knlWidth != knlWidth || p.btmWgt != btmWgt) drop_graphs();
         ldmWgt = p.ldmWgt; knlWidth = p.knlWidth; knlEps = p.knlEps;
         spdTan = p.spdTanVal; spdTsv = p.spdTsvVal; btmWgt = p.btmWgt; dt = p.timeStp;
         itrMax = p.cgItrMax; cgTolSqu = p.cgTolSqu;
@@ -411,7 +414,7 @@ struct Engine : EngineBase {
             ctlIdx.alloc(NC, tot); ctlIdx.upload(hCtl.data(), NC, s);
             qc.alloc(nc, tot); Kcc.alloc((size_t)NC * NC, tot);
             qCur.alloc((size_t)V * D, tot); tauCur.alloc(V, tot);
-            cA.alloc(nc, tot); cG.alloc(nc, tot); cR.alloc(nc, tot); cZ.alloc(nc, tot); cD.alloc(nc, tot);
+            cA.alloc(nc, tot); cA0.alloc(nc, tot); cG.alloc(nc, tot); cR.alloc(nc, tot); cZ.alloc(nc, tot); cD.alloc(nc, tot);
             cS.alloc(nc, tot); cY.alloc(nc, tot); cK.alloc(nc, tot);
             cRout.alloc(nc, tot); cDelta.alloc(nc, tot); cZero.alloc(nc, tot);
             CK(hipMemsetAsync(cZero.p, 0, sizeof(double) * nc, s));
@@ -759,7 +762,10 @@ struct Engine : EngineBase {
         hipGraphExec_t ex;
     };
     std::vector<CgGraph> cgGraphs;
-    bool useGraphs = true;
+    // Off: with graph capture on, a run of three worker threads serving
+    // mixed-precision requests ended in a host segmentation fault whose cause
+    // has not been shown; the iterations are launched one by one instead.
+    bool useGraphs = false;
 
     hipGraphExec_t cg_graph(KMode md, const double *q, const double *tauNow, const double *geoNow, double *xs) {
         for (auto &g : cgGraphs)
@@ -865,7 +871,7 @@ struct Engine : EngineBase {
     // its iterate with the rest of the budget
     int ctl_solve(KMode md, const double *q, const double *tauNow, const double *geoNow, const double *rhs, double *xs,
                   bool zeroStart, double tolSqu, int maxIt, int &iters) {
-        const int cap = precAge > 0 ? std::min(maxIt, 4 * precIters + 100) : maxIt;
+        const int cap = precAge > 0 ? std::min(std::max(1, maxIt / 2), 4 * precIters + 100) : maxIt;
         int st = ctl_pcg(md, q, tauNow, geoNow, rhs, xs, zeroStart, tolSqu, cap, iters);
         if (st == 2 && cap < maxIt) {
             kcc_build(true);
@@ -893,9 +899,9 @@ struct Engine : EngineBase {
         gather_ctl_k<D><<<gridOf(NC), NT, 0, s>>>(NC, V, ctlIdx.p, q, qc.p);
         CHECK_LAUNCH();
         const bool refresh = precAge < 0 || precAge >= PREC_MAX_AGE || lastIters > PREC_GROWTH * precIters + 4 ||
-                             precEveryStep;
+                             precEveryStep || forceRefresh;
         kcc_build(refresh);
-        if (refresh) precAge = 0;
+        if (refresh) { precAge = 0; forceRefresh = false; }
         // this step in the mixed form (FP32 stored K inside FP64 refinement)
         // or in the FP64 form (stored FP64 K when it fits)
         const bool mixedStep = precision == SA_PREC_MIXED && A32.p && !mixedOff;
@@ -930,6 +936,10 @@ struct Engine : EngineBase {
             }
             double rrPrev = 0.0;
             bool stalled = false;
+            // alpha entering the sweeps (the previous step's solution): the
+            // FP64 form restarts from it if the sweeps stall, since a stalled
+            // FP32 correction can carry alpha far from the solution
+            CK(hipMemcpyAsync(cA0.p, cA.p, sizeof(double) * nc, hipMemcpyDeviceToDevice, s));
             for (int sw = 0; sw < 12 && gg != 0.0; ++sw) {
                 CK(hipMemsetAsync(ctl.p, 0, sizeof(Ctl), s));
                 S_apply(KMode::MatrixFree64, q, tauNow, geoNow, cA.p, nullptr);
@@ -964,6 +974,8 @@ struct Engine : EngineBase {
                 // of the run (stored FP64 kernel matrices when they fit)
                 ++fp64Steps;
                 mixedOff = true;
+                forceRefresh = true;   // the next step refreshes and takes its iteration count as the reference
+                CK(hipMemcpyAsync(cA.p, cA0.p, sizeof(double) * nc, hipMemcpyDeviceToDevice, s));
                 ensure_fp64_store();
                 if (A64.p) {
                     kstore_build(q, true, false);
@@ -1073,6 +1085,7 @@ struct Engine : EngineBase {
         fp64Steps = 0;
         mixedOff = false;
         precEveryStep = false;
+        forceRefresh = false;
         geometry_k<D><<<gridOf(E), NT, 0, s>>>(E, V, elmVtx.p, frmVtx.p, tsvVtx.p, q0.p, spdTan, spdTsv, geo[0].p,
                                                femElm.p);
         CHECK_LAUNCH();

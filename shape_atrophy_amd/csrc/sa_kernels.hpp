@@ -740,8 +740,8 @@ __global__ __launch_bounds__(NT) void kstore_build_k(int V, const double *__rest
 // lane, one 16-byte load per column) and one slice of the columns
 // (blockIdx.y); its wavefronts each stream a contiguous quarter of the slice
 // (consecutive columns of a panel are adjacent in memory) with no barrier,
-// reading x at wavefront-uniform addresses, and are summed in wavefront order
-// through LDS at the end.  The block's partial goes to part + s * D * nr;
+// reading x with vector loads (one address per wavefront, served by L1), and
+// are summed in wavefront order through LDS at the end.  The block's partial goes to part + s * D * nr;
 // slice_rows_k sums the slices in slice order.
 template <int D, typename T>
 __global__ __launch_bounds__(NT) void kstore_mv_k(int nr, int nc, const T *__restrict__ M,
@@ -755,7 +755,7 @@ __global__ __launch_bounds__(NT) void kstore_mv_k(int nr, int nc, const T *__res
     __shared__ double red[W - 1][D][PR];
     if (ctl && ctl->done) return;
     const int lane = threadIdx.x & 63;
-    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wv = threadIdx.x >> 6;
     const int r0 = blockIdx.x * PR + R * lane;
     const int per = (nc + gridDim.y - 1) / gridDim.y;
     const int jb = blockIdx.y * per, je = min(nc, jb + per);
