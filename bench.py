@@ -85,6 +85,7 @@ def run_requests(api, meshes, reqs, precision, workers, mem_budget=0, rate=None,
         m = reqs[k][0]
         print(f"[bench]   {precision} req {k}: V={meshes[m]['q0'].shape[1]} d={meshes[m]['q0'].shape[0]} "
               f"steps={g['steps']} iters={g['pcg_iters']} sweeps={g['refine']} fp64_steps={g['fp64_steps']} "
+              f"fail={g['fail_reason']} "
               f"run={g['seconds']:.2f}s done_at={time.perf_counter() - t0:.1f}s rc={g['rc']} status={g['status']}",
               file=sys.stderr, flush=True)
     wall = time.perf_counter() - t0

@@ -128,7 +128,16 @@ typedef struct sa_stats {
     double operatorMs;           /* summed time of the elastic operator applies (element + gather kernels) */
     int operatorCount;
     int fp64Steps;               /* mixed precision: steps whose refinement stalled and were finished in FP64 */
+    int failReason;              /* status 1: SA_FAIL_* */
 } sa_stats;
+
+/* sa_stats.failReason */
+#define SA_FAIL_BREAKDOWN 1      /* velocity CG breakdown (zero curvature or zero r.z) */
+#define SA_FAIL_BUDGET    2      /* velocity CG iteration budget (cgItrMax) spent */
+#define SA_FAIL_CHOLESKY  3      /* control-point K_cc not positive definite (Cholesky) */
+#define SA_FAIL_TRTRI     4      /* inverse of the Cholesky factor failed */
+#define SA_FAIL_RD        5      /* reaction-diffusion CG did not converge */
+#define SA_FAIL_REFINE    6      /* mixed-precision refinement did not converge */
 
 /* deformation_{2D,3D}_frame (deformation.cu:14-199).
    tauVarMat: tauVarNum x candNum (arg 0); tauPrmVec (arg 1); lmkIniMat (arg 2).

@@ -65,7 +65,7 @@ class SaStats(C.Structure):
     _fields_ = [("status", C.c_int), ("steps", C.c_int), ("pcgIters", C.c_int), ("rdIters", C.c_int),
                 ("refineSweeps", C.c_int), ("seconds", C.c_double), ("kmatvecMs", C.c_double),
                 ("kmatvecCount", C.c_int), ("operatorMs", C.c_double), ("operatorCount", C.c_int),
-                ("fp64Steps", C.c_int)]
+                ("fp64Steps", C.c_int), ("failReason", C.c_int)]
 
 
 class SaError(RuntimeError):
@@ -334,7 +334,7 @@ class Session:
                                     _dp(grad), C.byref(st)))
         return dict(obj=float(obj[0]), lmkStk=lmk, tauMat=tau, grad=grad, status=st.status, steps=st.steps,
                     pcg_iters=st.pcgIters, rd_iters=st.rdIters, refine=st.refineSweeps, fp64_steps=st.fp64Steps,
-                    seconds=st.seconds,
+                    fail_reason=st.failReason, seconds=st.seconds,
                     kmatvec_ms=st.kmatvecMs, kmatvec_count=st.kmatvecCount, operator_ms=st.operatorMs,
                     operator_count=st.operatorCount)
 
@@ -438,7 +438,7 @@ class Server:
         rc = lib().sa_server_wait(self._h, C.c_longlong(req_id), _dp(obj), C.byref(st), _dp(times))
         return dict(rc=rc, error=lib().sa_last_error().decode() if rc else "", obj=float(obj[0]),
                     status=st.status, steps=st.steps, pcg_iters=st.pcgIters, refine=st.refineSweeps,
-                    fp64_steps=st.fp64Steps,
+                    fp64_steps=st.fp64Steps, fail_reason=st.failReason,
                     seconds=st.seconds, t_submit=times[0], t_start=times[1], t_end=times[2])
 
     def info(self) -> dict:

@@ -26,6 +26,19 @@ template <int D> struct GeoLayout {
 // reductions: wavefront butterfly then a fixed-order sum over the 4 waves.
 // The order depends only on the block shape, so results are reproducible.
 
+// Solver control values (flags, counters, the CG step history) are produced
+// by one kernel and read by the next, possibly on another XCD: they are
+// accessed with agent-scope atomic loads and stores, which are coherent
+// across the XCDs' L2 caches and never served by the scalar data cache.
+template <typename T>
+__device__ inline T cld(const T *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T, typename U>
+__device__ inline void cst(T *p, U v) {
+    __hip_atomic_store(p, (T)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 __device__ inline double wave_sum(double v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);

@@ -22,6 +22,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <mutex>
 #include <vector>
 
 #include <rocblas/rocblas.h>
@@ -35,6 +36,9 @@ namespace sa {
 namespace {
 
 thread_local std::string g_err;
+// rocBLAS / rocSOLVER calls of all sessions run one at a time (each to
+// completion on its stream before the next starts)
+std::mutex g_libMu;
 
 struct SaError : std::runtime_error {
     int code;
@@ -118,7 +122,7 @@ std::vector<int> incidence(const std::vector<int> &items, int nItems, int k, int
 }
 
 struct Stats {
-    int status = 0, steps = 0, pcgIters = 0, rdIters = 0, refine = 0, fp64Steps = 0;
+    int status = 0, steps = 0, pcgIters = 0, rdIters = 0, refine = 0, fp64Steps = 0, failReason = 0;
     double kmvMs = 0.0, opMs = 0.0;   // summed durations of the timed kernels (profiling on)
     int kmvCount = 0, opCount = 0;
 };
@@ -200,6 +204,7 @@ struct Engine : EngineBase {
     bool no64Store = false;      // the FP64 stored matrices of the fallback did not fit
     bool precEveryStep = false;  // a stale preconditioner failed in this run: refresh every step
     bool forceRefresh = false;   // refresh at the next step (after a stalled mixed step)
+    int failReason = 0;          // SA_FAIL_* of a failed run
 
     // mixed form: FP64 stored matrices for the steps that fall back to FP64,
     // allocated on the first such step (matrix-free FP64 products if they do
@@ -448,14 +453,16 @@ struct Engine : EngineBase {
             // about 1024 blocks per stored product (row blocks x column slices)
             const int prS = precision == SA_PREC_MIXED ? panel_rows<float>() : panel_rows<double>();
             const int rbA = gridOf(V, prS), rbT = gridOf(NC, prS), rbM = gridOf(NC, panel_rows<double>());
-            slicesA = std::max(1, std::min(64, (1024 + rbA - 1) / rbA));
-            slicesT = std::max(1, std::min(256, (1024 + rbT - 1) / rbT));
-            slicesM = std::max(1, std::min(256, (1024 + rbM - 1) / rbM));
+            // (and at least enough slices for the LDS copy of x: KS_MAX_COLS columns)
+            slicesA = std::max(gridOf(NC, KS_MAX_COLS), std::min(64, (1024 + rbA - 1) / rbA));
+            slicesT = std::max(gridOf(V, KS_MAX_COLS), std::min(256, (1024 + rbT - 1) / rbT));
+            slicesM = std::max(gridOf(NC, KS_MAX_COLS), std::min(256, (1024 + rbM - 1) / rbM));
             partS.alloc(std::max({(size_t)slicesA * D * V, (size_t)slicesT * nc, (size_t)slicesM * nc}), tot);
             int rb = gridOf(NC, MV_ROWS);
             slicesC = std::max(1, std::min(32, (1024 + rb - 1) / rb));
             part.alloc((size_t)slicesC * nc, tot);
             potrfInfo.alloc(2, tot);
+            std::lock_guard<std::mutex> lk(g_libMu);
             if (rocblas_create_handle(&blas) != rocblas_status_success) throw SaError(SA_E_DEVICE, "rocblas_create_handle failed");
             if (rocblas_set_stream(blas, s) != rocblas_status_success) throw SaError(SA_E_DEVICE, "rocblas_set_stream failed");
             // deterministic factorisation and products: no atomics in rocBLAS/rocSOLVER kernels
@@ -570,7 +577,7 @@ struct Engine : EngineBase {
                 }
             }
             if (h.done) break;
-            if (itEnd >= itrMax) { iters = itrMax; return 1; }
+            if (itEnd >= itrMax) { iters = itrMax; failReason = SA_FAIL_BUDGET; return 1; }
             it0 = itEnd;
             chunk = std::min(chunk * 2, 64);
         }
@@ -625,6 +632,7 @@ struct Engine : EngineBase {
 #undef SA_KB
         CHECK_LAUNCH();
         if (!refresh) return;
+        std::lock_guard<std::mutex> lk(g_libMu);
         if (rocsolver_dpotrf(blas, rocblas_fill_lower, NC, Kcc.p, NC, potrfInfo.p) != rocblas_status_success)
             throw SaError(SA_E_DEVICE, "rocsolver_dpotrf failed");
         if (rocsolver_dtrtri(blas, rocblas_fill_lower, rocblas_diagonal_non_unit, NC, Kcc.p, NC, potrfInfo.p + 1) !=
@@ -638,7 +646,11 @@ struct Engine : EngineBase {
             throw SaError(SA_E_DEVICE, "rocblas_dgemm failed");
         panelize_k<<<g, NT, 0, s>>>(NC, Mcol.p, NC, Mpre.p);
         CHECK_LAUNCH();
+        CK(hipStreamSynchronize(s));
     }
+
+    // dynamic LDS of kstore_mv_k: the D x slice copy of x
+    static size_t ks_lds(int ncol, int slices) { return sizeof(double) * D * (size_t)gridOf(ncol, slices); }
 
     template <typename T>
     static size_t padded_rows(int n) { return (size_t)gridOf(n, panel_rows<T>()) * panel_rows<T>(); }
@@ -646,7 +658,7 @@ struct Engine : EngineBase {
     // y = M x for a dense FP64 C x C panel-major matrix
     void cc_product(const double *M, const double *x, double *y) {
         dim3 g(gridOf(NC, panel_rows<double>()), slicesM);
-        kstore_mv_k<D, double><<<g, NT, 0, s>>>(NC, NC, M, x, partS.p, ctl.p);
+        kstore_mv_k<D, double><<<g, NT, ks_lds(NC, slicesM), s>>>(NC, NC, M, x, partS.p, ctl.p);
         CHECK_LAUNCH();
         slice_rows_k<<<gridOf(D * NC, 64), NT, 0, s>>>(D * NC, slicesM, partS.p, y, ctl.p);
         CHECK_LAUNCH();
@@ -707,10 +719,10 @@ struct Engine : EngineBase {
         if (ev) CK(hipEventRecord(ev[0], s));
         if (md == KMode::Stored64) {
             dim3 g(gridOf(nr, panel_rows<double>()), sl);
-            kstore_mv_k<D, double><<<g, NT, 0, s>>>(nr, ncol, xc ? A64.p : At64.p, u, partS.p, c);
+            kstore_mv_k<D, double><<<g, NT, ks_lds(ncol, sl), s>>>(nr, ncol, xc ? A64.p : At64.p, u, partS.p, c);
         } else {
             dim3 g(gridOf(nr, panel_rows<float>()), sl);
-            kstore_mv_k<D, float><<<g, NT, 0, s>>>(nr, ncol, xc ? A32.p : At32.p, u, partS.p, c);
+            kstore_mv_k<D, float><<<g, NT, ks_lds(ncol, sl), s>>>(nr, ncol, xc ? A32.p : At32.p, u, partS.p, c);
         }
         CHECK_LAUNCH();
         if (ev) CK(hipEventRecord(ev[1], s));
@@ -981,6 +993,9 @@ struct Engine : EngineBase {
                     kstore_build(q, true, false);
                     exact = KMode::Stored64;
                 }
+                // g = K_cx f and its |g|^2 partials (P0) for the FP64 solve; the
+                // sweeps left P0 holding the last residual's
+                kx_product(exact, false, q, f.p, cG.p, P0.p, nullptr);
                 chunkHint = 16;
                 int its = 0;
                 status = ctl_solve(exact, q, tauNow, geoNow, cG.p, cA.p, false, cgTolSqu,
@@ -993,9 +1008,12 @@ struct Engine : EngineBase {
             int info[2] = {0, 0};
             CK(hipMemcpyAsync(info, potrfInfo.p, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
             CK(hipStreamSynchronize(s));
-            if (info[0] != 0 || info[1] != 0) return 1;
+            if (info[0] != 0) { failReason = SA_FAIL_CHOLESKY; return 1; }
+            if (info[1] != 0) { failReason = SA_FAIL_TRTRI; return 1; }
+            return 0;
         }
-        return status == 0 ? 0 : 1;
+        failReason = status == 2 ? SA_FAIL_BUDGET : status == 1 ? SA_FAIL_BREAKDOWN : SA_FAIL_REFINE;
+        return 1;
     }
 
     int reaction_diffusion(const double *tauNow, double *tauNxt, const double *geoNow, const double *ppdNow,
@@ -1022,7 +1040,7 @@ struct Engine : EngineBase {
             }
             sync_ctl(rdCtl.p, &h);
             if (h.done) break;
-            if (itEnd >= rdMax) { iters = rdMax; return 1; }
+            if (itEnd >= rdMax) { iters = rdMax; failReason = SA_FAIL_RD; return 1; }
             it0 = itEnd;
             chunk = std::min(chunk * 2, 128);
         }
@@ -1086,6 +1104,7 @@ struct Engine : EngineBase {
         mixedOff = false;
         precEveryStep = false;
         forceRefresh = false;
+        failReason = 0;
         geometry_k<D><<<gridOf(E), NT, 0, s>>>(E, V, elmVtx.p, frmVtx.p, tsvVtx.p, q0.p, spdTan, spdTsv, geo[0].p,
                                                femElm.p);
         CHECK_LAUNCH();
@@ -1189,6 +1208,7 @@ struct Engine : EngineBase {
             cur = nx;
         }
         st.status = failed;
+        st.failReason = failed ? failReason : 0;
         if (obj) {
             if (failed) *obj = DBL_MAX;
             else if (hasTarget) *obj = objective(lmk.p + (size_t)(TT - 1) * n, gradOut);
@@ -1275,6 +1295,7 @@ int sa_session_run(sa_session *ss, const double *tauVarVec, int tauVarNum, const
             stats->rdIters = st.rdIters;
             stats->refineSweeps = st.refine;
             stats->fp64Steps = st.fp64Steps;
+            stats->failReason = st.failReason;
             stats->seconds = sec;
             stats->kmatvecMs = st.kmvMs;
             stats->kmatvecCount = st.kmvCount;

@@ -252,7 +252,7 @@ __global__ __launch_bounds__(NT) void elastic_elem_k(int E, int V, const int *__
                                                      const double *__restrict__ mod, T *__restrict__ frc,
                                                      const Ctl *ctl) {
     using G = GeoLayout<D>;
-    if (ctl && ctl->done) return;
+    if (ctl && cld(&ctl->done)) return;
     int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= E) return;
     int vtx[D + 1];
@@ -307,7 +307,11 @@ __global__ __launch_bounds__(NT) void elastic_elem_k(int E, int V, const int *__
         double f1[D], f2[D];
 #pragma unroll
         for (int c = 0; c < D; ++c) { f1[c] = geo[(G::F1 + c) * (size_t)E + e]; f2[c] = geo[(G::F2 + c) * (size_t)E + e]; }
-        frame_stress<D>(eps, f1, f2, mod, sig);
+        // moduli change between runs (host copies): read coherently
+        double md[D + 1];
+#pragma unroll
+        for (int m = 0; m <= D; ++m) md[m] = cld(&mod[m]);
+        frame_stress<D>(eps, f1, f2, md, sig);
     }
 #pragma unroll
     for (int c = 0; c < D; ++c) {
@@ -345,7 +349,7 @@ __global__ void btm_meas_k(int B, int V, const int *__restrict__ btmVtx, const d
 template <int D>
 __global__ void pen_elem_k(int B, int V, const int *__restrict__ btmVtx, const double *__restrict__ meas,
                            const double *__restrict__ x, double *__restrict__ pfrc, const Ctl *ctl) {
-    if (ctl && ctl->done) return;
+    if (ctl && cld(&ctl->done)) return;
     int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= B) return;
     double m = meas[b];
@@ -368,7 +372,7 @@ __global__ __launch_bounds__(NT) void gather_op_k(int V, int E, const int *__res
                                                   const double *__restrict__ w, double *__restrict__ out,
                                                   double *__restrict__ out2, double *__restrict__ P, const Ctl *ctl) {
     __shared__ double red[NT / 64];
-    if (ctl->done) return;
+    if (cld(&ctl->done)) return;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     double ss = 0.0;
     if (i < V) {
@@ -425,27 +429,27 @@ __global__ void pcg_start_k(const double *__restrict__ P, int nP, double tolSqu,
     __shared__ double red[NT / 64];
     double bb = reduce_partials(P, nP, red);
     if (threadIdx.x == 0) {
-        ctl->bb = bb;
-        ctl->tol = bb * tolSqu;
-        ctl->done = (bb == 0.0) ? 1 : 0;
-        ctl->status = 0;
-        ctl->iters = 0;
-        ctl->it = -1;
-        ctl->maxIt = maxIt;
+        cst(&ctl->bb, bb);
+        cst(&ctl->tol, bb * tolSqu);
+        cst(&ctl->done, (bb == 0.0) ? 1 : 0);
+        cst(&ctl->status, 0);
+        cst(&ctl->iters, 0);
+        cst(&ctl->it, -1);
+        cst(&ctl->maxIt, maxIt);
     }
 }
 
 // end of a control-point CG iteration: advance the iteration counter; the
 // budget spent stops the solve with status 2
 __global__ void ctl_next_k(Ctl *ctl) {
-    if (ctl->done) return;
-    const int it = ctl->it + 1;
-    ctl->it = it;
-    if (it >= ctl->maxIt) { ctl->done = 1; ctl->status = 2; ctl->iters = it; }
+    if (cld(&ctl->done)) return;
+    const int it = cld(&ctl->it) + 1;
+    cst(&ctl->it, it);
+    if (it >= cld(&ctl->maxIt)) { cst(&ctl->done, 1); cst(&ctl->status, 2); cst(&ctl->iters, it); }
 }
 
 __global__ void zero_if_empty_k(int n, double *__restrict__ x, double *__restrict__ a, const Ctl *ctl) {
-    if (ctl->bb != 0.0) return;
+    if (cld(&ctl->bb) != 0.0) return;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) { x[i] = 0.0; if (a) a[i] = 0.0; }
 }
@@ -458,13 +462,13 @@ __global__ __launch_bounds__(NT) void pcg_update_k(int n, const double *__restri
                                                    const double *__restrict__ Kid, const double *__restrict__ Ad,
                                                    double *__restrict__ P2, Ctl *ctl) {
     __shared__ double red[NT / 64];
-    if (ctl->done) return;
+    if (cld(&ctl->done)) return;
     double dAd = reduce_partials(P, nP, red);
     if (dAd == 0.0) {   // ipcg.cu:63-68
-        if (blockIdx.x == 0 && threadIdx.x == 0) { ctl->status = 1; ctl->done = 1; ctl->iters = it; }
+        if (blockIdx.x == 0 && threadIdx.x == 0) { cst(&ctl->status, 1); cst(&ctl->done, 1); cst(&ctl->iters, it); }
         return;
     }
-    double st = hist[it] / dAd;
+    double st = cld(&hist[it]) / dAd;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     double ss = 0.0;
     if (i < n) {
@@ -484,20 +488,20 @@ __global__ __launch_bounds__(NT) void pcg_dir_k(int n, const double *__restrict_
                                                 int it, const double *__restrict__ r, const double *__restrict__ Kr,
                                                 double *__restrict__ Kid, double *__restrict__ d, Ctl *ctl) {
     __shared__ double red[NT / 64];
-    if (ctl->done) return;
+    if (cld(&ctl->done)) return;
     double rKr = reduce_partials(P, nP, red);
     if (rKr == 0.0) {   // ipcg.cu:48-53, 83-88
-        if (blockIdx.x == 0 && threadIdx.x == 0) { ctl->status = 1; ctl->done = 1; ctl->iters = it + 1; }
+        if (blockIdx.x == 0 && threadIdx.x == 0) { cst(&ctl->status, 1); cst(&ctl->done, 1); cst(&ctl->iters, it + 1); }
         return;
     }
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (it < 0) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) hist[0] = rKr;
+        if (blockIdx.x == 0 && threadIdx.x == 0) cst(&hist[0], rKr);
         if (i < n) d[i] = Kr[i];
         return;
     }
-    double be = rKr / hist[it];
-    if (blockIdx.x == 0 && threadIdx.x == 0) hist[it + 1] = rKr;
+    double be = rKr / cld(&hist[it]);
+    if (blockIdx.x == 0 && threadIdx.x == 0) cst(&hist[it + 1], rKr);
     if (i < n) {
         Kid[i] = 1.0 * r[i] + be * Kid[i];
         d[i] = 1.0 * Kr[i] + be * d[i];
@@ -524,12 +528,12 @@ __global__ __launch_bounds__(NT) void kmatvec_k(int V, const double *__restrict_
     __shared__ double racc[MV_PH][D][MV_ROWS];
     __shared__ double red[NT / 64];
     __shared__ double etab[64];
-    if (ctl && ctl->done) return;
+    if (ctl && cld(&ctl->done)) return;
     init_exp_table(etab);
     if constexpr (CHECK) {
         double rr = reduce_partials(Prr, nPrr, red);
-        if (rr < ctl->tol) {
-            if (blockIdx.x == 0 && threadIdx.x == 0) { ctl->done = 1; ctl->iters = it + 1; }
+        if (rr < cld(&ctl->tol)) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) { cst(&ctl->done, 1); cst(&ctl->iters, it + 1); }
             return;
         }
     }
@@ -591,7 +595,7 @@ __global__ __launch_bounds__(NT) void kcross_k(int na, const double *__restrict_
     __shared__ double sx[D][NT];
     __shared__ double racc[MV_PH][D][MV_ROWS];
     __shared__ double etab[64];
-    if (ctl && ctl->done) return;
+    if (ctl && cld(&ctl->done)) return;
     init_exp_table(etab);
     const int lr = threadIdx.x & (MV_ROWS - 1), ph = threadIdx.x / MV_ROWS;
     const int i = blockIdx.x * MV_ROWS + lr;
@@ -638,7 +642,7 @@ __global__ __launch_bounds__(NT) void kcross_k(int na, const double *__restrict_
 __global__ __launch_bounds__(NT) void slice_sum_k(int n, int slices, const double *__restrict__ part,
                                                   double *__restrict__ y, double *__restrict__ P, const Ctl *ctl) {
     __shared__ double red[NT / 64];
-    if (ctl && ctl->done) return;
+    if (ctl && cld(&ctl->done)) return;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     double ss = 0.0;
     if (i < n) {
@@ -737,12 +741,15 @@ __global__ __launch_bounds__(NT) void kstore_build_k(int V, const double *__rest
 
 // y = M x for a stored panel-major M (nr x nc, see panel_at), x and y with D
 // columns (stride nc and nr).  A block owns one panel of 64 R rows (R per
-// lane, one 16-byte load per column) and one slice of the columns
-// (blockIdx.y); its wavefronts each stream a contiguous quarter of the slice
-// (consecutive columns of a panel are adjacent in memory) with no barrier,
-// reading x with vector loads (one address per wavefront, served by L1), and
-// are summed in wavefront order through LDS at the end.  The block's partial goes to part + s * D * nr;
-// slice_rows_k sums the slices in slice order.
+// lane, one 16-byte load per column) and one slice of at most KS_MAX_COLS
+// columns (blockIdx.y); it stages its slice of x in LDS (dynamic, D x slice
+// doubles), then its wavefronts each stream a contiguous quarter of the slice
+// (consecutive columns of a panel are adjacent in memory) with no further
+// barrier, reading x from LDS as a broadcast, and are summed in wavefront
+// order through LDS at the end.  The block's partial goes to
+// part + s * D * nr; slice_rows_k sums the slices in slice order.
+constexpr int KS_MAX_COLS = 1024;
+
 template <int D, typename T>
 __global__ __launch_bounds__(NT) void kstore_mv_k(int nr, int nc, const T *__restrict__ M,
                                                   const double *__restrict__ x, double *__restrict__ part,
@@ -752,21 +759,27 @@ __global__ __launch_bounds__(NT) void kstore_mv_k(int nr, int nc, const T *__res
     constexpr int W = NT / 64;
     constexpr int U = 8;   // columns whose loads are issued together
     using V4 = typename std::conditional<sizeof(T) == 4, float4, double2>::type;
+    extern __shared__ double sx[];   // D x (je - jb)
     __shared__ double red[W - 1][D][PR];
-    if (ctl && ctl->done) return;
-    const int lane = threadIdx.x & 63;
-    const int wv = threadIdx.x >> 6;
+    if (ctl && cld(&ctl->done)) return;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int r0 = blockIdx.x * PR + R * lane;
     const int per = (nc + gridDim.y - 1) / gridDim.y;
     const int jb = blockIdx.y * per, je = min(nc, jb + per);
-    const int pw = (je - jb + W - 1) / W;
-    const int wb = min(je, jb + wv * pw), we = min(je, wb + pw);
+    const int ns = max(0, je - jb);
+    for (int t = threadIdx.x; t < D * ns; t += NT) {
+        const int c = t / ns, j = t - c * ns;
+        sx[t] = x[c * nc + jb + j];
+    }
+    __syncthreads();
+    const int pw = (ns + W - 1) / W;
+    const int wb = min(ns, wv * pw), we = min(ns, wb + pw);   // slice-relative columns of this wavefront
     double a[R][D];
 #pragma unroll
     for (int k = 0; k < R; ++k)
 #pragma unroll
         for (int c = 0; c < D; ++c) a[k][c] = 0.0;
-    const T *col = M + (size_t)blockIdx.x * PR * nc + (size_t)wb * PR + R * lane;
+    const T *col = M + (size_t)blockIdx.x * PR * nc + (size_t)(jb + wb) * PR + R * lane;
     int j = wb;
     for (; j + U <= we; j += U, col += U * PR) {
         V4 m[U];
@@ -776,7 +789,7 @@ __global__ __launch_bounds__(NT) void kstore_mv_k(int nr, int nc, const T *__res
         for (int u = 0; u < U; ++u)
 #pragma unroll
             for (int c = 0; c < D; ++c) {
-                const double xv = x[c * nc + j + u];
+                const double xv = sx[c * ns + j + u];
                 if constexpr (sizeof(T) == 4) {
                     a[0][c] += (double)m[u].x * xv; a[1][c] += (double)m[u].y * xv;
                     a[2][c] += (double)m[u].z * xv; a[3][c] += (double)m[u].w * xv;
@@ -789,7 +802,7 @@ __global__ __launch_bounds__(NT) void kstore_mv_k(int nr, int nc, const T *__res
         V4 m = *reinterpret_cast<const V4 *>(col);
 #pragma unroll
         for (int c = 0; c < D; ++c) {
-            const double xv = x[c * nc + j];
+            const double xv = sx[c * ns + j];
             if constexpr (sizeof(T) == 4) {
                 a[0][c] += (double)m.x * xv; a[1][c] += (double)m.y * xv;
                 a[2][c] += (double)m.z * xv; a[3][c] += (double)m.w * xv;
@@ -834,7 +847,7 @@ __global__ void panelize_k(int n, const double *__restrict__ src, int ld, double
 __global__ __launch_bounds__(NT) void slice_rows_k(int n, int slices, const double *__restrict__ part,
                                                    double *__restrict__ y, const Ctl *ctl) {
     __shared__ double red[NT / 64][64];
-    if (ctl && ctl->done) return;
+    if (ctl && cld(&ctl->done)) return;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int i = blockIdx.x * 64 + lane;
     double s = 0.0;
@@ -870,7 +883,7 @@ __global__ __launch_bounds__(NT) void ctl_combine_k(int C, int V, const double *
                                                     double ldmWgt, double eps, double *__restrict__ out,
                                                     double *__restrict__ P, const Ctl *ctl) {
     __shared__ double red[NT / 64];
-    if (ctl->done) return;
+    if (cld(&ctl->done)) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     double ss = 0.0;
     if (i < C * D) {
@@ -890,14 +903,14 @@ __global__ __launch_bounds__(NT) void ctl_update_k(int n, const double *__restri
                                                    double *__restrict__ r, const double *__restrict__ d, const double *__restrict__ Sd,
                                                    double *__restrict__ P2, Ctl *ctl) {
     __shared__ double red[NT / 64];
-    if (ctl->done) return;
-    const int it = ctl->it;
+    if (cld(&ctl->done)) return;
+    const int it = cld(&ctl->it);
     double dSd = reduce_partials(P, nP, red);
     if (dSd == 0.0) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) { ctl->status = 1; ctl->done = 1; ctl->iters = it; }
+        if (blockIdx.x == 0 && threadIdx.x == 0) { cst(&ctl->status, 1); cst(&ctl->done, 1); cst(&ctl->iters, it); }
         return;
     }
-    double st = hist[it] / dSd;
+    double st = cld(&hist[it]) / dSd;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     double ss = 0.0;
     if (i < n) {
@@ -915,11 +928,11 @@ __global__ __launch_bounds__(NT) void ctl_dot_k(int n, const double *__restrict_
                                                 const double *__restrict__ r, const double *__restrict__ z,
                                                 double *__restrict__ P3, Ctl *ctl) {
     __shared__ double red[NT / 64];
-    if (ctl->done) return;
-    const int it = ctl->it;
+    if (cld(&ctl->done)) return;
+    const int it = cld(&ctl->it);
     double rr = reduce_partials(Prr, nP, red);
-    if (rr < ctl->tol) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) { ctl->done = 1; ctl->iters = it + 1; }
+    if (rr < cld(&ctl->tol)) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) { cst(&ctl->done, 1); cst(&ctl->iters, it + 1); }
         return;
     }
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -932,21 +945,21 @@ __global__ __launch_bounds__(NT) void ctl_dot_k(int n, const double *__restrict_
 __global__ __launch_bounds__(NT) void ctl_dir_k(int n, const double *__restrict__ P, int nP, double *__restrict__ hist,
                                                 const double *__restrict__ z, double *__restrict__ d, Ctl *ctl) {
     __shared__ double red[NT / 64];
-    if (ctl->done) return;
-    const int it = ctl->it;
+    if (cld(&ctl->done)) return;
+    const int it = cld(&ctl->it);
     double rz = reduce_partials(P, nP, red);
     if (rz == 0.0) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) { ctl->status = 1; ctl->done = 1; ctl->iters = it + 1; }
+        if (blockIdx.x == 0 && threadIdx.x == 0) { cst(&ctl->status, 1); cst(&ctl->done, 1); cst(&ctl->iters, it + 1); }
         return;
     }
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (it < 0) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) hist[0] = rz;
+        if (blockIdx.x == 0 && threadIdx.x == 0) cst(&hist[0], rz);
         if (i < n) d[i] = z[i];
         return;
     }
-    double be = rz / hist[it];
-    if (blockIdx.x == 0 && threadIdx.x == 0) hist[it + 1] = rz;
+    double be = rz / cld(&hist[it]);
+    if (blockIdx.x == 0 && threadIdx.x == 0) cst(&hist[it + 1], rz);
     if (i < n) d[i] = z[i] + be * d[i];
 }
 
@@ -1030,12 +1043,12 @@ __global__ void rd_fin_k(const double *__restrict__ P, int nP, double tolSqu, do
     double rz = reduce_partials(P + MAXP, nP, red);
     double bb = reduce_partials(P + 2 * MAXP, nP, red);
     if (threadIdx.x == 0) {
-        ctl->bb = bb;
-        ctl->tol = bb * tolSqu;
-        ctl->status = 0;
-        ctl->iters = 0;
-        ctl->done = (bb == 0.0 || rr < bb * tolSqu) ? 1 : 0;
-        hist[0] = rz;
+        cst(&ctl->bb, bb);
+        cst(&ctl->tol, bb * tolSqu);
+        cst(&ctl->status, 0);
+        cst(&ctl->iters, 0);
+        cst(&ctl->done, (bb == 0.0 || rr < bb * tolSqu) ? 1 : 0);
+        cst(&hist[0], rz);
     }
 }
 
@@ -1043,7 +1056,7 @@ __global__ __launch_bounds__(NT) void rd_spmv_k(int V, const int *__restrict__ r
                                                 const double *__restrict__ A, const double *__restrict__ d,
                                                 double *__restrict__ Ad, double *__restrict__ P, const Ctl *ctl) {
     __shared__ double red[NT / 64];
-    if (ctl->done) return;
+    if (cld(&ctl->done)) return;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     double ss = 0.0;
     if (i < V) {
@@ -1063,9 +1076,9 @@ __global__ __launch_bounds__(NT) void rd_update_k(int V, const double *__restric
                                                   const double *__restrict__ d, const double *__restrict__ Ad,
                                                   double *__restrict__ P2, Ctl *ctl) {
     __shared__ double red[NT / 64];
-    if (ctl->done) return;
+    if (cld(&ctl->done)) return;
     double dAd = reduce_partials(P, nP, red);
-    double al = hist[it] / dAd;
+    double al = cld(&hist[it]) / dAd;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     double s_rr = 0.0, s_rz = 0.0;
     if (i < V) {
@@ -1085,15 +1098,15 @@ __global__ __launch_bounds__(NT) void rd_update_k(int V, const double *__restric
 __global__ __launch_bounds__(NT) void rd_dir_k(int V, const double *__restrict__ P, int nP, double *__restrict__ hist,
                                                int it, const double *__restrict__ zv, double *__restrict__ d, Ctl *ctl) {
     __shared__ double red[NT / 64];
-    if (ctl->done) return;
+    if (cld(&ctl->done)) return;
     double rr = reduce_partials(P, nP, red);
-    if (rr < ctl->tol) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) { ctl->done = 1; ctl->iters = it + 1; }
+    if (rr < cld(&ctl->tol)) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) { cst(&ctl->done, 1); cst(&ctl->iters, it + 1); }
         return;
     }
     double rz = reduce_partials(P + MAXP, nP, red);
-    double be = rz / hist[it];
-    if (blockIdx.x == 0 && threadIdx.x == 0) hist[it + 1] = rz;
+    double be = rz / cld(&hist[it]);
+    if (blockIdx.x == 0 && threadIdx.x == 0) cst(&hist[it + 1], rz);
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < V) d[i] = zv[i] + be * d[i];
 }
