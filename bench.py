@@ -94,6 +94,52 @@ def run_requests(api, meshes, reqs, precision, workers, mem_budget=0, rate=None,
     return out, wall
 
 
+def mode_precision(args, api, W):
+    """Mixed form against the plain FP64 form on one workload: relative
+    errors of the final displacement, the concentration history, the
+    objective and its gradient (config c3: the 2D ring of c1 with the kernel
+    on every landmark as control points)."""
+    steps = args.steps if args.steps is not None else 20
+    if args.config == "c3":
+        prob = W.ring_problem()
+        prob["ctlIdx"] = np.arange(prob["q0"].shape[1], dtype=np.int32)
+        desc = "2D concentric-ring disk, 10 rings x 200 nodes, Matern nu=4 on every landmark (control points = landmarks)"
+    else:
+        prob, desc = workload(args.config, steps)
+    prob["params"] = dict(prob["params"], timeStp=1.0 / steps)
+    prob["timeNum"] = steps + 1
+    out = {}
+    for prec in ("fp64", "mixed"):
+        s = api.Session(prob, precision=prec)
+        t0 = time.perf_counter()
+        r = s.run(prob["tauVar"][:, 0], prob["tauPrm"], want_traj=True, want_grad=True)
+        r["wall"] = time.perf_counter() - t0
+        s.close()
+        out[prec] = r
+        print(f"[bench] {prec}: {r['steps']} steps {r['wall']:.3f}s obj={r['obj']:.15e} iters={r['pcg_iters']} "
+              f"sweeps={r['refine']} fp64_steps={r['fp64_steps']} status={r['status']}", file=sys.stderr, flush=True)
+    a, m = out["fp64"], out["mixed"]
+    disp = np.linalg.norm(a["lmkStk"][-1] - a["lmkStk"][0])
+    errs = {
+        "displacement": float(np.linalg.norm(m["lmkStk"][-1] - a["lmkStk"][-1]) / disp),
+        "concentration": float(np.linalg.norm(m["tauMat"] - a["tauMat"]) / np.linalg.norm(a["tauMat"])),
+        "objective": float(abs(m["obj"] - a["obj"]) / abs(a["obj"])),
+        "gradient": float(np.linalg.norm(m["grad"] - a["grad"]) / np.linalg.norm(a["grad"])),
+    }
+    return {
+        "metric": "relative error of the mixed form vs the plain FP64 form",
+        "value": max(errs.values()), "unit": "max relative error", "n_gpus": 1, "steps": steps, "warmup": 0,
+        "ms_per_step": round(1e3 * m["wall"] / max(m["steps"], 1), 3), "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64+f32", "data": "synthetic",
+        "config": {"workload": desc, "config": args.config, "landmarks": int(prob["q0"].shape[1]),
+                   "control_points": int(len(prob.get("ctlIdx", [])))},
+        "rel_err": errs,
+        "mixed": {"steps_per_s": round(m["steps"] / m["wall"], 3), "pcg_iters": m["pcg_iters"],
+                  "refine_sweeps": m["refine"], "fp64_steps": m["fp64_steps"]},
+        "fp64": {"steps_per_s": round(a["steps"] / a["wall"], 3), "pcg_iters": a["pcg_iters"]},
+    }
+
+
 def mode_batch(args, api, W):
     """64 independent meshes of uneven size (half 2D rings, half 3D shells),
     each with its own parameters, 20 steps, run concurrently through the
@@ -193,8 +239,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true",
                     help="time without the per-kernel events (graph-captured CG iterations); no roofline")
-    ap.add_argument("--cpu-iters", type=int, default=1)
-    ap.add_argument("--mode", default="single", choices=["single", "batch", "serve"],
+    ap.add_argument("--cpu-iters", type=int, default=0, help="CG iterations of the CPU sample (0: sized to the workload)")
+    ap.add_argument("--mode", default="single", choices=["single", "batch", "serve", "precision"],
                     help="single: one solve of --config; batch: 64 uneven meshes concurrently; serve: Poisson stream")
     ap.add_argument("--workers", type=int, default=8)
     ap.add_argument("--batch-n", type=int, default=64)
@@ -222,7 +268,7 @@ def main():
     from shape_atrophy_amd import api
     if args.mode != "single":
         from shape_atrophy_amd import workloads as W
-        res = (mode_batch if args.mode == "batch" else mode_serve)(args, api, W)
+        res = {"batch": mode_batch, "serve": mode_serve, "precision": mode_precision}[args.mode](args, api, W)
         print(json.dumps(res))
         return
     steps = args.steps if args.steps is not None else 5
@@ -276,6 +322,16 @@ def main():
                     "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": TRAFFIC.get((args.config, args.precision)),
                     "avg_ms": round(avg, 4), "launches": kmv_n,
                     "work": f"{esz} B x {V} x {nc} stored kernel entries + 8 B x {d} x ({V} + {nc}) vectors per launch"}
+        elif 8 * ((V + 127) // 128 * 128) * V <= (48 << 30):
+            # all landmarks, K stored per step (FP64, panel-major): each launch
+            # (K r plus the slice sum) reads K once
+            B = 8 * V * V + 8 * d * 3 * V
+            ach = B / (avg * 1e-3) / 1e9
+            roof = {"bound": "hbm", "kernel": f"kstore_mv_k<{d}, double> + kfin_k (stored K)",
+                    "achieved": round(ach, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                    "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": TRAFFIC.get((args.config, args.precision)),
+                    "avg_ms": round(avg, 4), "launches": kmv_n,
+                    "work": f"8 B x {V} x {V} stored kernel entries + 8 B x {d} x 3 x {V} vectors per product"}
         else:
             pairs = V * V
             flops = FLOP_PER_PAIR[order] * pairs
@@ -299,7 +355,9 @@ def main():
         import oracle as O
         O.build()
         threads = min(os.cpu_count() or 1, 16)
-        smp = O.pcg_sample(prob, args.cpu_iters, threads=threads)
+        pairs = V * (nc if nc else V)
+        n_it = args.cpu_iters or int(np.clip(5e8 / pairs, 1, 200))
+        smp = O.pcg_sample(prob, n_it, threads=threads)
         its_per_step = sum(o["pcg_iters"] for o in runs) / max(total_steps, 1)
         cpu_step = smp["setup_s"] + smp["cg_start_s"] + its_per_step * smp["per_iter_s"]
         cpu = {"value": round(1.0 / cpu_step, 6), "unit": "steps/s", "cores": threads, "kind": "port",

@@ -184,6 +184,8 @@ struct Engine : EngineBase {
     // control-point form
     DBuf<double> qCur, tauCur, qc, Kcc, Kc, Mpre, Mcol, cA, cA0, cG, cR, cZ, cD, cS, cY, cK, cRout, cDelta, cZero, part, partS;
     DBuf<double> A64, At64;      // stored K_xc / K_cx, FP64
+    DBuf<double> Kst;            // all-landmark form: stored K (V x V, panel-major FP64), rebuilt per step
+    int slicesK = 1;
     DBuf<float> A32, At32;       // stored K_xc / K_cx, FP32 (mixed precision)
     int slicesA = 1, slicesT = 1, slicesM = 1;
     DBuf<int> potrfInfo;
@@ -470,6 +472,17 @@ struct Engine : EngineBase {
                 throw SaError(SA_E_DEVICE, "rocblas_set_atomics_mode failed");
             require(gridOf((long)nc) <= MAXP, "too many control points for the partial-sum buffers");
         }
+        if (NC == 0) {
+            // all-landmark form: K stored per step when it fits (21 GB at V = 51,210)
+            const size_t nk = padded_rows<double>(V) * (size_t)V;
+            if (nk * sizeof(double) <= (size_t)48 << 30) {
+                Kst.alloc(nk, tot);
+                CK(hipMemsetAsync(Kst.p, 0, sizeof(double) * nk, s));
+                const int rbK = gridOf(V, panel_rows<double>());
+                slicesK = std::max(gridOf(V, KS_MAX_COLS), std::min(256, (2048 + rbK - 1) / rbK));
+                partS.alloc((size_t)slicesK * D * V, tot);
+            }
+        }
         require(gridOf(n) <= MAXP && gridOf(V, MV_ROWS) <= MAXP, "mesh too large for the partial-sum buffers");
         CK(hipStreamSynchronize(s));
     }
@@ -477,6 +490,16 @@ struct Engine : EngineBase {
     // ---------------- kernel-order dispatch
     template <bool CHECK>
     void kmatvec(const double *qq, const double *xin, double *yout, const double *Prr, int nPrr, int it) {
+        if (Kst.p) {
+            // stored K (built from qq by kst_build this step): HBM-bound product
+            dim3 g(gridOf(V, panel_rows<double>()), slicesK);
+            kstore_mv_k<D, double><<<g, NT, ks_lds(V, slicesK), s>>>(V, V, Kst.p, xin, partS.p, ctl.p);
+            CHECK_LAUNCH();
+            kfin_k<D, CHECK><<<gridOf(V), NT, 0, s>>>(V, slicesK, partS.p, xin, yout, knlEps, Prr, nPrr, P3.p, it,
+                                                      ctl.p);
+            CHECK_LAUNCH();
+            return;
+        }
         double invW = 1.0 / knlWidth;
         int g = gridOf(V, MV_ROWS);
         Ctl *c = ctl.p;
@@ -490,6 +513,26 @@ struct Engine : EngineBase {
             default: SA_MV(4); break;
         }
 #undef SA_MV
+        CHECK_LAUNCH();
+    }
+
+    // partial sums the kernel product leaves in P3 (consumed by pcg_dir_k)
+    int kmv_parts() const { return Kst.p ? gridOf(V) : gridOf(V, MV_ROWS); }
+
+    // stored K = k(|q_i - q_j|) (without eps) for the all-landmark form
+    void kst_build(const double *qq) {
+        const double invW = 1.0 / knlWidth;
+        dim3 g(gridOf(V, 64), gridOf(V, 64));
+#define SA_KS(O) kstore_build_k<D, O, double><<<g, NT, 0, s>>>(V, qq, V, qq, invW, Kst.p, nullptr)
+        switch (knlOrder) {
+            case -1: SA_KS(-1); break;
+            case 0: SA_KS(0); break;
+            case 1: SA_KS(1); break;
+            case 2: SA_KS(2); break;
+            case 3: SA_KS(3); break;
+            default: SA_KS(4); break;
+        }
+#undef SA_KS
         CHECK_LAUNCH();
     }
 
@@ -522,7 +565,8 @@ struct Engine : EngineBase {
     // velocity solve of one step; returns 0 ok, 1 failure
     int velocity(const double *q, const double *tauNow, const double *geoNow, bool warm, int &iters) {
         const int n = V * D;
-        const int gV = gridOf(V), gN = gridOf(n), gM = gridOf(V, MV_ROWS);
+        const int gV = gridOf(V), gN = gridOf(n), gM = kmv_parts();
+        if (Kst.p) kst_build(q);
         // f and |f|^2
         yank_elem_k<D><<<gridOf(E), NT, 0, s>>>(E, elmVtx.p, geoNow, ya.p, frc.p);
         CHECK_LAUNCH();
@@ -1128,6 +1172,7 @@ struct Engine : EngineBase {
                 break;
             case SA_PROBE_KMATVEC:
                 CK(hipMemcpyAsync(r.p, in, sizeof(double) * n, hipMemcpyHostToDevice, s));
+                if (Kst.p) kst_build(q0.p);
                 kmatvec<false>(q0.p, r.p, Kr.p, nullptr, 0, -1);
                 CK(hipMemcpyAsync(out, Kr.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
                 break;

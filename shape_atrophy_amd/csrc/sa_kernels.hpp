@@ -735,7 +735,7 @@ __global__ __launch_bounds__(NT) void kstore_build_k(int V, const double *__rest
     const int lj = threadIdx.x & 63, li0 = threadIdx.x >> 6;
     for (int l = li0; l < 64; l += 4) {
         int ii = i0 + l, jj = j0 + lj;
-        if (ii < V && jj < C) At[panel_at(jj, ii, PR, V)] = tile[lj][l];
+        if (At && ii < V && jj < C) At[panel_at(jj, ii, PR, V)] = tile[lj][l];
     }
 }
 
@@ -830,6 +830,42 @@ __global__ __launch_bounds__(NT) void kstore_mv_k(int nr, int nc, const T *__res
                 for (int w = 0; w < W - 1; ++w) sum += red[w][c][lane * R + k];
                 if (r0 + k < nr) po[c * nr + r0 + k] = sum;
             }
+    }
+}
+
+// all-landmark form: y = (K + eps I) x from the slice partials of the stored
+// K product (summed in slice order), and with CHECK the kmatvec_k contract:
+// stop when |r|^2 (partials Prr) is below the tolerance, else partials of x.y
+template <int D, bool CHECK>
+__global__ __launch_bounds__(NT) void kfin_k(int V, int slices, const double *__restrict__ part,
+                                             const double *__restrict__ x, double *__restrict__ y, double eps,
+                                             const double *__restrict__ Prr, int nPrr, double *__restrict__ P3,
+                                             int it, Ctl *ctl) {
+    __shared__ double red[NT / 64];
+    if (ctl && cld(&ctl->done)) return;
+    if constexpr (CHECK) {
+        double rr = reduce_partials(Prr, nPrr, red);
+        if (rr < cld(&ctl->tol)) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) { cst(&ctl->done, 1); cst(&ctl->iters, it + 1); }
+            return;
+        }
+    }
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double dotp = 0.0;
+    if (i < V) {
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            double s = 0.0;
+            for (int k = 0; k < slices; ++k) s += part[(size_t)k * D * V + (size_t)c * V + i];
+            const double xi = x[c * V + i];
+            s += eps * xi;
+            y[c * V + i] = s;
+            dotp += xi * s;
+        }
+    }
+    if constexpr (CHECK) {
+        double t = block_sum(dotp, red);
+        if (threadIdx.x == 0) P3[blockIdx.x] = t;
     }
 }
 
