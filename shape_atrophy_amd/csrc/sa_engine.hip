@@ -24,6 +24,8 @@
 #include <string>
 #include <mutex>
 #include <vector>
+#include <type_traits>
+#include <cstdlib>
 
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
@@ -74,6 +76,18 @@ struct DBuf {
             throw SaError(SA_E_ALLOC, "Fail to allocate device memory (" + std::to_string(sizeof(T) * count) + " bytes).");
         }
         total += (long long)(sizeof(T) * count);
+        // SA_POISON_ALLOC=1: floating-point buffers start as NaN, so a read
+        // of memory no kernel has written shows up in the results
+        if constexpr (std::is_floating_point<T>::value) {
+            static const bool poison = [] {
+                const char *e = std::getenv("SA_POISON_ALLOC");
+                return e && e[0] == '1';
+            }();
+            if (poison) {
+                CK(hipMemset(p, 0xff, sizeof(T) * count));
+                CK(hipDeviceSynchronize());   // before any upload on a session stream
+            }
+        }
     }
     void upload(const T *h, size_t count, hipStream_t s) {
         CK(hipMemcpyAsync(p, h, sizeof(T) * count, hipMemcpyHostToDevice, s));
