@@ -26,7 +26,7 @@ import os
 import numpy as np
 
 _PKG = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_PKG, "libsa_hip.so")
+LIB_PATH = os.environ.get("SA_HIP_LIB") or os.path.join(_PKG, "libsa_hip.so")   # SA_HIP_LIB: another build of the library
 _lib = None
 
 _P = C.POINTER

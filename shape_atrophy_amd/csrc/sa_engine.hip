@@ -466,13 +466,13 @@ struct Engine : EngineBase {
                     CK(hipMemsetAsync(At64.p, 0, sizeof(double) * nt, s));
                 }
             }
-            // about 1024 blocks per stored product (row blocks x column slices)
+            // about KS_BLOCKS blocks per stored product (row blocks x column slices)
             const int prS = precision == SA_PREC_MIXED ? panel_rows<float>() : panel_rows<double>();
             const int rbA = gridOf(V, prS), rbT = gridOf(NC, prS), rbM = gridOf(NC, panel_rows<double>());
             // (and at least enough slices for the LDS copy of x: KS_MAX_COLS columns)
-            slicesA = std::max(gridOf(NC, KS_MAX_COLS), std::min(64, (1024 + rbA - 1) / rbA));
-            slicesT = std::max(gridOf(V, KS_MAX_COLS), std::min(256, (1024 + rbT - 1) / rbT));
-            slicesM = std::max(gridOf(NC, KS_MAX_COLS), std::min(256, (1024 + rbM - 1) / rbM));
+            slicesA = std::max(gridOf(NC, KS_MAX_COLS), std::min(64, (KS_BLOCKS + rbA - 1) / rbA));
+            slicesT = std::max(gridOf(V, KS_MAX_COLS), std::min(256, (KS_BLOCKS + rbT - 1) / rbT));
+            slicesM = std::max(gridOf(NC, KS_MAX_COLS), std::min(256, (KS_BLOCKS + rbM - 1) / rbM));
             partS.alloc(std::max({(size_t)slicesA * D * V, (size_t)slicesT * nc, (size_t)slicesM * nc}), tot);
             int rb = gridOf(NC, MV_ROWS);
             slicesC = std::max(1, std::min(32, (1024 + rb - 1) / rb));

@@ -662,8 +662,11 @@ __global__ __launch_bounds__(NT) void slice_sum_k(int n, int slices, const doubl
 // a panel's columns follow one another, each holding the panel's rows, so a
 // wavefront streaming consecutive columns reads one contiguous range.  The
 // row count is padded to whole panels (padding rows hold zeros).
+#ifndef SA_PANEL_BYTES
+#define SA_PANEL_BYTES 1024
+#endif
 template <typename T>
-__host__ __device__ constexpr int panel_rows() { return 1024 / (int)sizeof(T); }
+__host__ __device__ constexpr int panel_rows() { return SA_PANEL_BYTES / (int)sizeof(T); }
 
 __host__ __device__ inline size_t panel_at(int i, int j, int PR, int ncols) {
     return (size_t)(i / PR) * PR * ncols + (size_t)j * PR + (i % PR);
@@ -749,16 +752,36 @@ __global__ __launch_bounds__(NT) void kstore_build_k(int V, const double *__rest
 // order through LDS at the end.  The block's partial goes to
 // part + s * D * nr; slice_rows_k sums the slices in slice order.
 constexpr int KS_MAX_COLS = 1024;
+#ifndef SA_KS_BLOCKS
+#define SA_KS_BLOCKS 1024
+#endif
+constexpr int KS_BLOCKS = SA_KS_BLOCKS;   // target blocks per stored product
+
+// a[r0 + k][c] += m.k * xv for the elements of one 16-byte load
+template <int R, int D>
+__device__ inline void fma_rows(double (&a)[R][D], int r0, int c, const float4 &m, double xv) {
+    a[r0][c] += (double)m.x * xv; a[r0 + 1][c] += (double)m.y * xv;
+    a[r0 + 2][c] += (double)m.z * xv; a[r0 + 3][c] += (double)m.w * xv;
+}
+template <int R, int D>
+__device__ inline void fma_rows(double (&a)[R][D], int r0, int c, const double2 &m, double xv) {
+    a[r0][c] += m.x * xv; a[r0 + 1][c] += m.y * xv;
+}
 
 template <int D, typename T>
 __global__ __launch_bounds__(NT) void kstore_mv_k(int nr, int nc, const T *__restrict__ M,
                                                   const double *__restrict__ x, double *__restrict__ part,
                                                   const Ctl *ctl) {
     constexpr int PR = panel_rows<T>();
-    constexpr int R = PR / 64;
+    constexpr int R = PR / 64;             // rows per lane
     constexpr int W = NT / 64;
-    constexpr int U = 8;   // columns whose loads are issued together
+#ifndef SA_KS_U
+#define SA_KS_U 4
+#endif
+    constexpr int U = SA_KS_U;             // columns whose loads are issued together
     using V4 = typename std::conditional<sizeof(T) == 4, float4, double2>::type;
+    constexpr int NV = R * (int)sizeof(T) / 16;   // 16-byte loads per lane and column
+    constexpr int EV = 16 / (int)sizeof(T);       // elements per 16-byte load
     extern __shared__ double sx[];   // D x (je - jb)
     __shared__ double red[W - 1][D][PR];
     if (ctl && cld(&ctl->done)) return;
@@ -782,33 +805,29 @@ __global__ __launch_bounds__(NT) void kstore_mv_k(int nr, int nc, const T *__res
     const T *col = M + (size_t)blockIdx.x * PR * nc + (size_t)(jb + wb) * PR + R * lane;
     int j = wb;
     for (; j + U <= we; j += U, col += U * PR) {
-        V4 m[U];
+        V4 m[U][NV];
 #pragma unroll
-        for (int u = 0; u < U; ++u) m[u] = *reinterpret_cast<const V4 *>(col + u * PR);
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int v = 0; v < NV; ++v) m[u][v] = *reinterpret_cast<const V4 *>(col + u * PR + v * EV);
 #pragma unroll
         for (int u = 0; u < U; ++u)
 #pragma unroll
             for (int c = 0; c < D; ++c) {
                 const double xv = sx[c * ns + j + u];
-                if constexpr (sizeof(T) == 4) {
-                    a[0][c] += (double)m[u].x * xv; a[1][c] += (double)m[u].y * xv;
-                    a[2][c] += (double)m[u].z * xv; a[3][c] += (double)m[u].w * xv;
-                } else {
-                    a[0][c] += m[u].x * xv; a[1][c] += m[u].y * xv;
-                }
+#pragma unroll
+                for (int v = 0; v < NV; ++v) fma_rows(a, v * EV, c, m[u][v], xv);
             }
     }
     for (; j < we; ++j, col += PR) {
-        V4 m = *reinterpret_cast<const V4 *>(col);
+        V4 m[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) m[v] = *reinterpret_cast<const V4 *>(col + v * EV);
 #pragma unroll
         for (int c = 0; c < D; ++c) {
             const double xv = sx[c * ns + j];
-            if constexpr (sizeof(T) == 4) {
-                a[0][c] += (double)m.x * xv; a[1][c] += (double)m.y * xv;
-                a[2][c] += (double)m.z * xv; a[3][c] += (double)m.w * xv;
-            } else {
-                a[0][c] += m.x * xv; a[1][c] += m.y * xv;
-            }
+#pragma unroll
+            for (int v = 0; v < NV; ++v) fma_rows(a, v * EV, c, m[v], xv);
         }
     }
     // wavefront order: ((w0 + w1) + w2) + w3
