@@ -196,7 +196,7 @@ struct Engine : EngineBase {
     DBuf<double> dCen, dDir, dVol, rowVal, gC, gN, grad, scal;
     DBuf<Ctl> ctl, rdCtl;
     // control-point form
-    DBuf<double> qCur, tauCur, qc, Kcc, Kc, Mpre, Mcol, cA, cA0, cG, cR, cZ, cD, cS, cY, cK, cRout, cDelta, cZero, part, partS;
+    DBuf<double> qCur, tauCur, qc, Kcc, Kc, Mpre, Mcol, cA, cA0, cAm1, cG, cR, cZ, cD, cS, cY, cK, cRout, cDelta, cZero, part, partS;
     DBuf<double> A64, At64;      // stored K_xc / K_cx, FP64
     DBuf<double> Kst;            // all-landmark form: stored K (V x V, panel-major FP64), rebuilt per step
     int slicesK = 1;
@@ -435,7 +435,7 @@ struct Engine : EngineBase {
             ctlIdx.alloc(NC, tot); ctlIdx.upload(hCtl.data(), NC, s);
             qc.alloc(nc, tot); Kcc.alloc((size_t)NC * NC, tot);
             qCur.alloc((size_t)V * D, tot); tauCur.alloc(V, tot);
-            cA.alloc(nc, tot); cA0.alloc(nc, tot); cG.alloc(nc, tot); cR.alloc(nc, tot); cZ.alloc(nc, tot); cD.alloc(nc, tot);
+            cA.alloc(nc, tot); cA0.alloc(nc, tot); cAm1.alloc(nc, tot); cG.alloc(nc, tot); cR.alloc(nc, tot); cZ.alloc(nc, tot); cD.alloc(nc, tot);
             cS.alloc(nc, tot); cY.alloc(nc, tot); cK.alloc(nc, tot);
             cRout.alloc(nc, tot); cDelta.alloc(nc, tot); cZero.alloc(nc, tot);
             CK(hipMemsetAsync(cZero.p, 0, sizeof(double) * nc, s));
@@ -1155,7 +1155,10 @@ struct Engine : EngineBase {
         CHECK_LAUNCH();
         CK(hipMemsetAsync(x.p, 0, sizeof(double) * n, s));
         CK(hipMemsetAsync(a.p, 0, sizeof(double) * n, s));
-        if (NC > 0) CK(hipMemsetAsync(cA.p, 0, sizeof(double) * NC * D, s));
+        if (NC > 0) {
+            CK(hipMemsetAsync(cA.p, 0, sizeof(double) * NC * D, s));
+            CK(hipMemsetAsync(cAm1.p, 0, sizeof(double) * NC * D, s));
+        }
         lastIters = 0;
         precAge = -1;
         fp64Steps = 0;
@@ -1242,6 +1245,11 @@ struct Engine : EngineBase {
             }
             int its = 0;
             int sweeps = 0;
+            if (NC > 0 && t >= 1) {
+                // initial guess: alpha_{t-1} extrapolated linearly from alpha_{t-2}
+                extrap_k<<<gridOf((long)NC * D), NT, 0, s>>>(NC * D, t >= 2 ? 1 : 0, cA.p, cAm1.p);
+                CHECK_LAUNCH();
+            }
             int vs = NC > 0 ? velocity_ctl(q, tn, geo[cur].p, its, sweeps) : velocity(q, tn, geo[cur].p, t > 0, its);
             st.refine += sweeps;
             st.fp64Steps = fp64Steps;

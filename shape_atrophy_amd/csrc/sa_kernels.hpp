@@ -1018,6 +1018,16 @@ __global__ __launch_bounds__(NT) void ctl_dir_k(int n, const double *__restrict_
     if (i < n) d[i] = z[i] + be * d[i];
 }
 
+// control-point form, start of a step: prev <- a and, with lin, a <- 2 a - prev
+// (the previous two solutions extrapolated to the new step)
+__global__ void extrap_k(int n, int lin, double *__restrict__ a, double *__restrict__ prev) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double ai = a[i];
+    if (lin) a[i] = 2.0 * ai - prev[i];
+    prev[i] = ai;
+}
+
 __global__ void axpby_k(int n, double *__restrict__ out, double a, const double *__restrict__ x, double b,
                         const double *__restrict__ y) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
