@@ -1018,14 +1018,18 @@ __global__ __launch_bounds__(NT) void ctl_dir_k(int n, const double *__restrict_
     if (i < n) d[i] = z[i] + be * d[i];
 }
 
-// control-point form, start of a step: prev <- a and, with lin, a <- 2 a - prev
-// (the previous two solutions extrapolated to the new step)
-__global__ void extrap_k(int n, int lin, double *__restrict__ a, double *__restrict__ prev) {
+// control-point form, start of a step: shift the solution history
+// (p2 <- p1 <- a) and extrapolate a to the new step with the given order:
+// 0 keeps a, 1: 2 a - p1, 2: 3 a - 3 p1 + p2
+__global__ void extrap_k(int n, int order, double *__restrict__ a, double *__restrict__ p1,
+                         double *__restrict__ p2) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const double ai = a[i];
-    if (lin) a[i] = 2.0 * ai - prev[i];
-    prev[i] = ai;
+    const double a0 = a[i], a1 = p1[i], a2 = p2[i];
+    if (order == 1) a[i] = 2.0 * a0 - a1;
+    else if (order >= 2) a[i] = 3.0 * (a0 - a1) + a2;
+    p2[i] = a1;
+    p1[i] = a0;
 }
 
 __global__ void axpby_k(int n, double *__restrict__ out, double a, const double *__restrict__ x, double b,
