@@ -2,7 +2,7 @@
 MI355X, with the roofline of its dominant kernel and the FP64 CPU oracle
 timed beside it.
 
-    python bench.py [--steps K] [--warmup W] [--repeats R] [--config c2|c2-all|c1]
+    python bench.py [--steps K (50)] [--warmup W] [--repeats R] [--config c2|c2-all|c1]
                     [--precision fp64|mixed] [--no-cpu]
 
 A "step" is one evolution time step (velocity solve, landmark update,
@@ -271,7 +271,7 @@ def main():
         res = {"batch": mode_batch, "serve": mode_serve, "precision": mode_precision}[args.mode](args, api, W)
         print(json.dumps(res))
         return
-    steps = args.steps if args.steps is not None else 5
+    steps = args.steps if args.steps is not None else 50   # the configured solve: 50 time steps
     prob, desc = workload(args.config, steps)
     d, V = prob["q0"].shape
     E = prob["elm"]["elmVtxMat"].shape[1]

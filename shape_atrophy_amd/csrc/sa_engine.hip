@@ -43,7 +43,7 @@ thread_local std::string g_err;
 std::mutex g_libMu;
 
 #ifndef SA_EXTRAP
-#define SA_EXTRAP 2   // order of the initial-guess extrapolation across steps
+#define SA_EXTRAP 3   // order of the initial-guess extrapolation across steps
 #endif
 
 struct SaError : std::runtime_error {
@@ -200,7 +200,8 @@ struct Engine : EngineBase {
     DBuf<double> dCen, dDir, dVol, rowVal, gC, gN, grad, scal;
     DBuf<Ctl> ctl, rdCtl;
     // control-point form
-    DBuf<double> qCur, tauCur, qc, Kcc, Kc, Mpre, Mcol, cA, cA0, cAm1, cAm2, cG, cR, cZ, cD, cS, cY, cK, cRout, cDelta, cZero, part, partS;
+    DBuf<double> qCur, tauCur, qc, Kcc, Kc, Mpre, Mcol, cG, cR, cZ, cD, cS, cY, cK, cRout, cDelta, cZero, part, partS;
+    DBuf<double> cA, cA0, cAm1, cAm2, cAm3;   // alpha, its value entering a step, the last three solutions
     DBuf<double> A64, At64;      // stored K_xc / K_cx, FP64
     DBuf<double> Kst;            // all-landmark form: stored K (V x V, panel-major FP64), rebuilt per step
     int slicesK = 1;
@@ -439,7 +440,8 @@ struct Engine : EngineBase {
             ctlIdx.alloc(NC, tot); ctlIdx.upload(hCtl.data(), NC, s);
             qc.alloc(nc, tot); Kcc.alloc((size_t)NC * NC, tot);
             qCur.alloc((size_t)V * D, tot); tauCur.alloc(V, tot);
-            cA.alloc(nc, tot); cA0.alloc(nc, tot); cAm1.alloc(nc, tot); cAm2.alloc(nc, tot); cG.alloc(nc, tot); cR.alloc(nc, tot); cZ.alloc(nc, tot); cD.alloc(nc, tot);
+            cA.alloc(nc, tot); cA0.alloc(nc, tot); cAm1.alloc(nc, tot); cAm2.alloc(nc, tot); cAm3.alloc(nc, tot);
+            cG.alloc(nc, tot); cR.alloc(nc, tot); cZ.alloc(nc, tot); cD.alloc(nc, tot);
             cS.alloc(nc, tot); cY.alloc(nc, tot); cK.alloc(nc, tot);
             cRout.alloc(nc, tot); cDelta.alloc(nc, tot); cZero.alloc(nc, tot);
             CK(hipMemsetAsync(cZero.p, 0, sizeof(double) * nc, s));
@@ -1163,6 +1165,7 @@ struct Engine : EngineBase {
             CK(hipMemsetAsync(cA.p, 0, sizeof(double) * NC * D, s));
             CK(hipMemsetAsync(cAm1.p, 0, sizeof(double) * NC * D, s));
             CK(hipMemsetAsync(cAm2.p, 0, sizeof(double) * NC * D, s));
+            CK(hipMemsetAsync(cAm3.p, 0, sizeof(double) * NC * D, s));
         }
         lastIters = 0;
         precAge = -1;
@@ -1253,7 +1256,8 @@ struct Engine : EngineBase {
             if (NC > 0 && t >= 1) {
                 // initial guess: the previous solutions extrapolated to this
                 // step (linear from two, quadratic from three)
-                extrap_k<<<gridOf((long)NC * D), NT, 0, s>>>(NC * D, std::min(t, SA_EXTRAP), cA.p, cAm1.p, cAm2.p);
+                extrap_k<<<gridOf((long)NC * D), NT, 0, s>>>(NC * D, std::min(t, SA_EXTRAP), cA.p, cAm1.p, cAm2.p,
+                                                             cAm3.p);
                 CHECK_LAUNCH();
             }
             int vs = NC > 0 ? velocity_ctl(q, tn, geo[cur].p, its, sweeps) : velocity(q, tn, geo[cur].p, t > 0, its);

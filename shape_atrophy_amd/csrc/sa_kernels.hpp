@@ -1019,15 +1019,18 @@ __global__ __launch_bounds__(NT) void ctl_dir_k(int n, const double *__restrict_
 }
 
 // control-point form, start of a step: shift the solution history
-// (p2 <- p1 <- a) and extrapolate a to the new step with the given order:
-// 0 keeps a, 1: 2 a - p1, 2: 3 a - 3 p1 + p2
+// (p3 <- p2 <- p1 <- a) and extrapolate a to the new step with the given
+// order (polynomial through the last order + 1 solutions): 0 keeps a,
+// 1: 2 a - p1, 2: 3 a - 3 p1 + p2, 3: 4 a - 6 p1 + 4 p2 - p3
 __global__ void extrap_k(int n, int order, double *__restrict__ a, double *__restrict__ p1,
-                         double *__restrict__ p2) {
+                         double *__restrict__ p2, double *__restrict__ p3) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const double a0 = a[i], a1 = p1[i], a2 = p2[i];
+    const double a0 = a[i], a1 = p1[i], a2 = p2[i], a3 = p3[i];
     if (order == 1) a[i] = 2.0 * a0 - a1;
-    else if (order >= 2) a[i] = 3.0 * (a0 - a1) + a2;
+    else if (order == 2) a[i] = 3.0 * (a0 - a1) + a2;
+    else if (order >= 3) a[i] = 4.0 * (a0 + a2) - 6.0 * a1 - a3;
+    p3[i] = a2;
     p2[i] = a1;
     p1[i] = a0;
 }
