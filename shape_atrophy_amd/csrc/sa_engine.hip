@@ -42,6 +42,9 @@ thread_local std::string g_err;
 // completion on its stream before the next starts)
 std::mutex g_libMu;
 
+#ifndef SA_MIXED_CC32
+#define SA_MIXED_CC32 1   // mixed form: K_cc from the FP32 stored rows, FP32 preconditioner
+#endif
 #ifndef SA_EXTRAP
 #define SA_EXTRAP 3   // order of the initial-guess extrapolation across steps
 #endif
@@ -206,6 +209,7 @@ struct Engine : EngineBase {
     DBuf<double> Kst;            // all-landmark form: stored K (V x V, panel-major FP64), rebuilt per step
     int slicesK = 1;
     DBuf<float> A32, At32;       // stored K_xc / K_cx, FP32 (mixed precision)
+    DBuf<float> Mpre32;          // FP32 panel copy of the preconditioner (mixed, SA_MIXED_CC32)
     int slicesA = 1, slicesT = 1, slicesM = 1;
     DBuf<int> potrfInfo;
     rocblas_handle blas = nullptr;
@@ -453,11 +457,18 @@ struct Engine : EngineBase {
             Mcol.alloc((size_t)NC * NC, tot);
             Mpre.alloc(pc, tot);
             CK(hipMemsetAsync(Mpre.p, 0, sizeof(double) * pc, s));
-            if (precision == SA_PREC_MIXED) {
+            if (precision == SA_PREC_MIXED && !SA_MIXED_CC32) {
                 // FP64 K_cc for the FP32 inner solve: its FP32 rounding need
                 // not be positive definite when K_cc is ill conditioned
                 Kc.alloc(pc, tot);
                 CK(hipMemsetAsync(Kc.p, 0, sizeof(double) * pc, s));
+            }
+            if (precision == SA_PREC_MIXED && SA_MIXED_CC32) {
+                // FP32 copy of the preconditioner for the FP32 inner solve (a
+                // step whose FP32 operator is unfit falls back to FP64)
+                const size_t pf = padded_rows<float>(NC) * (size_t)NC;
+                Mpre32.alloc(pf, tot);
+                CK(hipMemsetAsync(Mpre32.p, 0, sizeof(float) * pf, s));
             }
             if (precision == SA_PREC_MIXED) {
                 const size_t na = padded_rows<float>(V) * (size_t)NC, nt = padded_rows<float>(NC) * (size_t)V;
@@ -710,6 +721,10 @@ struct Engine : EngineBase {
             throw SaError(SA_E_DEVICE, "rocblas_dgemm failed");
         panelize_k<<<g, NT, 0, s>>>(NC, Mcol.p, NC, Mpre.p);
         CHECK_LAUNCH();
+        if (Mpre32.p) {
+            panel_to_float_k<<<g, NT, 0, s>>>(NC, Mpre.p, Mpre32.p);
+            CHECK_LAUNCH();
+        }
         CK(hipStreamSynchronize(s));
     }
 
@@ -718,6 +733,10 @@ struct Engine : EngineBase {
 
     template <typename T>
     static size_t padded_rows(int n) { return (size_t)gridOf(n, panel_rows<T>()) * panel_rows<T>(); }
+
+    // kernel-matrix products of the control-point form: matrix-free FP64, or
+    // the stored matrices (see kstore_build_k) in FP64 or FP32
+    enum class KMode { MatrixFree64, Stored64, Stored32 };
 
     // y = M x for a dense FP64 C x C panel-major matrix
     void cc_product(const double *M, const double *x, double *y) {
@@ -730,10 +749,18 @@ struct Engine : EngineBase {
 
     // z = M r, M = X^T X ~ K_cc^{-1}, in FP64 in every form: an FP32 rounding
     // of M need not be positive definite
-    void prec_apply(const double *r, double *z) { cc_product(Mpre.p, r, z); }
+    void prec_apply(KMode md, const double *r, double *z) {
+        if (md == KMode::Stored32 && Mpre32.p) {
+            dim3 g(gridOf(NC, panel_rows<float>()), slicesM);
+            kstore_mv_k<D, float><<<g, NT, ks_lds(NC, slicesM), s>>>(NC, NC, Mpre32.p, r, partS.p, ctl.p);
+            CHECK_LAUNCH();
+            slice_rows_k<<<gridOf(D * NC, 64), NT, 0, s>>>(D * NC, slicesM, partS.p, z, ctl.p);
+            CHECK_LAUNCH();
+            return;
+        }
+        cc_product(Mpre.p, r, z);
+    }
 
-    // stored kernel matrices of the control-point form (see kstore_build_k)
-    enum class KMode { MatrixFree64, Stored64, Stored32 };
 
     void kstore_build(const double *q, bool fp64, bool fp32) {
         double invW = 1.0 / knlWidth;
@@ -806,7 +833,7 @@ struct Engine : EngineBase {
         gather_op<2>(nullptr, nullptr, Ad.p, nullptr, nullptr);                 // B w
         if (ev) CK(hipEventRecord(ev[3], s));
         kx_product(md, false, q, Ad.p, cY.p, nullptr, ctl.p, ev ? ev + 4 : nullptr);   // K_cx B w
-        if (md == KMode::Stored32) cc_product(Kc.p, u, cK.p);
+        if (md == KMode::Stored32 && Kc.p) cc_product(Kc.p, u, cK.p);
     }
 
     // one control-point CG iteration on S x = rhs (x = xs): S d, step,
@@ -815,13 +842,13 @@ struct Engine : EngineBase {
     // no-ops.  ev: optional per-iteration events (6).
     void cg_iter(KMode md, const double *q, const double *tauNow, const double *geoNow, double *xs, hipEvent_t *ev) {
         const int nc = NC * D, gC = gridOf(nc);
-        const double *kdv = md == KMode::Stored32 ? cK.p : nullptr;
+        const double *kdv = md == KMode::Stored32 && Kc.p ? cK.p : nullptr;
         S_apply(md, q, tauNow, geoNow, cD.p, ev);
         ctl_combine_k<D, 0><<<gC, NT, 0, s>>>(NC, V, cY.p, kdv, Kr.p, ctlIdx.p, cD.p, nullptr, ldmWgt, knlEps, cS.p,
                                              P1.p, ctl.p);
         ctl_update_k<<<gC, NT, 0, s>>>(nc, P1.p, gC, hist.p, xs, cR.p, cD.p, cS.p, P2.p, ctl.p);
         CHECK_LAUNCH();
-        prec_apply(cR.p, cZ.p);
+        prec_apply(md, cR.p, cZ.p);
         ctl_dot_k<<<gC, NT, 0, s>>>(nc, P2.p, gC, cR.p, cZ.p, P3.p, ctl.p);
         ctl_dir_k<<<gC, NT, 0, s>>>(nc, P3.p, gC, hist.p, cZ.p, cD.p, ctl.p);
         ctl_next_k<<<1, 1, 0, s>>>(ctl.p);
@@ -879,7 +906,7 @@ struct Engine : EngineBase {
     int ctl_pcg(KMode md, const double *q, const double *tauNow, const double *geoNow, const double *rhs, double *xs,
                 bool zeroStart, double tolSqu, int maxIt, int &iters) {
         const int nc = NC * D, gC = gridOf(nc);
-        const double *kdv = md == KMode::Stored32 ? cK.p : nullptr;   // K_cc d (else gathered from K_xc d)
+        const double *kdv = md == KMode::Stored32 && Kc.p ? cK.p : nullptr;   // K_cc d (else gathered from K_xc d)
         pcg_start_k<<<1, NT, 0, s>>>(P0.p, gC, tolSqu, maxIt, ctl.p);
         CHECK_LAUNCH();
         zero_if_empty_k<<<gC, NT, 0, s>>>(nc, xs, nullptr, ctl.p);
@@ -894,7 +921,7 @@ struct Engine : EngineBase {
                                                  P2.p, ctl.p);
         }
         CHECK_LAUNCH();
-        prec_apply(cR.p, cZ.p);
+        prec_apply(md, cR.p, cZ.p);
         ctl_dot_k<<<gC, NT, 0, s>>>(nc, P2.p, gC, cR.p, cZ.p, P3.p, ctl.p);
         ctl_dir_k<<<gC, NT, 0, s>>>(nc, P3.p, gC, hist.p, cZ.p, cD.p, ctl.p);
         ctl_next_k<<<1, 1, 0, s>>>(ctl.p);

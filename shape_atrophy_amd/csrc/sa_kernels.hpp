@@ -888,6 +888,14 @@ __global__ __launch_bounds__(NT) void kfin_k(int V, int slices, const double *__
     }
 }
 
+// FP64 panel-major n x n matrix -> FP32 panel-major copy (its own panel height)
+__global__ void panel_to_float_k(int n, const double *__restrict__ src, float *__restrict__ dst) {
+    long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)n * n) return;
+    int j = (int)(idx / n), i = (int)(idx - (long)j * n);
+    dst[panel_at(i, j, panel_rows<float>(), n)] = (float)src[panel_at(i, j, panel_rows<double>(), n)];
+}
+
 // column-major (leading dimension ld) n x n matrix -> panel-major copy
 __global__ void panelize_k(int n, const double *__restrict__ src, int ld, double *__restrict__ dst) {
     long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
