@@ -207,6 +207,7 @@ struct Engine : EngineBase {
     DBuf<double> cA, cA0, cAm1, cAm2, cAm3;   // alpha, its value entering a step, the last three solutions
     DBuf<double> A64, At64;      // stored K_xc / K_cx, FP64
     DBuf<double> Kst;            // all-landmark form: stored K (V x V, panel-major FP64), rebuilt per step
+    DBuf<double> aM1, aM2, aM3;  // all-landmark form: the last three alpha = K^-1 v
     int slicesK = 1;
     DBuf<float> A32, At32;       // stored K_xc / K_cx, FP32 (mixed precision)
     DBuf<float> Mpre32;          // FP32 panel copy of the preconditioner (mixed, SA_MIXED_CC32)
@@ -504,6 +505,7 @@ struct Engine : EngineBase {
             require(gridOf((long)nc) <= MAXP, "too many control points for the partial-sum buffers");
         }
         if (NC == 0) {
+            aM1.alloc(n, tot); aM2.alloc(n, tot); aM3.alloc(n, tot);
             // all-landmark form: K stored per step when it fits (21 GB at V = 51,210)
             const size_t nk = padded_rows<double>(V) * (size_t)V;
             if (nk * sizeof(double) <= (size_t)48 << 30) {
@@ -1188,6 +1190,11 @@ struct Engine : EngineBase {
         CHECK_LAUNCH();
         CK(hipMemsetAsync(x.p, 0, sizeof(double) * n, s));
         CK(hipMemsetAsync(a.p, 0, sizeof(double) * n, s));
+        if (NC == 0) {
+            CK(hipMemsetAsync(aM1.p, 0, sizeof(double) * n, s));
+            CK(hipMemsetAsync(aM2.p, 0, sizeof(double) * n, s));
+            CK(hipMemsetAsync(aM3.p, 0, sizeof(double) * n, s));
+        }
         if (NC > 0) {
             CK(hipMemsetAsync(cA.p, 0, sizeof(double) * NC * D, s));
             CK(hipMemsetAsync(cAm1.p, 0, sizeof(double) * NC * D, s));
@@ -1280,6 +1287,12 @@ struct Engine : EngineBase {
             }
             int its = 0;
             int sweeps = 0;
+            if (NC == 0 && t >= 1) {
+                // all-landmark form: alpha = K^-1 v extrapolated the same way;
+                // the step then starts from x0 = K alpha
+                extrap_k<<<gridOf((long)n), NT, 0, s>>>((int)n, std::min(t, SA_EXTRAP), a.p, aM1.p, aM2.p, aM3.p);
+                CHECK_LAUNCH();
+            }
             if (NC > 0 && t >= 1) {
                 // initial guess: the previous solutions extrapolated to this
                 // step (linear from two, quadratic from three)
