@@ -1066,10 +1066,14 @@ struct Engine : EngineBase {
                 // inner tolerance: relative 1e-5 of this residual, never below the outer target
                 double inner = std::max(1e-10, cgTolSqu * gg / rr);
                 chunkHint = 16;
+                // an inner solve may not run past 3x the previous step's
+                // iterations (+200; 1/8 of the budget on a run's first step):
+                // an FP32 operator that does not converge counts as a stall
+                const int capIn = lastIters > 0 ? 3 * lastIters + 200 : std::max(1000, itrMax / 8);
                 int st = ctl_solve(KMode::Stored32, q, tauNow, geoNow, cRout.p, cDelta.p, true, inner,
-                                 std::max(1, itrMax - iters), its);
+                                   std::max(1, std::min(itrMax - iters, capIn)), its);
                 iters += its;
-                if (st == 1) { stalled = true; break; }   // breakdown of the FP32 solve
+                if (st != 0) { stalled = true; break; }   // breakdown or no convergence of the FP32 solve
                 axpby_k<<<gC, NT, 0, s>>>(nc, cA.p, 1.0, cA.p, 1.0, cDelta.p);
                 CHECK_LAUNCH();
                 if (iters >= itrMax) break;
