@@ -757,6 +757,31 @@ constexpr int KS_MAX_COLS = 1024;
 #endif
 constexpr int KS_BLOCKS = SA_KS_BLOCKS;   // target blocks per stored product
 
+// one 16-byte load of a stored matrix, streamed once per product: the
+// non-temporal hint (SA_KS_NT, default on) keeps it from displacing cached
+// data; measured 0.64 -> 0.69 (FP32) and 0.72 -> 0.80 (FP64) of HBM peak
+#ifndef SA_KS_NT
+#define SA_KS_NT 1
+#endif
+typedef float ks_f4 __attribute__((ext_vector_type(4)));
+typedef double ks_d2 __attribute__((ext_vector_type(2)));
+__device__ inline float4 ks_load(const float4 *p) {
+#if SA_KS_NT
+    ks_f4 v = __builtin_nontemporal_load(reinterpret_cast<const ks_f4 *>(p));
+    return make_float4(v.x, v.y, v.z, v.w);
+#else
+    return *p;
+#endif
+}
+__device__ inline double2 ks_load(const double2 *p) {
+#if SA_KS_NT
+    ks_d2 v = __builtin_nontemporal_load(reinterpret_cast<const ks_d2 *>(p));
+    return make_double2(v.x, v.y);
+#else
+    return *p;
+#endif
+}
+
 // a[r0 + k][c] += m.k * xv for the elements of one 16-byte load
 template <int R, int D>
 __device__ inline void fma_rows(double (&a)[R][D], int r0, int c, const float4 &m, double xv) {
@@ -809,7 +834,7 @@ __global__ __launch_bounds__(NT) void kstore_mv_k(int nr, int nc, const T *__res
 #pragma unroll
         for (int u = 0; u < U; ++u)
 #pragma unroll
-            for (int v = 0; v < NV; ++v) m[u][v] = *reinterpret_cast<const V4 *>(col + u * PR + v * EV);
+            for (int v = 0; v < NV; ++v) m[u][v] = ks_load(reinterpret_cast<const V4 *>(col + u * PR + v * EV));
 #pragma unroll
         for (int u = 0; u < U; ++u)
 #pragma unroll
