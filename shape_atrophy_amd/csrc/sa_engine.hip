@@ -211,7 +211,7 @@ struct Engine : EngineBase {
     int slicesK = 1;
     DBuf<float> A32, At32;       // stored K_xc / K_cx, FP32 (mixed precision)
     DBuf<float> Mpre32;          // FP32 panel copy of the preconditioner (mixed, SA_MIXED_CC32)
-    int slicesA = 1, slicesT = 1, slicesM = 1;
+    int slicesA = 1, slicesT = 1, slicesM = 1, slicesM32 = 1;
     DBuf<int> potrfInfo;
     rocblas_handle blas = nullptr;
     int slicesC = 1;
@@ -488,10 +488,15 @@ struct Engine : EngineBase {
             const int prS = precision == SA_PREC_MIXED ? panel_rows<float>() : panel_rows<double>();
             const int rbA = gridOf(V, prS), rbT = gridOf(NC, prS), rbM = gridOf(NC, panel_rows<double>());
             // (and at least enough slices for the LDS copy of x: KS_MAX_COLS columns)
-            slicesA = std::max(gridOf(NC, KS_MAX_COLS), std::min(64, (KS_BLOCKS + rbA - 1) / rbA));
-            slicesT = std::max(gridOf(V, KS_MAX_COLS), std::min(256, (KS_BLOCKS + rbT - 1) / rbT));
+            // FP32 products stream half the bytes per column: twice the blocks
+            const int tgt = precision == SA_PREC_MIXED ? 2 * KS_BLOCKS : KS_BLOCKS;
+            const int rbM32 = gridOf(NC, panel_rows<float>());
+            slicesA = std::max(gridOf(NC, KS_MAX_COLS), std::min(64, (tgt + rbA - 1) / rbA));
+            slicesT = std::max(gridOf(V, KS_MAX_COLS), std::min(256, (tgt + rbT - 1) / rbT));
             slicesM = std::max(gridOf(NC, KS_MAX_COLS), std::min(256, (KS_BLOCKS + rbM - 1) / rbM));
-            partS.alloc(std::max({(size_t)slicesA * D * V, (size_t)slicesT * nc, (size_t)slicesM * nc}), tot);
+            slicesM32 = std::max(gridOf(NC, KS_MAX_COLS), std::min(256, (2 * KS_BLOCKS + rbM32 - 1) / rbM32));
+            partS.alloc(std::max({(size_t)slicesA * D * V, (size_t)slicesT * nc, (size_t)slicesM * nc,
+                                  (size_t)slicesM32 * nc}), tot);
             int rb = gridOf(NC, MV_ROWS);
             slicesC = std::max(1, std::min(32, (1024 + rb - 1) / rb));
             part.alloc((size_t)slicesC * nc, tot);
@@ -753,10 +758,10 @@ struct Engine : EngineBase {
     // of M need not be positive definite
     void prec_apply(KMode md, const double *r, double *z) {
         if (md == KMode::Stored32 && Mpre32.p) {
-            dim3 g(gridOf(NC, panel_rows<float>()), slicesM);
-            kstore_mv_k<D, float><<<g, NT, ks_lds(NC, slicesM), s>>>(NC, NC, Mpre32.p, r, partS.p, ctl.p);
+            dim3 g(gridOf(NC, panel_rows<float>()), slicesM32);
+            kstore_mv_k<D, float><<<g, NT, ks_lds(NC, slicesM32), s>>>(NC, NC, Mpre32.p, r, partS.p, ctl.p);
             CHECK_LAUNCH();
-            slice_rows_k<<<gridOf(D * NC, 64), NT, 0, s>>>(D * NC, slicesM, partS.p, z, ctl.p);
+            slice_rows_k<<<gridOf(D * NC, 64), NT, 0, s>>>(D * NC, slicesM32, partS.p, z, ctl.p);
             CHECK_LAUNCH();
             return;
         }
