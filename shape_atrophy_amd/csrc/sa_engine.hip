@@ -494,7 +494,7 @@ struct Engine : EngineBase {
             slicesA = std::max(gridOf(NC, KS_MAX_COLS), std::min(64, (tgt + rbA - 1) / rbA));
             slicesT = std::max(gridOf(V, KS_MAX_COLS), std::min(256, (tgt + rbT - 1) / rbT));
             slicesM = std::max(gridOf(NC, KS_MAX_COLS), std::min(256, (KS_BLOCKS + rbM - 1) / rbM));
-            slicesM32 = std::max(gridOf(NC, KS_MAX_COLS), std::min(256, (2 * KS_BLOCKS + rbM32 - 1) / rbM32));
+            slicesM32 = std::max(gridOf(NC, KS_MAX_COLS), std::min(256, (KS_BLOCKS + rbM32 - 1) / rbM32));
             partS.alloc(std::max({(size_t)slicesA * D * V, (size_t)slicesT * nc, (size_t)slicesM * nc,
                                   (size_t)slicesM32 * nc}), tot);
             int rb = gridOf(NC, MV_ROWS);
