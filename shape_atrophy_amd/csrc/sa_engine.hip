@@ -507,7 +507,7 @@ struct Engine : EngineBase {
             // deterministic factorisation and products: no atomics in rocBLAS/rocSOLVER kernels
             if (rocblas_set_atomics_mode(blas, rocblas_atomics_not_allowed) != rocblas_status_success)
                 throw SaError(SA_E_DEVICE, "rocblas_set_atomics_mode failed");
-            require(gridOf((long)nc) <= MAXP, "too many control points for the partial-sum buffers");
+            require(gridOf((long)nc, 64) <= MAXP, "too many control points for the partial-sum buffers");
         }
         if (NC == 0) {
             aM1.alloc(n, tot); aM2.alloc(n, tot); aM3.alloc(n, tot);
@@ -754,18 +754,25 @@ struct Engine : EngineBase {
         CHECK_LAUNCH();
     }
 
-    // z = M r, M = X^T X ~ K_cc^{-1}, in FP64 in every form: an FP32 rounding
-    // of M need not be positive definite
-    void prec_apply(KMode md, const double *r, double *z) {
+    // z = M r, M = X^T X ~ K_cc^{-1} (its FP32 panel copy in the FP32 form),
+    // fused with the CG test that follows: stop on |r|^2 (gC partials in P2)
+    // under the tolerance, else the r.z partials into P3, gridOf(D NC, 64) of
+    // them
+    void prec_dot(KMode md, const double *r, double *z) {
+        const int nc = NC * D;
+        int sl;
         if (md == KMode::Stored32 && Mpre32.p) {
-            dim3 g(gridOf(NC, panel_rows<float>()), slicesM32);
-            kstore_mv_k<D, float><<<g, NT, ks_lds(NC, slicesM32), s>>>(NC, NC, Mpre32.p, r, partS.p, ctl.p);
-            CHECK_LAUNCH();
-            slice_rows_k<<<gridOf(D * NC, 64), NT, 0, s>>>(D * NC, slicesM32, partS.p, z, ctl.p);
-            CHECK_LAUNCH();
-            return;
+            sl = slicesM32;
+            dim3 g(gridOf(NC, panel_rows<float>()), sl);
+            kstore_mv_k<D, float><<<g, NT, ks_lds(NC, sl), s>>>(NC, NC, Mpre32.p, r, partS.p, ctl.p);
+        } else {
+            sl = slicesM;
+            dim3 g(gridOf(NC, panel_rows<double>()), sl);
+            kstore_mv_k<D, double><<<g, NT, ks_lds(NC, sl), s>>>(NC, NC, Mpre.p, r, partS.p, ctl.p);
         }
-        cc_product(Mpre.p, r, z);
+        CHECK_LAUNCH();
+        slice_dot_k<<<gridOf(nc, 64), NT, 0, s>>>(nc, sl, partS.p, P2.p, gridOf(nc), r, z, P3.p, ctl.p);
+        CHECK_LAUNCH();
     }
 
 
@@ -803,8 +810,11 @@ struct Engine : EngineBase {
     // y = K_xc u (xc = true, V rows) or y = K_cx u (C rows) in the given mode;
     // |y|^2 partials in P when given.  ev (optional): events recorded around
     // the stored-matrix product kernel alone.
+    // adv: control block whose iteration counter the stored product advances
+    // (first kernel of a CG iteration); sum = false leaves the stored
+    // product's slice partials in partS for a fused consumer.
     void kx_product(KMode md, bool xc, const double *q, const double *u, double *y, double *P, const Ctl *c,
-                    hipEvent_t *ev = nullptr) {
+                    hipEvent_t *ev = nullptr, Ctl *adv = nullptr, bool sum = true) {
         if (md == KMode::MatrixFree64) {
             if (ev) CK(hipEventRecord(ev[0], s));
             if (xc) kcross(V, q, NC, qc.p, u, y, 1, P, c);
@@ -817,48 +827,62 @@ struct Engine : EngineBase {
         if (ev) CK(hipEventRecord(ev[0], s));
         if (md == KMode::Stored64) {
             dim3 g(gridOf(nr, panel_rows<double>()), sl);
-            kstore_mv_k<D, double><<<g, NT, ks_lds(ncol, sl), s>>>(nr, ncol, xc ? A64.p : At64.p, u, partS.p, c);
+            kstore_mv_k<D, double><<<g, NT, ks_lds(ncol, sl), s>>>(nr, ncol, xc ? A64.p : At64.p, u, partS.p, c, adv);
         } else {
             dim3 g(gridOf(nr, panel_rows<float>()), sl);
-            kstore_mv_k<D, float><<<g, NT, ks_lds(ncol, sl), s>>>(nr, ncol, xc ? A32.p : At32.p, u, partS.p, c);
+            kstore_mv_k<D, float><<<g, NT, ks_lds(ncol, sl), s>>>(nr, ncol, xc ? A32.p : At32.p, u, partS.p, c, adv);
         }
         CHECK_LAUNCH();
         if (ev) CK(hipEventRecord(ev[1], s));
+        if (!sum) return;
         if (P) slice_sum_k<<<gridOf((long)D * nr), NT, 0, s>>>(D * nr, sl, partS.p, y, P, c);   // + |y|^2 partials
         else slice_rows_k<<<gridOf(D * nr, 64), NT, 0, s>>>(D * nr, sl, partS.p, y, c);
         CHECK_LAUNCH();
     }
 
     // S u = K_cx B K_xc u (-> cY); K_xc u stays in Kr.  Its control-point
-    // rows are K_cc u in the FP64 forms; the FP32 form takes K_cc u from the
-    // FP64 K_cc (-> cK).  ev: 6 events (product, operator, product).
+    // rows are K_cc u, except in the FP32 form with an FP64 K_cc, which
+    // takes K_cc u from that (-> cK).  ev: 6 events (product, operator,
+    // product).  fused: S_apply opens a CG iteration whose stored products
+    // feed slice_combine_k (see fused_iter); the K_cx product's partials stay
+    // in partS.
+    bool fused_iter(KMode md) const { return md != KMode::MatrixFree64 && !(md == KMode::Stored32 && Kc.p); }
+
     void S_apply(KMode md, const double *q, const double *tauNow, const double *geoNow, const double *u,
-                 hipEvent_t *ev) {
-        kx_product(md, true, q, u, Kr.p, nullptr, ctl.p, ev);                  // w = K_xc u
+                 hipEvent_t *ev, bool fused = false) {
+        kx_product(md, true, q, u, Kr.p, nullptr, ctl.p, ev, fused ? ctl.p : nullptr);   // w = K_xc u
         if (ev) CK(hipEventRecord(ev[2], s));
         elastic(Kr.p, tauNow, geoNow, ctl.p);
         gather_op<2>(nullptr, nullptr, Ad.p, nullptr, nullptr);                 // B w
         if (ev) CK(hipEventRecord(ev[3], s));
-        kx_product(md, false, q, Ad.p, cY.p, nullptr, ctl.p, ev ? ev + 4 : nullptr);   // K_cx B w
+        kx_product(md, false, q, Ad.p, cY.p, nullptr, ctl.p, ev ? ev + 4 : nullptr, nullptr, !fused);   // K_cx B w
         if (md == KMode::Stored32 && Kc.p) cc_product(Kc.p, u, cK.p);
     }
 
-    // one control-point CG iteration on S x = rhs (x = xs): S d, step,
-    // preconditioned residual, new direction, counter; every kernel leaves at
-    // once when the solve is done, so iterations queued past the end are
+    // one control-point CG iteration on S x = rhs (x = xs): counter (the
+    // iteration's first kernel advances it; the budget spent ends the solve),
+    // S d, step, preconditioned residual, new direction; every kernel leaves
+    // at once when the solve is done, so iterations queued past the end are
     // no-ops.  ev: optional per-iteration events (6).
     void cg_iter(KMode md, const double *q, const double *tauNow, const double *geoNow, double *xs, hipEvent_t *ev) {
         const int nc = NC * D, gC = gridOf(nc);
-        const double *kdv = md == KMode::Stored32 && Kc.p ? cK.p : nullptr;
-        S_apply(md, q, tauNow, geoNow, cD.p, ev);
-        ctl_combine_k<D, 0><<<gC, NT, 0, s>>>(NC, V, cY.p, kdv, Kr.p, ctlIdx.p, cD.p, nullptr, ldmWgt, knlEps, cS.p,
-                                             P1.p, ctl.p);
-        ctl_update_k<<<gC, NT, 0, s>>>(nc, P1.p, gC, hist.p, xs, cR.p, cD.p, cS.p, P2.p, ctl.p);
+        int gS = gC;   // d.Sd partials
+        if (fused_iter(md)) {
+            S_apply(md, q, tauNow, geoNow, cD.p, ev, true);
+            gS = gridOf(nc, 64);
+            slice_combine_k<D><<<gS, NT, 0, s>>>(NC, V, slicesT, partS.p, Kr.p, ctlIdx.p, cD.p, ldmWgt, knlEps, cS.p,
+                                                 P1.p, ctl.p);
+        } else {
+            ctl_next_k<<<1, 1, 0, s>>>(ctl.p);
+            const double *kdv = md == KMode::Stored32 && Kc.p ? cK.p : nullptr;
+            S_apply(md, q, tauNow, geoNow, cD.p, ev);
+            ctl_combine_k<D, 0><<<gC, NT, 0, s>>>(NC, V, cY.p, kdv, Kr.p, ctlIdx.p, cD.p, nullptr, ldmWgt, knlEps,
+                                                 cS.p, P1.p, ctl.p);
+        }
+        ctl_update_k<<<gC, NT, 0, s>>>(nc, P1.p, gS, hist.p, xs, cR.p, cD.p, cS.p, P2.p, ctl.p);
         CHECK_LAUNCH();
-        prec_apply(md, cR.p, cZ.p);
-        ctl_dot_k<<<gC, NT, 0, s>>>(nc, P2.p, gC, cR.p, cZ.p, P3.p, ctl.p);
-        ctl_dir_k<<<gC, NT, 0, s>>>(nc, P3.p, gC, hist.p, cZ.p, cD.p, ctl.p);
-        ctl_next_k<<<1, 1, 0, s>>>(ctl.p);
+        prec_dot(md, cR.p, cZ.p);
+        ctl_dir_k<<<gC, NT, 0, s>>>(nc, P3.p, gridOf(nc, 64), hist.p, cZ.p, cD.p, ctl.p);
         CHECK_LAUNCH();
     }
 
@@ -928,11 +952,10 @@ struct Engine : EngineBase {
                                                  P2.p, ctl.p);
         }
         CHECK_LAUNCH();
-        prec_apply(md, cR.p, cZ.p);
-        ctl_dot_k<<<gC, NT, 0, s>>>(nc, P2.p, gC, cR.p, cZ.p, P3.p, ctl.p);
-        ctl_dir_k<<<gC, NT, 0, s>>>(nc, P3.p, gC, hist.p, cZ.p, cD.p, ctl.p);
-        ctl_next_k<<<1, 1, 0, s>>>(ctl.p);
+        prec_dot(md, cR.p, cZ.p);
+        ctl_dir_k<<<gC, NT, 0, s>>>(nc, P3.p, gridOf(nc, 64), hist.p, cZ.p, cD.p, ctl.p);   // it = -1: d = z
         CHECK_LAUNCH();
+        if (maxIt < 1) ctl_next_k<<<1, 1, 0, s>>>(ctl.p);   // no budget: the counter check ends the solve here
         Ctl &h = hctl[0];
         if (!timer.on && useGraphs) {
             hipGraphExec_t ex = cg_graph(md, q, tauNow, geoNow, xs);

@@ -796,7 +796,7 @@ __device__ inline void fma_rows(double (&a)[R][D], int r0, int c, const double2 
 template <int D, typename T>
 __global__ __launch_bounds__(NT) void kstore_mv_k(int nr, int nc, const T *__restrict__ M,
                                                   const double *__restrict__ x, double *__restrict__ part,
-                                                  const Ctl *ctl) {
+                                                  const Ctl *ctl, Ctl *adv = nullptr) {
     constexpr int PR = panel_rows<T>();
     constexpr int R = PR / 64;             // rows per lane
     constexpr int W = NT / 64;
@@ -810,6 +810,13 @@ __global__ __launch_bounds__(NT) void kstore_mv_k(int nr, int nc, const T *__res
     extern __shared__ double sx[];   // D x (je - jb)
     __shared__ double red[W - 1][D][PR];
     if (ctl && cld(&ctl->done)) return;
+    if (adv && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+        // first kernel of a CG iteration: advance the iteration counter (no
+        // kernel of this launch reads it); the budget spent stops the solve
+        const int it = cld(&adv->it) + 1;
+        cst(&adv->it, it);
+        if (it >= cld(&adv->maxIt)) { cst(&adv->done, 1); cst(&adv->status, 2); cst(&adv->iters, it); }
+    }
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int r0 = blockIdx.x * PR + R * lane;
     const int per = (nc + gridDim.y - 1) / gridDim.y;
@@ -875,6 +882,80 @@ __global__ __launch_bounds__(NT) void kstore_mv_k(int nr, int nc, const T *__res
                 if (r0 + k < nr) po[c * nr + r0 + k] = sum;
             }
     }
+}
+
+// control-point CG, fused consumers of the stored-product slice partials
+// (64 outputs per block, wavefront w adding slices w, w + 4, ..., combined in
+// wavefront order as in slice_rows_k):
+//
+// slice_combine_k: y = sum of the K_cx (B K_xc d) partials; out = S d = y +
+// ldmWgt (K_cc d + eps d), K_cc d read from w = K_xc d at the control-point
+// rows; block partial of d.(S d).
+template <int D>
+__global__ __launch_bounds__(NT) void slice_combine_k(int C, int V, int slices, const double *__restrict__ part,
+                                                      const double *__restrict__ w, const int *__restrict__ idx,
+                                                      const double *__restrict__ d, double ldmWgt, double eps,
+                                                      double *__restrict__ out, double *__restrict__ P,
+                                                      const Ctl *ctl) {
+    __shared__ double red[NT / 64][64];
+    __shared__ double red2[NT / 64];
+    if (cld(&ctl->done)) return;
+    const int n = C * D;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int i = blockIdx.x * 64 + lane;
+    double s = 0.0;
+    if (i < n)
+        for (int k = wv; k < slices; k += NT / 64) s += part[(size_t)k * n + i];
+    red[wv][lane] = s;
+    __syncthreads();
+    double ss = 0.0;
+    if (wv == 0 && i < n) {
+        double y = red[0][lane];
+#pragma unroll
+        for (int ww = 1; ww < NT / 64; ++ww) y += red[ww][lane];
+        const int c = i / C;
+        const double kd = w[(size_t)c * V + idx[i - c * C]];
+        const double sd = y + ldmWgt * (kd + eps * d[i]);
+        out[i] = sd;
+        ss = d[i] * sd;
+    }
+    const double t = block_sum(ss, red2);
+    if (threadIdx.x == 0) P[blockIdx.x] = t;
+}
+
+// slice_dot_k: after the residual update, stop on |r|^2 (partials Prr) below
+// the tolerance; else z = M r from the preconditioner product's partials and
+// the block partial of r.z
+__global__ __launch_bounds__(NT) void slice_dot_k(int n, int slices, const double *__restrict__ part,
+                                                  const double *__restrict__ Prr, int nPrr,
+                                                  const double *__restrict__ r, double *__restrict__ z,
+                                                  double *__restrict__ P3, Ctl *ctl) {
+    __shared__ double red[NT / 64][64];
+    __shared__ double red2[NT / 64];
+    if (cld(&ctl->done)) return;
+    const int it = cld(&ctl->it);
+    const double rr = reduce_partials(Prr, nPrr, red2);
+    if (rr < cld(&ctl->tol)) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) { cst(&ctl->done, 1); cst(&ctl->iters, it + 1); }
+        return;
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int i = blockIdx.x * 64 + lane;
+    double s = 0.0;
+    if (i < n)
+        for (int k = wv; k < slices; k += NT / 64) s += part[(size_t)k * n + i];
+    red[wv][lane] = s;
+    __syncthreads();
+    double ss = 0.0;
+    if (wv == 0 && i < n) {
+        double zi = red[0][lane];
+#pragma unroll
+        for (int ww = 1; ww < NT / 64; ++ww) zi += red[ww][lane];
+        z[i] = zi;
+        ss = r[i] * zi;
+    }
+    const double t = block_sum(ss, red2);
+    if (threadIdx.x == 0) P3[blockIdx.x] = t;
 }
 
 // all-landmark form: y = (K + eps I) x from the slice partials of the stored
