@@ -228,6 +228,71 @@ def mode_serve(args, api, W):
     }
 
 
+def mode_fit(args, api, W):
+    """Fit of the four tau variables of the 2D ring (C1 mesh, 20 steps) to a
+    target generated by the forward model at known values, from a start 8-12 %
+    away: BFGS with strong-Wolfe line searches, each gradient a batch of nine
+    candidates (central differences) run concurrently on the request server.
+    Throughput in objective evaluations/s, against the same candidate batches
+    run one at a time on a session and against the FP64 oracle on the host."""
+    from shape_atrophy_amd import fit
+    prec = "fp64"   # all-landmark kernel: the mixed form applies to control points only
+    steps = args.steps if args.steps is not None else 20
+    p = W.ring_problem()
+    p["timeNum"] = steps + 1
+    th = p["tauVar"][:, 0].copy()
+    s = api.Session(p)
+    r = s.run(th, p["tauPrm"], want_traj=True)
+    p["target"] = fit.target_from_shape(p, r["lmkStk"][-1])
+    s.close()
+    x0 = th * np.array([1.08, 0.92, 1.12, 0.93])
+    print(f"[bench] fit: V={p['q0'].shape[1]} steps={steps} theta*={th} x0={x0}", file=sys.stderr, flush=True)
+    obj = fit.ServerObjective(p, precision=prec, workers=args.workers)
+    obj(fit.fd_candidates(x0, fit.fd_steps(x0, fit.OptData())))        # warm-up: sessions created
+    t0 = time.perf_counter()
+    res = fit.bfgs(obj, x0, fit.OptData(itrMax=60, tolVal=1e-7, verbose=True))
+    wall = time.perf_counter() - t0
+    obj.close()
+    err = float(np.abs(res.x - th).max())
+    print(f"[bench] fit: {res.status} after {res.iterations} iterations, {res.evaluations} evaluations in "
+          f"{wall:.2f}s; max |theta - theta*| {err:.3e}", file=sys.stderr, flush=True)
+    # the same candidate batch served one at a time
+    X = fit.fd_candidates(res.x, fit.fd_steps(res.x, fit.OptData()))
+    so = fit.SessionObjective(p, precision=prec)
+    so(X[:, :1])
+    t1 = time.perf_counter()
+    so(X)
+    serial = X.shape[1] / (time.perf_counter() - t1)
+    so.close()
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        O.build()
+        n_cpu = 2
+        t2 = time.perf_counter()
+        for j in range(n_cpu):
+            O.objfcn(dict(p, tauVar=X[:, j:j + 1].copy()), threads=os.cpu_count())
+        cpu = {"value": round(n_cpu / (time.perf_counter() - t2), 4), "unit": "evaluations/s",
+               "cores": os.cpu_count(), "kind": "port",
+               "sample": f"{n_cpu} objective evaluations of the same workload, FP64 oracle"}
+    return {
+        "metric": "objective evaluations/s while fitting tauVar (BFGS, central-difference gradients)",
+        "value": round(res.evaluations / wall, 3), "unit": "evaluations/s", "n_gpus": 1,
+        "steps": res.evaluations * steps, "warmup": 1, "ms_per_step": round(1e3 * wall / (res.evaluations * steps), 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "2D concentric-ring disk, 10 rings x 200 nodes, target from the forward model at "
+                               "known tau variables", "time_steps": steps, "workers": args.workers,
+                   "precision": prec},
+        "fit": {"status": res.status, "iterations": res.iterations, "evaluations": res.evaluations,
+                "batches": res.batches, "seconds": round(wall, 3), "objective": res.f,
+                "max_abs_err_theta": err},
+        "serial_session": {"value": round(serial, 3), "unit": "evaluations/s"},
+        "cpu_baseline": cpu,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -240,7 +305,7 @@ def main():
     ap.add_argument("--no-profile", action="store_true",
                     help="time without the per-kernel events (graph-captured CG iterations); no roofline")
     ap.add_argument("--cpu-iters", type=int, default=0, help="CG iterations of the CPU sample (0: sized to the workload)")
-    ap.add_argument("--mode", default="single", choices=["single", "batch", "serve", "precision"],
+    ap.add_argument("--mode", default="single", choices=["single", "batch", "serve", "precision", "fit"],
                     help="single: one solve of --config; batch: 64 uneven meshes concurrently; serve: Poisson stream")
     ap.add_argument("--workers", type=int, default=8)
     ap.add_argument("--batch-n", type=int, default=64)
@@ -268,7 +333,8 @@ def main():
     from shape_atrophy_amd import api
     if args.mode != "single":
         from shape_atrophy_amd import workloads as W
-        res = {"batch": mode_batch, "serve": mode_serve, "precision": mode_precision}[args.mode](args, api, W)
+        res = {"batch": mode_batch, "serve": mode_serve, "precision": mode_precision,
+               "fit": mode_fit}[args.mode](args, api, W)
         print(json.dumps(res))
         return
     steps = args.steps if args.steps is not None else 50   # the configured solve: 50 time steps
