@@ -45,6 +45,17 @@ std::mutex g_libMu;
 #ifndef SA_MIXED_CC32
 #define SA_MIXED_CC32 1   // mixed form: K_cc from the FP32 stored rows, FP32 preconditioner
 #endif
+// spectral preconditioner: iterations recorded, Ritz vectors kept
+#ifndef SA_SPEC_M
+#define SA_SPEC_M 256
+#endif
+#ifndef SA_SPEC_K
+#define SA_SPEC_K 96
+#endif
+// steps a control-point preconditioner is reused at most
+#ifndef SA_PREC_MAX_AGE
+#define SA_PREC_MAX_AGE 8
+#endif
 #ifndef SA_EXTRAP
 #define SA_EXTRAP 3   // order of the initial-guess extrapolation across steps
 #endif
@@ -211,6 +222,18 @@ struct Engine : EngineBase {
     int slicesK = 1;
     DBuf<float> A32, At32;       // stored K_xc / K_cx, FP32 (mixed precision)
     DBuf<float> Mpre32;          // FP32 panel copy of the preconditioner (mixed, SA_MIXED_CC32)
+    // spectral term of the control-point preconditioner, z = M r + U diag(cf) U^T r:
+    // U holds Ritz vectors of M S for its largest eigenvalues lambda_l (orthonormal
+    // in the M^-1 inner product), cf_l = theta / lambda_l - 1 maps them to theta,
+    // the next Ritz value.  They are harvested from the Krylov basis of the first
+    // solve after each refresh of M (Rayleigh-Ritz on the recorded d_j, S d_j, r_j)
+    // and used until M is rebuilt.  SA_SPECTRAL=0 switches the term off.
+    static constexpr int SPEC_M = SA_SPEC_M;   // iterations recorded
+    static constexpr int SPEC_K = SA_SPEC_K;   // Ritz vectors kept at most
+    DBuf<double> recD, recSD, recR, specU, specC, specW, specA;
+    DBuf<int> specInfo;
+    bool specEnabled = true, specOn = false, specRecord = false, recActive = false;
+    int specK = 0, specMode = -1;
     int slicesA = 1, slicesT = 1, slicesM = 1, slicesM32 = 1;
     DBuf<int> potrfInfo;
     rocblas_handle blas = nullptr;
@@ -222,7 +245,7 @@ struct Engine : EngineBase {
     // rebuilt on a run's first step, after PREC_MAX_AGE steps, or when a
     // step needs PREC_GROWTH times the CG iterations of the first step that
     // used it (any SPD preconditioner leaves the solution unchanged)
-    static constexpr int PREC_MAX_AGE = 8;
+    static constexpr int PREC_MAX_AGE = SA_PREC_MAX_AGE;
     static constexpr double PREC_GROWTH = 1.2;
     int precAge = -1, precIters = 0;
     int fp64Steps = 0;           // mixed-form steps finished by the FP64 form (this run)
@@ -450,6 +473,16 @@ struct Engine : EngineBase {
             cS.alloc(nc, tot); cY.alloc(nc, tot); cK.alloc(nc, tot);
             cRout.alloc(nc, tot); cDelta.alloc(nc, tot); cZero.alloc(nc, tot);
             CK(hipMemsetAsync(cZero.p, 0, sizeof(double) * nc, s));
+            {
+                const char *e = std::getenv("SA_SPECTRAL");
+                specEnabled = !(e && e[0] == '0');
+            }
+            if (specEnabled) {
+                recD.alloc(nc * SPEC_M, tot); recSD.alloc(nc * SPEC_M, tot); recR.alloc(nc * SPEC_M, tot);
+                specU.alloc(nc * SPEC_K, tot); specC.alloc(SPEC_K, tot); specW.alloc(SPEC_K, tot);
+                specA.alloc(2 * (size_t)SPEC_M * SPEC_M + 2 * SPEC_M, tot);
+                specInfo.alloc(1, tot);
+            }
             // stored kernel matrices: FP64 for the plain form, FP32 for the mixed one
             // dense products are panel-major with rows padded to whole panels
             // (zero padding rows); the preconditioner leaves rocBLAS
@@ -714,6 +747,10 @@ struct Engine : EngineBase {
 #undef SA_KB
         CHECK_LAUNCH();
         if (!refresh) return;
+        // a new M: the spectral term (orthonormal in the old M's inner
+        // product) is dropped and harvested again from the next solve
+        specOn = false;
+        specRecord = specEnabled;
         std::lock_guard<std::mutex> lk(g_libMu);
         if (rocsolver_dpotrf(blas, rocblas_fill_lower, NC, Kcc.p, NC, potrfInfo.p) != rocblas_status_success)
             throw SaError(SA_E_DEVICE, "rocsolver_dpotrf failed");
@@ -760,6 +797,11 @@ struct Engine : EngineBase {
     // them
     void prec_dot(KMode md, const double *r, double *z) {
         const int nc = NC * D;
+        const bool sp = specOn && !recActive && specMode == (int)md;
+        if (sp) {
+            spec_w_k<<<specK, NT, 0, s>>>(nc, specU.p, r, specW.p, ctl.p);
+            CHECK_LAUNCH();
+        }
         int sl;
         if (md == KMode::Stored32 && Mpre32.p) {
             sl = slicesM32;
@@ -771,7 +813,8 @@ struct Engine : EngineBase {
             kstore_mv_k<D, double><<<g, NT, ks_lds(NC, sl), s>>>(NC, NC, Mpre.p, r, partS.p, ctl.p);
         }
         CHECK_LAUNCH();
-        slice_dot_k<<<gridOf(nc, 64), NT, 0, s>>>(nc, sl, partS.p, P2.p, gridOf(nc), r, z, P3.p, ctl.p);
+        slice_dot_k<<<gridOf(nc, 64), NT, 0, s>>>(nc, sl, partS.p, P2.p, gridOf(nc), r, z, P3.p,
+                                                  sp ? specU.p : nullptr, specC.p, specW.p, specK, ctl.p);
         CHECK_LAUNCH();
     }
 
@@ -871,7 +914,8 @@ struct Engine : EngineBase {
             S_apply(md, q, tauNow, geoNow, cD.p, ev, true);
             gS = gridOf(nc, 64);
             slice_combine_k<D><<<gS, NT, 0, s>>>(NC, V, slicesT, partS.p, Kr.p, ctlIdx.p, cD.p, ldmWgt, knlEps, cS.p,
-                                                 P1.p, ctl.p);
+                                                 P1.p, recActive ? recD.p : nullptr, recSD.p, recR.p, cR.p, SPEC_M,
+                                                 ctl.p);
         } else {
             ctl_next_k<<<1, 1, 0, s>>>(ctl.p);
             const double *kdv = md == KMode::Stored32 && Kc.p ? cK.p : nullptr;
@@ -934,8 +978,152 @@ struct Engine : EngineBase {
     // the kernel timer the iterations run as replays of a captured graph of
     // GRAPH_ITERS iterations, the host checking the device control block
     // between launches.
+    // ctl_pcg_run, recording the Krylov basis when a spectral harvest is due
+    // (first solve after a refresh of M) and harvesting from it afterwards
     int ctl_pcg(KMode md, const double *q, const double *tauNow, const double *geoNow, const double *rhs, double *xs,
                 bool zeroStart, double tolSqu, int maxIt, int &iters) {
+        recActive = specRecord && fused_iter(md);
+        int st;
+        try {
+            st = ctl_pcg_run(md, q, tauNow, geoNow, rhs, xs, zeroStart, tolSqu, maxIt, iters);
+        } catch (...) {
+            recActive = false;
+            throw;
+        }
+        if (recActive) {
+            recActive = false;
+            if (st != 1) spec_harvest(md, std::min(iters, SPEC_M));
+        }
+        return st;
+    }
+
+    // Rayleigh-Ritz for M S on the recorded Krylov basis (m iterations).
+    // z_j = M r_j = d_j - beta_{j-1} d_{j-1}, so Z = D B with B upper
+    // bidiagonal; the pencil is H = Z^T S Z = B^T (D^T SD) B against the Gram
+    // matrix G = Z^T M^-1 Z = B^T (D^T R), both scaled to the Lanczos vectors
+    // z_j / sqrt(r_j.z_j).  Directions where G is numerically singular (lost
+    // orthogonality) are dropped; the leading Ritz vectors U = D B P c, with
+    // G = Q g Q^T, P = Q g^-1/2 and c the eigenvectors of P^T H P, become the
+    // spectral term.
+    void spec_harvest(KMode md, int m) {
+        const int nc = NC * D;
+        if (m < 24) return;   // too short a basis: the next solve records again
+        const size_t mm = (size_t)m * m;
+        double *dA1 = specA.p, *dA2 = specA.p + (size_t)SPEC_M * SPEC_M, *dEv = dA2 + (size_t)SPEC_M * SPEC_M;
+        double *dE = dEv + SPEC_M;
+        std::vector<double> A1(mm), A2(mm), rz(m);
+        const double one = 1.0, zero = 0.0;
+        {
+            std::lock_guard<std::mutex> lk(g_libMu);
+            if (rocblas_dgemm(blas, rocblas_operation_transpose, rocblas_operation_none, m, m, nc, &one, recD.p, nc,
+                              recR.p, nc, &zero, dA1, m) != rocblas_status_success ||
+                rocblas_dgemm(blas, rocblas_operation_transpose, rocblas_operation_none, m, m, nc, &one, recD.p, nc,
+                              recSD.p, nc, &zero, dA2, m) != rocblas_status_success)
+                throw SaError(SA_E_DEVICE, "rocblas_dgemm failed");
+            CK(hipStreamSynchronize(s));
+        }
+        CK(hipMemcpyAsync(A1.data(), dA1, sizeof(double) * mm, hipMemcpyDeviceToHost, s));
+        CK(hipMemcpyAsync(A2.data(), dA2, sizeof(double) * mm, hipMemcpyDeviceToHost, s));
+        CK(hipMemcpyAsync(rz.data(), hist.p, sizeof(double) * m, hipMemcpyDeviceToHost, s));
+        CK(hipStreamSynchronize(s));
+        specRecord = false;
+        for (int j = 0; j < m; ++j)
+            if (!(rz[j] > 0.0) || !std::isfinite(rz[j])) return;
+        std::vector<double> beta(m, 0.0), sc(m);
+        for (int j = 0; j + 1 < m; ++j) beta[j] = rz[j + 1] / rz[j];
+        for (int j = 0; j < m; ++j) sc[j] = 1.0 / std::sqrt(rz[j]);
+        auto at = [m](std::vector<double> &A, int i, int j) -> double & { return A[i + (size_t)j * m]; };
+        // G = B^T A1, H = B^T A2 B (column-major), symmetrised and scaled by sc
+        std::vector<double> G(mm), Tm(mm), H(mm);
+        for (int j = 0; j < m; ++j)
+            for (int i = 0; i < m; ++i) {
+                at(G, i, j) = at(A1, i, j) - (i > 0 ? beta[i - 1] * at(A1, i - 1, j) : 0.0);
+                at(Tm, i, j) = at(A2, i, j) - (j > 0 ? beta[j - 1] * at(A2, i, j - 1) : 0.0);
+            }
+        for (int j = 0; j < m; ++j)
+            for (int i = 0; i < m; ++i) at(H, i, j) = at(Tm, i, j) - (i > 0 ? beta[i - 1] * at(Tm, i - 1, j) : 0.0);
+        for (int j = 0; j < m; ++j)
+            for (int i = 0; i <= j; ++i) {
+                const double f = sc[i] * sc[j];
+                const double g = 0.5 * (at(G, i, j) + at(G, j, i)) * f, h = 0.5 * (at(H, i, j) + at(H, j, i)) * f;
+                at(G, i, j) = at(G, j, i) = g;
+                at(H, i, j) = at(H, j, i) = h;
+            }
+        // eigenpairs of the Gram matrix; keep directions above 1e-10 of its largest
+        std::vector<double> ev(m);
+        auto syevd = [&](std::vector<double> &A, int n) -> bool {
+            int info = 0;
+            CK(hipMemcpyAsync(dA1, A.data(), sizeof(double) * n * n, hipMemcpyHostToDevice, s));
+            {
+                std::lock_guard<std::mutex> lk(g_libMu);
+                if (rocsolver_dsyevd(blas, rocblas_evect_original, rocblas_fill_lower, n, dA1, n, dEv, dE,
+                                     specInfo.p) != rocblas_status_success)
+                    throw SaError(SA_E_DEVICE, "rocsolver_dsyevd failed");
+                CK(hipStreamSynchronize(s));
+            }
+            CK(hipMemcpyAsync(A.data(), dA1, sizeof(double) * n * n, hipMemcpyDeviceToHost, s));
+            CK(hipMemcpyAsync(ev.data(), dEv, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+            CK(hipMemcpyAsync(&info, specInfo.p, sizeof(int), hipMemcpyDeviceToHost, s));
+            CK(hipStreamSynchronize(s));
+            return info == 0;
+        };
+        if (!syevd(G, m) || !(ev[m - 1] > 0.0)) return;
+        std::vector<int> keep;
+        for (int i = 0; i < m; ++i)
+            if (ev[i] > 1e-10 * ev[m - 1]) keep.push_back(i);
+        const int r = (int)keep.size();
+        const int kk = std::min(SPEC_K, 2 * r / 3);
+        if (kk < 8) return;
+        std::vector<double> P((size_t)m * r);   // m x r: Q_keep / sqrt(g)
+        for (int l = 0; l < r; ++l) {
+            const double f = 1.0 / std::sqrt(ev[keep[l]]);
+            for (int i = 0; i < m; ++i) P[i + (size_t)l * m] = G[i + (size_t)keep[l] * m] * f;
+        }
+        std::vector<double> HP((size_t)m * r), Hr((size_t)r * r);
+        for (int l = 0; l < r; ++l)
+            for (int j = 0; j < m; ++j) {
+                const double pj = P[j + (size_t)l * m];
+                if (pj == 0.0) continue;
+                for (int i = 0; i < m; ++i) HP[i + (size_t)l * m] += at(H, i, j) * pj;
+            }
+        for (int b = 0; b < r; ++b)
+            for (int a = 0; a <= b; ++a) {
+                double t = 0.0;
+                for (int i = 0; i < m; ++i) t += P[i + (size_t)a * m] * HP[i + (size_t)b * m];
+                Hr[a + (size_t)b * r] = Hr[b + (size_t)a * r] = t;
+            }
+        if (!syevd(Hr, r)) return;
+        const double theta = ev[r - 1 - kk];
+        if (!(theta > 0.0) || !(ev[r - 1] > theta)) return;
+        // W = B diag(sc) P c_top (m x kk); U = D W
+        std::vector<double> Y((size_t)m * kk, 0.0), W((size_t)m * kk), cf(kk);
+        for (int l = 0; l < kk; ++l) {
+            const double *c = &Hr[(size_t)(r - 1 - l) * r];
+            for (int a = 0; a < r; ++a) {
+                const double ca = c[a];
+                for (int i = 0; i < m; ++i) Y[i + (size_t)l * m] += P[i + (size_t)a * m] * ca;
+            }
+            for (int i = 0; i < m; ++i) Y[i + (size_t)l * m] *= sc[i];
+            for (int i = 0; i < m; ++i)
+                W[i + (size_t)l * m] = Y[i + (size_t)l * m] - (i + 1 < m ? beta[i] * Y[i + 1 + (size_t)l * m] : 0.0);
+            cf[l] = theta / ev[r - 1 - l] - 1.0;
+        }
+        CK(hipMemcpyAsync(dA2, W.data(), sizeof(double) * m * kk, hipMemcpyHostToDevice, s));
+        CK(hipMemcpyAsync(specC.p, cf.data(), sizeof(double) * kk, hipMemcpyHostToDevice, s));
+        {
+            std::lock_guard<std::mutex> lk(g_libMu);
+            if (rocblas_dgemm(blas, rocblas_operation_none, rocblas_operation_none, nc, kk, m, &one, recD.p, nc, dA2, m,
+                              &zero, specU.p, nc) != rocblas_status_success)
+                throw SaError(SA_E_DEVICE, "rocblas_dgemm failed");
+            CK(hipStreamSynchronize(s));
+        }
+        specOn = true;
+        specMode = (int)md;
+        specK = kk;
+    }
+
+    int ctl_pcg_run(KMode md, const double *q, const double *tauNow, const double *geoNow, const double *rhs, double *xs,
+                    bool zeroStart, double tolSqu, int maxIt, int &iters) {
         const int nc = NC * D, gC = gridOf(nc);
         const double *kdv = md == KMode::Stored32 && Kc.p ? cK.p : nullptr;   // K_cc d (else gathered from K_xc d)
         pcg_start_k<<<1, NT, 0, s>>>(P0.p, gC, tolSqu, maxIt, ctl.p);
@@ -1509,6 +1697,8 @@ long long sa_session_footprint(const sa_mesh *m, const sa_target *t, const sa_pa
         b += 8 * (2 * C * C + pad(C, 128) * C + 12 * C * D + 256 * C * D + 64 * D * V + 32 * C * D);
         if (p->precision == SA_PREC_MIXED) b += 4 * (pad(V, 256) * C + pad(C, 256) * V) + 8 * pad(C, 128) * C;
         else if ((pad(V, 128) * C + pad(C, 128) * V) * 8 <= (48LL << 30)) b += 8 * (pad(V, 128) * C + pad(C, 128) * V);
+        // spectral preconditioner: recorded Krylov basis, Ritz vectors, projected matrices
+        b += 8 * ((3 * SA_SPEC_M + SA_SPEC_K) * C * D + 2 * SA_SPEC_M * SA_SPEC_M + 2 * SA_SPEC_M + 2 * SA_SPEC_K);
     }
     return b;
 }

@@ -890,17 +890,23 @@ __global__ __launch_bounds__(NT) void kstore_mv_k(int nr, int nc, const T *__res
 //
 // slice_combine_k: y = sum of the K_cx (B K_xc d) partials; out = S d = y +
 // ldmWgt (K_cc d + eps d), K_cc d read from w = K_xc d at the control-point
-// rows; block partial of d.(S d).
+// rows; block partial of d.(S d).  recD non-null: iteration it < recMax also
+// records d, S d and the residual r entering the iteration in column it of
+// recD, recSD, recR (n x recMax each), the Krylov basis the spectral
+// preconditioner is harvested from.
 template <int D>
 __global__ __launch_bounds__(NT) void slice_combine_k(int C, int V, int slices, const double *__restrict__ part,
                                                       const double *__restrict__ w, const int *__restrict__ idx,
                                                       const double *__restrict__ d, double ldmWgt, double eps,
                                                       double *__restrict__ out, double *__restrict__ P,
-                                                      const Ctl *ctl) {
+                                                      double *__restrict__ recD, double *__restrict__ recSD,
+                                                      double *__restrict__ recR, const double *__restrict__ r,
+                                                      int recMax, const Ctl *ctl) {
     __shared__ double red[NT / 64][64];
     __shared__ double red2[NT / 64];
     if (cld(&ctl->done)) return;
     const int n = C * D;
+    const int it = recD ? cld(&ctl->it) : recMax;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int i = blockIdx.x * 64 + lane;
     double s = 0.0;
@@ -918,18 +924,40 @@ __global__ __launch_bounds__(NT) void slice_combine_k(int C, int V, int slices, 
         const double sd = y + ldmWgt * (kd + eps * d[i]);
         out[i] = sd;
         ss = d[i] * sd;
+        if (it < recMax) {
+            const size_t o = (size_t)it * n + i;
+            recD[o] = d[i];
+            recSD[o] = sd;
+            recR[o] = r[i];
+        }
     }
     const double t = block_sum(ss, red2);
     if (threadIdx.x == 0) P[blockIdx.x] = t;
 }
 
+// spectral term of the preconditioner, first half: w = U^T r for the k
+// columns of U (n x k), one block per column
+__global__ __launch_bounds__(NT) void spec_w_k(int n, const double *__restrict__ U, const double *__restrict__ r,
+                                               double *__restrict__ w, const Ctl *ctl) {
+    __shared__ double red[NT / 64];
+    if (cld(&ctl->done)) return;
+    const double *u = U + (size_t)blockIdx.x * n;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += NT) s += u[i] * r[i];
+    s = block_sum(s, red);
+    if (threadIdx.x == 0) w[blockIdx.x] = s;
+}
+
 // slice_dot_k: after the residual update, stop on |r|^2 (partials Prr) below
-// the tolerance; else z = M r from the preconditioner product's partials and
-// the block partial of r.z
+// the tolerance; else z = M r from the preconditioner product's partials
+// (plus, with U, the spectral term U diag(cf) w, w = U^T r) and the block
+// partial of r.z
 __global__ __launch_bounds__(NT) void slice_dot_k(int n, int slices, const double *__restrict__ part,
                                                   const double *__restrict__ Prr, int nPrr,
                                                   const double *__restrict__ r, double *__restrict__ z,
-                                                  double *__restrict__ P3, Ctl *ctl) {
+                                                  double *__restrict__ P3, const double *__restrict__ U,
+                                                  const double *__restrict__ cf, const double *__restrict__ w,
+                                                  int k, Ctl *ctl) {
     __shared__ double red[NT / 64][64];
     __shared__ double red2[NT / 64];
     if (cld(&ctl->done)) return;
@@ -951,6 +979,11 @@ __global__ __launch_bounds__(NT) void slice_dot_k(int n, int slices, const doubl
         double zi = red[0][lane];
 #pragma unroll
         for (int ww = 1; ww < NT / 64; ++ww) zi += red[ww][lane];
+        if (U) {
+            double t = 0.0;
+            for (int b = 0; b < k; ++b) t += U[(size_t)b * n + i] * (cf[b] * w[b]);
+            zi += t;
+        }
         z[i] = zi;
         ss = r[i] * zi;
     }

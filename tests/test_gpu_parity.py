@@ -214,3 +214,21 @@ def test_mixed_precision_falls_back_to_fp64():
     assert rm["status"] == 0 and r64["status"] == 0
     assert rm["fp64_steps"] > 0
     assert abs(rm["obj"] - r64["obj"]) <= 1e-7 * abs(r64["obj"])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_spectral_preconditioner(precision, monkeypatch):
+    """The spectral term of the control-point preconditioner (Ritz vectors of
+    M S harvested after each refresh of M) changes the CG iterate path but not
+    the converged solution: the objective agrees with the plain
+    preconditioner's to the CG tolerance, in fewer iterations."""
+    p = W.shell_problem(level=3, n_layers=5, time_steps=12, n_ctl=321)
+    print(f"[gpu] spectral {precision} V={p['q0'].shape[1]}", flush=True)
+    monkeypatch.setenv("SA_SPECTRAL", "0")
+    plain = api.Session(p, precision=precision).run(p["tauVar"][:, 0], p["tauPrm"])
+    monkeypatch.setenv("SA_SPECTRAL", "1")
+    spec = api.Session(p, precision=precision).run(p["tauVar"][:, 0], p["tauPrm"])
+    print("iterations plain", plain["pcg_iters"], "spectral", spec["pcg_iters"])
+    assert plain["status"] == 0 and spec["status"] == 0
+    assert abs(spec["obj"] - plain["obj"]) <= 1e-9 * abs(plain["obj"])
+    assert spec["pcg_iters"] < 0.9 * plain["pcg_iters"]
