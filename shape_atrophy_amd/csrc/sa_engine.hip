@@ -230,6 +230,7 @@ struct Engine : EngineBase {
     // and used until M is rebuilt.  SA_SPECTRAL=0 switches the term off.
     static constexpr int SPEC_M = SA_SPEC_M;   // iterations recorded
     static constexpr int SPEC_K = SA_SPEC_K;   // Ritz vectors kept at most
+    static_assert(SPEC_K <= SPEC_KMAX, "SA_SPEC_K above the kernels' LDS capacity");
     DBuf<double> recD, recSD, recR, specU, specC, specW, specA;
     DBuf<int> specInfo;
     bool specEnabled = true, specOn = false, specRecord = false, recActive = false;
@@ -479,7 +480,8 @@ struct Engine : EngineBase {
             }
             if (specEnabled) {
                 recD.alloc(nc * SPEC_M, tot); recSD.alloc(nc * SPEC_M, tot); recR.alloc(nc * SPEC_M, tot);
-                specU.alloc(nc * SPEC_K, tot); specC.alloc(SPEC_K, tot); specW.alloc(SPEC_K, tot);
+                specU.alloc(nc * SPEC_K, tot); specC.alloc(SPEC_K, tot);
+                specW.alloc((size_t)SPEC_K * gridOf((long)nc, SPW_ROWS), tot);
                 specA.alloc(2 * (size_t)SPEC_M * SPEC_M + 2 * SPEC_M, tot);
                 specInfo.alloc(1, tot);
             }
@@ -795,11 +797,16 @@ struct Engine : EngineBase {
     // fused with the CG test that follows: stop on |r|^2 (gC partials in P2)
     // under the tolerance, else the r.z partials into P3, gridOf(D NC, 64) of
     // them
+    // the spectral term applies to this solve (harvested in the same mode,
+    // not recording)
+    bool spec_active(KMode md) const { return specOn && !recActive && specMode == (int)md; }
+
     void prec_dot(KMode md, const double *r, double *z) {
         const int nc = NC * D;
-        const bool sp = specOn && !recActive && specMode == (int)md;
+        const bool sp = spec_active(md);
+        const int nPw = gridOf(nc, SPW_ROWS);
         if (sp) {
-            spec_w_k<<<specK, NT, 0, s>>>(nc, specU.p, r, specW.p, ctl.p);
+            spec_wp_k<<<dim3(nPw, gridOf(specK, SPW_COLS)), NT, 0, s>>>(nc, specU.p, specK, r, specW.p, ctl.p);
             CHECK_LAUNCH();
         }
         int sl;
@@ -814,7 +821,7 @@ struct Engine : EngineBase {
         }
         CHECK_LAUNCH();
         slice_dot_k<<<gridOf(nc, 64), NT, 0, s>>>(nc, sl, partS.p, P2.p, gridOf(nc), r, z, P3.p,
-                                                  sp ? specU.p : nullptr, specC.p, specW.p, specK, ctl.p);
+                                                  sp ? specU.p : nullptr, specC.p, specW.p, nPw, specK, ctl.p);
         CHECK_LAUNCH();
     }
 

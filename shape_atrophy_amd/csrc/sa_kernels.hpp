@@ -752,6 +752,8 @@ __global__ __launch_bounds__(NT) void kstore_build_k(int V, const double *__rest
 // order through LDS at the end.  The block's partial goes to
 // part + s * D * nr; slice_rows_k sums the slices in slice order.
 constexpr int KS_MAX_COLS = 1024;
+// most Ritz vectors in the spectral term of the control-point preconditioner
+constexpr int SPEC_KMAX = 128;
 #ifndef SA_KS_BLOCKS
 #define SA_KS_BLOCKS 1024
 #endif
@@ -935,31 +937,19 @@ __global__ __launch_bounds__(NT) void slice_combine_k(int C, int V, int slices, 
     if (threadIdx.x == 0) P[blockIdx.x] = t;
 }
 
-// spectral term of the preconditioner, first half: w = U^T r for the k
-// columns of U (n x k), one block per column
-__global__ __launch_bounds__(NT) void spec_w_k(int n, const double *__restrict__ U, const double *__restrict__ r,
-                                               double *__restrict__ w, const Ctl *ctl) {
-    __shared__ double red[NT / 64];
-    if (cld(&ctl->done)) return;
-    const double *u = U + (size_t)blockIdx.x * n;
-    double s = 0.0;
-    for (int i = threadIdx.x; i < n; i += NT) s += u[i] * r[i];
-    s = block_sum(s, red);
-    if (threadIdx.x == 0) w[blockIdx.x] = s;
-}
-
 // slice_dot_k: after the residual update, stop on |r|^2 (partials Prr) below
 // the tolerance; else z = M r from the preconditioner product's partials
-// (plus, with U, the spectral term U diag(cf) w, w = U^T r) and the block
-// partial of r.z
+// (plus, with U, the spectral term U diag(cf) w, w = U^T r summed from the
+// nPw block partials wp of ctl_update_k) and the block partial of r.z
 __global__ __launch_bounds__(NT) void slice_dot_k(int n, int slices, const double *__restrict__ part,
                                                   const double *__restrict__ Prr, int nPrr,
                                                   const double *__restrict__ r, double *__restrict__ z,
                                                   double *__restrict__ P3, const double *__restrict__ U,
-                                                  const double *__restrict__ cf, const double *__restrict__ w,
-                                                  int k, Ctl *ctl) {
+                                                  const double *__restrict__ cf, const double *__restrict__ wp,
+                                                  int nPw, int k, Ctl *ctl) {
     __shared__ double red[NT / 64][64];
     __shared__ double red2[NT / 64];
+    __shared__ double cw[SPEC_KMAX];
     if (cld(&ctl->done)) return;
     const int it = cld(&ctl->it);
     const double rr = reduce_partials(Prr, nPrr, red2);
@@ -969,9 +959,25 @@ __global__ __launch_bounds__(NT) void slice_dot_k(int n, int slices, const doubl
     }
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int i = blockIdx.x * 64 + lane;
+    if (U) {
+        for (int b = threadIdx.x; b < k; b += NT) {
+            double s = 0.0;
+#pragma unroll 16
+            for (int j = 0; j < nPw; ++j) s += wp[(size_t)j * k + b];
+            cw[b] = cf[b] * s;
+        }
+        __syncthreads();
+    }
     double s = 0.0;
-    if (i < n)
-        for (int k = wv; k < slices; k += NT / 64) s += part[(size_t)k * n + i];
+    if (i < n) {
+        for (int q = wv; q < slices; q += NT / 64) s += part[(size_t)q * n + i];
+        // wavefront wv's share of the spectral term: columns wv, wv + 4, ...
+        if (U) {
+            double t = 0.0;
+            for (int b = wv; b < k; b += NT / 64) t += U[(size_t)b * n + i] * cw[b];
+            s += t;
+        }
+    }
     red[wv][lane] = s;
     __syncthreads();
     double ss = 0.0;
@@ -979,11 +985,6 @@ __global__ __launch_bounds__(NT) void slice_dot_k(int n, int slices, const doubl
         double zi = red[0][lane];
 #pragma unroll
         for (int ww = 1; ww < NT / 64; ++ww) zi += red[ww][lane];
-        if (U) {
-            double t = 0.0;
-            for (int b = 0; b < k; ++b) t += U[(size_t)b * n + i] * (cf[b] * w[b]);
-            zi += t;
-        }
         z[i] = zi;
         ss = r[i] * zi;
     }
@@ -1097,6 +1098,44 @@ __global__ __launch_bounds__(NT) void ctl_combine_k(int C, int V, const double *
     }
     double t = block_sum(ss, red);
     if (threadIdx.x == 0) P[blockIdx.x] = t;
+}
+
+// spectral term of the preconditioner, first half: partials of w = U^T r
+// (U: n x k).  Block (c, g) covers rows [c * SPW_ROWS, (c + 1) * SPW_ROWS)
+// and columns [g * SPW_COLS, (g + 1) * SPW_COLS): wp[c * k + b].
+constexpr int SPW_ROWS = 4 * NT, SPW_COLS = 4;
+__global__ __launch_bounds__(NT) void spec_wp_k(int n, const double *__restrict__ U, int k,
+                                                const double *__restrict__ r, double *__restrict__ wp,
+                                                const Ctl *ctl) {
+    __shared__ double red[NT / 64][SPW_COLS];
+    if (cld(&ctl->done)) return;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int b0 = blockIdx.y * SPW_COLS;
+    double acc[SPW_COLS];
+#pragma unroll
+    for (int c = 0; c < SPW_COLS; ++c) acc[c] = 0.0;
+#pragma unroll
+    for (int q = 0; q < SPW_ROWS / NT; ++q) {
+        const int i = blockIdx.x * SPW_ROWS + q * NT + threadIdx.x;
+        if (i < n) {
+            const double ri = r[i];
+#pragma unroll
+            for (int c = 0; c < SPW_COLS; ++c)
+                if (b0 + c < k) acc[c] += U[(size_t)(b0 + c) * n + i] * ri;
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < SPW_COLS; ++c) {
+        const double v = wave_sum(acc[c]);
+        if (lane == 0) red[wv][c] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < SPW_COLS && b0 + (int)threadIdx.x < k) {
+        double s = red[0][threadIdx.x];
+#pragma unroll
+        for (int w = 1; w < NT / 64; ++w) s += red[w][threadIdx.x];
+        wp[(size_t)blockIdx.x * k + b0 + threadIdx.x] = s;
+    }
 }
 
 // a += st d; r -= st Sd (st = rz / d.Sd); partial |r|^2
