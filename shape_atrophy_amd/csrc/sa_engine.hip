@@ -1074,7 +1074,14 @@ struct Engine : EngineBase {
             CK(hipStreamSynchronize(s));
             return info == 0;
         };
-        if (!syevd(G, m) || !(ev[m - 1] > 0.0)) return;
+        auto finite = [](const std::vector<double> &A) {
+            for (double v : A)
+                if (!std::isfinite(v)) return false;
+            return true;
+        };
+        // a solve that went non-finite leaves nothing to harvest
+        if (!finite(G) || !finite(H)) return;
+        if (!syevd(G, m) || !(ev[m - 1] > 0.0) || !finite(G)) return;
         std::vector<int> keep;
         for (int i = 0; i < m; ++i)
             if (ev[i] > 1e-10 * ev[m - 1]) keep.push_back(i);
@@ -1099,7 +1106,7 @@ struct Engine : EngineBase {
                 for (int i = 0; i < m; ++i) t += P[i + (size_t)a * m] * HP[i + (size_t)b * m];
                 Hr[a + (size_t)b * r] = Hr[b + (size_t)a * r] = t;
             }
-        if (!syevd(Hr, r)) return;
+        if (!finite(Hr) || !syevd(Hr, r) || !finite(Hr)) return;
         const double theta = ev[r - 1 - kk];
         if (!(theta > 0.0) || !(ev[r - 1] > theta)) return;
         // W = B diag(sc) P c_top (m x kk); U = D W
@@ -1168,7 +1175,10 @@ struct Engine : EngineBase {
         int it0 = 0;
         int chunk = std::max(4, std::min(64, chunkHint));
         while (true) {
-            int itEnd = std::min(maxIt, it0 + chunk);
+            // the counter check that ends a solve whose budget is spent runs
+            // at the start of an iteration: iteration maxIt is launched too
+            // (its first kernel marks the solve done, the rest are no-ops)
+            int itEnd = std::min(maxIt + 1, it0 + chunk);
             if (timer.on) timer.ensure(itEnd - it0);
             for (int it = it0; it < itEnd; ++it)
                 cg_iter(md, q, tauNow, geoNow, xs, timer.on ? &timer.ev[6 * (it - it0)] : nullptr);

@@ -232,3 +232,20 @@ def test_spectral_preconditioner(precision, monkeypatch):
     assert plain["status"] == 0 and spec["status"] == 0
     assert abs(spec["obj"] - plain["obj"]) <= 1e-9 * abs(plain["obj"])
     assert spec["pcg_iters"] < 0.9 * plain["pcg_iters"]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_control_point_budget_spent_ends_the_solve(precision):
+    """A control-point solve whose CG budget (cgItrMax) runs out ends with a
+    failed status (objective DBL_MAX, failReason budget) instead of waiting
+    for a convergence that never comes."""
+    p = small_shell()
+    V = p["q0"].shape[1]
+    p["ctlIdx"] = np.sort(np.random.default_rng(3).choice(V, V // 3, replace=False)).astype(np.int32)
+    p["params"] = dict(p["params"], cgItrMax=3)
+    print(f"[gpu] budget {precision} V={V}", flush=True)
+    r = api.Session(p, precision=precision).run(p["tauVar"][:, 0], p["tauPrm"])
+    print("status", r["status"], "fail", r["fail_reason"], "steps", r["steps"], "iters", r["pcg_iters"])
+    assert r["status"] != 0
+    assert r["obj"] == np.finfo(np.float64).max
+    assert r["fail_reason"] == 2   # SA_FAIL_BUDGET
