@@ -940,6 +940,7 @@ struct Engine : EngineBase {
     // HIP graphs of GRAPH_ITERS CG iterations, one per operand set (captured
     // on first use, kept for the session: the buffers never move)
     static constexpr int GRAPH_ITERS = 16;
+    static constexpr int TIMER_EVERY = 4;
     struct CgGraph {
         KMode md;
         const double *q, *tau, *geo;
@@ -1180,12 +1181,16 @@ struct Engine : EngineBase {
             // (its first kernel marks the solve done, the rest are no-ops)
             int itEnd = std::min(maxIt + 1, it0 + chunk);
             if (timer.on) timer.ensure(itEnd - it0);
+            // the kernel timer samples every TIMER_EVERY-th iteration (its
+            // events serialise the launches around them)
             for (int it = it0; it < itEnd; ++it)
-                cg_iter(md, q, tauNow, geoNow, xs, timer.on ? &timer.ev[6 * (it - it0)] : nullptr);
+                cg_iter(md, q, tauNow, geoNow, xs,
+                        timer.on && it % TIMER_EVERY == 0 ? &timer.ev[6 * (it - it0)] : nullptr);
             sync_ctl(ctl.p, &h);
             if (timer.on) {
                 int last = h.done ? h.iters : itEnd;   // iterations it < last applied S
                 for (int it = it0; it < std::min(itEnd, last); ++it) {
+                    if (it % TIMER_EVERY) continue;
                     hipEvent_t *ev = &timer.ev[6 * (it - it0)];
                     float a0, a1, a2;
                     CK(hipEventElapsedTime(&a0, ev[0], ev[1]));
