@@ -206,6 +206,11 @@ struct Engine : EngineBase {
     // tables
     DBuf<int> elmVtx, vtxElm, frmVtx, tsvVtx, btmVtx, vtxBtm, femIfo, rowPtr, colIdx, diagIdx, vfdVtx, vfdAdj, elmLyr,
         ctlIdx;
+    // vertex-ordered element forces of the operator apply: elmSlot[l E + e]
+    // is the position of (e, l) in the concatenated vtxElmMat lists,
+    // vtxStart[i] where vertex i's list starts (absent when an (e, l) is
+    // listed twice: the element-ordered gather then runs instead)
+    DBuf<int> elmSlot, vtxStart;
     DBuf<double> q0, dmod, tCen, tDir, tVol;
     // state
     DBuf<double> lmk, tau, geo[2], femElm, ppd[2], lhs, ya, ra, f, x, a, r, Kid, Kr, d, Ad, frc, pfrc, meas;
@@ -216,6 +221,7 @@ struct Engine : EngineBase {
     // control-point form
     DBuf<double> qCur, tauCur, qc, Kcc, Kc, Mpre, Mcol, cG, cR, cZ, cD, cS, cY, cK, cRout, cDelta, cZero, part, partS;
     DBuf<double> cA, cA0, cAm1, cAm2, cAm3;   // alpha, its value entering a step, the last three solutions
+    DBuf<double> cR2, cD2;       // second residual / direction buffers (iteration parity, fused CG)
     DBuf<double> A64, At64;      // stored K_xc / K_cx, FP64
     DBuf<double> Kst;            // all-landmark form: stored K (V x V, panel-major FP64), rebuilt per step
     DBuf<double> aM1, aM2, aM3;  // all-landmark form: the last three alpha = K^-1 v
@@ -404,6 +410,26 @@ struct Engine : EngineBase {
         long long &tot = bytes;
         elmVtx.alloc(hElm.size(), tot); elmVtx.upload(hElm.data(), hElm.size(), s);
         vtxElm.alloc(hVtxElm.size(), tot); vtxElm.upload(hVtxElm.data(), hVtxElm.size(), s);
+        {
+            std::vector<int> hSlot((size_t)(D + 1) * E, -1), hStart((size_t)V + 1, 0);
+            bool once = true;
+            for (int i = 0; i < V; ++i) {
+                const int adj = hVtxElm[i];
+                hStart[i + 1] = hStart[i] + adj;
+                for (int a = 0; a < adj && once; ++a) {
+                    const int e = hVtxElm[(size_t)(1 + 2 * a) * V + i], l = hVtxElm[(size_t)(2 + 2 * a) * V + i];
+                    int &sl = hSlot[(size_t)l * E + e];
+                    if (sl >= 0) once = false;
+                    sl = hStart[i] + a;
+                }
+            }
+            // the slot layout must fit the force buffer ((D + 1) D doubles per element)
+            if (once && hStart[V] <= (D + 1) * E) {
+                elmSlot.alloc(hSlot.size(), tot); elmSlot.upload(hSlot.data(), hSlot.size(), s);
+                vtxStart.alloc(hStart.size(), tot); vtxStart.upload(hStart.data(), hStart.size(), s);
+                CK(hipStreamSynchronize(s));   // the host vectors go out of scope
+            }
+        }
         frmVtx.alloc(hFrm.size(), tot); frmVtx.upload(hFrm.data(), hFrm.size(), s);
         tsvVtx.alloc(hTsv.size(), tot); tsvVtx.upload(hTsv.data(), hTsv.size(), s);
         femIfo.alloc(hFemIfo.size(), tot); femIfo.upload(hFemIfo.data(), hFemIfo.size(), s);
@@ -471,6 +497,7 @@ struct Engine : EngineBase {
             qCur.alloc((size_t)V * D, tot); tauCur.alloc(V, tot);
             cA.alloc(nc, tot); cA0.alloc(nc, tot); cAm1.alloc(nc, tot); cAm2.alloc(nc, tot); cAm3.alloc(nc, tot);
             cG.alloc(nc, tot); cR.alloc(nc, tot); cZ.alloc(nc, tot); cD.alloc(nc, tot);
+            cR2.alloc(nc, tot); cD2.alloc(nc, tot);
             cS.alloc(nc, tot); cY.alloc(nc, tot); cK.alloc(nc, tot);
             cRout.alloc(nc, tot); cDelta.alloc(nc, tot); cZero.alloc(nc, tot);
             CK(hipMemsetAsync(cZero.p, 0, sizeof(double) * nc, s));
@@ -481,7 +508,7 @@ struct Engine : EngineBase {
             if (specEnabled) {
                 recD.alloc(nc * SPEC_M, tot); recSD.alloc(nc * SPEC_M, tot); recR.alloc(nc * SPEC_M, tot);
                 specU.alloc(nc * SPEC_K, tot); specC.alloc(SPEC_K, tot);
-                specW.alloc((size_t)SPEC_K * gridOf((long)nc, SPW_ROWS), tot);
+                specW.alloc((size_t)SPEC_K * gridOf((long)nc), tot);
                 specA.alloc(2 * (size_t)SPEC_M * SPEC_M + 2 * SPEC_M, tot);
                 specInfo.alloc(1, tot);
             }
@@ -609,12 +636,16 @@ struct Engine : EngineBase {
         CHECK_LAUNCH();
     }
 
+    // element pass of E x with the geometry geoNow; the forces land in vertex
+    // order when the slot map exists
     void elastic(const double *xin, const double *tauNow, const double *geoNow, const Ctl *c) {
         int g = gridOf(E);
         if (elaType == SA_ELA_FRAME)
-            elastic_elem_k<D, false, double><<<g, NT, 0, s>>>(E, V, elmVtx.p, geoNow, xin, tauNow, elmLyr.p, dmod.p, frc.p, c);
+            elastic_elem_k<D, false, double><<<g, NT, 0, s>>>(E, V, elmVtx.p, geoNow, xin, tauNow, elmLyr.p, dmod.p,
+                                                              frc.p, c, elmSlot.p);
         else
-            elastic_elem_k<D, true, double><<<g, NT, 0, s>>>(E, V, elmVtx.p, geoNow, xin, tauNow, elmLyr.p, dmod.p, frc.p, c);
+            elastic_elem_k<D, true, double><<<g, NT, 0, s>>>(E, V, elmVtx.p, geoNow, xin, tauNow, elmLyr.p, dmod.p,
+                                                             frc.p, c, elmSlot.p);
         CHECK_LAUNCH();
         if (btmWgt != 0.0 && B > 0) {
             pen_elem_k<D><<<gridOf(B), NT, 0, s>>>(B, V, btmVtx.p, meas.p, xin, pfrc.p, c);
@@ -626,7 +657,7 @@ struct Engine : EngineBase {
     void gather_op(const double *u, const double *w, double *out, double *out2, double *P) {
         const double *pf = (btmWgt != 0.0 && B > 0) ? pfrc.p : nullptr;
         gather_op_k<D, MODE><<<gridOf(V), NT, 0, s>>>(V, E, vtxElm.p, frc.p, BL, B, vtxBtm.p, pf, btmWgt, ldmWgt, u, w,
-                                                     out, out2, P, ctl.p);
+                                                     out, out2, P, ctl.p, vtxStart.p);
         CHECK_LAUNCH();
     }
 
@@ -801,11 +832,14 @@ struct Engine : EngineBase {
     // not recording)
     bool spec_active(KMode md) const { return specOn && !recActive && specMode == (int)md; }
 
-    void prec_dot(KMode md, const double *r, double *z) {
+    // wpReady > 0: the U^T r partials (that many blocks) are already in specW
+    // (ctl_update_k wrote them); else spec_wp_k computes them here
+    void prec_dot(KMode md, const double *r, double *z, int wpReady = 0) {
         const int nc = NC * D;
         const bool sp = spec_active(md);
-        const int nPw = gridOf(nc, SPW_ROWS);
-        if (sp) {
+        int nPw = wpReady;
+        if (sp && !wpReady) {
+            nPw = gridOf(nc, SPW_ROWS);
             spec_wp_k<<<dim3(nPw, gridOf(specK, SPW_COLS)), NT, 0, s>>>(nc, specU.p, specK, r, specW.p, ctl.p);
             CHECK_LAUNCH();
         }
@@ -863,8 +897,17 @@ struct Engine : EngineBase {
     // adv: control block whose iteration counter the stored product advances
     // (first kernel of a CG iteration); sum = false leaves the stored
     // product's slice partials in partS for a fused consumer.
+    // dir: the product's input is the CG direction of iteration dir->j,
+    // formed while staging it (kstore_mv_k's dz ... dout arguments)
+    struct DirArgs {
+        const double *z, *P;
+        int nP;
+        double *hist;
+        int j;
+        double *dout;
+    };
     void kx_product(KMode md, bool xc, const double *q, const double *u, double *y, double *P, const Ctl *c,
-                    hipEvent_t *ev = nullptr, Ctl *adv = nullptr, bool sum = true) {
+                    hipEvent_t *ev = nullptr, Ctl *adv = nullptr, bool sum = true, const DirArgs *dir = nullptr) {
         if (md == KMode::MatrixFree64) {
             if (ev) CK(hipEventRecord(ev[0], s));
             if (xc) kcross(V, q, NC, qc.p, u, y, 1, P, c);
@@ -877,10 +920,14 @@ struct Engine : EngineBase {
         if (ev) CK(hipEventRecord(ev[0], s));
         if (md == KMode::Stored64) {
             dim3 g(gridOf(nr, panel_rows<double>()), sl);
-            kstore_mv_k<D, double><<<g, NT, ks_lds(ncol, sl), s>>>(nr, ncol, xc ? A64.p : At64.p, u, partS.p, c, adv);
+            kstore_mv_k<D, double><<<g, NT, ks_lds(ncol, sl), s>>>(
+                nr, ncol, xc ? A64.p : At64.p, u, partS.p, c, adv, dir ? dir->z : nullptr, dir ? dir->P : nullptr,
+                dir ? dir->nP : 0, dir ? dir->hist : nullptr, dir ? dir->j : 0, dir ? dir->dout : nullptr);
         } else {
             dim3 g(gridOf(nr, panel_rows<float>()), sl);
-            kstore_mv_k<D, float><<<g, NT, ks_lds(ncol, sl), s>>>(nr, ncol, xc ? A32.p : At32.p, u, partS.p, c, adv);
+            kstore_mv_k<D, float><<<g, NT, ks_lds(ncol, sl), s>>>(
+                nr, ncol, xc ? A32.p : At32.p, u, partS.p, c, adv, dir ? dir->z : nullptr, dir ? dir->P : nullptr,
+                dir ? dir->nP : 0, dir ? dir->hist : nullptr, dir ? dir->j : 0, dir ? dir->dout : nullptr);
         }
         CHECK_LAUNCH();
         if (ev) CK(hipEventRecord(ev[1], s));
@@ -899,8 +946,8 @@ struct Engine : EngineBase {
     bool fused_iter(KMode md) const { return md != KMode::MatrixFree64 && !(md == KMode::Stored32 && Kc.p); }
 
     void S_apply(KMode md, const double *q, const double *tauNow, const double *geoNow, const double *u,
-                 hipEvent_t *ev, bool fused = false) {
-        kx_product(md, true, q, u, Kr.p, nullptr, ctl.p, ev, fused ? ctl.p : nullptr);   // w = K_xc u
+                 hipEvent_t *ev, bool fused = false, const DirArgs *dir = nullptr) {
+        kx_product(md, true, q, u, Kr.p, nullptr, ctl.p, ev, fused ? ctl.p : nullptr, true, dir);   // w = K_xc u
         if (ev) CK(hipEventRecord(ev[2], s));
         elastic(Kr.p, tauNow, geoNow, ctl.p);
         gather_op<2>(nullptr, nullptr, Ad.p, nullptr, nullptr);                 // B w
@@ -914,23 +961,41 @@ struct Engine : EngineBase {
     // S d, step, preconditioned residual, new direction; every kernel leaves
     // at once when the solve is done, so iterations queued past the end are
     // no-ops.  ev: optional per-iteration events (6).
-    void cg_iter(KMode md, const double *q, const double *tauNow, const double *geoNow, double *xs, hipEvent_t *ev) {
+    //
+    // Fused form (stored products): iteration j keeps its direction and
+    // residual in the buffers of parity j (cD/cR even, cD2/cR2 odd).  The
+    // K_xc product forms d_j = z_j + beta d_(j-1) while staging it (the
+    // start of the solve formed d_0), ctl_update_k writes r_(j+1) to the
+    // other buffer and, with the spectral term, the partials of U^T r_(j+1);
+    // no separate direction kernel.
+    void cg_iter(KMode md, const double *q, const double *tauNow, const double *geoNow, double *xs, hipEvent_t *ev,
+                 int j) {
         const int nc = NC * D, gC = gridOf(nc);
-        int gS = gC;   // d.Sd partials
         if (fused_iter(md)) {
-            S_apply(md, q, tauNow, geoNow, cD.p, ev, true);
-            gS = gridOf(nc, 64);
-            slice_combine_k<D><<<gS, NT, 0, s>>>(NC, V, slicesT, partS.p, Kr.p, ctlIdx.p, cD.p, ldmWgt, knlEps, cS.p,
-                                                 P1.p, recActive ? recD.p : nullptr, recSD.p, recR.p, cR.p, SPEC_M,
+            double *dj = (j & 1) ? cD2.p : cD.p, *dprev = (j & 1) ? cD.p : cD2.p;
+            double *rj = (j & 1) ? cR2.p : cR.p, *rnext = (j & 1) ? cR.p : cR2.p;
+            DirArgs dir{cZ.p, P3.p, gridOf(nc, 64), hist.p, j, dj};
+            S_apply(md, q, tauNow, geoNow, j > 0 ? dprev : dj, ev, true, j > 0 ? &dir : nullptr);
+            const int gS = gridOf(nc, 64);   // d.Sd partials
+            slice_combine_k<D><<<gS, NT, 0, s>>>(NC, V, slicesT, partS.p, Kr.p, ctlIdx.p, dj, ldmWgt, knlEps, cS.p,
+                                                 P1.p, recActive ? recD.p : nullptr, recSD.p, recR.p, rj, SPEC_M,
                                                  ctl.p);
-        } else {
+            const bool sp = spec_active(md);
+            ctl_update_k<<<dim3(gC, sp ? gridOf(specK, SPW_COLS) : 1), NT, 0, s>>>(
+                nc, P1.p, gS, hist.p, xs, rj, rnext, dj, cS.p, P2.p, sp ? specU.p : nullptr, specK, specW.p, ctl.p);
+            CHECK_LAUNCH();
+            prec_dot(md, rnext, cZ.p, sp ? gC : 0);
+            return;
+        }
+        {
             ctl_next_k<<<1, 1, 0, s>>>(ctl.p);
             const double *kdv = md == KMode::Stored32 && Kc.p ? cK.p : nullptr;
             S_apply(md, q, tauNow, geoNow, cD.p, ev);
             ctl_combine_k<D, 0><<<gC, NT, 0, s>>>(NC, V, cY.p, kdv, Kr.p, ctlIdx.p, cD.p, nullptr, ldmWgt, knlEps,
                                                  cS.p, P1.p, ctl.p);
         }
-        ctl_update_k<<<gC, NT, 0, s>>>(nc, P1.p, gS, hist.p, xs, cR.p, cD.p, cS.p, P2.p, ctl.p);
+        ctl_update_k<<<gC, NT, 0, s>>>(nc, P1.p, gC, hist.p, xs, cR.p, cR.p, cD.p, cS.p, P2.p, nullptr, 0, nullptr,
+                                       ctl.p);
         CHECK_LAUNCH();
         prec_dot(md, cR.p, cZ.p);
         ctl_dir_k<<<gC, NT, 0, s>>>(nc, P3.p, gridOf(nc, 64), hist.p, cZ.p, cD.p, ctl.p);
@@ -960,7 +1025,7 @@ struct Engine : EngineBase {
         hipGraph_t gr = nullptr;
         CK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         try {
-            for (int k = 0; k < GRAPH_ITERS; ++k) cg_iter(md, q, tauNow, geoNow, xs, nullptr);
+            for (int k = 0; k < GRAPH_ITERS; ++k) cg_iter(md, q, tauNow, geoNow, xs, nullptr, k);
         } catch (...) {
             (void)hipStreamEndCapture(s, &gr);
             if (gr) (void)hipGraphDestroy(gr);
@@ -1160,7 +1225,9 @@ struct Engine : EngineBase {
         CHECK_LAUNCH();
         if (maxIt < 1) ctl_next_k<<<1, 1, 0, s>>>(ctl.p);   // no budget: the counter check ends the solve here
         Ctl &h = hctl[0];
-        if (!timer.on && useGraphs) {
+        // (graph replays would repeat the captured iteration indices, which
+        // the fused iteration passes as kernel arguments)
+        if (!timer.on && useGraphs && !fused_iter(md)) {
             hipGraphExec_t ex = cg_graph(md, q, tauNow, geoNow, xs);
             // first sync after about the previous solve's iteration count
             int launches = std::max(1, std::min(8, chunkHint / GRAPH_ITERS));
@@ -1185,7 +1252,7 @@ struct Engine : EngineBase {
             // events serialise the launches around them)
             for (int it = it0; it < itEnd; ++it)
                 cg_iter(md, q, tauNow, geoNow, xs,
-                        timer.on && it % TIMER_EVERY == 0 ? &timer.ev[6 * (it - it0)] : nullptr);
+                        timer.on && it % TIMER_EVERY == 0 ? &timer.ev[6 * (it - it0)] : nullptr, it);
             sync_ctl(ctl.p, &h);
             if (timer.on) {
                 int last = h.done ? h.iters : itEnd;   // iterations it < last applied S
@@ -1256,6 +1323,7 @@ struct Engine : EngineBase {
         kx_product(exact, false, q, f.p, cG.p, P0.p, nullptr);
         int status;
         sweeps = 0;
+        bool kaReady = false;   // Kr holds K_xc alpha (FP64) for the final alpha
         if (!mixedStep) {
             if (precision == SA_PREC_MIXED) ++fp64Steps;
             chunkHint = lastIters - 4;
@@ -1293,7 +1361,7 @@ struct Engine : EngineBase {
                 CHECK_LAUNCH();
                 sync_ctl(ctl.p, &hctl[0]);
                 double rr = hctl[0].bb;
-                if (rr < cgTolSqu * gg) { status = 0; break; }
+                if (rr < cgTolSqu * gg) { status = 0; kaReady = true; break; }
                 // a sweep that does not cut the FP64 residual tenfold: the
                 // FP32 operator is too far from the FP64 one (ill-conditioned
                 // K_cc); the step is finished by the FP64 form
@@ -1339,7 +1407,9 @@ struct Engine : EngineBase {
             }
         }
         if (status == 0) {
-            kx_product(exact, true, q, cA.p, x.p, nullptr, nullptr);            // v = K_xc alpha
+            // v = K_xc alpha; the converged residual check already formed it in FP64
+            if (kaReady) CK(hipMemcpyAsync(x.p, Kr.p, sizeof(double) * V * D, hipMemcpyDeviceToDevice, s));
+            else kx_product(exact, true, q, cA.p, x.p, nullptr, nullptr);
             int info[2] = {0, 0};
             CK(hipMemcpyAsync(info, potrfInfo.p, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
             CK(hipStreamSynchronize(s));
@@ -1467,7 +1537,9 @@ struct Engine : EngineBase {
             case SA_PROBE_ELASTIC:
                 CK(hipMemcpyAsync(x.p, in, sizeof(double) * n, hipMemcpyHostToDevice, s));
                 elastic(x.p, tau.p, geo[0].p, ctl.p);
-                gather_k<D><<<gridOf(V), NT, 0, s>>>(V, E, vtxElm.p, frc.p, f.p, P0.p);
+                // E x alone (no penalty term)
+                gather_op_k<D, 2><<<gridOf(V), NT, 0, s>>>(V, E, vtxElm.p, frc.p, BL, B, vtxBtm.p, nullptr, 0.0, 0.0,
+                                                           nullptr, nullptr, f.p, nullptr, nullptr, ctl.p, vtxStart.p);
                 CHECK_LAUNCH();
                 CK(hipMemcpyAsync(out, f.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
                 break;
@@ -1716,7 +1788,7 @@ long long sa_session_footprint(const sa_mesh *m, const sa_target *t, const sa_pa
         // rows padded to whole panels of 256 (FP32) / 128 (FP64) rows
         auto pad = [](long long n, long long pr) { return (n + pr - 1) / pr * pr; };
         // K_cc, rocBLAS output, preconditioner, vectors, slice partials, cross partials
-        b += 8 * (2 * C * C + pad(C, 128) * C + 12 * C * D + 256 * C * D + 64 * D * V + 32 * C * D);
+        b += 8 * (2 * C * C + pad(C, 128) * C + 14 * C * D + 256 * C * D + 64 * D * V + 32 * C * D);
         if (p->precision == SA_PREC_MIXED) b += 4 * (pad(V, 256) * C + pad(C, 256) * V) + 8 * pad(C, 128) * C;
         else if ((pad(V, 128) * C + pad(C, 128) * V) * 8 <= (48LL << 30)) b += 8 * (pad(V, 128) * C + pad(C, 128) * V);
         // spectral preconditioner: recorded Krylov basis, Ritz vectors, projected matrices

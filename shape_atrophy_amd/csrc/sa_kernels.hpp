@@ -245,12 +245,15 @@ __global__ __launch_bounds__(NT) void gather_k(int V, int E, const int *__restri
 // computeStrainTensor.cu:4-53|4-61 + frmElasticComputeKernel
 // (applyFrameElasticOperator.cu:41-85|113-164).  ISO selects the per-layer
 // isotropic law sigma = (1 - 0.5 u_bar)(lambda tr(eps) I + 2 mu eps).
+// slot non-null: the force on local
+// vertex l goes to frc[slot[l E + e] * D + c], the position of (e, l) in the
+// vertex-ordered list gather_op_k reads contiguously (slot < 0: not listed).
 template <int D, bool ISO, typename T>
 __global__ __launch_bounds__(NT) void elastic_elem_k(int E, int V, const int *__restrict__ elmVtx,
                                                      const double *__restrict__ geo, const T *__restrict__ x,
                                                      const double *__restrict__ tau, const int *__restrict__ elmLyr,
                                                      const double *__restrict__ mod, T *__restrict__ frc,
-                                                     const Ctl *ctl) {
+                                                     const Ctl *ctl, const int *__restrict__ slot = nullptr) {
     using G = GeoLayout<D>;
     if (ctl && cld(&ctl->done)) return;
     int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -263,7 +266,7 @@ __global__ __launch_bounds__(NT) void elastic_elem_k(int E, int V, const int *__
     for (int a = 0; a < D; ++a)
 #pragma unroll
         for (int b = 0; b < D; ++b) inv[a][b] = geo[(G::INV + a * D + b) * (size_t)E + e];
-    double vol = elem_vol<D>(geo[G::DET * (size_t)E + e]);
+    const double vol = elem_vol<D>(geo[G::DET * (size_t)E + e]);
     double x0[D];
 #pragma unroll
     for (int c = 0; c < D; ++c) x0[c] = (double)x[c * V + vtx[0]];
@@ -313,6 +316,7 @@ __global__ __launch_bounds__(NT) void elastic_elem_k(int E, int V, const int *__
         for (int m = 0; m <= D; ++m) md[m] = cld(&mod[m]);
         frame_stress<D>(eps, f1, f2, md, sig);
     }
+    double fo[D + 1][D];
 #pragma unroll
     for (int c = 0; c < D; ++c) {
         double s0 = 0.0;
@@ -322,9 +326,23 @@ __global__ __launch_bounds__(NT) void elastic_elem_k(int E, int V, const int *__
 #pragma unroll
             for (int j = 0; j < D; ++j) s += inv[k][j] * sig[j][c];
             s0 -= s;
-            frc[((size_t)e * (D + 1) + k + 1) * D + c] = (T)(s * vol);
+            fo[k + 1][c] = s * vol;
         }
-        frc[(size_t)e * (D + 1) * D + c] = (T)(s0 * vol);
+        fo[0][c] = s0 * vol;
+    }
+    if (slot) {
+#pragma unroll
+        for (int l = 0; l <= D; ++l) {
+            const int sl = slot[l * E + e];
+            if (sl < 0) continue;
+#pragma unroll
+            for (int c = 0; c < D; ++c) frc[(size_t)sl * D + c] = (T)fo[l][c];
+        }
+    } else {
+#pragma unroll
+        for (int l = 0; l <= D; ++l)
+#pragma unroll
+            for (int c = 0; c < D; ++c) frc[((size_t)e * (D + 1) + l) * D + c] = (T)fo[l][c];
     }
 }
 
@@ -370,7 +388,8 @@ __global__ __launch_bounds__(NT) void gather_op_k(int V, int E, const int *__res
                                                   const int *__restrict__ vtxBtm, const double *__restrict__ pfrc,
                                                   double btmWgt, double ldmWgt, const double *__restrict__ u,
                                                   const double *__restrict__ w, double *__restrict__ out,
-                                                  double *__restrict__ out2, double *__restrict__ P, const Ctl *ctl) {
+                                                  double *__restrict__ out2, double *__restrict__ P, const Ctl *ctl,
+                                                  const int *__restrict__ vstart = nullptr) {
     __shared__ double red[NT / 64];
     if (cld(&ctl->done)) return;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -379,11 +398,20 @@ __global__ __launch_bounds__(NT) void gather_op_k(int V, int E, const int *__res
         double acc[D];
 #pragma unroll
         for (int c = 0; c < D; ++c) acc[c] = 0.0;
-        int adj = vtxElm[i];
-        for (int a = 0; a < adj; ++a) {
-            int e = vtxElm[(1 + 2 * a) * V + i], l = vtxElm[(2 + 2 * a) * V + i];
+        if (vstart) {
+            // forces in vertex order (elastic_elem_k with slots): this
+            // vertex's list is contiguous, in the order vtxElmMat lists it
+            const int a1 = vstart[i + 1];
+            for (int a = vstart[i]; a < a1; ++a)
 #pragma unroll
-            for (int c = 0; c < D; ++c) acc[c] += frc[((size_t)e * (D + 1) + l) * D + c];
+                for (int c = 0; c < D; ++c) acc[c] += frc[(size_t)a * D + c];
+        } else {
+            int adj = vtxElm[i];
+            for (int a = 0; a < adj; ++a) {
+                int e = vtxElm[(1 + 2 * a) * V + i], l = vtxElm[(2 + 2 * a) * V + i];
+#pragma unroll
+                for (int c = 0; c < D; ++c) acc[c] += frc[((size_t)e * (D + 1) + l) * D + c];
+            }
         }
         if (pfrc && i < BL) {
             double pa[D];
@@ -798,7 +826,11 @@ __device__ inline void fma_rows(double (&a)[R][D], int r0, int c, const double2 
 template <int D, typename T>
 __global__ __launch_bounds__(NT) void kstore_mv_k(int nr, int nc, const T *__restrict__ M,
                                                   const double *__restrict__ x, double *__restrict__ part,
-                                                  const Ctl *ctl, Ctl *adv = nullptr) {
+                                                  const Ctl *ctl, Ctl *adv = nullptr,
+                                                  const double *__restrict__ dz = nullptr,
+                                                  const double *__restrict__ dP = nullptr, int dnP = 0,
+                                                  double *__restrict__ dhist = nullptr, int dj = 0,
+                                                  double *__restrict__ dout = nullptr) {
     constexpr int PR = panel_rows<T>();
     constexpr int R = PR / 64;             // rows per lane
     constexpr int W = NT / 64;
@@ -811,7 +843,23 @@ __global__ __launch_bounds__(NT) void kstore_mv_k(int nr, int nc, const T *__res
     constexpr int EV = 16 / (int)sizeof(T);       // elements per 16-byte load
     extern __shared__ double sx[];   // D x (je - jb)
     __shared__ double red[W - 1][D][PR];
+    __shared__ double redd[W];
     if (ctl && cld(&ctl->done)) return;
+    double be = 0.0;
+    if (dz) {
+        // the CG direction of iteration dj, d = z + beta d_old with beta =
+        // rz_dj / rz_(dj-1) (rz_dj: the r.z partials dP), staged here and
+        // written to dout by the first row panel (as ctl_dir_k would)
+        const double rz = reduce_partials(dP, dnP, redd);
+        if (rz == 0.0) {
+            if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+                cst(&adv->status, 1); cst(&adv->done, 1); cst(&adv->iters, dj);
+            }
+            return;
+        }
+        be = rz / cld(&dhist[dj - 1]);
+        if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) cst(&dhist[dj], rz);
+    }
     if (adv && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
         // first kernel of a CG iteration: advance the iteration counter (no
         // kernel of this launch reads it); the budget spent stops the solve
@@ -826,7 +874,14 @@ __global__ __launch_bounds__(NT) void kstore_mv_k(int nr, int nc, const T *__res
     const int ns = max(0, je - jb);
     for (int t = threadIdx.x; t < D * ns; t += NT) {
         const int c = t / ns, j = t - c * ns;
-        sx[t] = x[c * nc + jb + j];
+        const size_t o = (size_t)c * nc + jb + j;
+        if (dz) {
+            const double v = dz[o] + be * x[o];
+            sx[t] = v;
+            if (blockIdx.x == 0) dout[o] = v;
+        } else {
+            sx[t] = x[o];
+        }
     }
     __syncthreads();
     const int pw = (ns + W - 1) / W;
@@ -1138,30 +1193,55 @@ __global__ __launch_bounds__(NT) void spec_wp_k(int n, const double *__restrict_
     }
 }
 
-// a += st d; r -= st Sd (st = rz / d.Sd); partial |r|^2
+// a += st d; rout = rin - st Sd (st = rz / d.Sd); partial |rout|^2 (rin and
+// rout may be one buffer when gridDim.y == 1).  Blocks with blockIdx.y = g
+// > 0 only recompute rout for the spectral term: with U (n x k), block
+// (x, g) writes the partials of w = U^T rout for columns 4g .. 4g + 3 to
+// wp[x * k + b].
 __global__ __launch_bounds__(NT) void ctl_update_k(int n, const double *__restrict__ P, int nP,
                                                    const double *__restrict__ hist, double *__restrict__ a,
-                                                   double *__restrict__ r, const double *__restrict__ d, const double *__restrict__ Sd,
-                                                   double *__restrict__ P2, Ctl *ctl) {
+                                                   const double *__restrict__ rin, double *__restrict__ rout,
+                                                   const double *__restrict__ d, const double *__restrict__ Sd,
+                                                   double *__restrict__ P2, const double *__restrict__ U, int k,
+                                                   double *__restrict__ wp, Ctl *ctl) {
     __shared__ double red[NT / 64];
+    __shared__ double redw[NT / 64][SPW_COLS];
     if (cld(&ctl->done)) return;
     const int it = cld(&ctl->it);
     double dSd = reduce_partials(P, nP, red);
     if (dSd == 0.0) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) { cst(&ctl->status, 1); cst(&ctl->done, 1); cst(&ctl->iters, it); }
+        if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+            cst(&ctl->status, 1); cst(&ctl->done, 1); cst(&ctl->iters, it);
+        }
         return;
     }
     double st = cld(&hist[it]) / dSd;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    double ss = 0.0;
-    if (i < n) {
-        a[i] += st * d[i];
-        double ri = r[i] - st * Sd[i];
-        r[i] = ri;
-        ss = ri * ri;
+    const double ri = i < n ? rin[i] - st * Sd[i] : 0.0;
+    if (blockIdx.y == 0) {
+        if (i < n) {
+            a[i] += st * d[i];
+            rout[i] = ri;
+        }
+        double t = block_sum(ri * ri, red);
+        if (threadIdx.x == 0) P2[blockIdx.x] = t;
     }
-    double t = block_sum(ss, red);
-    if (threadIdx.x == 0) P2[blockIdx.x] = t;
+    if (U) {
+        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+        const int b0 = blockIdx.y * SPW_COLS;
+#pragma unroll
+        for (int c = 0; c < SPW_COLS; ++c) {
+            const double v = wave_sum(i < n && b0 + c < k ? U[(size_t)(b0 + c) * n + i] * ri : 0.0);
+            if (lane == 0) redw[wv][c] = v;
+        }
+        __syncthreads();
+        if (threadIdx.x < SPW_COLS && b0 + (int)threadIdx.x < k) {
+            double s = redw[0][threadIdx.x];
+#pragma unroll
+            for (int w = 1; w < NT / 64; ++w) s += redw[w][threadIdx.x];
+            wp[(size_t)blockIdx.x * k + b0 + threadIdx.x] = s;
+        }
+    }
 }
 
 // after z = M^{-1} r: stop on |r|^2 < tol, else partial r.z
