@@ -52,6 +52,10 @@ std::mutex g_libMu;
 #ifndef SA_SPEC_K
 #define SA_SPEC_K 96
 #endif
+// how M = X^T X is formed: 0 dgemm, 1 dsyrk, 2 lauum (in place)
+#ifndef SA_MPRE_BUILD
+#define SA_MPRE_BUILD 1
+#endif
 // steps a control-point preconditioner is reused at most
 #ifndef SA_PREC_MAX_AGE
 #define SA_PREC_MAX_AGE 8
@@ -237,7 +241,7 @@ struct Engine : EngineBase {
     static constexpr int SPEC_M = SA_SPEC_M;   // iterations recorded
     static constexpr int SPEC_K = SA_SPEC_K;   // Ritz vectors kept at most
     static_assert(SPEC_K <= SPEC_KMAX, "SA_SPEC_K above the kernels' LDS capacity");
-    DBuf<double> recD, recSD, recR, specU, specC, specW, specA;
+    DBuf<double> recD, recSD, recR, specU, specC, specW, specA, specScr;
     DBuf<int> specInfo;
     bool specEnabled = true, specOn = false, specRecord = false, recActive = false;
     int specK = 0, specMode = -1;
@@ -509,7 +513,8 @@ struct Engine : EngineBase {
                 recD.alloc(nc * SPEC_M, tot); recSD.alloc(nc * SPEC_M, tot); recR.alloc(nc * SPEC_M, tot);
                 specU.alloc(nc * SPEC_K, tot); specC.alloc(SPEC_K, tot);
                 specW.alloc((size_t)SPEC_K * gridOf((long)nc), tot);
-                specA.alloc(2 * (size_t)SPEC_M * SPEC_M + 2 * SPEC_M, tot);
+                specA.alloc(6 * (size_t)SPEC_M * SPEC_M + 2 * SPEC_M, tot);
+                specScr.alloc((size_t)(SPEC_CHUNKS + 1) * SPEC_M * SPEC_M, tot);
                 specInfo.alloc(1, tot);
             }
             // stored kernel matrices: FP64 for the plain form, FP32 for the mixed one
@@ -793,10 +798,23 @@ struct Engine : EngineBase {
         zero_upper_k<<<g, NT, 0, s>>>(NC, Kcc.p);
         CHECK_LAUNCH();
         const double one = 1.0, zero = 0.0;
+        // M = X^T X (lower triangle is all panelize_k reads)
+#if SA_MPRE_BUILD == 2
+        if (rocsolver_dlauum(blas, rocblas_fill_lower, NC, Kcc.p, NC) != rocblas_status_success)
+            throw SaError(SA_E_DEVICE, "rocsolver_dlauum failed");
+        const double *Mlow = Kcc.p;
+#elif SA_MPRE_BUILD == 1
+        if (rocblas_dsyrk(blas, rocblas_fill_lower, rocblas_operation_transpose, NC, NC, &one, Kcc.p, NC, &zero,
+                          Mcol.p, NC) != rocblas_status_success)
+            throw SaError(SA_E_DEVICE, "rocblas_dsyrk failed");
+        const double *Mlow = Mcol.p;
+#else
         if (rocblas_dgemm(blas, rocblas_operation_transpose, rocblas_operation_none, NC, NC, NC, &one, Kcc.p, NC,
                           Kcc.p, NC, &zero, Mcol.p, NC) != rocblas_status_success)
             throw SaError(SA_E_DEVICE, "rocblas_dgemm failed");
-        panelize_k<<<g, NT, 0, s>>>(NC, Mcol.p, NC, Mpre.p);
+        const double *Mlow = Mcol.p;
+#endif
+        panelize_k<<<g, NT, 0, s>>>(NC, Mlow, NC, Mpre.p);
         CHECK_LAUNCH();
         if (Mpre32.p) {
             panel_to_float_k<<<g, NT, 0, s>>>(NC, Mpre.p, Mpre32.p);
@@ -1078,21 +1096,53 @@ struct Engine : EngineBase {
     // orthogonality) are dropped; the leading Ritz vectors U = D B P c, with
     // G = Q g Q^T, P = Q g^-1/2 and c the eigenvectors of P^T H P, become the
     // spectral term.
+    // C = recD^T X (m x m) over the nc recorded rows: row chunks as one
+    // batched product (a single m x m x nc product leaves most of the chip
+    // idle), the chunk results summed in chunk order by slice_rows_k
+    static constexpr int SPEC_CHUNKS = 64;
+    void spec_gram(const double *X, double *C, int m) {
+        const int nc = NC * D;
+        const int P = std::max(1, std::min(SPEC_CHUNKS, nc / 128));
+        const int cs = nc / P, rem = nc - P * cs;
+        const size_t mm = (size_t)m * m;
+        double *scr = specScr.p;
+        const double one = 1.0, zero = 0.0;
+        if (rocblas_dgemm_strided_batched(blas, rocblas_operation_transpose, rocblas_operation_none, m, m, cs, &one,
+                                          recD.p, nc, cs, X, nc, cs, &zero, scr, m, (rocblas_stride)mm, P) !=
+            rocblas_status_success)
+            throw SaError(SA_E_DEVICE, "rocblas_dgemm_strided_batched failed");
+        int parts = P;
+        if (rem > 0) {
+            if (rocblas_dgemm(blas, rocblas_operation_transpose, rocblas_operation_none, m, m, rem, &one,
+                              recD.p + (size_t)P * cs, nc, X + (size_t)P * cs, nc, &zero, scr + (size_t)P * mm, m) !=
+                rocblas_status_success)
+                throw SaError(SA_E_DEVICE, "rocblas_dgemm failed");
+            ++parts;
+        }
+        slice_rows_k<<<gridOf((long)mm, 64), NT, 0, s>>>((int)mm, parts, scr, C, nullptr);
+        CHECK_LAUNCH();
+    }
+
+    // Rayleigh-Ritz for M S on the recorded Krylov basis (m iterations).
+    // z_j = M r_j = d_j - beta_{j-1} d_{j-1}, so Z = D B with B upper
+    // bidiagonal; the pencil is H = Z^T S Z = B^T (D^T SD) B against the Gram
+    // matrix G = Z^T M^-1 Z = B^T (D^T R), both scaled to the Lanczos vectors
+    // z_j / sqrt(r_j.z_j).  Directions where G is numerically singular (lost
+    // orthogonality) are dropped; the leading Ritz vectors U = D B P c, with
+    // G = Q g Q^T, P = Q g^-1/2 and c the eigenvectors of P^T H P, become the
+    // spectral term.  The m x m and m x r products run on the device.
     void spec_harvest(KMode md, int m) {
         const int nc = NC * D;
         if (m < 24) return;   // too short a basis: the next solve records again
-        const size_t mm = (size_t)m * m;
-        double *dA1 = specA.p, *dA2 = specA.p + (size_t)SPEC_M * SPEC_M, *dEv = dA2 + (size_t)SPEC_M * SPEC_M;
-        double *dE = dEv + SPEC_M;
+        const size_t mm = (size_t)m * m, MM = (size_t)SPEC_M * SPEC_M;
+        double *dA1 = specA.p, *dA2 = dA1 + MM, *dP = dA2 + MM, *dHP = dP + MM, *dHr = dHP + MM, *dY = dHr + MM;
+        double *dEv = dY + MM, *dE = dEv + SPEC_M;
         std::vector<double> A1(mm), A2(mm), rz(m);
         const double one = 1.0, zero = 0.0;
         {
             std::lock_guard<std::mutex> lk(g_libMu);
-            if (rocblas_dgemm(blas, rocblas_operation_transpose, rocblas_operation_none, m, m, nc, &one, recD.p, nc,
-                              recR.p, nc, &zero, dA1, m) != rocblas_status_success ||
-                rocblas_dgemm(blas, rocblas_operation_transpose, rocblas_operation_none, m, m, nc, &one, recD.p, nc,
-                              recSD.p, nc, &zero, dA2, m) != rocblas_status_success)
-                throw SaError(SA_E_DEVICE, "rocblas_dgemm failed");
+            spec_gram(recR.p, dA1, m);
+            spec_gram(recSD.p, dA2, m);
             CK(hipStreamSynchronize(s));
         }
         CK(hipMemcpyAsync(A1.data(), dA1, sizeof(double) * mm, hipMemcpyDeviceToHost, s));
@@ -1122,32 +1172,36 @@ struct Engine : EngineBase {
                 at(G, i, j) = at(G, j, i) = g;
                 at(H, i, j) = at(H, j, i) = h;
             }
-        // eigenpairs of the Gram matrix; keep directions above 1e-10 of its largest
+        auto finite = [](const double *A, size_t n) {
+            for (size_t i = 0; i < n; ++i)
+                if (!std::isfinite(A[i])) return false;
+            return true;
+        };
+        // a solve that went non-finite leaves nothing to harvest
+        if (!finite(G.data(), mm) || !finite(H.data(), mm)) return;
+        // eigenpairs of a symmetric n x n matrix on the device (in place in A);
+        // eigenvalues ascending into ev
         std::vector<double> ev(m);
-        auto syevd = [&](std::vector<double> &A, int n) -> bool {
+        auto syevd = [&](double *A, int n) -> bool {
             int info = 0;
-            CK(hipMemcpyAsync(dA1, A.data(), sizeof(double) * n * n, hipMemcpyHostToDevice, s));
             {
                 std::lock_guard<std::mutex> lk(g_libMu);
-                if (rocsolver_dsyevd(blas, rocblas_evect_original, rocblas_fill_lower, n, dA1, n, dEv, dE,
+                if (rocsolver_dsyevd(blas, rocblas_evect_original, rocblas_fill_lower, n, A, n, dEv, dE,
                                      specInfo.p) != rocblas_status_success)
                     throw SaError(SA_E_DEVICE, "rocsolver_dsyevd failed");
                 CK(hipStreamSynchronize(s));
             }
-            CK(hipMemcpyAsync(A.data(), dA1, sizeof(double) * n * n, hipMemcpyDeviceToHost, s));
             CK(hipMemcpyAsync(ev.data(), dEv, sizeof(double) * n, hipMemcpyDeviceToHost, s));
             CK(hipMemcpyAsync(&info, specInfo.p, sizeof(int), hipMemcpyDeviceToHost, s));
             CK(hipStreamSynchronize(s));
-            return info == 0;
+            return info == 0 && finite(ev.data(), n);
         };
-        auto finite = [](const std::vector<double> &A) {
-            for (double v : A)
-                if (!std::isfinite(v)) return false;
-            return true;
-        };
-        // a solve that went non-finite leaves nothing to harvest
-        if (!finite(G) || !finite(H)) return;
-        if (!syevd(G, m) || !(ev[m - 1] > 0.0) || !finite(G)) return;
+        // eigenpairs of the Gram matrix; keep directions above 1e-10 of its largest
+        CK(hipMemcpyAsync(dA1, G.data(), sizeof(double) * mm, hipMemcpyHostToDevice, s));
+        if (!syevd(dA1, m) || !(ev[m - 1] > 0.0)) return;
+        CK(hipMemcpyAsync(G.data(), dA1, sizeof(double) * mm, hipMemcpyDeviceToHost, s));
+        CK(hipStreamSynchronize(s));
+        if (!finite(G.data(), mm)) return;
         std::vector<int> keep;
         for (int i = 0; i < m; ++i)
             if (ev[i] > 1e-10 * ev[m - 1]) keep.push_back(i);
@@ -1159,33 +1213,38 @@ struct Engine : EngineBase {
             const double f = 1.0 / std::sqrt(ev[keep[l]]);
             for (int i = 0; i < m; ++i) P[i + (size_t)l * m] = G[i + (size_t)keep[l] * m] * f;
         }
-        std::vector<double> HP((size_t)m * r), Hr((size_t)r * r);
-        for (int l = 0; l < r; ++l)
-            for (int j = 0; j < m; ++j) {
-                const double pj = P[j + (size_t)l * m];
-                if (pj == 0.0) continue;
-                for (int i = 0; i < m; ++i) HP[i + (size_t)l * m] += at(H, i, j) * pj;
-            }
-        for (int b = 0; b < r; ++b)
-            for (int a = 0; a <= b; ++a) {
-                double t = 0.0;
-                for (int i = 0; i < m; ++i) t += P[i + (size_t)a * m] * HP[i + (size_t)b * m];
-                Hr[a + (size_t)b * r] = Hr[b + (size_t)a * r] = t;
-            }
-        if (!finite(Hr) || !syevd(Hr, r) || !finite(Hr)) return;
+        // P^T H P on the device, then its eigenpairs, then Y = P c (the kk
+        // leading eigenvectors, ascending order: columns r - kk .. r - 1)
+        CK(hipMemcpyAsync(dA2, H.data(), sizeof(double) * mm, hipMemcpyHostToDevice, s));
+        CK(hipMemcpyAsync(dP, P.data(), sizeof(double) * m * r, hipMemcpyHostToDevice, s));
+        {
+            std::lock_guard<std::mutex> lk(g_libMu);
+            if (rocblas_dgemm(blas, rocblas_operation_none, rocblas_operation_none, m, r, m, &one, dA2, m, dP, m,
+                              &zero, dHP, m) != rocblas_status_success ||
+                rocblas_dgemm(blas, rocblas_operation_transpose, rocblas_operation_none, r, r, m, &one, dP, m, dHP, m,
+                              &zero, dHr, r) != rocblas_status_success)
+                throw SaError(SA_E_DEVICE, "rocblas_dgemm failed");
+            CK(hipStreamSynchronize(s));
+        }
+        if (!syevd(dHr, r)) return;
         const double theta = ev[r - 1 - kk];
         if (!(theta > 0.0) || !(ev[r - 1] > theta)) return;
-        // W = B diag(sc) P c_top (m x kk); U = D W
-        std::vector<double> Y((size_t)m * kk, 0.0), W((size_t)m * kk), cf(kk);
+        {
+            std::lock_guard<std::mutex> lk(g_libMu);
+            if (rocblas_dgemm(blas, rocblas_operation_none, rocblas_operation_none, m, kk, r, &one, dP, m,
+                              dHr + (size_t)(r - kk) * r, r, &zero, dY, m) != rocblas_status_success)
+                throw SaError(SA_E_DEVICE, "rocblas_dgemm failed");
+            CK(hipStreamSynchronize(s));
+        }
+        std::vector<double> Y((size_t)m * kk), W((size_t)m * kk), cf(kk);
+        CK(hipMemcpyAsync(Y.data(), dY, sizeof(double) * m * kk, hipMemcpyDeviceToHost, s));
+        CK(hipStreamSynchronize(s));
+        if (!finite(Y.data(), Y.size())) return;
+        // W = B diag(sc) Y, largest Ritz value first; U = D W
         for (int l = 0; l < kk; ++l) {
-            const double *c = &Hr[(size_t)(r - 1 - l) * r];
-            for (int a = 0; a < r; ++a) {
-                const double ca = c[a];
-                for (int i = 0; i < m; ++i) Y[i + (size_t)l * m] += P[i + (size_t)a * m] * ca;
-            }
-            for (int i = 0; i < m; ++i) Y[i + (size_t)l * m] *= sc[i];
+            const double *y = &Y[(size_t)(kk - 1 - l) * m];
             for (int i = 0; i < m; ++i)
-                W[i + (size_t)l * m] = Y[i + (size_t)l * m] - (i + 1 < m ? beta[i] * Y[i + 1 + (size_t)l * m] : 0.0);
+                W[i + (size_t)l * m] = sc[i] * y[i] - (i + 1 < m ? beta[i] * sc[i + 1] * y[i + 1] : 0.0);
             cf[l] = theta / ev[r - 1 - l] - 1.0;
         }
         CK(hipMemcpyAsync(dA2, W.data(), sizeof(double) * m * kk, hipMemcpyHostToDevice, s));
@@ -1792,7 +1851,8 @@ long long sa_session_footprint(const sa_mesh *m, const sa_target *t, const sa_pa
         if (p->precision == SA_PREC_MIXED) b += 4 * (pad(V, 256) * C + pad(C, 256) * V) + 8 * pad(C, 128) * C;
         else if ((pad(V, 128) * C + pad(C, 128) * V) * 8 <= (48LL << 30)) b += 8 * (pad(V, 128) * C + pad(C, 128) * V);
         // spectral preconditioner: recorded Krylov basis, Ritz vectors, projected matrices
-        b += 8 * ((3 * SA_SPEC_M + SA_SPEC_K) * C * D + 2 * SA_SPEC_M * SA_SPEC_M + 2 * SA_SPEC_M + 2 * SA_SPEC_K);
+        b += 8 * ((3 * SA_SPEC_M + SA_SPEC_K) * C * D + (6 + 65) * SA_SPEC_M * SA_SPEC_M + 2 * SA_SPEC_M +
+                  2 * SA_SPEC_K);
     }
     return b;
 }

@@ -1092,11 +1092,12 @@ __global__ void panel_to_float_k(int n, const double *__restrict__ src, float *_
 }
 
 // column-major (leading dimension ld) n x n matrix -> panel-major copy
+// src: column-major symmetric matrix of which only the lower triangle is read
 __global__ void panelize_k(int n, const double *__restrict__ src, int ld, double *__restrict__ dst) {
     long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (long)n * n) return;
     int j = (int)(idx / n), i = (int)(idx - (long)j * n);
-    dst[panel_at(i, j, panel_rows<double>(), n)] = src[(size_t)j * ld + i];
+    dst[panel_at(i, j, panel_rows<double>(), n)] = i >= j ? src[(size_t)j * ld + i] : src[(size_t)i * ld + j];
 }
 
 // y[i] = sum over the slices s (in slice order) of part[s * n + i]: 64
