@@ -84,7 +84,7 @@ def test_probe_yank(prob, oracle):
     assert rel(out[:d * V].reshape(d, V), ref) < 1e-12
 
 
-@pytest.mark.parametrize("order", [-1, 0, 1, 3, 4])
+@pytest.mark.parametrize("order", [-1, 0, 1, 2, 3, 4])
 def test_probe_kmatvec(order, oracle):
     for p in (small_ring(), small_shell()):
         p["name"] = f"kmatvec{p['dim']}d_o{order}"
@@ -106,6 +106,22 @@ def test_probe_varifold(prob, oracle):
     val, grad = oracle.varifold(prob, q, with_grad=True)
     assert abs(out[0] - val) <= 1e-12 * abs(val)
     assert rel(out[1:].reshape(q.shape), grad) < 1e-11
+
+
+@pytest.mark.parametrize("ck,dk", [("C", "L"), ("G", "O"), ("C", "U")])
+def test_probe_varifold_kernel_types(ck, dk, oracle):
+    """The Cauchy centre kernel and the linear, oriented and unoriented
+    direction kernels (varifold.cu's kernel switches), value and gradient."""
+    for p in (small_ring(), small_shell()):
+        p["vfd"] = dict(p["vfd"], cenKnlType=ck, dirKnlType=dk, dirKnlWidth=0.7)
+        p["name"] = f"varifold{p['dim']}d_{ck}{dk}"
+        s = _session(p)
+        rng = np.random.default_rng(11)
+        q = p["q0"] + 0.01 * rng.standard_normal(p["q0"].shape)
+        out = s.probe("varifold", p["tauVar"][:, 0], p["tauPrm"], q)
+        val, grad = oracle.varifold(p, q, with_grad=True)
+        assert abs(out[0] - val) <= 1e-12 * abs(val), (ck, dk, p["dim"])
+        assert rel(out[1:].reshape(q.shape), grad) < 1e-11, (ck, dk, p["dim"])
 
 
 def test_full_solve_matches_oracle(prob, oracle):
