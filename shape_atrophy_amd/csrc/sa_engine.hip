@@ -508,6 +508,10 @@ struct Engine : EngineBase {
             {
                 const char *e = std::getenv("SA_SPECTRAL");
                 specEnabled = !(e && e[0] == '0');
+                // SA_MATRIX_FREE=1: FP64 kernel products stay matrix-free (as
+                // above the 48 GB cap), for testing that form at small sizes
+                const char *mf = std::getenv("SA_MATRIX_FREE");
+                no64Store = mf && mf[0] == '1';
             }
             if (specEnabled) {
                 recD.alloc(nc * SPEC_M, tot); recSD.alloc(nc * SPEC_M, tot); recR.alloc(nc * SPEC_M, tot);
@@ -545,7 +549,7 @@ struct Engine : EngineBase {
                 CK(hipMemsetAsync(At32.p, 0, sizeof(float) * nt, s));
             } else {
                 const size_t na = padded_rows<double>(V) * (size_t)NC, nt = padded_rows<double>(NC) * (size_t)V;
-                if ((na + nt) * sizeof(double) <= (size_t)48 << 30) {
+                if ((na + nt) * sizeof(double) <= (size_t)48 << 30 && !no64Store) {
                     A64.alloc(na, tot); At64.alloc(nt, tot);
                     CK(hipMemsetAsync(A64.p, 0, sizeof(double) * na, s));
                     CK(hipMemsetAsync(At64.p, 0, sizeof(double) * nt, s));

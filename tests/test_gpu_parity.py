@@ -265,3 +265,20 @@ def test_control_point_budget_spent_ends_the_solve(precision):
     assert r["status"] != 0
     assert r["obj"] == np.finfo(np.float64).max
     assert r["fail_reason"] == 2   # SA_FAIL_BUDGET
+
+
+def test_matrix_free_control_point_form(oracle, monkeypatch):
+    """The FP64 control-point form with matrix-free kernel products (the form
+    used above the 48 GB store cap; SA_MATRIX_FREE=1 forces it here) agrees
+    with the oracle to the CG tolerance."""
+    p = small_shell()
+    V = p["q0"].shape[1]
+    p["ctlIdx"] = np.sort(np.random.default_rng(3).choice(V, V // 3, replace=False)).astype(np.int32)
+    p["name"] = "shell3d_ctl_matrix_free"
+    monkeypatch.setenv("SA_MATRIX_FREE", "1")
+    r = _session(p).run(p["tauVar"][:, 0], p["tauPrm"], want_traj=True)
+    o = oracle.objfcn(p)
+    assert r["status"] == 0 and o["status"] == 0
+    disp = np.linalg.norm(o["lmkStk"][-1] - o["lmkStk"][0])
+    assert np.linalg.norm(r["lmkStk"][-1] - o["lmkStk"][-1]) / disp < 1e-8
+    assert abs(r["obj"] - o["obj"]) <= 1e-8 * abs(o["obj"])
