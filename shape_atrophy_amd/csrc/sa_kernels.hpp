@@ -694,7 +694,12 @@ __global__ __launch_bounds__(NT) void slice_sum_k(int n, int slices, const doubl
 #define SA_PANEL_BYTES 1024
 #endif
 template <typename T>
-__host__ __device__ constexpr int panel_rows() { return SA_PANEL_BYTES / (int)sizeof(T); }
+#ifndef SA_PANEL_BYTES32
+#define SA_PANEL_BYTES32 SA_PANEL_BYTES
+#endif
+__host__ __device__ constexpr int panel_rows() {
+    return (sizeof(T) == 4 ? SA_PANEL_BYTES32 : SA_PANEL_BYTES) / (int)sizeof(T);
+}
 
 __host__ __device__ inline size_t panel_at(int i, int j, int PR, int ncols) {
     return (size_t)(i / PR) * PR * ncols + (size_t)j * PR + (i % PR);
@@ -837,7 +842,10 @@ __global__ __launch_bounds__(NT) void kstore_mv_k(int nr, int nc, const T *__res
 #ifndef SA_KS_U
 #define SA_KS_U 4
 #endif
-    constexpr int U = SA_KS_U;             // columns whose loads are issued together
+#ifndef SA_KS_U32
+#define SA_KS_U32 SA_KS_U
+#endif
+    constexpr int U = sizeof(T) == 4 ? SA_KS_U32 : SA_KS_U;   // columns whose loads are issued together
     using V4 = typename std::conditional<sizeof(T) == 4, float4, double2>::type;
     constexpr int NV = R * (int)sizeof(T) / 16;   // 16-byte loads per lane and column
     constexpr int EV = 16 / (int)sizeof(T);       // elements per 16-byte load
