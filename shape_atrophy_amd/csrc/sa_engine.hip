@@ -772,6 +772,33 @@ struct Engine : EngineBase {
     // rocBLAS dgemm); Kc (FP32 inner solve only) keeps K_cc without eps for
     // the products K_cc u and is rebuilt every step
     void kcc_build(bool refresh) {
+        kcc_build_once(refresh);
+        if (!refresh) return;
+        // a factorisation that reports failure is repeated once from a fresh
+        // K_cc: with several sessions on the device a K_cc that factorises
+        // alone every time has been reported indefinite (rarely; cause not
+        // found); a matrix that is indefinite fails again and the solve
+        // reports SA_FAIL_CHOLESKY as before
+        int info[2] = {0, 0};
+        CK(hipMemcpyAsync(info, potrfInfo.p, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
+        CK(hipStreamSynchronize(s));
+        if (info[0] != 0 || info[1] != 0) {
+            static const bool trace = [] {
+                const char *e = std::getenv("SA_TRACE_RETRY");
+                return e && e[0] == '1';
+            }();
+            kcc_build_once(true);
+            if (trace) {
+                int again[2] = {0, 0};
+                CK(hipMemcpyAsync(again, potrfInfo.p, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
+                CK(hipStreamSynchronize(s));
+                std::fprintf(stderr, "[sa] K_cc factorisation retried: info %d %d -> %d %d (C = %d)\n", info[0],
+                             info[1], again[0], again[1], NC);
+            }
+        }
+    }
+
+    void kcc_build_once(bool refresh) {
         if (!refresh && !Kc.p) return;
         double invW = 1.0 / knlWidth;
         long nn = (long)NC * NC;
@@ -1473,13 +1500,14 @@ struct Engine : EngineBase {
             // v = K_xc alpha; the converged residual check already formed it in FP64
             if (kaReady) CK(hipMemcpyAsync(x.p, Kr.p, sizeof(double) * V * D, hipMemcpyDeviceToDevice, s));
             else kx_product(exact, true, q, cA.p, x.p, nullptr, nullptr);
-            int info[2] = {0, 0};
-            CK(hipMemcpyAsync(info, potrfInfo.p, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
-            CK(hipStreamSynchronize(s));
-            if (info[0] != 0) { failReason = SA_FAIL_CHOLESKY; return 1; }
-            if (info[1] != 0) { failReason = SA_FAIL_TRTRI; return 1; }
-            return 0;
         }
+        // a failed factorisation of K_cc explains a failed solve first
+        int info[2] = {0, 0};
+        CK(hipMemcpyAsync(info, potrfInfo.p, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
+        CK(hipStreamSynchronize(s));
+        if (info[0] != 0) { failReason = SA_FAIL_CHOLESKY; return 1; }
+        if (info[1] != 0) { failReason = SA_FAIL_TRTRI; return 1; }
+        if (status == 0) return 0;
         failReason = status == 2 ? SA_FAIL_BUDGET : status == 1 ? SA_FAIL_BREAKDOWN : SA_FAIL_REFINE;
         return 1;
     }
