@@ -1372,7 +1372,17 @@ struct Engine : EngineBase {
     int ctl_solve(KMode md, const double *q, const double *tauNow, const double *geoNow, const double *rhs, double *xs,
                   bool zeroStart, double tolSqu, int maxIt, int &iters) {
         const int cap = precAge > 0 ? std::min(std::max(1, maxIt / 2), 4 * precIters + 100) : maxIt;
+        const bool spUsed = specOn && specMode == (int)md && !(specRecord && fused_iter(md));
         int st = ctl_pcg(md, q, tauNow, geoNow, rhs, xs, zeroStart, tolSqu, cap, iters);
+        if (st == 1 && spUsed && iters < cap) {
+            // a breakdown with the spectral term: continue from this iterate
+            // with the plain preconditioner (the term returns at the next refresh)
+            specOn = false;
+            int more = 0;
+            chunkHint = 16;
+            st = ctl_pcg(md, q, tauNow, geoNow, rhs, xs, false, tolSqu, std::max(1, cap - iters), more);
+            iters += more;
+        }
         if (st == 2 && cap < maxIt) {
             kcc_build(true);
             precAge = 0;
