@@ -71,8 +71,8 @@ typedef struct sa_mesh {
     const int *vtxElmMat;  /* arg 4  lmkNum x vtxElmCols */
     const int *frmVtxMat;  /* arg 5  tanVtxMat (2D, elmNum x 4) | nmlVtxMat (3D, elmNum x 6) */
     const int *tsvVtxMat;  /* arg 6  elmNum x 4 (2D) | 6 (3D) */
-    const int *bdrVtxMat;  /* arg 7  (unused by the path, accepted for call compatibility) */
-    const int *vtxBdrMat;  /* arg 8  (unused by the path) */
+    const int *bdrVtxMat;  /* arg 7  (read only with sa_params.bdrYank; accepted for call compatibility) */
+    const int *vtxBdrMat;  /* arg 8  (read only with sa_params.bdrYank) */
     const int *btmVtxMat;  /* arg 9  btmElmNum x dim */
     const int *vtxBtmMat;  /* arg 10 btmLmkNum x vtxBtmCols */
     const int *femVtxMat;  /* arg 11 nzrNum x 2, sorted (row, col) */
@@ -102,6 +102,10 @@ typedef struct sa_params {
     double cgTolSqu;             /* arg 26 = cgTolVal^2 */
     int precision;               /* SA_PREC_FP64 | SA_PREC_MIXED */
     double rdTolVal;             /* relative residual of the sparse reaction-diffusion solve (0 -> 1e-13) */
+    int bdrYank;                 /* 1: add the boundary yank the reference keeps commented out
+                                    (computeExternalYank.cu:49-84 | 3D :58-99, gathered through
+                                    vtxBdrMat :106-116 | :125-135); needs bdrVtxMat/vtxBdrMat.
+                                    0: as the reference runs. */
 } sa_params;
 
 typedef struct sa_target {

@@ -29,7 +29,8 @@ class OMesh(C.Structure):
                 ("btmLmkNum", _I), ("vfdElmNum", _I), ("vtxElmRows", _I), ("vtxBtmRows", _I),
                 ("femIfoRows", _I), ("elmVtx", _PI), ("vtxElm", _PI), ("frmVtx", _PI), ("tsvVtx", _PI),
                 ("btmVtx", _PI), ("vtxBtm", _PI), ("femVtx", _PI), ("femIfo", _PI), ("vfdVtx", _PI),
-                ("elmLyr", _PI), ("ctlNum", _I), ("ctlIdx", _PI)]
+                ("elmLyr", _PI), ("ctlNum", _I), ("ctlIdx", _PI), ("bdrElmNum", _I), ("vtxBdrRows", _I),
+                ("bdrVtx", _PI), ("vtxBdr", _PI), ("bdrYank", _I)]
 
 
 class OParams(C.Structure):
@@ -116,6 +117,12 @@ def make_mesh(prob: dict, keep: _Keep) -> OMesh:
         m.ctlIdx = _ip(c)
     else:
         m.ctlNum = 0
+    if prob["params"].get("bdrYank", 0) and "bdrVtxMat" in e and "vtxBdrMat" in e:
+        bdr = arr("bdrVtxMat"); vb = arr("vtxBdrMat")
+        m.bdrElmNum = bdr.shape[1]
+        m.vtxBdrRows = vb.shape[0]
+        m.bdrVtx, m.vtxBdr = _ip(bdr), _ip(vb)
+        m.bdrYank = 1
     return m
 
 

@@ -38,6 +38,11 @@ struct OMesh {
     const int *elmLyr;     // per-element material layer (isotropic mode), may be null
     int ctlNum;            // control points (0 = kernel over all landmarks)
     const int *ctlIdx;     // landmark index of each control point
+    // boundary facets (bdrElmNum x dim) and their vertex adjacency (count-
+    // prefixed, vtxBdrRows x lmkNum); bdrYank = 1 adds the boundary yank
+    int bdrElmNum, vtxBdrRows;
+    const int *bdrVtx, *vtxBdr;
+    int bdrYank;
 };
 
 struct OParams {
@@ -406,6 +411,39 @@ void external_yank(const OMesh &m, const double *q, const double *act, std::vect
         }
     }
     gather_vertex<D>(m, frc, out);
+    if (!m.bdrYank || !m.bdrVtx) return;
+    // boundary yank, commented out in the reference (computeExternalYank.cu
+    // :49-84 | 3D :58-99): facet normal n = (t_y, -t_x), t = q1 - q0 (2D) or
+    // (q1 - q0) x (q2 - q0) (3D); vertex k of the facet gets act(q_k) n /
+    // (VTXNUM - 1), VTXNUM - 1 = D; gathered after the element forces in
+    // vtxBdrMat order (:106-116 | :125-135)
+    const int Bn = m.bdrElmNum;
+    std::vector<double> bf((size_t)D * D * Bn);
+    for (int b = 0; b < Bn; ++b) {
+        int v[D];
+        for (int k = 0; k < D; ++k) v[k] = m.bdrVtx[k * Bn + b];
+        double n[D];
+        if (D == 2) {
+            const double t0 = q[v[1]] - q[v[0]], t1 = q[V + v[1]] - q[V + v[0]];
+            n[0] = t1;
+            n[1] = -t0;
+        } else {
+            double u[3], w[3];
+            for (int c = 0; c < 3; ++c) { u[c] = q[c * V + v[1]] - q[c * V + v[0]]; w[c] = q[c * V + v[D - 1]] - q[c * V + v[0]]; }
+            n[0] = u[1] * w[2] - u[2] * w[1];
+            n[1] = u[2] * w[0] - u[0] * w[2];
+            n[D - 1] = u[0] * w[1] - u[1] * w[0];
+        }
+        for (int k = 0; k < D; ++k)
+            for (int c = 0; c < D; ++c) bf[(size_t)(k * D + c) * Bn + b] = act[v[k]] * n[c] / (double)D;
+    }
+    for (int i = 0; i < V; ++i) {
+        const int adj = m.vtxBdr[i];
+        for (int a = 0; a < adj; ++a) {
+            const int b = m.vtxBdr[(size_t)(1 + 2 * a) * V + i], l = m.vtxBdr[(size_t)(2 + 2 * a) * V + i];
+            for (int c = 0; c < D; ++c) out[c * V + i] += bf[(size_t)(l * D + c) * Bn + b];
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------

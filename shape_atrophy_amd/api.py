@@ -52,7 +52,7 @@ class SaParams(C.Structure):
                 ("knlEps", C.c_double), ("modVec", _PD), ("modNum", C.c_int), ("spdTanVal", C.c_double),
                 ("spdTsvVal", C.c_double), ("ynkActPrmVec", _PD), ("reaActPrmVec", _PD), ("btmWgt", C.c_double),
                 ("timeStp", C.c_double), ("timeNum", C.c_int), ("cgItrMax", C.c_int), ("cgTolSqu", C.c_double),
-                ("precision", C.c_int), ("rdTolVal", C.c_double)]
+                ("precision", C.c_int), ("rdTolVal", C.c_double), ("bdrYank", C.c_int)]
 
 
 class SaTarget(C.Structure):
@@ -187,7 +187,7 @@ def make_mesh(elmObj: dict, lmkIni: np.ndarray, keep: _Keep, with_vfd: bool) -> 
 
 def make_params(elaType, ldmWgt, knlOrder, knlWidth, knlEps, modVec, ynkActPrmVec, spdTanVal, spdTsvVal,
                 reaActPrmVec, btmWgt, timeStp, timeEnd, cgItrMax, cgTolVal, keep, precision="fp64",
-                rdTolVal=1e-13) -> SaParams:
+                rdTolVal=1e-13, bdrYank=0) -> SaParams:
     et = str(elaType).lower()
     if et not in ("frame", "isotropic"):
         raise ValueError(f"Unknown elasticity type {elaType}.")
@@ -211,6 +211,7 @@ def make_params(elaType, ldmWgt, knlOrder, knlWidth, knlEps, modVec, ynkActPrmVe
     p.cgTolSqu = float(cgTolVal) ** 2
     p.precision = {"fp64": SA_PREC_FP64, "mixed": SA_PREC_MIXED}[precision]
     p.rdTolVal = float(rdTolVal)
+    p.bdrYank = int(bool(bdrYank))
     return p
 
 
@@ -306,7 +307,7 @@ class Session:
                                p["modMat"] if p["elaType"] == "isotropic" else p["modVec"], p["ynkActPrmVec"],
                                p["spdTanVal"], p["spdTsvVal"], p["reaActPrmVec"], p["btmWgt"], p["timeStp"],
                                p["timeEnd"], p["cgItrMax"], p["cgTolVal"], keep, precision,
-                               p.get("rdTolVal", 1e-13))
+                               p.get("rdTolVal", 1e-13), p.get("bdrYank", 0))
         if prob.get("timeNum"):
             self.prm.timeNum = int(prob["timeNum"])
         with_target = with_target and "target" in prob
@@ -384,7 +385,8 @@ def params_struct(p: dict, keep, precision: str = "fp64", timeNum: int | None = 
     prm = make_params(p["elaType"], p["ldmWgt"], p["knlOrder"], p["knlWidth"], p["knlEps"],
                       p["modMat"] if p["elaType"] == "isotropic" else p["modVec"], p["ynkActPrmVec"],
                       p["spdTanVal"], p["spdTsvVal"], p["reaActPrmVec"], p["btmWgt"], p["timeStp"], p["timeEnd"],
-                      p["cgItrMax"], p["cgTolVal"], keep, precision, p.get("rdTolVal", 1e-13))
+                      p["cgItrMax"], p["cgTolVal"], keep, precision, p.get("rdTolVal", 1e-13),
+                      p.get("bdrYank", 0))
     if timeNum:
         prm.timeNum = int(timeNum)
     return prm

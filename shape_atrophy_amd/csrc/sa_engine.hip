@@ -215,6 +215,12 @@ struct Engine : EngineBase {
     // vtxStart[i] where vertex i's list starts (absent when an (e, l) is
     // listed twice: the element-ordered gather then runs instead)
     DBuf<int> elmSlot, vtxStart;
+    // boundary yank (sa_params.bdrYank): boundary facets, their vertex
+    // adjacency, the facet forces; Bn = 0 when the tables are absent or unusable
+    DBuf<int> bdrVtx, vtxBdr;
+    DBuf<double> bfrc;
+    int Bn = 0;
+    bool bdrYank = false;
     DBuf<double> q0, dmod, tCen, tDir, tVol;
     // state
     DBuf<double> lmk, tau, geo[2], femElm, ppd[2], lhs, ya, ra, f, x, a, r, Kid, Kr, d, Ad, frc, pfrc, meas;
@@ -301,6 +307,8 @@ struct Engine : EngineBase {
         require(p.knlWidth > 0, "knlWidth must be positive");
         require(p.timeNum >= 1 && p.timeNum <= T, "timeNum exceeds the session's allocation");
         require(p.cgItrMax >= 1 && p.cgItrMax <= itrMax0, "cgItrMax exceeds the session's allocation");
+        require(!p.bdrYank || Bn > 0, "bdrYank needs consistent bdrVtxMat / vtxBdrMat");
+        bdrYank = p.bdrYank != 0;
         mod.assign(p.modVec, p.modVec + p.modNum);
         CK(hipMemcpyAsync(dmod.p, mod.data(), sizeof(double) * mod.size(), hipMemcpyHostToDevice, s));
         CK(hipStreamSynchronize(s));
@@ -357,6 +365,28 @@ struct Engine : EngineBase {
             hBtm = to_host(m.btmVtxMat, (size_t)B * D, dev);
             hVtxBtm = to_host(m.vtxBtmMat, (size_t)BL * vtxBtmRows, dev);
         }
+        // boundary facets (only the optional boundary yank reads them; the
+        // reference passes them and leaves them unused): kept when consistent
+        std::vector<int> hBdr, hVtxBdr;
+        if (m.bdrElmNum > 0 && m.bdrVtxMat && m.vtxBdrMat && m.vtxBdrCols > 0) {
+            const int nb = m.bdrElmNum, rows = m.vtxBdrCols;
+            hBdr = to_host(m.bdrVtxMat, (size_t)nb * D, dev);
+            hVtxBdr = to_host(m.vtxBdrMat, (size_t)V * rows, dev);
+            bool ok = true;
+            for (int v : hBdr) ok = ok && v >= 0 && v < V;
+            for (int i = 0; i < V && ok; ++i) {
+                const int adj = hVtxBdr[i];
+                ok = adj >= 0 && 1 + 2 * adj <= rows;
+                for (int a = 0; a < adj && ok; ++a) {
+                    const int b = hVtxBdr[(size_t)(1 + 2 * a) * V + i], l = hVtxBdr[(size_t)(2 + 2 * a) * V + i];
+                    ok = b >= 0 && b < nb && l >= 0 && l < D && hBdr[(size_t)l * nb + b] == i;
+                }
+            }
+            if (ok) Bn = nb;
+            else { hBdr.clear(); hVtxBdr.clear(); }
+        }
+        bdrYank = p.bdrYank != 0;
+        require(!bdrYank || Bn > 0, "bdrYank needs consistent bdrVtxMat / vtxBdrMat");
         for (int v : hElm) require(v >= 0 && v < V, "elmVtxMat index out of range");
         for (int v : hFrm) require(v >= 0 && v < V, "frame stencil index out of range");
         for (int v : hTsv) require(v >= 0 && v < V, "tsvVtxMat index out of range");
@@ -443,6 +473,12 @@ struct Engine : EngineBase {
         if (B > 0) {
             btmVtx.alloc(hBtm.size(), tot); btmVtx.upload(hBtm.data(), hBtm.size(), s);
             vtxBtm.alloc(hVtxBtm.size(), tot); vtxBtm.upload(hVtxBtm.data(), hVtxBtm.size(), s);
+        }
+        if (Bn > 0) {
+            bdrVtx.alloc(hBdr.size(), tot); bdrVtx.upload(hBdr.data(), hBdr.size(), s);
+            vtxBdr.alloc(hVtxBdr.size(), tot); vtxBdr.upload(hVtxBdr.data(), hVtxBdr.size(), s);
+            bfrc.alloc((size_t)D * D * Bn, tot);
+            CK(hipStreamSynchronize(s));   // the host tables go out of scope
         }
         if (!hLyr.empty()) { elmLyr.alloc(E, tot); elmLyr.upload(hLyr.data(), E, s); }
         dmod.alloc(mod.size(), tot); dmod.upload(mod.data(), mod.size(), s);
@@ -645,6 +681,20 @@ struct Engine : EngineBase {
         CHECK_LAUNCH();
     }
 
+    // external yank f (and its |f|^2 partials in P) at positions q: element
+    // forces, plus the boundary forces when bdrYank is on
+    void yank(const double *q, const double *geoNow, double *fOut, double *P) {
+        yank_elem_k<D><<<gridOf(E), NT, 0, s>>>(E, elmVtx.p, geoNow, ya.p, frc.p);
+        CHECK_LAUNCH();
+        if (bdrYank) {
+            bdr_yank_k<D><<<gridOf(Bn), NT, 0, s>>>(Bn, V, bdrVtx.p, q, ya.p, bfrc.p);
+            CHECK_LAUNCH();
+        }
+        gather_k<D><<<gridOf(V), NT, 0, s>>>(V, E, vtxElm.p, frc.p, fOut, P, bdrYank ? vtxBdr.p : nullptr, bfrc.p,
+                                             Bn);
+        CHECK_LAUNCH();
+    }
+
     // element pass of E x with the geometry geoNow; the forces land in vertex
     // order when the slot map exists
     void elastic(const double *xin, const double *tauNow, const double *geoNow, const Ctl *c) {
@@ -681,10 +731,7 @@ struct Engine : EngineBase {
         const int gV = gridOf(V), gN = gridOf(n), gM = kmv_parts();
         if (Kst.p) kst_build(q);
         // f and |f|^2
-        yank_elem_k<D><<<gridOf(E), NT, 0, s>>>(E, elmVtx.p, geoNow, ya.p, frc.p);
-        CHECK_LAUNCH();
-        gather_k<D><<<gV, NT, 0, s>>>(V, E, vtxElm.p, frc.p, f.p, P0.p);
-        CHECK_LAUNCH();
+        yank(q, geoNow, f.p, P0.p);
         pcg_start_k<<<1, NT, 0, s>>>(P0.p, gV, cgTolSqu, itrMax, ctl.p);
         CHECK_LAUNCH();
         zero_if_empty_k<<<gN, NT, 0, s>>>(n, x.p, a.p, ctl.p);
@@ -1402,10 +1449,7 @@ struct Engine : EngineBase {
         CK(hipMemcpyAsync(qCur.p, qStep, sizeof(double) * V * D, hipMemcpyDeviceToDevice, s));
         CK(hipMemcpyAsync(tauCur.p, tauStep, sizeof(double) * V, hipMemcpyDeviceToDevice, s));
         const double *q = qCur.p, *tauNow = tauCur.p;
-        yank_elem_k<D><<<gridOf(E), NT, 0, s>>>(E, elmVtx.p, geoNow, ya.p, frc.p);
-        CHECK_LAUNCH();
-        gather_k<D><<<gridOf(V), NT, 0, s>>>(V, E, vtxElm.p, frc.p, f.p, P0.p);
-        CHECK_LAUNCH();
+        yank(q, geoNow, f.p, P0.p);
         gather_ctl_k<D><<<gridOf(NC), NT, 0, s>>>(NC, V, ctlIdx.p, q, qc.p);
         CHECK_LAUNCH();
         const bool refresh = precAge < 0 || precAge >= PREC_MAX_AGE || lastIters > PREC_GROWTH * precIters + 4 ||
@@ -1653,9 +1697,7 @@ struct Engine : EngineBase {
             case SA_PROBE_YANK:
                 activity_k<<<gridOf(V), NT, 0, s>>>(V, tau.p, ynk[0], ynk[1], ynk[2], ynk[3], rea[0], rea[1], rea[2],
                                                    rea[3], rea[4], ya.p, ra.p);
-                yank_elem_k<D><<<gridOf(E), NT, 0, s>>>(E, elmVtx.p, geo[0].p, ya.p, frc.p);
-                gather_k<D><<<gridOf(V), NT, 0, s>>>(V, E, vtxElm.p, frc.p, f.p, P0.p);
-                CHECK_LAUNCH();
+                yank(q0.p, geo[0].p, f.p, P0.p);
                 CK(hipMemcpyAsync(out, f.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
                 CK(hipMemcpyAsync(out + n, tau.p, sizeof(double) * V, hipMemcpyDeviceToHost, s));
                 break;
@@ -1879,6 +1921,9 @@ long long sa_session_footprint(const sa_mesh *m, const sa_target *t, const sa_pa
     long long b = 0;
     b += 4 * (E * (D + 1) + V * m->vtxElmCols + E * (D == 2 ? 8 : 12) + Z * m->femIfoCols + 2 * Z + 2 * V);
     b += 8 * (V * D * T + V * T + 2 * GN * E + (1 + NS) * E + 3 * Z + 2 * V + 9 * V * D + (D + 1) * D * E + 5 * V);
+    // vertex-ordered force slots; boundary facets, their adjacency and forces
+    b += 4 * ((D + 1) * E + V + 1) + 4 * ((long long)m->bdrElmNum * D + V * m->vtxBdrCols) +
+         8 * D * D * (long long)m->bdrElmNum;
     b += 8 * (7 * sa::MAXP + p->cgItrMax + 200010);
     if (t) {
         long long F = m->vfdElmNum, Ft = t->tgtElmNum;

@@ -215,12 +215,48 @@ __global__ __launch_bounds__(NT) void yank_elem_k(int E, const int *__restrict__
     }
 }
 
+// boundary yank, the term the reference keeps commented out
+// (computeExternalYank.cu:49-84 | 3D :58-99): on boundary facet b with
+// vertices q_0..q_{D-1}, n = (t_y, -t_x) with t = q_1 - q_0 (2D) or
+// n = (q_1 - q_0) x (q_2 - q_0) (3D), and the force on its vertex k is
+// act(q_k) n / D (VTXNUM - 1 = D).  bf[(k D + c) Bn + b].
+template <int D>
+__global__ void bdr_yank_k(int Bn, int V, const int *__restrict__ bdrVtx, const double *__restrict__ q,
+                           const double *__restrict__ act, double *__restrict__ bf) {
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= Bn) return;
+    int v[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) v[k] = bdrVtx[k * Bn + b];
+    double n[D];
+    if constexpr (D == 2) {
+        const double t0 = q[v[1]] - q[v[0]], t1 = q[V + v[1]] - q[V + v[0]];
+        n[0] = t1;
+        n[1] = -t0;
+    } else {
+        double u[3], w[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) { u[c] = q[c * V + v[1]] - q[c * V + v[0]]; w[c] = q[c * V + v[2]] - q[c * V + v[0]]; }
+        n[0] = u[1] * w[2] - u[2] * w[1];
+        n[1] = u[2] * w[0] - u[0] * w[2];
+        n[2] = u[0] * w[1] - u[1] * w[0];
+    }
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        const double f = act[v[k]];
+#pragma unroll
+        for (int c = 0; c < D; ++c) bf[(size_t)(k * D + c) * Bn + b] = f * n[c] / (double)D;
+    }
+}
+
 // vertex gather of element vectors (yankGatherKernel / frmElasticGatherKernel)
-// with a partial sum of |out|^2 per block
+// with a partial sum of |out|^2 per block: the element forces in vtxElmMat
+// order, then (with vtxBdr) the boundary forces in vtxBdrMat order
 template <int D>
 __global__ __launch_bounds__(NT) void gather_k(int V, int E, const int *__restrict__ vtxElm,
                                                const double *__restrict__ frc, double *__restrict__ out,
-                                               double *__restrict__ P) {
+                                               double *__restrict__ P, const int *__restrict__ vtxBdr = nullptr,
+                                               const double *__restrict__ bf = nullptr, int Bn = 0) {
     __shared__ double red[NT / 64];
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     double ss = 0.0;
@@ -233,6 +269,14 @@ __global__ __launch_bounds__(NT) void gather_k(int V, int E, const int *__restri
             int e = vtxElm[(1 + 2 * a) * V + i], l = vtxElm[(2 + 2 * a) * V + i];
 #pragma unroll
             for (int c = 0; c < D; ++c) acc[c] += frc[((size_t)e * (D + 1) + l) * D + c];
+        }
+        if (vtxBdr) {
+            const int nb = vtxBdr[i];
+            for (int a = 0; a < nb; ++a) {
+                const int bb = vtxBdr[(1 + 2 * a) * V + i], l = vtxBdr[(2 + 2 * a) * V + i];
+#pragma unroll
+                for (int c = 0; c < D; ++c) acc[c] += bf[(size_t)(l * D + c) * Bn + bb];
+            }
         }
 #pragma unroll
         for (int c = 0; c < D; ++c) { out[c * V + i] = acc[c]; ss += acc[c] * acc[c]; }

@@ -22,6 +22,7 @@ __all__ = [
     "vertex_adjacency",
     "fem_sparsity",
     "free_boundary",
+    "orient_boundary",
     "generate_element_object_2d",
     "generate_element_object_3d",
     "tri2tetra",
@@ -132,7 +133,9 @@ def free_boundary(elm: np.ndarray) -> np.ndarray:
 
     Stands in for MATLAB `freeBoundary`; the reference's active code never
     reads bdrVtxMat/vtxBdrMat (the boundary-yank kernel is commented out,
-    computeExternalYank.cu:49-84), so only the set matters, not its order."""
+    computeExternalYank.cu:49-84), so only the set matters here; once
+    positions exist, orient_boundary turns every facet outward, the
+    orientation that kernel assumes."""
     elm = np.asarray(elm, dtype=np.int64)
     nv = elm.shape[1]
     faces = []
@@ -142,6 +145,57 @@ def free_boundary(elm: np.ndarray) -> np.ndarray:
     f = np.sort(np.concatenate(faces, axis=0), axis=1)
     uniq, cnt = np.unique(f, axis=0, return_counts=True)
     return uniq[cnt == 1]
+
+
+def orient_boundary(elm_obj: dict, pos: np.ndarray) -> dict:
+    """Orient the boundary facets of elm_obj outward for positions pos (V, d)
+    and rebuild vtxBdrMat: the facet normal of the boundary-yank kernel,
+    n = (t_y, -t_x) with t = q_1 - q_0 in 2D and (q_1 - q_0) x (q_2 - q_0) in
+    3D (computeExternalYank.cu:49-84 | 3D :58-99), points away from the
+    element the facet belongs to (MATLAB freeBoundary's outward orientation).
+    Facets keep their rows; a facet pointing inward has its last two vertices
+    swapped."""
+    pos = np.asarray(pos, dtype=np.float64)
+    d = pos.shape[1]
+    bdr = np.asarray(elm_obj["bdrVtxMat"], dtype=np.int64).T.copy()      # (nB, d)
+    elm = np.asarray(elm_obj["elmVtxMat"], dtype=np.int64).T             # (E, d + 1)
+    if bdr.size == 0:
+        return elm_obj
+    # the vertex opposite each boundary facet in its element: facets as
+    # integer keys of their sorted vertices, matched by binary search
+    V = pos.shape[0]
+    def keys(f):
+        f = np.sort(f, axis=1)
+        out = np.zeros(f.shape[0], dtype=np.int64)
+        for c in range(f.shape[1]):
+            out = out * V + f[:, c]
+        return out
+    fk, fo = [], []
+    for drop in range(d + 1):
+        fk.append(keys(np.delete(elm, drop, axis=1)))
+        fo.append(elm[:, drop])
+    fk = np.concatenate(fk)
+    fo = np.concatenate(fo)
+    order = np.argsort(fk, kind="stable")
+    fk, fo = fk[order], fo[order]
+    bk = keys(bdr)
+    at = np.searchsorted(fk, bk)
+    if np.any(at >= fk.size) or np.any(fk[np.minimum(at, fk.size - 1)] != bk):
+        raise ValueError("a boundary facet is not a facet of any element")
+    opp = fo[at]
+    p = [pos[bdr[:, k]] for k in range(d)]
+    if d == 2:
+        t = p[1] - p[0]
+        n = np.stack([t[:, 1], -t[:, 0]], axis=1)
+    else:
+        n = np.cross(p[1] - p[0], p[2] - p[0])
+    out = (sum(p) / d) - pos[opp]
+    flip = (n * out).sum(axis=1) < 0
+    bdr[flip, d - 2], bdr[flip, d - 1] = bdr[flip, d - 1].copy(), bdr[flip, d - 2].copy()
+    elm_obj = dict(elm_obj)
+    elm_obj["bdrVtxMat"] = np.ascontiguousarray(bdr.T, dtype=np.int32)
+    elm_obj["vtxBdrMat"] = vertex_adjacency(bdr, pos.shape[0])
+    return elm_obj
 
 
 def _pack(elm_obj: dict) -> dict:

@@ -107,7 +107,7 @@ def strip_problem(vfd_layers=(1, 31)) -> dict:
     x = -3.0 + np.arange(L) * 6.0 / (L - 1)
     y = -1.0 + np.arange(C) * 2.0 / (C - 1)
     pos = np.stack([np.tile(x, C), np.repeat(y, L)], axis=1)
-    elm = _mesh.generate_element_object_2d(L, C, list(vfd_layers))
+    elm = _mesh.orient_boundary(_mesh.generate_element_object_2d(L, C, list(vfd_layers)), pos)
     return dict(dim=2, elm=elm, q0=pos.T.copy(), params=demo_params(),
                 tauVar=np.array([[0.5], [0.5], [0.2], [0.3]]), tauPrm=np.array([1.0]))
 
@@ -132,7 +132,7 @@ def ring_problem(n_rings: int = 10, n_per_ring: int = 200, seed: int = 1, target
     # keep the outer boundary on its radius so the varifold curve is a circle
     rr[(C - 1) * L:] = r_outer
     pos = np.stack([rr * np.cos(tt), rr * np.sin(tt)], axis=1)
-    elm = _mesh.generate_element_object_2d(L, C, [C], periodic=True)
+    elm = _mesh.orient_boundary(_mesh.generate_element_object_2d(L, C, [C], periodic=True), pos)
     cen = pos[elm["elmVtxMat"].T].mean(axis=1)
     rad = np.sqrt((cen ** 2).sum(1))
     elm["elmLyrVec"] = np.ascontiguousarray(np.digitize(rad, [0.6, 0.8]).astype(np.int32))
@@ -177,7 +177,7 @@ def shell_problem(level: int = 5, n_layers: int = 5, scales=None, axes=(1.0, 0.8
     L = sph.shape[0]
     pos = np.concatenate([s * sph * ax for s in scales], axis=0)
     tet, nml, tsv = _mesh.tri2tetra(tri, n_layers)
-    elm = _mesh.generate_element_object_3d(tet, tri, nml, tsv, [n_layers])
+    elm = _mesh.orient_boundary(_mesh.generate_element_object_3d(tet, tri, nml, tsv, [n_layers]), pos)
     n_slab = n_layers - 1
     slab = np.arange(tet.shape[0]) // (3 * tri.shape[0])
     elm["elmLyrVec"] = np.ascontiguousarray(np.minimum(slab * 3 // n_slab, 2).astype(np.int32))

@@ -282,3 +282,27 @@ def test_matrix_free_control_point_form(oracle, monkeypatch):
     disp = np.linalg.norm(o["lmkStk"][-1] - o["lmkStk"][0])
     assert np.linalg.norm(r["lmkStk"][-1] - o["lmkStk"][-1]) / disp < 1e-8
     assert abs(r["obj"] - o["obj"]) <= 1e-8 * abs(o["obj"])
+
+
+@pytest.mark.parametrize("which", ["ring2d", "shell3d", "shell3d_ctl"])
+def test_boundary_yank(which, oracle):
+    """sa_params.bdrYank = 1 adds the boundary yank the reference keeps
+    commented out: the yank vector and a full solve against the oracle's
+    same term."""
+    p = small_ring() if which == "ring2d" else small_shell()
+    if which == "shell3d_ctl":
+        V = p["q0"].shape[1]
+        p["ctlIdx"] = np.sort(np.random.default_rng(3).choice(V, V // 3, replace=False)).astype(np.int32)
+    p["params"] = dict(p["params"], bdrYank=1)
+    p["name"] = which + "_bdr_yank"
+    s = _session(p)
+    d, V = p["q0"].shape
+    out = s.probe("yank", p["tauVar"][:, 0], p["tauPrm"])
+    ya, _ = oracle.activities(oracle.tau_initial(p), p["params"]["ynkActPrmVec"], p["params"]["reaActPrmVec"])
+    assert rel(out[:d * V].reshape(d, V), oracle.external_yank(p, p["q0"], ya)) < 1e-12
+    r = s.run(p["tauVar"][:, 0], p["tauPrm"], want_traj=True)
+    o = oracle.objfcn(p)
+    assert r["status"] == 0 and o["status"] == 0
+    disp = np.linalg.norm(o["lmkStk"][-1] - o["lmkStk"][0])
+    assert np.linalg.norm(r["lmkStk"][-1] - o["lmkStk"][-1]) / disp < 1e-8
+    assert abs(r["obj"] - o["obj"]) <= 1e-8 * abs(o["obj"])

@@ -212,6 +212,31 @@ def test_yank_work_identity(oracle, maker):
 
 
 @pytest.mark.parametrize("maker", [small_ring, small_shell])
+def test_boundary_yank_closed_forms(oracle, maker):
+    """The boundary yank the reference keeps commented out, on outward
+    facets (orient_boundary): for a constant activity a the element yank at
+    a vertex is -a times the integral of grad(lambda_k), i.e. -a sum over its
+    boundary facets of |F| n_out / d; the boundary term adds a n / (VTXNUM-1)
+    per facet, where |n| = |F| in 2D and 2|F| in 3D.  So the two cancel in
+    2D, and in 3D the boundary term is -2 times the element term."""
+    p = maker()
+    d, V = p["q0"].shape
+    act = np.full(V, 0.7)
+    f0 = oracle.external_yank(p, p["q0"], act)
+    fb = oracle.external_yank(dict(p, params=dict(p["params"], bdrYank=1)), p["q0"], act) - f0
+    scale = np.abs(f0).max()
+    assert scale > 0
+    if d == 2:
+        assert np.abs(f0 + fb).max() < 1e-13 * scale
+    else:
+        assert np.abs(fb + 2.0 * f0).max() < 1e-13 * scale
+    # the oriented table still lists exactly the free boundary
+    from shape_atrophy_amd import mesh as M
+    bdr = p["elm"]["bdrVtxMat"].T
+    assert set(map(tuple, np.sort(bdr, 1))) == set(map(tuple, M.free_boundary(p["elm"]["elmVtxMat"].T)))
+
+
+@pytest.mark.parametrize("maker", [small_ring, small_shell])
 def test_fem_matrices(oracle, maker):
     p = maker()
     q = p["q0"]
