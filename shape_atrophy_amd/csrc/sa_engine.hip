@@ -911,13 +911,20 @@ struct Engine : EngineBase {
     // the stored matrices (see kstore_build_k) in FP64 or FP32
     enum class KMode { MatrixFree64, Stored64, Stored32 };
 
+    // y[i] = sum of the slice partials part[k * n + i] (slice_rows_k; four
+    // wavefronts per 64 rows once the rows alone give 1,024 blocks)
+    void slice_rows(int n, int slices, const double *part, double *y, const Ctl *c) {
+        if (gridOf(n, 64) >= 1024) slice_rows_k<4><<<gridOf(n, 64), 256, 0, s>>>(n, slices, part, y, c);
+        else slice_rows_k<SLW><<<gridOf(n, 64), NTS, 0, s>>>(n, slices, part, y, c);
+        CHECK_LAUNCH();
+    }
+
     // y = M x for a dense FP64 C x C panel-major matrix
     void cc_product(const double *M, const double *x, double *y) {
         dim3 g(gridOf(NC, panel_rows<double>()), slicesM);
         kstore_mv_k<D, double><<<g, NT, ks_lds(NC, slicesM), s>>>(NC, NC, M, x, partS.p, ctl.p);
         CHECK_LAUNCH();
-        slice_rows_k<<<gridOf(D * NC, 64), NT, 0, s>>>(D * NC, slicesM, partS.p, y, ctl.p);
-        CHECK_LAUNCH();
+        slice_rows(D * NC, slicesM, partS.p, y, ctl.p);
     }
 
     // z = M r, M = X^T X ~ K_cc^{-1} (its FP32 panel copy in the FP32 form),
@@ -950,7 +957,7 @@ struct Engine : EngineBase {
             kstore_mv_k<D, double><<<g, NT, ks_lds(NC, sl), s>>>(NC, NC, Mpre.p, r, partS.p, ctl.p);
         }
         CHECK_LAUNCH();
-        slice_dot_k<<<gridOf(nc, 64), NT, 0, s>>>(nc, sl, partS.p, P2.p, gridOf(nc), r, z, P3.p,
+        slice_dot_k<<<gridOf(nc, 64), NTS, 0, s>>>(nc, sl, partS.p, P2.p, gridOf(nc), r, z, P3.p,
                                                   sp ? specU.p : nullptr, specC.p, specW.p, nPw, specK, ctl.p);
         CHECK_LAUNCH();
     }
@@ -1029,7 +1036,7 @@ struct Engine : EngineBase {
         if (ev) CK(hipEventRecord(ev[1], s));
         if (!sum) return;
         if (P) slice_sum_k<<<gridOf((long)D * nr), NT, 0, s>>>(D * nr, sl, partS.p, y, P, c);   // + |y|^2 partials
-        else slice_rows_k<<<gridOf(D * nr, 64), NT, 0, s>>>(D * nr, sl, partS.p, y, c);
+        else slice_rows(D * nr, sl, partS.p, y, c);
         CHECK_LAUNCH();
     }
 
@@ -1073,7 +1080,7 @@ struct Engine : EngineBase {
             DirArgs dir{cZ.p, P3.p, gridOf(nc, 64), hist.p, j, dj};
             S_apply(md, q, tauNow, geoNow, j > 0 ? dprev : dj, ev, true, j > 0 ? &dir : nullptr);
             const int gS = gridOf(nc, 64);   // d.Sd partials
-            slice_combine_k<D><<<gS, NT, 0, s>>>(NC, V, slicesT, partS.p, Kr.p, ctlIdx.p, dj, ldmWgt, knlEps, cS.p,
+            slice_combine_k<D><<<gS, NTS, 0, s>>>(NC, V, slicesT, partS.p, Kr.p, ctlIdx.p, dj, ldmWgt, knlEps, cS.p,
                                                  P1.p, recActive ? recD.p : nullptr, recSD.p, recR.p, rj, SPEC_M,
                                                  ctl.p);
             const bool sp = spec_active(md);
@@ -1197,8 +1204,7 @@ struct Engine : EngineBase {
                 throw SaError(SA_E_DEVICE, "rocblas_dgemm failed");
             ++parts;
         }
-        slice_rows_k<<<gridOf((long)mm, 64), NT, 0, s>>>((int)mm, parts, scr, C, nullptr);
-        CHECK_LAUNCH();
+        slice_rows((int)mm, parts, scr, C, nullptr);
     }
 
     // Rayleigh-Ritz for M S on the recorded Krylov basis (m iterations).

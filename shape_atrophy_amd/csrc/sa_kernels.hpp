@@ -831,6 +831,18 @@ __global__ __launch_bounds__(NT) void kstore_build_k(int V, const double *__rest
 constexpr int KS_MAX_COLS = 1024;
 // most Ritz vectors in the spectral term of the control-point preconditioner
 constexpr int SPEC_KMAX = 128;
+// slice-sum consumers (slice_rows_k, slice_combine_k, slice_dot_k): a block
+// sums the slice partials of 64 rows with SLW wavefronts, wavefront w adding
+// slices w, w + SLW, ..., the wavefront sums then added in order.  Sixteen
+// wavefronts keep enough loads in flight for the 100-250 slices of the
+// control-point products (a block per 64 rows gives only ~1 wavefront per
+// SIMD at C = 5,000); vectors of more than 64 Ki rows fill the chip with
+// four (slice_rows_k<4>).
+#ifndef SA_SLICE_WAVES
+#define SA_SLICE_WAVES 16
+#endif
+constexpr int SLW = SA_SLICE_WAVES;
+constexpr int NTS = 64 * SLW;
 #ifndef SA_KS_BLOCKS
 #define SA_KS_BLOCKS 1024
 #endif
@@ -994,8 +1006,8 @@ __global__ __launch_bounds__(NT) void kstore_mv_k(int nr, int nc, const T *__res
 }
 
 // control-point CG, fused consumers of the stored-product slice partials
-// (64 outputs per block, wavefront w adding slices w, w + 4, ..., combined in
-// wavefront order as in slice_rows_k):
+// (64 outputs per block, wavefront w adding slices w, w + SLW, ..., combined
+// in wavefront order as in slice_rows_k):
 //
 // slice_combine_k: y = sum of the K_cx (B K_xc d) partials; out = S d = y +
 // ldmWgt (K_cc d + eps d), K_cc d read from w = K_xc d at the control-point
@@ -1004,15 +1016,15 @@ __global__ __launch_bounds__(NT) void kstore_mv_k(int nr, int nc, const T *__res
 // recD, recSD, recR (n x recMax each), the Krylov basis the spectral
 // preconditioner is harvested from.
 template <int D>
-__global__ __launch_bounds__(NT) void slice_combine_k(int C, int V, int slices, const double *__restrict__ part,
-                                                      const double *__restrict__ w, const int *__restrict__ idx,
-                                                      const double *__restrict__ d, double ldmWgt, double eps,
-                                                      double *__restrict__ out, double *__restrict__ P,
-                                                      double *__restrict__ recD, double *__restrict__ recSD,
-                                                      double *__restrict__ recR, const double *__restrict__ r,
-                                                      int recMax, const Ctl *ctl) {
-    __shared__ double red[NT / 64][64];
-    __shared__ double red2[NT / 64];
+__global__ __launch_bounds__(NTS) void slice_combine_k(int C, int V, int slices, const double *__restrict__ part,
+                                                       const double *__restrict__ w, const int *__restrict__ idx,
+                                                       const double *__restrict__ d, double ldmWgt, double eps,
+                                                       double *__restrict__ out, double *__restrict__ P,
+                                                       double *__restrict__ recD, double *__restrict__ recSD,
+                                                       double *__restrict__ recR, const double *__restrict__ r,
+                                                       int recMax, const Ctl *ctl) {
+    __shared__ double red[SLW][64];
+    __shared__ double red2[SLW];
     if (cld(&ctl->done)) return;
     const int n = C * D;
     const int it = recD ? cld(&ctl->it) : recMax;
@@ -1020,14 +1032,14 @@ __global__ __launch_bounds__(NT) void slice_combine_k(int C, int V, int slices, 
     const int i = blockIdx.x * 64 + lane;
     double s = 0.0;
     if (i < n)
-        for (int k = wv; k < slices; k += NT / 64) s += part[(size_t)k * n + i];
+        for (int k = wv; k < slices; k += SLW) s += part[(size_t)k * n + i];
     red[wv][lane] = s;
     __syncthreads();
     double ss = 0.0;
     if (wv == 0 && i < n) {
         double y = red[0][lane];
 #pragma unroll
-        for (int ww = 1; ww < NT / 64; ++ww) y += red[ww][lane];
+        for (int ww = 1; ww < SLW; ++ww) y += red[ww][lane];
         const int c = i / C;
         const double kd = w[(size_t)c * V + idx[i - c * C]];
         const double sd = y + ldmWgt * (kd + eps * d[i]);
@@ -1048,14 +1060,17 @@ __global__ __launch_bounds__(NT) void slice_combine_k(int C, int V, int slices, 
 // the tolerance; else z = M r from the preconditioner product's partials
 // (plus, with U, the spectral term U diag(cf) w, w = U^T r summed from the
 // nPw block partials wp of ctl_update_k) and the block partial of r.z
-__global__ __launch_bounds__(NT) void slice_dot_k(int n, int slices, const double *__restrict__ part,
-                                                  const double *__restrict__ Prr, int nPrr,
-                                                  const double *__restrict__ r, double *__restrict__ z,
-                                                  double *__restrict__ P3, const double *__restrict__ U,
-                                                  const double *__restrict__ cf, const double *__restrict__ wp,
-                                                  int nPw, int k, Ctl *ctl) {
-    __shared__ double red[NT / 64][64];
-    __shared__ double red2[NT / 64];
+__global__ __launch_bounds__(NTS) void slice_dot_k(int n, int slices, const double *__restrict__ part,
+                                                   const double *__restrict__ Prr, int nPrr,
+                                                   const double *__restrict__ r, double *__restrict__ z,
+                                                   double *__restrict__ P3, const double *__restrict__ U,
+                                                   const double *__restrict__ cf, const double *__restrict__ wp,
+                                                   int nPw, int k, Ctl *ctl) {
+    constexpr int G = NTS / SPEC_KMAX;   // threads summing each w entry
+    static_assert(G >= 1, "slice-consumer block smaller than the spectral term");
+    __shared__ double red[SLW][64];
+    __shared__ double red2[SLW];
+    __shared__ double cwp[G][SPEC_KMAX];
     __shared__ double cw[SPEC_KMAX];
     if (cld(&ctl->done)) return;
     const int it = cld(&ctl->it);
@@ -1067,21 +1082,30 @@ __global__ __launch_bounds__(NT) void slice_dot_k(int n, int slices, const doubl
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int i = blockIdx.x * 64 + lane;
     if (U) {
-        for (int b = threadIdx.x; b < k; b += NT) {
+        // w = U^T r from the nPw block partials: thread (g, b) sums partials
+        // g, g + G, ...; the G sums are added in order (the same in every block)
+        const int b = threadIdx.x % SPEC_KMAX, g = threadIdx.x / SPEC_KMAX;
+        if (b < k) {
             double s = 0.0;
-#pragma unroll 16
-            for (int j = 0; j < nPw; ++j) s += wp[(size_t)j * k + b];
-            cw[b] = cf[b] * s;
+            for (int j = g; j < nPw; j += G) s += wp[(size_t)j * k + b];
+            cwp[g][b] = s;
+        }
+        __syncthreads();
+        if (threadIdx.x < k) {
+            double s = cwp[0][threadIdx.x];
+#pragma unroll
+            for (int gg = 1; gg < G; ++gg) s += cwp[gg][threadIdx.x];
+            cw[threadIdx.x] = cf[threadIdx.x] * s;
         }
         __syncthreads();
     }
     double s = 0.0;
     if (i < n) {
-        for (int q = wv; q < slices; q += NT / 64) s += part[(size_t)q * n + i];
-        // wavefront wv's share of the spectral term: columns wv, wv + 4, ...
+        for (int q = wv; q < slices; q += SLW) s += part[(size_t)q * n + i];
+        // wavefront wv's share of the spectral term: columns wv, wv + SLW, ...
         if (U) {
             double t = 0.0;
-            for (int b = wv; b < k; b += NT / 64) t += U[(size_t)b * n + i] * cw[b];
+            for (int b = wv; b < k; b += SLW) t += U[(size_t)b * n + i] * cw[b];
             s += t;
         }
     }
@@ -1091,7 +1115,7 @@ __global__ __launch_bounds__(NT) void slice_dot_k(int n, int slices, const doubl
     if (wv == 0 && i < n) {
         double zi = red[0][lane];
 #pragma unroll
-        for (int ww = 1; ww < NT / 64; ++ww) zi += red[ww][lane];
+        for (int ww = 1; ww < SLW; ++ww) zi += red[ww][lane];
         z[i] = zi;
         ss = r[i] * zi;
     }
@@ -1152,24 +1176,25 @@ __global__ void panelize_k(int n, const double *__restrict__ src, int ld, double
     dst[panel_at(i, j, panel_rows<double>(), n)] = i >= j ? src[(size_t)j * ld + i] : src[(size_t)i * ld + j];
 }
 
-// y[i] = sum over the slices s (in slice order) of part[s * n + i]: 64
-// outputs per block, wavefront w adding slices w, w + 4, ..., combined in
-// wavefront order
-__global__ __launch_bounds__(NT) void slice_rows_k(int n, int slices, const double *__restrict__ part,
-                                                   double *__restrict__ y, const Ctl *ctl) {
-    __shared__ double red[NT / 64][64];
+// y[i] = sum of the slice partials part[s * n + i]: 64 outputs per block,
+// wavefront w of W adding slices w, w + W, ..., combined in wavefront order
+// (W = SLW for short vectors; 4 when the rows alone give enough blocks)
+template <int W>
+__global__ __launch_bounds__(64 * W) void slice_rows_k(int n, int slices, const double *__restrict__ part,
+                                                       double *__restrict__ y, const Ctl *ctl) {
+    __shared__ double red[W][64];
     if (ctl && cld(&ctl->done)) return;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int i = blockIdx.x * 64 + lane;
     double s = 0.0;
     if (i < n)
-        for (int k = wv; k < slices; k += NT / 64) s += part[(size_t)k * n + i];
+        for (int k = wv; k < slices; k += W) s += part[(size_t)k * n + i];
     red[wv][lane] = s;
     __syncthreads();
     if (wv == 0 && i < n) {
         double t = red[0][lane];
 #pragma unroll
-        for (int w = 1; w < NT / 64; ++w) t += red[w][lane];
+        for (int w = 1; w < W; ++w) t += red[w][lane];
         y[i] = t;
     }
 }
