@@ -57,6 +57,15 @@ std::mutex g_libMu;
 #define SA_MPRE_BUILD 1
 #endif
 // steps a control-point preconditioner is reused at most
+// mixed form: refinement residuals from the FP32 matrices and their FP32
+// remainders (1) or from matrix-free FP64 kernel products (0)
+#ifndef SA_SPLIT_RESID
+#define SA_SPLIT_RESID 1
+#endif
+// the remainders are stored when both fit in this many bytes
+#ifndef SA_SPLIT_MAX_GB
+#define SA_SPLIT_MAX_GB 8
+#endif
 #ifndef SA_PREC_MAX_AGE
 #define SA_PREC_MAX_AGE 8
 #endif
@@ -237,6 +246,7 @@ struct Engine : EngineBase {
     DBuf<double> aM1, aM2, aM3;  // all-landmark form: the last three alpha = K^-1 v
     int slicesK = 1;
     DBuf<float> A32, At32;       // stored K_xc / K_cx, FP32 (mixed precision)
+    DBuf<float> A32lo, At32lo;   // their FP32 remainders (split products of the refinement residuals)
     DBuf<float> Mpre32;          // FP32 panel copy of the preconditioner (mixed, SA_MIXED_CC32)
     // spectral term of the control-point preconditioner, z = M r + U diag(cf) U^T r:
     // U holds Ritz vectors of M S for its largest eigenvalues lambda_l (orthonormal
@@ -583,6 +593,11 @@ struct Engine : EngineBase {
                 A32.alloc(na, tot); At32.alloc(nt, tot);
                 CK(hipMemsetAsync(A32.p, 0, sizeof(float) * na, s));
                 CK(hipMemsetAsync(At32.p, 0, sizeof(float) * nt, s));
+                if (SA_SPLIT_RESID && !no64Store && (na + nt) * sizeof(float) <= ((size_t)SA_SPLIT_MAX_GB << 30)) {
+                    A32lo.alloc(na, tot); At32lo.alloc(nt, tot);
+                    CK(hipMemsetAsync(A32lo.p, 0, sizeof(float) * na, s));
+                    CK(hipMemsetAsync(At32lo.p, 0, sizeof(float) * nt, s));
+                }
             } else {
                 const size_t na = padded_rows<double>(V) * (size_t)NC, nt = padded_rows<double>(NC) * (size_t)V;
                 if ((na + nt) * sizeof(double) <= (size_t)48 << 30 && !no64Store) {
@@ -602,7 +617,8 @@ struct Engine : EngineBase {
             slicesT = std::max(gridOf(V, KS_MAX_COLS), std::min(256, (tgt + rbT - 1) / rbT));
             slicesM = std::max(gridOf(NC, KS_MAX_COLS), std::min(256, (KS_BLOCKS + rbM - 1) / rbM));
             slicesM32 = std::max(gridOf(NC, KS_MAX_COLS), std::min(256, (KS_BLOCKS + rbM32 - 1) / rbM32));
-            partS.alloc(std::max({(size_t)slicesA * D * V, (size_t)slicesT * nc, (size_t)slicesM * nc,
+            // (twice the product slices for the split residual products)
+            partS.alloc(std::max({(size_t)2 * slicesA * D * V, (size_t)2 * slicesT * nc, (size_t)slicesM * nc,
                                   (size_t)slicesM32 * nc}), tot);
             int rb = gridOf(NC, MV_ROWS);
             slicesC = std::max(1, std::min(32, (1024 + rb - 1) / rb));
@@ -909,7 +925,9 @@ struct Engine : EngineBase {
 
     // kernel-matrix products of the control-point form: matrix-free FP64, or
     // the stored matrices (see kstore_build_k) in FP64 or FP32
-    enum class KMode { MatrixFree64, Stored64, Stored32 };
+    // Split32: the FP32 matrices plus their FP32 remainders, K = hi + lo to
+    // about 2^-48 relative (residual products of the mixed form only)
+    enum class KMode { MatrixFree64, Stored64, Stored32, Split32 };
 
     // y[i] = sum of the slice partials part[k * n + i] (slice_rows_k; four
     // wavefronts per 64 rows once the rows alone give 1,024 blocks)
@@ -980,7 +998,7 @@ struct Engine : EngineBase {
             CHECK_LAUNCH();
         }
         if (fp32 && A32.p) {
-#define SA_KS(O) kstore_build_k<D, O, float><<<g, NT, 0, s>>>(V, q, NC, qc.p, invW, A32.p, At32.p)
+#define SA_KS(O) kstore_build_k<D, O, float><<<g, NT, 0, s>>>(V, q, NC, qc.p, invW, A32.p, At32.p, A32lo.p, At32lo.p)
             switch (knlOrder) {
                 case -1: SA_KS(-1); break;
                 case 0: SA_KS(0); break;
@@ -1020,6 +1038,18 @@ struct Engine : EngineBase {
         }
         const int nr = xc ? V : NC, ncol = xc ? NC : V;
         const int sl = xc ? slicesA : slicesT;
+        if (md == KMode::Split32) {
+            // hi product into slices [0, sl), lo product into [sl, 2 sl), summed together
+            dim3 g(gridOf(nr, panel_rows<float>()), sl);
+            kstore_mv_k<D, float><<<g, NT, ks_lds(ncol, sl), s>>>(nr, ncol, xc ? A32.p : At32.p, u, partS.p, c);
+            kstore_mv_k<D, float><<<g, NT, ks_lds(ncol, sl), s>>>(nr, ncol, xc ? A32lo.p : At32lo.p, u,
+                                                                   partS.p + (size_t)sl * D * nr, c);
+            CHECK_LAUNCH();
+            if (P) slice_sum_k<<<gridOf((long)D * nr), NT, 0, s>>>(D * nr, 2 * sl, partS.p, y, P, c);
+            else slice_rows(D * nr, 2 * sl, partS.p, y, c);
+            CHECK_LAUNCH();
+            return;
+        }
         if (ev) CK(hipEventRecord(ev[0], s));
         if (md == KMode::Stored64) {
             dim3 g(gridOf(nr, panel_rows<double>()), sl);
@@ -1469,7 +1499,10 @@ struct Engine : EngineBase {
         kstore_build(q, !mixedStep, mixedStep);
         CK(hipMemsetAsync(ctl.p, 0, sizeof(Ctl), s));
         // g = K_cx f in FP64 (stored FP64 matrix when present), |g|^2 partials
-        KMode exact = !mixedStep && A64.p ? KMode::Stored64 : KMode::MatrixFree64;
+        // exact products: the stored FP64 matrices (plain form), else the FP32
+        // matrices with their remainders (mixed form), else matrix-free FP64
+        KMode exact = !mixedStep ? (A64.p ? KMode::Stored64 : KMode::MatrixFree64)
+                                 : (A32lo.p ? KMode::Split32 : KMode::MatrixFree64);
         kx_product(exact, false, q, f.p, cG.p, P0.p, nullptr);
         int status;
         sweeps = 0;
@@ -1479,7 +1512,8 @@ struct Engine : EngineBase {
             chunkHint = lastIters - 4;
             status = ctl_solve(exact, q, tauNow, geoNow, cG.p, cA.p, false, cgTolSqu, itrMax, iters);
         } else {
-            // iterative refinement: FP64 residual r = g - S alpha, inner
+            // iterative refinement: residual r = g - S alpha (FP64 arithmetic,
+            // kernel entries exact to ~2^-48), inner
             // solve of S~ delta = r with the FP32 stored K (relative 1e-5),
             // alpha += delta, until |r|^2 < cgTolSqu |g|^2
             double gg = 0.0;
@@ -1503,7 +1537,7 @@ struct Engine : EngineBase {
             CK(hipMemcpyAsync(cA0.p, cA.p, sizeof(double) * nc, hipMemcpyDeviceToDevice, s));
             for (int sw = 0; sw < 12 && gg != 0.0; ++sw) {
                 CK(hipMemsetAsync(ctl.p, 0, sizeof(Ctl), s));
-                S_apply(KMode::MatrixFree64, q, tauNow, geoNow, cA.p, nullptr);
+                S_apply(exact, q, tauNow, geoNow, cA.p, nullptr);
                 ctl_combine_k<D, 1><<<gC, NT, 0, s>>>(NC, V, cY.p, nullptr, Kr.p, ctlIdx.p, cA.p, cG.p, ldmWgt, knlEps, cRout.p,
                                                      P0.p, ctl.p);
                 CHECK_LAUNCH();
@@ -1542,6 +1576,7 @@ struct Engine : EngineBase {
                 forceRefresh = true;   // the next step refreshes and takes its iteration count as the reference
                 CK(hipMemcpyAsync(cA.p, cA0.p, sizeof(double) * nc, hipMemcpyDeviceToDevice, s));
                 ensure_fp64_store();
+                exact = KMode::MatrixFree64;
                 if (A64.p) {
                     kstore_build(q, true, false);
                     exact = KMode::Stored64;
@@ -1940,8 +1975,12 @@ long long sa_session_footprint(const sa_mesh *m, const sa_target *t, const sa_pa
         // rows padded to whole panels of 256 (FP32) / 128 (FP64) rows
         auto pad = [](long long n, long long pr) { return (n + pr - 1) / pr * pr; };
         // K_cc, rocBLAS output, preconditioner, vectors, slice partials, cross partials
-        b += 8 * (2 * C * C + pad(C, 128) * C + 14 * C * D + 256 * C * D + 64 * D * V + 32 * C * D);
-        if (p->precision == SA_PREC_MIXED) b += 4 * (pad(V, 256) * C + pad(C, 256) * V) + 8 * pad(C, 128) * C;
+        b += 8 * (2 * C * C + pad(C, 128) * C + 14 * C * D + 512 * C * D + 128 * D * V + 32 * C * D);
+        if (p->precision == SA_PREC_MIXED) {
+            const long long a32 = 4 * (pad(V, 256) * C + pad(C, 256) * V);
+            b += a32 + 8 * pad(C, 128) * C;
+            if (SA_SPLIT_RESID && a32 <= ((long long)SA_SPLIT_MAX_GB << 30)) b += a32;   // FP32 remainders
+        }
         else if ((pad(V, 128) * C + pad(C, 128) * V) * 8 <= (48LL << 30)) b += 8 * (pad(V, 128) * C + pad(C, 128) * V);
         // spectral preconditioner: recorded Krylov basis, Ritz vectors, projected matrices
         b += 8 * ((3 * SA_SPEC_M + SA_SPEC_K) * C * D + (6 + 65) * SA_SPEC_M * SA_SPEC_M + 2 * SA_SPEC_M +

@@ -782,14 +782,19 @@ __global__ void kcc_build_k(int C, const double *__restrict__ qc, double invW, d
 // the same rounded values, so the two products of a CG iteration use exactly
 // K and K^T.  T = double (plain FP64 form) or
 // float (mixed form).  A 64 x 64 tile per block; At is written through LDS so
-// both stores are coalesced.
+// both stores are coalesced.  For T = float, Alo / Atlo (optional) receive the
+// FP32 remainder K - (double)A of each entry in the same layout: the pair
+// gives the refinement residuals of the mixed form products accurate to about
+// 2^-48 relative per entry from two FP32 matrix reads.
 template <int D, int ORD, typename T>
 __global__ __launch_bounds__(NT) void kstore_build_k(int V, const double *__restrict__ q, int C,
                                                      const double *__restrict__ qc, double invW, T *__restrict__ A,
-                                                     T *__restrict__ At) {
+                                                     T *__restrict__ At, T *__restrict__ Alo = nullptr,
+                                                     T *__restrict__ Atlo = nullptr) {
     constexpr int PR = panel_rows<T>();
     __shared__ double etab[64];
     __shared__ T tile[64][65];
+    __shared__ T tlo[sizeof(T) == 4 ? 64 : 1][65];
     init_exp_table(etab);
     __syncthreads();
     const int i0 = blockIdx.x * 64, j0 = blockIdx.y * 64;
@@ -800,15 +805,22 @@ __global__ __launch_bounds__(NT) void kstore_build_k(int V, const double *__rest
     for (int c = 0; c < D; ++c) qi[c] = i < V ? q[c * V + i] : 0.0;
     for (int lj = lj0; lj < 64; lj += 4) {
         int j = j0 + lj;
-        T v = 0;
+        T v = 0, vl = 0;
         if (i < V && j < C) {
             double d2 = 0.0;
 #pragma unroll
             for (int c = 0; c < D; ++c) { double dd = qi[c] - qc[c * C + j]; d2 += dd * dd; }
-            v = (T)rkhs_kernel<D, ORD>(d2, invW, etab);
+            const double kv = rkhs_kernel<D, ORD>(d2, invW, etab);
+            v = (T)kv;
             A[panel_at(i, j, PR, C)] = v;
+            if constexpr (sizeof(T) == 4) {
+                // FP32 remainder: v + vl equals kv to about 2^-48 relative
+                vl = (T)(kv - (double)v);
+                if (Alo) Alo[panel_at(i, j, PR, C)] = vl;
+            }
         }
         tile[lj][li] = v;
+        if constexpr (sizeof(T) == 4) tlo[lj][li] = vl;
     }
     __syncthreads();
     // transposed store: lanes run along j
@@ -816,6 +828,8 @@ __global__ __launch_bounds__(NT) void kstore_build_k(int V, const double *__rest
     for (int l = li0; l < 64; l += 4) {
         int ii = i0 + l, jj = j0 + lj;
         if (At && ii < V && jj < C) At[panel_at(jj, ii, PR, V)] = tile[lj][l];
+        if constexpr (sizeof(T) == 4)
+            if (Atlo && ii < V && jj < C) Atlo[panel_at(jj, ii, PR, V)] = tlo[lj][l];
     }
 }
 

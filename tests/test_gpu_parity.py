@@ -213,6 +213,31 @@ def test_mixed_precision_control_points(which, oracle):
     assert abs(r["obj"] - o["obj"]) <= 1e-7 * abs(o["obj"])
 
 
+def test_mixed_residual_products_split_and_matrix_free(oracle, monkeypatch):
+    """The mixed form's refinement residuals use the FP32 stored matrices plus
+    their FP32 remainders (each entry exact to ~2^-48); with SA_MATRIX_FREE=1
+    they are matrix-free FP64 kernel products instead.  Both meet the same
+    FP64 residual test, so the two runs agree with each other and with the
+    oracle to the CG tolerance."""
+    p = small_shell()
+    V = p["q0"].shape[1]
+    p["ctlIdx"] = np.sort(np.random.default_rng(3).choice(V, V // 3, replace=False)).astype(np.int32)
+    p["name"] = "shell3d_mixed_split"
+    print(f"[gpu] session {p['name']} V={V}", flush=True)
+    rs = api.Session(p, precision="mixed").run(p["tauVar"][:, 0], p["tauPrm"], want_traj=True)
+    monkeypatch.setenv("SA_MATRIX_FREE", "1")
+    print(f"[gpu] session {p['name']} matrix-free V={V}", flush=True)
+    rm = api.Session(p, precision="mixed").run(p["tauVar"][:, 0], p["tauPrm"], want_traj=True)
+    o = oracle.objfcn(p)
+    assert rs["status"] == 0 and rm["status"] == 0 and o["status"] == 0
+    disp = np.linalg.norm(o["lmkStk"][-1] - o["lmkStk"][0])
+    assert np.linalg.norm(rs["lmkStk"][-1] - rm["lmkStk"][-1]) / disp < 1e-8
+    assert abs(rs["obj"] - rm["obj"]) <= 1e-8 * abs(o["obj"])
+    for r in (rs, rm):
+        assert np.linalg.norm(r["lmkStk"][-1] - o["lmkStk"][-1]) / disp < 1e-7
+        assert abs(r["obj"] - o["obj"]) <= 1e-7 * abs(o["obj"])
+
+
 def test_mixed_precision_falls_back_to_fp64():
     """2D Matern nu=4 on 10% of the landmarks: K_cc is so ill conditioned that
     FP32 kernel products cannot drive the FP64 residual down; such steps are
