@@ -57,6 +57,13 @@ std::mutex g_libMu;
 #define SA_MPRE_BUILD 1
 #endif
 // steps a control-point preconditioner is reused at most
+// mixed form: squared relative tolerance of an inner FP32 solve (at most;
+// looser when the outer target needs less): 1e-9, a residual cut of about
+// 3e-5 per sweep, measured best on the 51k-landmark shell (1e-10 and 1e-8
+// took 1.5 % and 0.6 % longer per step)
+#ifndef SA_INNER_TOL2
+#define SA_INNER_TOL2 1e-9
+#endif
 // mixed form: refinement residuals from the FP32 matrices and their FP32
 // remainders (1) or from matrix-free FP64 kernel products (0)
 #ifndef SA_SPLIT_RESID
@@ -1514,7 +1521,7 @@ struct Engine : EngineBase {
         } else {
             // iterative refinement: residual r = g - S alpha (FP64 arithmetic,
             // kernel entries exact to ~2^-48), inner
-            // solve of S~ delta = r with the FP32 stored K (relative 1e-5),
+            // solve of S~ delta = r with the FP32 stored K (relative ~3e-5),
             // alpha += delta, until |r|^2 < cgTolSqu |g|^2
             double gg = 0.0;
             {
@@ -1553,8 +1560,8 @@ struct Engine : EngineBase {
                 rrPrev = rr;
                 ++sweeps;
                 int its = 0;
-                // inner tolerance: relative 1e-5 of this residual, never below the outer target
-                double inner = std::max(1e-10, cgTolSqu * gg / rr);
+                // inner tolerance: relative sqrt(SA_INNER_TOL2) of this residual, never below the outer target
+                double inner = std::max(SA_INNER_TOL2, cgTolSqu * gg / rr);
                 chunkHint = 16;
                 // an inner solve may not run past 3x the previous step's
                 // iterations (+200; 1/8 of the budget on a run's first step):
