@@ -73,6 +73,17 @@ std::mutex g_libMu;
 #ifndef SA_SPLIT_MAX_GB
 #define SA_SPLIT_MAX_GB 8
 #endif
+// refreshes of the control-point preconditioner in a row that carry the
+// spectral term over instead of harvesting a new one: term harvested from
+// FP32 inner solves (mixed form) / from FP64 solves.  Measured at the 51k
+// shell: mixed 13.80 (never) / 13.89 (1 or 2) / 13.67 (always) steps/s;
+// FP64 9.79 / 10.19 (1) / 10.36 (2) / 10.69 (always)
+#ifndef SA_SPEC_CARRY
+#define SA_SPEC_CARRY 2
+#endif
+#ifndef SA_SPEC_CARRY64
+#define SA_SPEC_CARRY64 1000000
+#endif
 #ifndef SA_PREC_MAX_AGE
 #define SA_PREC_MAX_AGE 8
 #endif
@@ -267,6 +278,12 @@ struct Engine : EngineBase {
     DBuf<double> recD, recSD, recR, specU, specC, specW, specA, specScr;
     DBuf<int> specInfo;
     bool specEnabled = true, specOn = false, specRecord = false, recActive = false;
+    // the next refresh of M carries the spectral term over (spec_carry_finish)
+    // instead of recording a new basis; specCarried counts carried refreshes
+    // since the last harvest
+    bool specCarry = false;
+    int specCarried = 0;
+    static constexpr double SPEC_CARRY_GMIN = 0.5;   // Gram eigenvalues kept within [0.5, 2]
     int specK = 0, specMode = -1;
     int slicesA = 1, slicesT = 1, slicesM = 1, slicesM32 = 1;
     DBuf<int> potrfInfo;
@@ -844,6 +861,10 @@ struct Engine : EngineBase {
     void kcc_build(bool refresh) {
         kcc_build_once(refresh);
         if (!refresh) return;
+        struct CarryDone {
+            Engine *e;
+            ~CarryDone() { e->specCarry = false; }
+        } carryDone{this};
         // a factorisation that reports failure is repeated once from a fresh
         // K_cc: with several sessions on the device a K_cc that factorises
         // alone every time has been reported indefinite (rarely; cause not
@@ -866,6 +887,60 @@ struct Engine : EngineBase {
                              info[1], again[0], again[1], NC);
             }
         }
+        if (specCarry) spec_carry_finish();
+    }
+
+    // U <- U G^-1/2 (G = Q g Q^T, the Gram matrix kcc_build_once formed):
+    // the carried Ritz vectors become orthonormal in the new M^-1 inner
+    // product, so M + U diag(cf) U^T stays positive definite (each cf in
+    // (-1, 0)); the symmetric (Loewdin) choice keeps each vector closest to
+    // the one it replaces, so its coefficient still fits.  A Gram matrix far
+    // from the identity (the subspace no longer near the old one) drops the
+    // term and records a new basis instead.
+    void spec_carry_finish() {
+        const int nc = NC * D, k = specK;
+        const size_t kk = (size_t)k * k, MM = (size_t)SPEC_M * SPEC_M;
+        double *dG = specA.p, *dW = dG + MM, *dEv = dG + 6 * MM, *dE = dEv + SPEC_M;
+        std::vector<double> Q(kk), ev(k), W(kk, 0.0);
+        int info = 0;
+        {
+            std::lock_guard<std::mutex> lk(g_libMu);
+            if (rocsolver_dsyevd(blas, rocblas_evect_original, rocblas_fill_lower, k, dG, k, dEv, dE, specInfo.p) !=
+                rocblas_status_success)
+                throw SaError(SA_E_DEVICE, "rocsolver_dsyevd failed");
+            CK(hipStreamSynchronize(s));
+        }
+        CK(hipMemcpyAsync(Q.data(), dG, sizeof(double) * kk, hipMemcpyDeviceToHost, s));
+        CK(hipMemcpyAsync(ev.data(), dEv, sizeof(double) * k, hipMemcpyDeviceToHost, s));
+        CK(hipMemcpyAsync(&info, specInfo.p, sizeof(int), hipMemcpyDeviceToHost, s));
+        CK(hipStreamSynchronize(s));
+        bool ok = info == 0;
+        for (int i = 0; i < k && ok; ++i) ok = std::isfinite(ev[i]) && ev[i] > SPEC_CARRY_GMIN && ev[i] < 1.0 / SPEC_CARRY_GMIN;
+        for (size_t i = 0; i < kk && ok; ++i) ok = std::isfinite(Q[i]);
+        if (!ok) {
+            specOn = false;
+            specRecord = specEnabled;
+            return;
+        }
+        // W = Q g^-1/2 Q^T
+        for (int l = 0; l < k; ++l) {
+            const double f = 1.0 / std::sqrt(ev[l]);
+            for (int j = 0; j < k; ++j) {
+                const double qj = Q[j + (size_t)l * k] * f;
+                for (int i = 0; i < k; ++i) W[i + (size_t)j * k] += Q[i + (size_t)l * k] * qj;
+            }
+        }
+        CK(hipMemcpyAsync(dW, W.data(), sizeof(double) * kk, hipMemcpyHostToDevice, s));
+        const double one = 1.0, zero = 0.0;
+        {
+            std::lock_guard<std::mutex> lk(g_libMu);
+            if (rocblas_dgemm(blas, rocblas_operation_none, rocblas_operation_none, nc, k, k, &one, specU.p, nc, dW, k,
+                              &zero, recD.p, nc) != rocblas_status_success)
+                throw SaError(SA_E_DEVICE, "rocblas_dgemm failed");
+            CK(hipMemcpyAsync(specU.p, recD.p, sizeof(double) * nc * k, hipMemcpyDeviceToDevice, s));
+            CK(hipStreamSynchronize(s));
+        }
+        ++specCarried;
     }
 
     void kcc_build_once(bool refresh) {
@@ -886,11 +961,25 @@ struct Engine : EngineBase {
 #undef SA_KB
         CHECK_LAUNCH();
         if (!refresh) return;
-        // a new M: the spectral term (orthonormal in the old M's inner
-        // product) is dropped and harvested again from the next solve
-        specOn = false;
-        specRecord = specEnabled;
         std::lock_guard<std::mutex> lk(g_libMu);
+        if (specCarry) {
+            // the spectral term carried over to the new M: its Gram matrix in
+            // the new M^-1 = K_cc + eps I inner product, G = U^T (K_cc + eps I) U,
+            // while Kcc still holds K_cc + eps I (spec_carry_finish
+            // re-orthonormalises U with it)
+            const int nc = NC * D, kd = specK * D;
+            const double one = 1.0, zero = 0.0;
+            if (rocblas_dgemm(blas, rocblas_operation_none, rocblas_operation_none, NC, kd, NC, &one, Kcc.p, NC,
+                              specU.p, NC, &zero, recD.p, NC) != rocblas_status_success ||
+                rocblas_dgemm(blas, rocblas_operation_transpose, rocblas_operation_none, specK, specK, nc, &one,
+                              specU.p, nc, recD.p, nc, &zero, specA.p, specK) != rocblas_status_success)
+                throw SaError(SA_E_DEVICE, "rocblas_dgemm failed");
+        } else {
+            // a new M: the spectral term (orthonormal in the old M's inner
+            // product) is dropped and harvested again from the next solve
+            specOn = false;
+            specRecord = specEnabled;
+        }
         if (rocsolver_dpotrf(blas, rocblas_fill_lower, NC, Kcc.p, NC, potrfInfo.p) != rocblas_status_success)
             throw SaError(SA_E_DEVICE, "rocsolver_dpotrf failed");
         if (rocsolver_dtrtri(blas, rocblas_fill_lower, rocblas_diagonal_non_unit, NC, Kcc.p, NC, potrfInfo.p + 1) !=
@@ -1380,6 +1469,7 @@ struct Engine : EngineBase {
         specOn = true;
         specMode = (int)md;
         specK = kk;
+        specCarried = 0;
     }
 
     int ctl_pcg_run(KMode md, const double *q, const double *tauNow, const double *geoNow, const double *rhs, double *xs,
@@ -1497,6 +1587,11 @@ struct Engine : EngineBase {
         CHECK_LAUNCH();
         const bool refresh = precAge < 0 || precAge >= PREC_MAX_AGE || lastIters > PREC_GROWTH * precIters + 4 ||
                              precEveryStep || forceRefresh;
+        // a scheduled refresh inside a run keeps the spectral term (at most
+        // SA_SPEC_CARRY times in a row); a run's first step, a growth-driven
+        // refresh and every other case harvest a new one
+        specCarry = refresh && precAge >= PREC_MAX_AGE && specOn &&
+                    specCarried < (specMode == (int)KMode::Stored32 ? SA_SPEC_CARRY : SA_SPEC_CARRY64);
         kcc_build(refresh);
         if (refresh) { precAge = 0; forceRefresh = false; }
         // this step in the mixed form (FP32 stored K inside FP64 refinement)

@@ -94,3 +94,41 @@ def test_spectral_term_cuts_iterations():
     x1, its, _ = pcg(S, Mf, b)
     assert its < 0.6 * itp
     assert np.linalg.norm(x1 - x0) <= 1e-8 * np.linalg.norm(x0)
+
+
+def carry(U, Mi_new):
+    """The term carried to a new preconditioner (engine: spec_carry_finish):
+    U G^-1/2 with G = U^T M_new^-1 U (symmetric square root)."""
+    g, Q = np.linalg.eigh(U.T @ Mi_new @ U)
+    return U @ (Q @ np.diag(1.0 / np.sqrt(g)) @ Q.T), g
+
+
+def test_carried_term_stays_positive_definite_and_useful():
+    S, M, Mi, _ = system()
+    n = S.shape[0]
+    rng = np.random.default_rng(5)
+    _, _, (D, SD, R, rz) = pcg(S, lambda r: M @ r, rng.standard_normal(n), rec=True)
+    lam, U = harvest(D, SD, R, rz, 40)
+    kk = U.shape[1]
+    cf = lam[kk] / lam[:kk] - 1.0
+    # the operator moves a little and the new M^-1 grows by 1.6: without
+    # re-orthonormalisation M_new + U diag(cf) U^T is indefinite
+    # (1/1.6 + cf < 0 along the leading directions)
+    E = 0.02 * rng.standard_normal((n, n))
+    Mi2 = 1.6 * (Mi + E @ E.T)
+    M2 = np.linalg.inv(Mi2)
+    S2 = S + 0.01 * (E + E.T) @ (E + E.T)
+    assert np.linalg.eigvalsh(M2 + U @ np.diag(cf) @ U.T).min() < 0
+    U2, g = carry(U, Mi2)
+    # within the engine's acceptance window for a carried term
+    assert g.min() > 0.5 and g.max() < 2.0
+    assert np.abs(U2.T @ Mi2 @ U2 - np.eye(kk)).max() < 1e-10
+    Mp = M2 + U2 @ np.diag(cf) @ U2.T
+    assert np.linalg.eigvalsh(0.5 * (Mp + Mp.T)).min() > 0
+    # it still cuts the iterations of a solve with the new operator, and the
+    # solution is the same
+    b = rng.standard_normal(n)
+    x0, itp, _ = pcg(S2, lambda r: M2 @ r, b)
+    x1, its, _ = pcg(S2, lambda r: M2 @ r + U2 @ (cf * (U2.T @ r)), b)
+    assert its < 0.7 * itp
+    assert np.linalg.norm(x1 - x0) <= 1e-8 * np.linalg.norm(x0)
