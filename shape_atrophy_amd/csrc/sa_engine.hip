@@ -47,10 +47,10 @@ std::mutex g_libMu;
 #endif
 // spectral preconditioner: iterations recorded, Ritz vectors kept
 #ifndef SA_SPEC_M
-#define SA_SPEC_M 256
+#define SA_SPEC_M 384
 #endif
 #ifndef SA_SPEC_K
-#define SA_SPEC_K 96
+#define SA_SPEC_K 128
 #endif
 // how M = X^T X is formed: 0 dgemm, 1 dsyrk, 2 lauum (in place)
 #ifndef SA_MPRE_BUILD
@@ -64,6 +64,14 @@ std::mutex g_libMu;
 #ifndef SA_INNER_TOL2
 #define SA_INNER_TOL2 1e-9
 #endif
+// mixed form: squared relative tolerance of an inner solve that records the
+// Krylov basis of a spectral harvest (never below the outer target): a
+// longer basis gives a stronger term for all the solves that carry it.
+// Measured at the 51k shell: 13.85 (1e-9, as the other inner solves) /
+// 14.40 (1e-11) / 14.77 (1e-12) / 14.80 (1e-13) / 15.13 (1e-14) steps/s
+#ifndef SA_REC_TOL2
+#define SA_REC_TOL2 1e-14
+#endif
 // mixed form: refinement residuals from the FP32 matrices and their FP32
 // remainders (1) or from matrix-free FP64 kernel products (0)
 #ifndef SA_SPLIT_RESID
@@ -74,15 +82,13 @@ std::mutex g_libMu;
 #define SA_SPLIT_MAX_GB 8
 #endif
 // refreshes of the control-point preconditioner in a row that carry the
-// spectral term over instead of harvesting a new one: term harvested from
-// FP32 inner solves (mixed form) / from FP64 solves.  Measured at the 51k
-// shell: mixed 13.80 (never) / 13.89 (1 or 2) / 13.67 (always) steps/s;
-// FP64 9.79 / 10.19 (1) / 10.36 (2) / 10.69 (always)
+// spectral term over instead of harvesting a new one (default: all of a
+// run's scheduled refreshes).  Measured at the 51k shell, FP64 form: 9.79
+// (never) / 10.19 (1) / 10.36 (2) / 10.69 (always) steps/s; mixed form with
+// a longer recording solve (SA_REC_TOL2): 14.28 (never, 1e-13) / 15.37 (4,
+// 1e-16) / 15.62 (always, 1e-16)
 #ifndef SA_SPEC_CARRY
-#define SA_SPEC_CARRY 2
-#endif
-#ifndef SA_SPEC_CARRY64
-#define SA_SPEC_CARRY64 1000000
+#define SA_SPEC_CARRY 1000000
 #endif
 #ifndef SA_PREC_MAX_AGE
 #define SA_PREC_MAX_AGE 8
@@ -1590,8 +1596,7 @@ struct Engine : EngineBase {
         // a scheduled refresh inside a run keeps the spectral term (at most
         // SA_SPEC_CARRY times in a row); a run's first step, a growth-driven
         // refresh and every other case harvest a new one
-        specCarry = refresh && precAge >= PREC_MAX_AGE && specOn &&
-                    specCarried < (specMode == (int)KMode::Stored32 ? SA_SPEC_CARRY : SA_SPEC_CARRY64);
+        specCarry = refresh && precAge >= PREC_MAX_AGE && specOn && specCarried < SA_SPEC_CARRY;
         kcc_build(refresh);
         if (refresh) { precAge = 0; forceRefresh = false; }
         // this step in the mixed form (FP32 stored K inside FP64 refinement)
@@ -1657,6 +1662,8 @@ struct Engine : EngineBase {
                 int its = 0;
                 // inner tolerance: relative sqrt(SA_INNER_TOL2) of this residual, never below the outer target
                 double inner = std::max(SA_INNER_TOL2, cgTolSqu * gg / rr);
+                // a solve that records the Krylov basis of a spectral harvest runs longer
+                if (specRecord) inner = std::max(std::min(inner, SA_REC_TOL2), cgTolSqu * gg / rr);
                 chunkHint = 16;
                 // an inner solve may not run past 3x the previous step's
                 // iterations (+200; 1/8 of the budget on a run's first step):
