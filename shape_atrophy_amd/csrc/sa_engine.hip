@@ -1897,13 +1897,14 @@ struct Engine : EngineBase {
             if (NC == 0 && t >= 1) {
                 // all-landmark form: alpha = K^-1 v extrapolated the same way;
                 // the step then starts from x0 = K alpha
-                extrap_k<<<gridOf((long)n), NT, 0, s>>>((int)n, std::min(t, SA_EXTRAP), a.p, aM1.p, aM2.p, aM3.p);
+                extrap_k<<<gridOf((long)n), NT, 0, s>>>((int)n, std::min(t - 1, SA_EXTRAP), a.p, aM1.p, aM2.p, aM3.p);
                 CHECK_LAUNCH();
             }
             if (NC > 0 && t >= 1) {
                 // initial guess: the previous solutions extrapolated to this
-                // step (linear from two, quadratic from three)
-                extrap_k<<<gridOf((long)NC * D), NT, 0, s>>>(NC * D, std::min(t, SA_EXTRAP), cA.p, cAm1.p, cAm2.p,
+                // step (step t has t of them: the last one alone, then
+                // linear from two, quadratic from three, cubic from four)
+                extrap_k<<<gridOf((long)NC * D), NT, 0, s>>>(NC * D, std::min(t - 1, SA_EXTRAP), cA.p, cAm1.p, cAm2.p,
                                                              cAm3.p);
                 CHECK_LAUNCH();
             }
