@@ -332,6 +332,7 @@ struct Engine : EngineBase {
         }
     }
     int chunkHint = 4;           // first launch chunk of the next control-point CG
+    int sweepIters[4] = {0, 0, 0, 0};   // inner iterations of the last step's refinement sweeps
     double ttSum = 0.0;
     bool ttReady = false;
 
@@ -1545,7 +1546,7 @@ struct Engine : EngineBase {
             }
             if (h.done) break;
             it0 = itEnd;
-            chunk = 16;
+            chunk = 8;
         }
         iters = h.iters;
         return h.status;
@@ -1664,7 +1665,8 @@ struct Engine : EngineBase {
                 double inner = std::max(SA_INNER_TOL2, cgTolSqu * gg / rr);
                 // a solve that records the Krylov basis of a spectral harvest runs longer
                 if (specRecord) inner = std::max(std::min(inner, SA_REC_TOL2), cgTolSqu * gg / rr);
-                chunkHint = 16;
+                // first chunk: about the previous step's iterations in this sweep
+                chunkHint = sw < 4 && sweepIters[sw] > 0 && !specRecord ? std::max(4, sweepIters[sw] - 2) : 16;
                 // an inner solve may not run past 3x the previous step's
                 // iterations (+200; 1/8 of the budget on a run's first step):
                 // an FP32 operator that does not converge counts as a stall
@@ -1672,6 +1674,7 @@ struct Engine : EngineBase {
                 int st = ctl_solve(KMode::Stored32, q, tauNow, geoNow, cRout.p, cDelta.p, true, inner,
                                    std::max(1, std::min(itrMax - iters, capIn)), its);
                 iters += its;
+                if (sw < 4) sweepIters[sw] = its;
                 if (st != 0) { stalled = true; break; }   // breakdown or no convergence of the FP32 solve
                 axpby_k<<<gC, NT, 0, s>>>(nc, cA.p, 1.0, cA.p, 1.0, cDelta.p);
                 CHECK_LAUNCH();
@@ -1809,6 +1812,7 @@ struct Engine : EngineBase {
             CK(hipMemsetAsync(cAm3.p, 0, sizeof(double) * NC * D, s));
         }
         lastIters = 0;
+        for (int &w : sweepIters) w = 0;
         precAge = -1;
         fp64Steps = 0;
         mixedOff = false;
