@@ -1597,12 +1597,14 @@ struct Engine : EngineBase {
         // a scheduled refresh inside a run keeps the spectral term (at most
         // SA_SPEC_CARRY times in a row); a run's first step, a growth-driven
         // refresh and every other case harvest a new one
-        specCarry = refresh && precAge >= PREC_MAX_AGE && specOn && specCarried < SA_SPEC_CARRY;
-        kcc_build(refresh);
-        if (refresh) { precAge = 0; forceRefresh = false; }
         // this step in the mixed form (FP32 stored K inside FP64 refinement)
         // or in the FP64 form (stored FP64 K when it fits)
         const bool mixedStep = precision == SA_PREC_MIXED && A32.p && !mixedOff;
+        const KMode cgMode = mixedStep ? KMode::Stored32 : A64.p ? KMode::Stored64 : KMode::MatrixFree64;
+        specCarry = refresh && precAge >= PREC_MAX_AGE && specOn && specMode == (int)cgMode &&
+                    specCarried < SA_SPEC_CARRY;
+        kcc_build(refresh);
+        if (refresh) { precAge = 0; forceRefresh = false; }
         if (!mixedStep && precision == SA_PREC_MIXED) ensure_fp64_store();
         kstore_build(q, !mixedStep, mixedStep);
         CK(hipMemsetAsync(ctl.p, 0, sizeof(Ctl), s));
