@@ -335,6 +335,7 @@ struct Engine : EngineBase {
     int sweepIters[4] = {0, 0, 0, 0};   // inner iterations of the last step's refinement sweeps
     double ttSum = 0.0;
     bool ttReady = false;
+    bool vfdMfma = false;        // varifold value pair contractions on MFMA
 
     int timeLevels() const override { return T; }
 
@@ -553,6 +554,10 @@ struct Engine : EngineBase {
             rowVal.alloc((size_t)F + Ft, tot);
             gC.alloc((size_t)F * D, tot); gN.alloc((size_t)F * D, tot);
             grad.alloc((size_t)V * D, tot);
+            // the value on FP64 matrix cores (vfd_rows_mfma_k) in the mixed
+            // form; SA_VFD_MFMA=0 / 1 forces it off / on in either form
+            const char *e = std::getenv("SA_VFD_MFMA");
+            vfdMfma = e && (e[0] == '0' || e[0] == '1') ? e[0] == '1' : precision == SA_PREC_MIXED;
         }
         size_t n = (size_t)V * D;
         lmk.alloc(n * T, tot);
@@ -1761,6 +1766,9 @@ struct Engine : EngineBase {
         if (gradOut)
             vfd_rows_k<D, true><<<nbD + nbT, NT, 0, s>>>(F, Ft, nbD, dCen.p, dDir.p, dVol.p, tCen.p, tDir.p, tVol.p, vp,
                                                         rowVal.p, gC.p, gN.p);
+        else if (vfdMfma)
+            vfd_rows_mfma_k<D><<<nbD + nbT, NT, 0, s>>>(F, Ft, nbD, dCen.p, dDir.p, dVol.p, tCen.p, tDir.p, tVol.p, vp,
+                                                       rowVal.p);
         else
             vfd_rows_k<D, false><<<nbD + nbT, NT, 0, s>>>(F, Ft, nbD, dCen.p, dDir.p, dVol.p, tCen.p, tDir.p, tVol.p, vp,
                                                          rowVal.p, gC.p, gN.p);
@@ -1868,7 +1876,9 @@ struct Engine : EngineBase {
                 require(hasTarget, "no target in this session");
                 double *qT = lmk.p + (size_t)(T - 1) * n;
                 CK(hipMemcpyAsync(qT, in, sizeof(double) * n, hipMemcpyHostToDevice, s));
-                out[0] = objective(qT, out + 1);
+                // the value as a run computes it (no gradient), then the gradient
+                out[0] = objective(qT, nullptr);
+                (void)objective(qT, out + 1);
                 break;
             }
             default:

@@ -1679,6 +1679,125 @@ __global__ __launch_bounds__(NT) void vfd_rows_k(int F, int Ft, int nbD, const d
     }
 }
 
+// Row sums of the varifold pair kernel (value only; the same rows, columns
+// and signs as vfd_rows_k<D, false>) with both pair contractions on FP64
+// matrix cores.  Each wavefront takes a strip of 16 rows; per tile of 16
+// columns, three v_mfma_f64_16x16x4f64 (K = 4: D coordinates, padded) give
+//   d2[i][j]  = |c_i - o|^2 + |c_j - o|^2 - 2 (c_i - o).(c_j - o)
+//             (K slots [c_i - o, 1] x [-2 (c_j - o), |c_j - o|^2], then
+//             [|c_i - o|^2, 0..] x [1, 0..] into the same accumulator)
+//   ang[i][j] = u_i . u_j
+// and the VALU evaluates only the centre and direction kernels.  The origin o
+// is the strip's first row centre: for the pairs the kernel weighs (|c_i -
+// c_j| a few widths) the expanded form then cancels no more than the direct
+// one (|c_j - o| <= |c_j - c_i| + the strip's extent).  MFMA operand layout
+// (16x16x4 FP64): lane l holds A[l % 16][l / 16] and B[l / 16][l % 16]; the
+// result D[4 r + l / 16][l % 16] is element r of the lane's 4-vector (rows
+// interleaved by register: the FP64 shape has one-row groups).
+typedef double vfd_d4 __attribute__((ext_vector_type(4)));
+
+template <int D>
+__global__ __launch_bounds__(NT) void vfd_rows_mfma_k(int F, int Ft, int nbD, const double *__restrict__ dc_,
+                                                      const double *__restrict__ du_, const double *__restrict__ dw_,
+                                                      const double *__restrict__ tc_, const double *__restrict__ tu_,
+                                                      const double *__restrict__ tw_, VfdPrm vp,
+                                                      double *__restrict__ rowVal) {
+    static_assert(D == 2 || D == 3, "K = 4 holds D coordinates and the constant 1");
+    static_assert(NT == 4 * 64 && MV_ROWS == 64, "four wavefronts of 16-row strips per block");
+    __shared__ double sc[D][NT], su[D][NT], sw[NT];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const bool drow = blockIdx.x < nbD;
+    const int nrow = drow ? F : Ft;
+    const int i0 = (drow ? blockIdx.x : blockIdx.x - nbD) * MV_ROWS + wv * 16;
+    const double *rc = drow ? dc_ : tc_, *ru = drow ? du_ : tu_, *rw = drow ? dw_ : tw_;
+    // origin: the strip's first row centre (the last row of a strip past the end)
+    const int io = min(i0, nrow - 1);
+    double o[D];
+#pragma unroll
+    for (int c = 0; c < D; ++c) o[c] = rc[c * nrow + io];
+    // A operands: row i0 + li, K slot lk
+    const int ia = i0 + li;
+    double aC = 0.0, aU = 0.0, aN = 0.0;
+    if (ia < nrow) {
+        if (lk < D) {
+            aC = rc[lk * nrow + ia] - o[lk];
+            aU = ru[lk * nrow + ia];
+        } else if (lk == D) {
+            aC = 1.0;
+        }
+        if (lk == 0) {
+#pragma unroll
+            for (int c = 0; c < D; ++c) {
+                const double t = rc[c * nrow + ia] - o[c];
+                aN += t * t;
+            }
+        }
+    }
+    const double bN = lk == 0 ? 1.0 : 0.0;
+    double val[4] = {0.0, 0.0, 0.0, 0.0};
+    const int npass = drow ? 2 : 1;
+    for (int pass = 0; pass < npass; ++pass) {
+        const bool tgt = (!drow) || pass == 1;
+        const double *pc = tgt ? tc_ : dc_, *pu = tgt ? tu_ : du_, *pw = tgt ? tw_ : dw_;
+        const int n = tgt ? Ft : F;
+        const double sg = (drow && pass == 1) ? -2.0 : 1.0;
+        double pv[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int j0 = 0; j0 < n; j0 += NT) {
+            const int j = j0 + threadIdx.x;
+            __syncthreads();
+#pragma unroll
+            for (int c = 0; c < D; ++c) {
+                sc[c][threadIdx.x] = j < n ? pc[c * n + j] : 0.0;
+                su[c][threadIdx.x] = j < n ? pu[c * n + j] : 0.0;
+            }
+            sw[threadIdx.x] = j < n ? pw[j] : 0.0;   // columns past the end weigh 0
+            __syncthreads();
+            const int ntile = (min(NT, n - j0) + 15) >> 4;
+            for (int t = 0; t < ntile; ++t) {
+                const int jj = t * 16 + li;   // this lane's column, as B operand and in the result
+                double bC = 0.0, bU = 0.0;
+                if (lk < D) {
+                    bC = -2.0 * (sc[lk][jj] - o[lk]);
+                    bU = su[lk][jj];
+                } else if (lk == D) {
+#pragma unroll
+                    for (int c = 0; c < D; ++c) {
+                        const double tj = sc[c][jj] - o[c];
+                        bC += tj * tj;
+                    }
+                }
+                const vfd_d4 z = {0.0, 0.0, 0.0, 0.0};
+                vfd_d4 d2 = __builtin_amdgcn_mfma_f64_16x16x4f64(aC, bC, z, 0, 0, 0);
+                d2 = __builtin_amdgcn_mfma_f64_16x16x4f64(aN, bN, d2, 0, 0, 0);
+                const vfd_d4 an = __builtin_amdgcn_mfma_f64_16x16x4f64(aU, bU, z, 0, 0, 0);
+                const double wj = sw[jj];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const double kc = cen_kernel(vp.cenType, fmax(d2[r], 0.0), vp.cenInvW2);
+                    const double ku = dir_kernel(vp.dirType, an[r], vp.dirInvW2);
+                    pv[r] += kc * ku * wj;
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) val[r] += sg * pv[r];
+    }
+    // row 4 r + lk: sum over the 16 lanes of the group (columns l % 16), fixed order
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+#pragma unroll
+        for (int m = 8; m > 0; m >>= 1) val[r] += __shfl_xor(val[r], m, 64);
+    }
+    if (li == 0) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int i = i0 + 4 * r + lk;
+            if (i < nrow) rowVal[(drow ? 0 : F) + i] = val[r] * rw[i];
+        }
+    }
+}
+
 // facet -> vertex gradient (dqVfdGatherKernel, varifold.cu:472-498|493-530)
 template <int D>
 __global__ void vfd_grad_k(int V, int F, const int *__restrict__ adjTbl, const int *__restrict__ vfdVtx,

@@ -124,6 +124,35 @@ def test_probe_varifold_kernel_types(ck, dk, oracle):
         assert rel(out[1:].reshape(q.shape), grad) < 1e-11, (ck, dk, p["dim"])
 
 
+@pytest.mark.parametrize("ck,dk", [("G", "B"), ("C", "L"), ("G", "O"), ("C", "U")])
+def test_varifold_value_on_mfma(ck, dk, oracle, monkeypatch):
+    """The varifold value with its pair contractions on FP64 matrix cores
+    (vfd_rows_mfma_k; SA_VFD_MFMA=1 turns it on in an FP64 session as the
+    mixed form does by default) against the oracle, on facet counts that
+    leave partial 16-row strips, partial 16-column tiles and several
+    256-column chunks.  Tolerance 1e-11 relative: the expanded distance
+    |c_i - o|^2 + |c_j - o|^2 - 2 (c_i - o).(c_j - o) (o the strip's first
+    centre) loses the bits of |c - o|^2 / |c_i - c_j|^2; an FP64 emulation of
+    it on these shapes stays within 1.5e-13 of the oracle."""
+    monkeypatch.setenv("SA_VFD_MFMA", "1")
+    shapes = (small_ring(), small_shell(), W.ring_problem(n_rings=4, n_per_ring=300),
+              W.shell_problem(level=3, n_layers=2, time_steps=2))
+    for p in shapes:
+        p["vfd"] = dict(p["vfd"], cenKnlType=ck, dirKnlType=dk,
+                        dirKnlWidth=p["vfd"]["dirKnlWidth"] if dk == "B" else 0.7)
+        p["name"] = f"varifold{p['dim']}d_mfma_{ck}{dk}"
+        s = _session(p)
+        rng = np.random.default_rng(9)
+        q = p["q0"] + 0.01 * rng.standard_normal(p["q0"].shape)
+        out = s.probe("varifold", p["tauVar"][:, 0], p["tauPrm"], q)
+        s.close()
+        val, grad = oracle.varifold(p, q, with_grad=True)
+        print(f"[gpu] {p['name']}: value rel {abs(out[0] - val) / abs(val):.2e}", flush=True)
+        assert abs(out[0] - val) <= 1e-11 * abs(val), (ck, dk, p["dim"])
+        # the gradient stays on the VALU kernel
+        assert rel(out[1:].reshape(q.shape), grad) < 1e-11, (ck, dk, p["dim"])
+
+
 def test_full_solve_matches_oracle(prob, oracle):
     s = _session(prob)
     r = s.run(prob["tauVar"][:, 0], prob["tauPrm"], want_traj=True)
