@@ -417,6 +417,25 @@ def main():
               "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(B / (avg * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
               "avg_ms": round(avg, 4), "algorithmic_bytes": B}
 
+    # the objective's varifold row sums (one launch per solve).  In the mixed
+    # form the pair contractions run on FP64 MFMA: three
+    # v_mfma_f64_16x16x4f64 of 2,048 flop per 16 x 16 tile of pairs, 24 MFMA
+    # flop per pair; the centre and direction kernels (an exponential per
+    # pair) stay on the FP64 VALU, which bounds the launch
+    vfd = None
+    vfd_ms = sum(o["varifold_ms"] for o in runs)
+    vfd_pairs = sum(o["varifold_pairs"] for o in runs)
+    if vfd_ms > 0 and vfd_pairs:
+        on_mfma = all(o["varifold_mfma"] for o in runs)
+        t = vfd_ms * 1e-3
+        vfd = {"kernel": f"vfd_rows_mfma_k<{d}>" if on_mfma else f"vfd_rows_k<{d}, false>", "bound": "fp64 VALU",
+               "pairs": vfd_pairs, "ms": round(vfd_ms, 4), "gpairs_per_s": round(vfd_pairs / t / 1e9, 2)}
+        if on_mfma:
+            ach = 24 * vfd_pairs / t / 1e12
+            vfd.update({"mfma_achieved": round(ach, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                        "mfma_frac": round(ach / PEAK_FP64_TFLOPS, 4),
+                        "work": "24 MFMA flop per pair (3 x 16x16x4 FP64 per 256 pairs)"})
+
     cpu = None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -472,6 +491,7 @@ def main():
         "objective": runs[-1]["obj"],
         "roofline": roof,
         "operator_roofline": op,
+        "varifold": vfd,
         "cpu_baseline": cpu,
         "fp64_form": ref64,
     }

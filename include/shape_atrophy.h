@@ -133,6 +133,9 @@ typedef struct sa_stats {
     int operatorCount;
     int fp64Steps;               /* mixed precision: steps whose refinement stalled and were finished in FP64 */
     int failReason;              /* status 1: SA_FAIL_* */
+    double varifoldMs;           /* with profiling on: hipEvent time of the objective's varifold row-sum launch */
+    long long varifoldPairs;     /*   ... the facet pairs it summed (F^2 + F Ft, + Ft^2 on a session's first) */
+    int varifoldMfma;            /*   ... 1 when the pair contractions ran on the matrix cores */
 } sa_stats;
 
 /* sa_stats.failReason */
@@ -190,7 +193,8 @@ int sa_session_set_params(sa_session *s, const sa_params *prm);
      SA_PROBE_YANK      out = [f(q0, tau0) (lmkNum x dim), tau0 (lmkNum)]
      SA_PROBE_FEM       out = [M(q0) (nzrNum), M + dt K_D (nzrNum)] in femVtxMat order
      SA_PROBE_GEOMETRY  out = per-element [Q^-1 (dim^2), det Q, frame 1 (dim), frame 2 (dim)] x elmNum
-     SA_PROBE_VARIFOLD  in = landmarks (lmkNum x dim); out = [objective, gradient (lmkNum x dim)] */
+     SA_PROBE_VARIFOLD  in = landmarks (lmkNum x dim); out = [objective, gradient (lmkNum x dim)]
+                        (the objective by the kernel a run uses, the gradient by the FP64 VALU kernel) */
 #define SA_PROBE_ELASTIC 1
 #define SA_PROBE_KMATVEC 2
 #define SA_PROBE_YANK 3

@@ -65,7 +65,8 @@ class SaStats(C.Structure):
     _fields_ = [("status", C.c_int), ("steps", C.c_int), ("pcgIters", C.c_int), ("rdIters", C.c_int),
                 ("refineSweeps", C.c_int), ("seconds", C.c_double), ("kmatvecMs", C.c_double),
                 ("kmatvecCount", C.c_int), ("operatorMs", C.c_double), ("operatorCount", C.c_int),
-                ("fp64Steps", C.c_int), ("failReason", C.c_int)]
+                ("fp64Steps", C.c_int), ("failReason", C.c_int), ("varifoldMs", C.c_double),
+                ("varifoldPairs", C.c_longlong), ("varifoldMfma", C.c_int)]
 
 
 class SaError(RuntimeError):
@@ -337,7 +338,8 @@ class Session:
                     pcg_iters=st.pcgIters, rd_iters=st.rdIters, refine=st.refineSweeps, fp64_steps=st.fp64Steps,
                     fail_reason=st.failReason, seconds=st.seconds,
                     kmatvec_ms=st.kmatvecMs, kmatvec_count=st.kmatvecCount, operator_ms=st.operatorMs,
-                    operator_count=st.operatorCount)
+                    operator_count=st.operatorCount, varifold_ms=st.varifoldMs, varifold_pairs=st.varifoldPairs,
+                    varifold_mfma=st.varifoldMfma)
 
     def set_params(self, params: dict):
         """New scalar parameters for the following runs (sa_session_set_params)."""

@@ -194,6 +194,9 @@ struct Stats {
     int status = 0, steps = 0, pcgIters = 0, rdIters = 0, refine = 0, fp64Steps = 0, failReason = 0;
     double kmvMs = 0.0, opMs = 0.0;   // summed durations of the timed kernels (profiling on)
     int kmvCount = 0, opCount = 0;
+    double vfdMs = 0.0;               // the varifold row-sum launch of the objective (profiling on)
+    long long vfdPairs = 0;           //   ... the facet pairs it summed
+    int vfdMfma = 0;                  //   ... 1 when it ran on the matrix cores (vfd_rows_mfma_k)
 };
 
 // hipEvent pairs around the kernel-matrix product and the elastic operator
@@ -208,7 +211,16 @@ struct KernelTimer {
             ev.push_back(e);
         }
     }
-    ~KernelTimer() { for (auto e : ev) (void)hipEventDestroy(e); }
+    hipEvent_t vfd[2] = {nullptr, nullptr};   // around the varifold row sums
+    void ensure_vfd() {
+        for (auto &e : vfd)
+            if (!e && hipEventCreate(&e) != hipSuccess) throw SaError(SA_E_DEVICE, "hipEventCreate failed");
+    }
+    ~KernelTimer() {
+        for (auto e : ev) (void)hipEventDestroy(e);
+        for (auto e : vfd)
+            if (e) (void)hipEventDestroy(e);
+    }
 };
 
 struct EngineBase {
@@ -1759,20 +1771,28 @@ struct Engine : EngineBase {
         return 0;
     }
 
-    double objective(const double *qT, double *gradOut) {
+    double objective(const double *qT, double *gradOut, Stats *prof = nullptr) {
         facets_k<D><<<gridOf(F), NT, 0, s>>>(F, V, vfdVtx.p, qT, dCen.p, dDir.p, dVol.p);
         CHECK_LAUNCH();
         int nbD = gridOf(F, MV_ROWS), nbT = ttReady ? 0 : gridOf(Ft, MV_ROWS);
+        const bool timed = prof && timer.on;
+        if (timed) {
+            timer.ensure_vfd();
+            CK(hipEventRecord(timer.vfd[0], s));
+        }
         if (gradOut)
             vfd_rows_k<D, true><<<nbD + nbT, NT, 0, s>>>(F, Ft, nbD, dCen.p, dDir.p, dVol.p, tCen.p, tDir.p, tVol.p, vp,
                                                         rowVal.p, gC.p, gN.p);
-        else if (vfdMfma)
-            vfd_rows_mfma_k<D><<<nbD + nbT, NT, 0, s>>>(F, Ft, nbD, dCen.p, dDir.p, dVol.p, tCen.p, tDir.p, tVol.p, vp,
-                                                       rowVal.p);
+        else if (vfdMfma) {
+            const int mD = gridOf(F, VFD_MFMA_ROWS), mT = ttReady ? 0 : gridOf(Ft, VFD_MFMA_ROWS);
+            vfd_rows_mfma_k<D><<<mD + mT, NT, 0, s>>>(F, Ft, mD, dCen.p, dDir.p, dVol.p, tCen.p, tDir.p, tVol.p, vp,
+                                                     rowVal.p);
+        }
         else
             vfd_rows_k<D, false><<<nbD + nbT, NT, 0, s>>>(F, Ft, nbD, dCen.p, dDir.p, dVol.p, tCen.p, tDir.p, tVol.p, vp,
                                                          rowVal.p, gC.p, gN.p);
         CHECK_LAUNCH();
+        if (timed) CK(hipEventRecord(timer.vfd[1], s));
         sum_k<<<1, NT, 0, s>>>(rowVal.p, F, scal.p);
         CHECK_LAUNCH();
         if (!ttReady) {
@@ -1787,6 +1807,13 @@ struct Engine : EngineBase {
             CK(hipMemcpyAsync(gradOut, grad.p, sizeof(double) * V * D, hipMemcpyDeviceToHost, s));
         }
         CK(hipStreamSynchronize(s));
+        if (timed) {
+            float ms = 0.f;
+            CK(hipEventElapsedTime(&ms, timer.vfd[0], timer.vfd[1]));
+            prof->vfdMs += ms;
+            prof->vfdPairs += (long long)F * (F + Ft) + (ttReady ? 0LL : (long long)Ft * Ft);
+            prof->vfdMfma = vfdMfma && !gradOut;
+        }
         if (!ttReady) { ttSum = hs[1]; ttReady = true; }
         return hs[0] + ttSum;
     }
@@ -1952,7 +1979,7 @@ struct Engine : EngineBase {
         st.failReason = failed ? failReason : 0;
         if (obj) {
             if (failed) *obj = DBL_MAX;
-            else if (hasTarget) *obj = objective(lmk.p + (size_t)(TT - 1) * n, gradOut);
+            else if (hasTarget) *obj = objective(lmk.p + (size_t)(TT - 1) * n, gradOut, &st);
             else *obj = 0.0;
         }
         if (lmkOut) CK(hipMemcpyAsync(lmkOut, lmk.p, sizeof(double) * n * TT, hipMemcpyDeviceToHost, s));
@@ -2042,6 +2069,9 @@ int sa_session_run(sa_session *ss, const double *tauVarVec, int tauVarNum, const
             stats->kmatvecCount = st.kmvCount;
             stats->operatorMs = st.opMs;
             stats->operatorCount = st.opCount;
+            stats->varifoldMs = st.vfdMs;
+            stats->varifoldPairs = st.vfdPairs;
+            stats->varifoldMfma = st.vfdMfma;
         }
     });
 }

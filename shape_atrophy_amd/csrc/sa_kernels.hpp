@@ -1681,20 +1681,23 @@ __global__ __launch_bounds__(NT) void vfd_rows_k(int F, int Ft, int nbD, const d
 
 // Row sums of the varifold pair kernel (value only; the same rows, columns
 // and signs as vfd_rows_k<D, false>) with both pair contractions on FP64
-// matrix cores.  Each wavefront takes a strip of 16 rows; per tile of 16
-// columns, three v_mfma_f64_16x16x4f64 (K = 4: D coordinates, padded) give
+// matrix cores.  Per strip of 16 rows and tile of 16 columns, three v_mfma_f64_16x16x4f64 (K = 4: D coordinates, padded) give
 //   d2[i][j]  = |c_i - o|^2 + |c_j - o|^2 - 2 (c_i - o).(c_j - o)
 //             (K slots [c_i - o, 1] x [-2 (c_j - o), |c_j - o|^2], then
 //             [|c_i - o|^2, 0..] x [1, 0..] into the same accumulator)
 //   ang[i][j] = u_i . u_j
-// and the VALU evaluates only the centre and direction kernels.  The origin o
-// is the strip's first row centre: for the pairs the kernel weighs (|c_i -
+// and the VALU evaluates only the centre and direction kernels.  A block
+// takes one strip of VFD_MFMA_ROWS = 16 rows (F / 16 blocks fill the chip
+// where F / 64 did not); its four wavefronts take every fourth column tile
+// and their row sums are added in wavefront order.  The origin o is the
+// strip's first row centre: for the pairs the kernel weighs (|c_i -
 // c_j| a few widths) the expanded form then cancels no more than the direct
 // one (|c_j - o| <= |c_j - c_i| + the strip's extent).  MFMA operand layout
 // (16x16x4 FP64): lane l holds A[l % 16][l / 16] and B[l / 16][l % 16]; the
 // result D[4 r + l / 16][l % 16] is element r of the lane's 4-vector (rows
 // interleaved by register: the FP64 shape has one-row groups).
 typedef double vfd_d4 __attribute__((ext_vector_type(4)));
+constexpr int VFD_MFMA_ROWS = 16;
 
 template <int D>
 __global__ __launch_bounds__(NT) void vfd_rows_mfma_k(int F, int Ft, int nbD, const double *__restrict__ dc_,
@@ -1703,13 +1706,14 @@ __global__ __launch_bounds__(NT) void vfd_rows_mfma_k(int F, int Ft, int nbD, co
                                                       const double *__restrict__ tw_, VfdPrm vp,
                                                       double *__restrict__ rowVal) {
     static_assert(D == 2 || D == 3, "K = 4 holds D coordinates and the constant 1");
-    static_assert(NT == 4 * 64 && MV_ROWS == 64, "four wavefronts of 16-row strips per block");
+    static_assert(NT == 4 * 64, "four wavefronts per block");
     __shared__ double sc[D][NT], su[D][NT], sw[NT];
+    __shared__ double wsum[4][VFD_MFMA_ROWS];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int li = lane & 15, lk = lane >> 4;
     const bool drow = blockIdx.x < nbD;
     const int nrow = drow ? F : Ft;
-    const int i0 = (drow ? blockIdx.x : blockIdx.x - nbD) * MV_ROWS + wv * 16;
+    const int i0 = (drow ? blockIdx.x : blockIdx.x - nbD) * VFD_MFMA_ROWS;
     const double *rc = drow ? dc_ : tc_, *ru = drow ? du_ : tu_, *rw = drow ? dw_ : tw_;
     // origin: the strip's first row centre (the last row of a strip past the end)
     const int io = min(i0, nrow - 1);
@@ -1754,7 +1758,7 @@ __global__ __launch_bounds__(NT) void vfd_rows_mfma_k(int F, int Ft, int nbD, co
             sw[threadIdx.x] = j < n ? pw[j] : 0.0;   // columns past the end weigh 0
             __syncthreads();
             const int ntile = (min(NT, n - j0) + 15) >> 4;
-            for (int t = 0; t < ntile; ++t) {
+            for (int t = wv; t < ntile; t += 4) {
                 const int jj = t * 16 + li;   // this lane's column, as B operand and in the result
                 double bC = 0.0, bU = 0.0;
                 if (lk < D) {
@@ -1791,10 +1795,14 @@ __global__ __launch_bounds__(NT) void vfd_rows_mfma_k(int F, int Ft, int nbD, co
     }
     if (li == 0) {
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int i = i0 + 4 * r + lk;
-            if (i < nrow) rowVal[(drow ? 0 : F) + i] = val[r] * rw[i];
-        }
+        for (int r = 0; r < 4; ++r) wsum[wv][4 * r + lk] = val[r];
+    }
+    __syncthreads();
+    if (threadIdx.x < VFD_MFMA_ROWS) {
+        const int i = i0 + threadIdx.x;
+        if (i < nrow)
+            rowVal[(drow ? 0 : F) + i] =
+                (((wsum[0][threadIdx.x] + wsum[1][threadIdx.x]) + wsum[2][threadIdx.x]) + wsum[3][threadIdx.x]) * rw[i];
     }
 }
 
