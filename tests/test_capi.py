@@ -76,3 +76,27 @@ def test_server_failure_path_without_device():
     if g["rc"] == -1:
         assert "range" in g["error"]
     srv.close()
+
+
+@pytest.mark.parametrize("cname,pyname", [("sa_mesh", "SaMesh"), ("sa_params", "SaParams"),
+                                          ("sa_target", "SaTarget"), ("sa_stats", "SaStats")])
+def test_struct_layout_matches_header(cname, pyname, tmp_path):
+    """The ctypes mirrors in api.py have the size and field offsets that the C
+    compiler gives the header's structs (a field added on one side only would
+    shift every later field)."""
+    py = getattr(api, pyname)
+    fields = [f[0] for f in py._fields_]
+    src = tmp_path / "layout.c"
+    lines = ["#include <stddef.h>", "#include <stdio.h>", '#include "shape_atrophy.h"', "int main(void) {",
+             f'    printf("size %zu\\n", sizeof({cname}));']
+    lines += [f'    printf("{f} %zu\\n", offsetof({cname}, {f}));' for f in fields]
+    lines += ["    return 0;", "}"]
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    import subprocess
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = dict(l.split() for l in subprocess.run([str(exe)], check=True, capture_output=True,
+                                                 text=True).stdout.splitlines())
+    assert int(out["size"]) == C.sizeof(py), (cname, out["size"], C.sizeof(py))
+    for f in fields:
+        assert int(out[f]) == getattr(py, f).offset, (cname, f)
