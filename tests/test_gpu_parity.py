@@ -132,8 +132,9 @@ def test_varifold_value_on_mfma(ck, dk, oracle, monkeypatch):
     leave partial 16-row strips, partial 16-column tiles and several
     256-column chunks.  Tolerance 1e-11 relative: the expanded distance
     |c_i - o|^2 + |c_j - o|^2 - 2 (c_i - o).(c_j - o) (o the strip's first
-    centre) loses the bits of |c - o|^2 / |c_i - c_j|^2; an FP64 emulation of
-    it on these shapes stays within 1.5e-13 of the oracle."""
+    centre) loses the bits of |c - o|^2 / |c_i - c_j|^2; its FP64 restatement
+    (test_oracle.py::test_expanded_distance_varifold_accuracy) stays within
+    1.5e-13 of the oracle on these shapes."""
     monkeypatch.setenv("SA_VFD_MFMA", "1")
     shapes = (small_ring(), small_shell(), W.ring_problem(n_rings=4, n_per_ring=300),
               W.shell_problem(level=3, n_layers=2, time_steps=2))

@@ -389,6 +389,46 @@ def test_varifold_value_and_gradient(oracle, maker, ck, dk):
             assert grad[k, i] == pytest.approx(fd, rel=1e-5, abs=1e-7 * np.abs(grad).max())
 
 
+def strip_expanded_varifold(c1, u1, w1, c2, u2, w2, vfd):
+    """The pair sum as the MFMA varifold kernel forms it: rows in strips of
+    16, each strip's squared distances expanded about its first centre o as
+    |c_i - o|^2 + (|c_j - o|^2 - 2 (c_i - o).(c_j - o)), clamped at 0."""
+    tot = 0.0
+    for s0 in range(0, c1.shape[1], 16):
+        o = c1[:, s0:s0 + 1]
+        a, b = c1[:, s0:s0 + 16] - o, c2 - o
+        d2 = np.maximum((a * a).sum(0)[:, None] + ((b * b).sum(0)[None, :] - 2 * a.T @ b), 0.0)
+        ang = u1[:, s0:s0 + 16].T @ u2
+        kc = np.exp(-d2 / vfd["cenKnlWidth"] ** 2) if vfd["cenKnlType"] == "G" else 1 / (1 + d2 / vfd["cenKnlWidth"] ** 2)
+        s2 = vfd["dirKnlWidth"] ** 2
+        ku = {"B": ang ** 2, "L": ang, "O": np.exp(2 * (ang - 1) / s2),
+              "U": np.exp(2 * (ang ** 2 - 1) / s2)}[vfd["dirKnlType"]]
+        tot += (kc * ku * w2[None, :]).sum(1) @ w1[s0:s0 + 16]
+    return tot
+
+
+@pytest.mark.parametrize("maker", [small_ring, small_shell, lambda: W.ring_problem(n_rings=4, n_per_ring=300),
+                                   lambda: W.shell_problem(level=3, n_layers=2, time_steps=2)])
+@pytest.mark.parametrize("ck,dk", [("G", "B"), ("C", "L"), ("G", "O"), ("C", "U")])
+def test_expanded_distance_varifold_accuracy(oracle, maker, ck, dk):
+    """The arithmetic of the MFMA varifold kernel (expanded squared distances
+    about each 16-row strip's first centre) in FP64 stays within 1e-12 of the
+    oracle's direct differences on the shapes its GPU test uses (measured:
+    at most 1.5e-13 on the coarse ring and shell, ~1e-15 on the finer ones),
+    which leaves the GPU test's 1e-11 tolerance a wide margin."""
+    p = maker()
+    p["vfd"] = dict(p["vfd"], cenKnlType=ck, dirKnlType=dk,
+                    dirKnlWidth=p["vfd"]["dirKnlWidth"] if dk == "B" else 0.7)
+    q = p["q0"] + 0.01 * np.random.default_rng(9).standard_normal(p["q0"].shape)
+    val = oracle.varifold(p, q)
+    c, u, w = oracle.facets(p, q)
+    t = p["target"]
+    got = (strip_expanded_varifold(c, u, w, c, u, w, p["vfd"])
+           - 2 * strip_expanded_varifold(c, u, w, t["cen"], t["dir"], t["vol"], p["vfd"])
+           + strip_expanded_varifold(t["cen"], t["dir"], t["vol"], t["cen"], t["dir"], t["vol"], p["vfd"]))
+    assert abs(got - val) <= 1e-12 * abs(val)
+
+
 def test_varifold_self_distance_zero(oracle):
     p = small_shell()
     c, u, w = oracle.facets(p, p["q0"])
