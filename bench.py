@@ -452,6 +452,13 @@ def main():
                           f"{smp['setup_s']:.2f}s, CG start {smp['cg_start_s']:.2f}s, {smp['iters']} CG "
                           f"iterations at {smp['per_iter_s']:.2f}s each, scaled to the GPU-measured "
                           f"{its_per_step:.1f} CG iterations per step (reaction-diffusion not included)")}
+        # the operator alone: median of 20 applies on one core, then of 100 on
+        # all the cores above (the second call leaves that thread count set)
+        one = O.time_elastic_apply(prob, reps=20, threads=1)
+        allc = O.time_elastic_apply(prob, reps=100, threads=threads)
+        cpu["operator_apply"] = {"median_ms": round(1e3 * allc, 3), "cores": threads, "reps": 100,
+                                 "single_core_median_ms": round(1e3 * one, 3), "single_core_reps": 20,
+                                 "gpu_avg_ms": op["avg_ms"] if op else None}
 
     # accuracy of the timed form against the plain FP64 GPU form on the same
     # solve (the FP64 form itself is held to the FP64 oracle by the parity tests)

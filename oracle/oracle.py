@@ -62,6 +62,7 @@ def lib():
         _lib.or_objfcn.restype = _I
         _lib.or_varifold.restype = _D
         _lib.or_num_threads.restype = _I
+        _lib.or_time_elastic_apply.restype = _D
     return _lib
 
 
@@ -259,6 +260,21 @@ def elastic_apply(prob: dict, q, tau, v) -> np.ndarray:
     out = np.empty_like(v)
     lib().or_elastic_apply(C.byref(m), C.byref(p), _dp(q), _dp(tau), _dp(v), _dp(out))
     return out
+
+
+def time_elastic_apply(prob: dict, reps: int = 100, threads: int | None = None) -> float:
+    """Median wall time (s) of reps elastic-operator applies at q0 and the
+    initial concentration, on the given OpenMP thread count (the CPU-baseline
+    sample of the operator alone)."""
+    L = lib()
+    if threads:
+        L.or_set_threads(int(threads))
+    keep, m = _mesh_call(prob)
+    p = make_params(prob, keep)
+    q = np.ascontiguousarray(prob["q0"], dtype=np.float64)
+    tau = tau_initial(prob)
+    v = np.ascontiguousarray(np.random.default_rng(0).standard_normal(q.shape))
+    return float(L.or_time_elastic_apply(C.byref(m), C.byref(p), _dp(q), _dp(tau), _dp(v), int(reps)))
 
 
 def penalty_apply(prob: dict, q, v) -> np.ndarray:

@@ -1276,6 +1276,24 @@ void or_elastic_apply(const OMesh *m, const OParams *p, const double *q, const d
     else { compute_frames<3>(*m, q, ws.f1, ws.f2); elastic_apply<3>(*m, *p, q, tau, ws, frc, v, out); }
 }
 
+// CPU-baseline sample of the operator alone: median wall time (s) of reps
+// elastic applies E(q) v (frames formed once, as a solve forms them once per
+// step) on the current OpenMP thread count
+double or_time_elastic_apply(const OMesh *m, const OParams *p, const double *q, const double *tau, const double *v,
+                             int reps) {
+    std::vector<double> frc, out((size_t)m->dim * m->lmkNum), t((size_t)std::max(reps, 1));
+    Workspace ws;
+    if (m->dim == 2) compute_frames<2>(*m, q, ws.f1, ws.f2); else compute_frames<3>(*m, q, ws.f1, ws.f2);
+    for (auto &tk : t) {
+        auto t0 = std::chrono::steady_clock::now();
+        if (m->dim == 2) elastic_apply<2>(*m, *p, q, tau, ws, frc, v, out.data());
+        else elastic_apply<3>(*m, *p, q, tau, ws, frc, v, out.data());
+        tk = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+    std::nth_element(t.begin(), t.begin() + t.size() / 2, t.end());
+    return t[t.size() / 2];
+}
+
 void or_penalty_apply(const OMesh *m, const double *q, const double *v, double *out) {
     if (m->dim == 2) penalty_apply<2>(*m, q, v, out); else penalty_apply<3>(*m, q, v, out);
 }

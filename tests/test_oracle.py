@@ -500,3 +500,17 @@ def test_control_point_first_step_dense(oracle):
     v = (KX @ alpha).reshape(d, V)
     v_or = (o["lmkStk"][1] - o["lmkStk"][0]) / prm["timeStp"]
     assert np.linalg.norm(v_or - v) / np.linalg.norm(v) < 1e-7
+
+
+def test_operator_apply_timing_sample(oracle):
+    """The CPU-baseline sample of the operator alone returns a positive
+    median and leaves the result of the apply unchanged (the timed applies
+    only read the mesh)."""
+    p = small_shell()
+    tau0 = oracle.tau_initial(p)
+    v = np.random.default_rng(0).standard_normal(p["q0"].shape)
+    before = oracle.elastic_apply(p, p["q0"], tau0, v)
+    t1 = oracle.time_elastic_apply(p, reps=5, threads=1)
+    t2 = oracle.time_elastic_apply(p, reps=5, threads=2)
+    assert 0.0 < t1 < 10.0 and 0.0 < t2 < 10.0
+    np.testing.assert_array_equal(oracle.elastic_apply(p, p["q0"], tau0, v), before)
