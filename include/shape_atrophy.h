@@ -136,6 +136,8 @@ typedef struct sa_stats {
     double varifoldMs;           /* with profiling on: hipEvent time of the objective's varifold row-sum launch */
     long long varifoldPairs;     /*   ... the facet pairs it summed (F^2 + F Ft, + Ft^2 on a session's first) */
     int varifoldMfma;            /*   ... 1 when the pair contractions ran on the matrix cores */
+    long long deviceBytes;       /* device memory the session holds (= sa_session_footprint of its mesh) */
+    int factorRetries;           /* control-point K_cc factorisations repeated after a reported failure */
 } sa_stats;
 
 /* sa_stats.failReason */
@@ -144,7 +146,8 @@ typedef struct sa_stats {
 #define SA_FAIL_CHOLESKY  3      /* control-point K_cc not positive definite (Cholesky) */
 #define SA_FAIL_TRTRI     4      /* inverse of the Cholesky factor failed */
 #define SA_FAIL_RD        5      /* reaction-diffusion CG did not converge */
-#define SA_FAIL_REFINE    6      /* mixed-precision refinement did not converge */
+#define SA_FAIL_REFINE    6      /* not produced: a mixed-precision step whose refinement stalls
+                                    is finished by the FP64 form, which reports its own reason */
 
 /* deformation_{2D,3D}_frame (deformation.cu:14-199).
    tauVarMat: tauVarNum x candNum (arg 0); tauPrmVec (arg 1); lmkIniMat (arg 2).
@@ -194,13 +197,15 @@ int sa_session_set_params(sa_session *s, const sa_params *prm);
      SA_PROBE_FEM       out = [M(q0) (nzrNum), M + dt K_D (nzrNum)] in femVtxMat order
      SA_PROBE_GEOMETRY  out = per-element [Q^-1 (dim^2), det Q, frame 1 (dim), frame 2 (dim)] x elmNum
      SA_PROBE_VARIFOLD  in = landmarks (lmkNum x dim); out = [objective, gradient (lmkNum x dim)]
-                        (the objective by the kernel a run uses, the gradient by the FP64 VALU kernel) */
+                        (the objective by the kernel a run uses, the gradient by the FP64 VALU kernel)
+     SA_PROBE_PENALTY   out = P(q0) in, the bottom penalty without btmWgt (lmkNum x dim) */
 #define SA_PROBE_ELASTIC 1
 #define SA_PROBE_KMATVEC 2
 #define SA_PROBE_YANK 3
 #define SA_PROBE_FEM 4
 #define SA_PROBE_GEOMETRY 5
 #define SA_PROBE_VARIFOLD 6
+#define SA_PROBE_PENALTY 7
 int sa_session_probe(sa_session *s, int what, const double *tauVarVec, int tauVarNum, const double *tauPrmVec,
                      int tauPrmNum, const double *in, double *out);
 /* bytes of device memory a session of this mesh holds */

@@ -66,7 +66,8 @@ class SaStats(C.Structure):
                 ("refineSweeps", C.c_int), ("seconds", C.c_double), ("kmatvecMs", C.c_double),
                 ("kmatvecCount", C.c_int), ("operatorMs", C.c_double), ("operatorCount", C.c_int),
                 ("fp64Steps", C.c_int), ("failReason", C.c_int), ("varifoldMs", C.c_double),
-                ("varifoldPairs", C.c_longlong), ("varifoldMfma", C.c_int)]
+                ("varifoldPairs", C.c_longlong), ("varifoldMfma", C.c_int), ("deviceBytes", C.c_longlong),
+                ("factorRetries", C.c_int)]
 
 
 class SaError(RuntimeError):
@@ -339,7 +340,7 @@ class Session:
                     fail_reason=st.failReason, seconds=st.seconds,
                     kmatvec_ms=st.kmatvecMs, kmatvec_count=st.kmatvecCount, operator_ms=st.operatorMs,
                     operator_count=st.operatorCount, varifold_ms=st.varifoldMs, varifold_pairs=st.varifoldPairs,
-                    varifold_mfma=st.varifoldMfma)
+                    varifold_mfma=st.varifoldMfma, device_bytes=st.deviceBytes, factor_retries=st.factorRetries)
 
     def set_params(self, params: dict):
         """New scalar parameters for the following runs (sa_session_set_params)."""
@@ -354,7 +355,7 @@ class Session:
     def set_profiling(self, on: bool = True):
         _check(lib().sa_session_set_profiling(self._h, 1 if on else 0))
 
-    PROBES = {"elastic": 1, "kmatvec": 2, "yank": 3, "fem": 4, "geometry": 5, "varifold": 6}
+    PROBES = {"elastic": 1, "kmatvec": 2, "yank": 3, "fem": 4, "geometry": 5, "varifold": 6, "penalty": 7}
 
     def probe(self, what: str, tauVar, tauPrm, x=None) -> np.ndarray:
         """One operator at the initial state (sa_session_probe)."""
@@ -362,7 +363,7 @@ class Session:
         tp = np.ascontiguousarray(np.asarray(tauPrm, dtype=np.float64).reshape(-1))
         d, V = self.dim, self.V
         E, Z = self.mesh.elmNum, self.mesh.nzrNum
-        size = {"elastic": d * V, "kmatvec": d * V, "yank": d * V + V, "fem": 2 * Z,
+        size = {"elastic": d * V, "kmatvec": d * V, "penalty": d * V, "yank": d * V + V, "fem": 2 * Z,
                 "geometry": (d * d + 1 + 2 * d) * E, "varifold": 1 + d * V}[what]
         out = np.zeros(size)
         xin = None if x is None else np.ascontiguousarray(x, dtype=np.float64)
@@ -442,7 +443,7 @@ class Server:
         rc = lib().sa_server_wait(self._h, C.c_longlong(req_id), _dp(obj), C.byref(st), _dp(times))
         return dict(rc=rc, error=lib().sa_last_error().decode() if rc else "", obj=float(obj[0]),
                     status=st.status, steps=st.steps, pcg_iters=st.pcgIters, refine=st.refineSweeps,
-                    fp64_steps=st.fp64Steps, fail_reason=st.failReason,
+                    fp64_steps=st.fp64Steps, fail_reason=st.failReason, factor_retries=st.factorRetries,
                     seconds=st.seconds, t_submit=times[0], t_start=times[1], t_end=times[2])
 
     def info(self) -> dict:

@@ -114,6 +114,15 @@ struct SaError : std::runtime_error {
 
 inline int gridOf(long n, int nt = NT) { return (int)((n + nt - 1) / nt); }
 
+// Set while a constructor only sizes a session (sa_session_footprint): every
+// allocation is counted and nothing touches the device, so the footprint is
+// exactly what a session of the same mesh and parameters allocates.
+thread_local bool g_sizing = false;
+struct SizingScope {
+    SizingScope() { g_sizing = true; }
+    ~SizingScope() { g_sizing = false; }
+};
+
 template <typename T>
 struct DBuf {
     T *p = nullptr;
@@ -126,6 +135,10 @@ struct DBuf {
         if (p) { (void)hipFree(p); p = nullptr; }
         n = count;
         if (count == 0) return;
+        if (g_sizing) {
+            total += (long long)(sizeof(T) * count);
+            return;
+        }
         if (hipMalloc((void **)&p, sizeof(T) * count) != hipSuccess) {
             p = nullptr;
             throw SaError(SA_E_ALLOC, "Fail to allocate device memory (" + std::to_string(sizeof(T) * count) + " bytes).");
@@ -145,8 +158,19 @@ struct DBuf {
         }
     }
     void upload(const T *h, size_t count, hipStream_t s) {
+        if (!p || count == 0) return;   // sizing, or nothing to copy
         CK(hipMemcpyAsync(p, h, sizeof(T) * count, hipMemcpyHostToDevice, s));
     }
+    void zero(hipStream_t s) {
+        if (p) CK(hipMemsetAsync(p, 0, sizeof(T) * n, s));
+    }
+};
+
+// non-owning view of part of a DBuf (the stored kernel matrices share one
+// allocation; p == nullptr means absent)
+template <typename T>
+struct View {
+    T *p = nullptr;
 };
 
 // host copy of an input array that may live on the device
@@ -197,6 +221,7 @@ struct Stats {
     double vfdMs = 0.0;               // the varifold row-sum launch of the objective (profiling on)
     long long vfdPairs = 0;           //   ... the facet pairs it summed
     int vfdMfma = 0;                  //   ... 1 when it ran on the matrix cores (vfd_rows_mfma_k)
+    int factorRetries = 0;            // K_cc factorisations repeated after a reported failure
 };
 
 // hipEvent pairs around the kernel-matrix product and the elastic operator
@@ -274,15 +299,24 @@ struct Engine : EngineBase {
     DBuf<double> dCen, dDir, dVol, rowVal, gC, gN, grad, scal;
     DBuf<Ctl> ctl, rdCtl;
     // control-point form
-    DBuf<double> qCur, tauCur, qc, Kcc, Kc, Mpre, Mcol, cG, cR, cZ, cD, cS, cY, cK, cRout, cDelta, cZero, part, partS;
+    DBuf<double> qc, Kcc, Kc, Mpre, Mcol, cG, cR, cZ, cD, cS, cY, cK, cRout, cDelta, cZero, part, partS;
     DBuf<double> cA, cA0, cAm1, cAm2, cAm3;   // alpha, its value entering a step, the last three solutions
     DBuf<double> cR2, cD2;       // second residual / direction buffers (iteration parity, fused CG)
-    DBuf<double> A64, At64;      // stored K_xc / K_cx, FP64
+    // stored K_xc (V x C) and K_cx (C x V) of the control-point form, views
+    // into kmem: FP64 (plain form) or FP32 with their FP32 remainders (mixed
+    // form).  A mixed session whose remainders are stored keeps the FP64 pair
+    // of its fallback steps in the same memory (alias64): the FP32 quartet
+    // takes 8 bytes per entry pair, the FP64 pair too (use_fp64_store)
+    DBuf<double> kmem;
+    View<double> A64, At64;      // stored K_xc / K_cx, FP64
+    size_t na32 = 0, nt32 = 0, na64 = 0, nt64 = 0;
+    bool alias64 = false;
+    bool matrixFree = false;     // SA_MATRIX_FREE=1: FP64 kernel products never stored
     DBuf<double> Kst;            // all-landmark form: stored K (V x V, panel-major FP64), rebuilt per step
     DBuf<double> aM1, aM2, aM3;  // all-landmark form: the last three alpha = K^-1 v
     int slicesK = 1;
-    DBuf<float> A32, At32;       // stored K_xc / K_cx, FP32 (mixed precision)
-    DBuf<float> A32lo, At32lo;   // their FP32 remainders (split products of the refinement residuals)
+    View<float> A32, At32;       // stored K_xc / K_cx, FP32 (mixed precision)
+    View<float> A32lo, At32lo;   // their FP32 remainders (split products of the refinement residuals)
     DBuf<float> Mpre32;          // FP32 panel copy of the preconditioner (mixed, SA_MIXED_CC32)
     // spectral term of the control-point preconditioner, z = M r + U diag(cf) U^T r:
     // U holds Ritz vectors of M S for its largest eigenvalues lambda_l (orthonormal
@@ -319,29 +353,33 @@ struct Engine : EngineBase {
     int precAge = -1, precIters = 0;
     int fp64Steps = 0;           // mixed-form steps finished by the FP64 form (this run)
     bool mixedOff = false;       // a step of this run stalled: the rest run in FP64
-    bool no64Store = false;      // the FP64 stored matrices of the fallback did not fit
     bool precEveryStep = false;  // a stale preconditioner failed in this run: refresh every step
     bool forceRefresh = false;   // refresh at the next step (after a stalled mixed step)
     int failReason = 0;          // SA_FAIL_* of a failed run
 
-    // mixed form: FP64 stored matrices for the steps that fall back to FP64,
-    // allocated on the first such step (matrix-free FP64 products if they do
-    // not fit in device memory)
-    void ensure_fp64_store() {
-        if (A64.p || no64Store) return;
-        const size_t na = padded_rows<double>(V) * (size_t)NC, nt = padded_rows<double>(NC) * (size_t)V;
-        long long tot = 0;
-        try {
-            A64.alloc(na, tot);
-            At64.alloc(nt, tot);
-            CK(hipMemsetAsync(A64.p, 0, sizeof(double) * na, s));
-            CK(hipMemsetAsync(At64.p, 0, sizeof(double) * nt, s));
-        } catch (const SaError &e) {
-            if (e.code != SA_E_ALLOC) throw;
-            A64.alloc(0, tot);
-            At64.alloc(0, tot);
-            no64Store = true;
-        }
+    // mixed form, fallback to FP64: the FP64 pair takes the memory of the
+    // FP32 quartet for the rest of the run (matrix-free FP64 products when
+    // the session has no quartet); restore_fp32_store undoes it at the
+    // start of the next run.  Padding rows of either layout must be zero.
+    void use_fp64_store() {
+        if (!alias64 || A64.p) return;
+        kmem.zero(s);
+        A64.p = kmem.p;
+        At64.p = kmem.p + na64;
+        A32.p = At32.p = A32lo.p = At32lo.p = nullptr;
+    }
+    void set_fp32_views() {
+        float *f = reinterpret_cast<float *>(kmem.p);
+        A32.p = f;
+        At32.p = f + na32;
+        A32lo.p = alias64 ? At32.p + nt32 : nullptr;
+        At32lo.p = alias64 ? A32lo.p + na32 : nullptr;
+    }
+    void restore_fp32_store() {
+        if (!alias64 || !A64.p) return;
+        kmem.zero(s);
+        A64.p = At64.p = nullptr;
+        set_fp32_views();
     }
     int chunkHint = 4;           // first launch chunk of the next control-point CG
     int sweepIters[4] = {0, 0, 0, 0};   // inner iterations of the last step's refinement sweeps
@@ -366,8 +404,6 @@ struct Engine : EngineBase {
         mod.assign(p.modVec, p.modVec + p.modNum);
         CK(hipMemcpyAsync(dmod.p, mod.data(), sizeof(double) * mod.size(), hipMemcpyHostToDevice, s));
         CK(hipStreamSynchronize(s));
-        // scalars baked into the captured CG graphs
-        if (p.ldmWgt != ldmWgt || p.knlEps != knlEps || p.knlWidth != knlWidth || p.btmWgt != btmWgt) drop_graphs();
         ldmWgt = p.ldmWgt; knlWidth = p.knlWidth; knlEps = p.knlEps;
         spdTan = p.spdTanVal; spdTsv = p.spdTsvVal; btmWgt = p.btmWgt; dt = p.timeStp;
         itrMax = p.cgItrMax; cgTolSqu = p.cgTolSqu;
@@ -379,8 +415,11 @@ struct Engine : EngineBase {
     }
 
     ~Engine() override {
-        drop_graphs();
-        if (blas) (void)rocblas_destroy_handle(blas);
+        if (blas) {
+            // handle creation and every library call hold the library lock
+            std::lock_guard<std::mutex> lk(g_libMu);
+            (void)rocblas_destroy_handle(blas);
+        }
         if (hctl) (void)hipHostFree(hctl);
         if (s) (void)hipStreamDestroy(s);
     }
@@ -493,8 +532,17 @@ struct Engine : EngineBase {
             for (int v : hCtl) require(v >= 0 && v < V, "ctlIdxVec index out of range");
         }
 
-        CK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-        CK(hipHostMalloc((void **)&hctl, sizeof(Ctl) * 2));
+        {
+            // SA_MATRIX_FREE=1: FP64 kernel products stay matrix-free (as
+            // above the 48 GB store cap), for testing that form at small sizes
+            const char *mf = std::getenv("SA_MATRIX_FREE");
+            matrixFree = mf && mf[0] == '1';
+        }
+        const bool dryRun = g_sizing;   // sizing only: no stream, no handles, no copies
+        if (!dryRun) {
+            CK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+            CK(hipHostMalloc((void **)&hctl, sizeof(Ctl) * 2));
+        }
         long long &tot = bytes;
         elmVtx.alloc(hElm.size(), tot); elmVtx.upload(hElm.data(), hElm.size(), s);
         vtxElm.alloc(hVtxElm.size(), tot); vtxElm.upload(hVtxElm.data(), hVtxElm.size(), s);
@@ -515,7 +563,7 @@ struct Engine : EngineBase {
             if (once && hStart[V] <= (D + 1) * E) {
                 elmSlot.alloc(hSlot.size(), tot); elmSlot.upload(hSlot.data(), hSlot.size(), s);
                 vtxStart.alloc(hStart.size(), tot); vtxStart.upload(hStart.data(), hStart.size(), s);
-                CK(hipStreamSynchronize(s));   // the host vectors go out of scope
+                if (!dryRun) CK(hipStreamSynchronize(s));   // the host vectors go out of scope
             }
         }
         frmVtx.alloc(hFrm.size(), tot); frmVtx.upload(hFrm.data(), hFrm.size(), s);
@@ -532,12 +580,16 @@ struct Engine : EngineBase {
             bdrVtx.alloc(hBdr.size(), tot); bdrVtx.upload(hBdr.data(), hBdr.size(), s);
             vtxBdr.alloc(hVtxBdr.size(), tot); vtxBdr.upload(hVtxBdr.data(), hVtxBdr.size(), s);
             bfrc.alloc((size_t)D * D * Bn, tot);
-            CK(hipStreamSynchronize(s));   // the host tables go out of scope
+            if (!dryRun) CK(hipStreamSynchronize(s));   // the host tables go out of scope
         }
         if (!hLyr.empty()) { elmLyr.alloc(E, tot); elmLyr.upload(hLyr.data(), E, s); }
         dmod.alloc(mod.size(), tot); dmod.upload(mod.data(), mod.size(), s);
-        auto hq0 = to_host(lmkIni, (size_t)V * D, dev);
-        q0.alloc((size_t)V * D, tot); q0.upload(hq0.data(), hq0.size(), s);
+        q0.alloc((size_t)V * D, tot);
+        if (!dryRun) {
+            auto hq0 = to_host(lmkIni, (size_t)V * D, dev);
+            q0.upload(hq0.data(), hq0.size(), s);
+            CK(hipStreamSynchronize(s));   // hq0 goes out of scope
+        }
 
         if (tg) {
             hasTarget = true;
@@ -592,20 +644,15 @@ struct Engine : EngineBase {
             size_t nc = (size_t)NC * D;
             ctlIdx.alloc(NC, tot); ctlIdx.upload(hCtl.data(), NC, s);
             qc.alloc(nc, tot); Kcc.alloc((size_t)NC * NC, tot);
-            qCur.alloc((size_t)V * D, tot); tauCur.alloc(V, tot);
             cA.alloc(nc, tot); cA0.alloc(nc, tot); cAm1.alloc(nc, tot); cAm2.alloc(nc, tot); cAm3.alloc(nc, tot);
             cG.alloc(nc, tot); cR.alloc(nc, tot); cZ.alloc(nc, tot); cD.alloc(nc, tot);
             cR2.alloc(nc, tot); cD2.alloc(nc, tot);
             cS.alloc(nc, tot); cY.alloc(nc, tot); cK.alloc(nc, tot);
             cRout.alloc(nc, tot); cDelta.alloc(nc, tot); cZero.alloc(nc, tot);
-            CK(hipMemsetAsync(cZero.p, 0, sizeof(double) * nc, s));
+            cZero.zero(s);
             {
                 const char *e = std::getenv("SA_SPECTRAL");
                 specEnabled = !(e && e[0] == '0');
-                // SA_MATRIX_FREE=1: FP64 kernel products stay matrix-free (as
-                // above the 48 GB cap), for testing that form at small sizes
-                const char *mf = std::getenv("SA_MATRIX_FREE");
-                no64Store = mf && mf[0] == '1';
             }
             if (specEnabled) {
                 recD.alloc(nc * SPEC_M, tot); recSD.alloc(nc * SPEC_M, tot); recR.alloc(nc * SPEC_M, tot);
@@ -622,37 +669,40 @@ struct Engine : EngineBase {
             const size_t pc = padded_rows<double>(NC) * (size_t)NC;
             Mcol.alloc((size_t)NC * NC, tot);
             Mpre.alloc(pc, tot);
-            CK(hipMemsetAsync(Mpre.p, 0, sizeof(double) * pc, s));
+            Mpre.zero(s);
             if (precision == SA_PREC_MIXED && !SA_MIXED_CC32) {
                 // FP64 K_cc for the FP32 inner solve: its FP32 rounding need
                 // not be positive definite when K_cc is ill conditioned
                 Kc.alloc(pc, tot);
-                CK(hipMemsetAsync(Kc.p, 0, sizeof(double) * pc, s));
+                Kc.zero(s);
             }
             if (precision == SA_PREC_MIXED && SA_MIXED_CC32) {
                 // FP32 copy of the preconditioner for the FP32 inner solve (a
                 // step whose FP32 operator is unfit falls back to FP64)
                 const size_t pf = padded_rows<float>(NC) * (size_t)NC;
                 Mpre32.alloc(pf, tot);
-                CK(hipMemsetAsync(Mpre32.p, 0, sizeof(float) * pf, s));
+                Mpre32.zero(s);
             }
+            na32 = padded_rows<float>(V) * (size_t)NC;
+            nt32 = padded_rows<float>(NC) * (size_t)V;
+            na64 = padded_rows<double>(V) * (size_t)NC;
+            nt64 = padded_rows<double>(NC) * (size_t)V;
             if (precision == SA_PREC_MIXED) {
-                const size_t na = padded_rows<float>(V) * (size_t)NC, nt = padded_rows<float>(NC) * (size_t)V;
-                A32.alloc(na, tot); At32.alloc(nt, tot);
-                CK(hipMemsetAsync(A32.p, 0, sizeof(float) * na, s));
-                CK(hipMemsetAsync(At32.p, 0, sizeof(float) * nt, s));
-                if (SA_SPLIT_RESID && !no64Store && (na + nt) * sizeof(float) <= ((size_t)SA_SPLIT_MAX_GB << 30)) {
-                    A32lo.alloc(na, tot); At32lo.alloc(nt, tot);
-                    CK(hipMemsetAsync(A32lo.p, 0, sizeof(float) * na, s));
-                    CK(hipMemsetAsync(At32lo.p, 0, sizeof(float) * nt, s));
-                }
-            } else {
-                const size_t na = padded_rows<double>(V) * (size_t)NC, nt = padded_rows<double>(NC) * (size_t)V;
-                if ((na + nt) * sizeof(double) <= (size_t)48 << 30 && !no64Store) {
-                    A64.alloc(na, tot); At64.alloc(nt, tot);
-                    CK(hipMemsetAsync(A64.p, 0, sizeof(double) * na, s));
-                    CK(hipMemsetAsync(At64.p, 0, sizeof(double) * nt, s));
-                }
+                // FP32 pair, plus the FP32 remainders when both fit in
+                // SA_SPLIT_MAX_GB; the quartet then also holds the FP64 pair
+                // of fallback steps (padded to 256 rows it is at least as
+                // large as the FP64 pair padded to 128)
+                alias64 = SA_SPLIT_RESID && !matrixFree &&
+                          (na32 + nt32) * sizeof(float) <= ((size_t)SA_SPLIT_MAX_GB << 30);
+                const size_t nf = (alias64 ? 2 : 1) * (na32 + nt32);
+                kmem.alloc((nf + 1) / 2, tot);
+                kmem.zero(s);
+                if (kmem.p) set_fp32_views();
+            } else if ((na64 + nt64) * sizeof(double) <= (size_t)48 << 30 && !matrixFree) {
+                kmem.alloc(na64 + nt64, tot);
+                kmem.zero(s);
+                A64.p = kmem.p;
+                At64.p = kmem.p ? kmem.p + na64 : nullptr;
             }
             // about KS_BLOCKS blocks per stored product (row blocks x column slices)
             const int prS = precision == SA_PREC_MIXED ? panel_rows<float>() : panel_rows<double>();
@@ -672,28 +722,41 @@ struct Engine : EngineBase {
             slicesC = std::max(1, std::min(32, (1024 + rb - 1) / rb));
             part.alloc((size_t)slicesC * nc, tot);
             potrfInfo.alloc(2, tot);
-            std::lock_guard<std::mutex> lk(g_libMu);
-            if (rocblas_create_handle(&blas) != rocblas_status_success) throw SaError(SA_E_DEVICE, "rocblas_create_handle failed");
-            if (rocblas_set_stream(blas, s) != rocblas_status_success) throw SaError(SA_E_DEVICE, "rocblas_set_stream failed");
-            // deterministic factorisation and products: no atomics in rocBLAS/rocSOLVER kernels
-            if (rocblas_set_atomics_mode(blas, rocblas_atomics_not_allowed) != rocblas_status_success)
-                throw SaError(SA_E_DEVICE, "rocblas_set_atomics_mode failed");
             require(gridOf((long)nc, 64) <= MAXP, "too many control points for the partial-sum buffers");
+            if (!dryRun) {
+                std::lock_guard<std::mutex> lk(g_libMu);
+                if (rocblas_create_handle(&blas) != rocblas_status_success)
+                    throw SaError(SA_E_DEVICE, "rocblas_create_handle failed");
+                if (rocblas_set_stream(blas, s) != rocblas_status_success)
+                    throw SaError(SA_E_DEVICE, "rocblas_set_stream failed");
+                // deterministic factorisation and products: no atomics in rocBLAS/rocSOLVER kernels
+                if (rocblas_set_atomics_mode(blas, rocblas_atomics_not_allowed) != rocblas_status_success)
+                    throw SaError(SA_E_DEVICE, "rocblas_set_atomics_mode failed");
+            }
         }
         if (NC == 0) {
             aM1.alloc(n, tot); aM2.alloc(n, tot); aM3.alloc(n, tot);
             // all-landmark form: K stored per step when it fits (21 GB at V = 51,210)
+            // (matrix-free kmatvec_k products when it does not fit, with
+            // SA_MATRIX_FREE=1, or when the device refuses the allocation)
             const size_t nk = padded_rows<double>(V) * (size_t)V;
-            if (nk * sizeof(double) <= (size_t)48 << 30) {
-                Kst.alloc(nk, tot);
-                CK(hipMemsetAsync(Kst.p, 0, sizeof(double) * nk, s));
-                const int rbK = gridOf(V, panel_rows<double>());
-                slicesK = std::max(gridOf(V, KS_MAX_COLS), std::min(256, (2048 + rbK - 1) / rbK));
-                partS.alloc((size_t)slicesK * D * V, tot);
+            if (nk * sizeof(double) <= (size_t)48 << 30 && !matrixFree) {
+                try {
+                    Kst.alloc(nk, tot);
+                } catch (const SaError &e) {
+                    if (e.code != SA_E_ALLOC) throw;
+                    Kst.n = 0;
+                }
+                if (Kst.n) {
+                    Kst.zero(s);
+                    const int rbK = gridOf(V, panel_rows<double>());
+                    slicesK = std::max(gridOf(V, KS_MAX_COLS), std::min(256, (2048 + rbK - 1) / rbK));
+                    partS.alloc((size_t)slicesK * D * V, tot);
+                }
             }
         }
         require(gridOf(n) <= MAXP && gridOf(V, MV_ROWS) <= MAXP, "mesh too large for the partial-sum buffers");
-        CK(hipStreamSynchronize(s));
+        if (!dryRun) CK(hipStreamSynchronize(s));
     }
 
     // ---------------- kernel-order dispatch
@@ -902,6 +965,7 @@ struct Engine : EngineBase {
                 const char *e = std::getenv("SA_TRACE_RETRY");
                 return e && e[0] == '1';
             }();
+            if (prof) prof->factorRetries++;
             kcc_build_once(true);
             if (trace) {
                 int again[2] = {0, 0};
@@ -1255,55 +1319,13 @@ struct Engine : EngineBase {
         CHECK_LAUNCH();
     }
 
-    // HIP graphs of GRAPH_ITERS CG iterations, one per operand set (captured
-    // on first use, kept for the session: the buffers never move)
-    static constexpr int GRAPH_ITERS = 16;
     static constexpr int TIMER_EVERY = 4;
-    struct CgGraph {
-        KMode md;
-        const double *q, *tau, *geo;
-        double *xs;
-        hipGraphExec_t ex;
-    };
-    std::vector<CgGraph> cgGraphs;
-    // Off: with graph capture on, a run of three worker threads serving
-    // mixed-precision requests ended in a host segmentation fault whose cause
-    // has not been shown; the iterations are launched one by one instead.
-    bool useGraphs = false;
-
-    hipGraphExec_t cg_graph(KMode md, const double *q, const double *tauNow, const double *geoNow, double *xs) {
-        for (auto &g : cgGraphs)
-            if (g.md == md && g.q == q && g.tau == tauNow && g.geo == geoNow && g.xs == xs) return g.ex;
-        if (cgGraphs.size() >= 16) drop_graphs();
-        hipGraph_t gr = nullptr;
-        CK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        try {
-            for (int k = 0; k < GRAPH_ITERS; ++k) cg_iter(md, q, tauNow, geoNow, xs, nullptr, k);
-        } catch (...) {
-            (void)hipStreamEndCapture(s, &gr);
-            if (gr) (void)hipGraphDestroy(gr);
-            throw;
-        }
-        CK(hipStreamEndCapture(s, &gr));
-        hipGraphExec_t ex = nullptr;
-        hipError_t e = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(gr);
-        CK(e);
-        cgGraphs.push_back({md, q, tauNow, geoNow, xs, ex});
-        return ex;
-    }
-
-    void drop_graphs() {
-        for (auto &g : cgGraphs) (void)hipGraphExecDestroy(g.ex);
-        cgGraphs.clear();
-    }
 
     // PCG on S x = rhs (|rhs|^2 partials in P0, gC of them), preconditioned by
     // X^T X; x in/out (x0 = 0 when zeroStart); stop when |r|^2 < tolSqu |rhs|^2.
-    // Returns 0 converged, 1 breakdown, 2 iteration budget spent.  Without
-    // the kernel timer the iterations run as replays of a captured graph of
-    // GRAPH_ITERS iterations, the host checking the device control block
-    // between launches.
+    // Returns 0 converged, 1 breakdown, 2 iteration budget spent.  The host
+    // launches the iterations in chunks and reads the device control block
+    // between chunks.
     // ctl_pcg_run, recording the Krylov basis when a spectral harvest is due
     // (first solve after a refresh of M) and harvesting from it afterwards
     int ctl_pcg(KMode md, const double *q, const double *tauNow, const double *geoNow, const double *rhs, double *xs,
@@ -1519,21 +1541,6 @@ struct Engine : EngineBase {
         CHECK_LAUNCH();
         if (maxIt < 1) ctl_next_k<<<1, 1, 0, s>>>(ctl.p);   // no budget: the counter check ends the solve here
         Ctl &h = hctl[0];
-        // (graph replays would repeat the captured iteration indices, which
-        // the fused iteration passes as kernel arguments)
-        if (!timer.on && useGraphs && !fused_iter(md)) {
-            hipGraphExec_t ex = cg_graph(md, q, tauNow, geoNow, xs);
-            // first sync after about the previous solve's iteration count
-            int launches = std::max(1, std::min(8, chunkHint / GRAPH_ITERS));
-            while (true) {
-                for (int k = 0; k < launches; ++k) CK(hipGraphLaunch(ex, s));
-                sync_ctl(ctl.p, &h);
-                if (h.done) break;
-                launches = 1;
-            }
-            iters = h.iters;
-            return h.status;
-        }
         int it0 = 0;
         int chunk = std::max(4, std::min(64, chunkHint));
         while (true) {
@@ -1601,11 +1608,7 @@ struct Engine : EngineBase {
 
     int velocity_ctl(const double *qStep, const double *tauStep, const double *geoNow, int &iters, int &sweeps) {
         const int nc = NC * D, gC = gridOf(nc);
-        // this step's positions and concentration in fixed buffers, so the
-        // captured CG graphs stay valid from step to step
-        CK(hipMemcpyAsync(qCur.p, qStep, sizeof(double) * V * D, hipMemcpyDeviceToDevice, s));
-        CK(hipMemcpyAsync(tauCur.p, tauStep, sizeof(double) * V, hipMemcpyDeviceToDevice, s));
-        const double *q = qCur.p, *tauNow = tauCur.p;
+        const double *q = qStep, *tauNow = tauStep;
         yank(q, geoNow, f.p, P0.p);
         gather_ctl_k<D><<<gridOf(NC), NT, 0, s>>>(NC, V, ctlIdx.p, q, qc.p);
         CHECK_LAUNCH();
@@ -1622,7 +1625,7 @@ struct Engine : EngineBase {
                     specCarried < SA_SPEC_CARRY;
         kcc_build(refresh);
         if (refresh) { precAge = 0; forceRefresh = false; }
-        if (!mixedStep && precision == SA_PREC_MIXED) ensure_fp64_store();
+        if (!mixedStep && precision == SA_PREC_MIXED) use_fp64_store();
         kstore_build(q, !mixedStep, mixedStep);
         CK(hipMemsetAsync(ctl.p, 0, sizeof(Ctl), s));
         // g = K_cx f in FP64 (stored FP64 matrix when present), |g|^2 partials
@@ -1699,14 +1702,17 @@ struct Engine : EngineBase {
                 CHECK_LAUNCH();
                 if (iters >= itrMax) break;
             }
-            if (status != 0 && (stalled || iters < itrMax)) {
+            // not converged: the iteration budget is spent (status 2), or the
+            // sweeps stalled or ran out (finished by the FP64 form below)
+            if (status != 0 && iters >= itrMax) status = 2;
+            if (status != 0 && status != 2) {
                 // FP64 PCG from the current alpha, for this step and the rest
                 // of the run (stored FP64 kernel matrices when they fit)
                 ++fp64Steps;
                 mixedOff = true;
                 forceRefresh = true;   // the next step refreshes and takes its iteration count as the reference
                 CK(hipMemcpyAsync(cA.p, cA0.p, sizeof(double) * nc, hipMemcpyDeviceToDevice, s));
-                ensure_fp64_store();
+                use_fp64_store();
                 exact = KMode::MatrixFree64;
                 if (A64.p) {
                     kstore_build(q, true, false);
@@ -1734,7 +1740,7 @@ struct Engine : EngineBase {
         if (info[0] != 0) { failReason = SA_FAIL_CHOLESKY; return 1; }
         if (info[1] != 0) { failReason = SA_FAIL_TRTRI; return 1; }
         if (status == 0) return 0;
-        failReason = status == 2 ? SA_FAIL_BUDGET : status == 1 ? SA_FAIL_BREAKDOWN : SA_FAIL_REFINE;
+        failReason = status == 2 ? SA_FAIL_BUDGET : SA_FAIL_BREAKDOWN;
         return 1;
     }
 
@@ -1850,6 +1856,7 @@ struct Engine : EngineBase {
         }
         lastIters = 0;
         for (int &w : sweepIters) w = 0;
+        restore_fp32_store();   // a fallback of the last run took the FP32 store
         precAge = -1;
         fp64Steps = 0;
         mixedOff = false;
@@ -1875,6 +1882,19 @@ struct Engine : EngineBase {
                 elastic(x.p, tau.p, geo[0].p, ctl.p);
                 // E x alone (no penalty term)
                 gather_op_k<D, 2><<<gridOf(V), NT, 0, s>>>(V, E, vtxElm.p, frc.p, BL, B, vtxBtm.p, nullptr, 0.0, 0.0,
+                                                           nullptr, nullptr, f.p, nullptr, nullptr, ctl.p, vtxStart.p);
+                CHECK_LAUNCH();
+                CK(hipMemcpyAsync(out, f.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+                break;
+            case SA_PROBE_PENALTY:
+                // P(q0) in alone (the bottom penalty without btmWgt): the
+                // element pass and the vertex gather of a solve, element forces zero
+                require(B > 0, "the mesh has no bottom facets");
+                CK(hipMemcpyAsync(x.p, in, sizeof(double) * n, hipMemcpyHostToDevice, s));
+                btm_meas_k<D><<<gridOf(B), NT, 0, s>>>(B, V, btmVtx.p, q0.p, meas.p);
+                pen_elem_k<D><<<gridOf(B), NT, 0, s>>>(B, V, btmVtx.p, meas.p, x.p, pfrc.p, ctl.p);
+                frc.zero(s);
+                gather_op_k<D, 2><<<gridOf(V), NT, 0, s>>>(V, E, vtxElm.p, frc.p, BL, B, vtxBtm.p, pfrc.p, 1.0, 0.0,
                                                            nullptr, nullptr, f.p, nullptr, nullptr, ctl.p, vtxStart.p);
                 CHECK_LAUNCH();
                 CK(hipMemcpyAsync(out, f.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
@@ -1919,6 +1939,10 @@ struct Engine : EngineBase {
         const size_t n = (size_t)V * D;
         st = Stats{};
         prof = &st;
+        struct ProfReset {
+            Engine *e;
+            ~ProfReset() { e->prof = nullptr; }
+        } profReset{this};
         const int TT = (Trun > 0 && Trun <= T) ? Trun : T;
         start(tauVar, nVar, tauPrm, nPrm);
         int cur = 0;
@@ -1989,7 +2013,7 @@ struct Engine : EngineBase {
 };
 
 EngineBase *make_engine(const sa_mesh *m, const sa_target *t, const sa_params *p, const double *lmkIni) {
-    if (!m || !p || !lmkIni) throw SaError(SA_E_ARG, "null mesh, params or landmarks");
+    if (!m || !p || (!lmkIni && !g_sizing)) throw SaError(SA_E_ARG, "null mesh, params or landmarks");
     if (m->dim == 2) return new Engine<2>(*m, t, *p, lmkIni);
     if (m->dim == 3) return new Engine<3>(*m, t, *p, lmkIni);
     throw SaError(SA_E_ARG, "Dimension " + std::to_string(m->dim) + " is not supported.");
@@ -2072,6 +2096,8 @@ int sa_session_run(sa_session *ss, const double *tauVarVec, int tauVarNum, const
             stats->varifoldMs = st.vfdMs;
             stats->varifoldPairs = st.vfdPairs;
             stats->varifoldMfma = st.vfdMfma;
+            stats->deviceBytes = ss->eng->bytes;
+            stats->factorRetries = st.factorRetries;
         }
     });
 }
@@ -2112,37 +2138,15 @@ int sa_session_probe(sa_session *ss, int what, const double *tauVarVec, int tauV
 }
 
 long long sa_session_footprint(const sa_mesh *m, const sa_target *t, const sa_params *p) {
-    if (!m || !p) return -1;
-    long long D = m->dim, V = m->lmkNum, E = m->elmNum, Z = m->nzrNum, T = p->timeNum;
-    long long NS = (D + 1) * (D + 2) / 2, GN = D * D + 1 + 2 * D;
-    long long b = 0;
-    b += 4 * (E * (D + 1) + V * m->vtxElmCols + E * (D == 2 ? 8 : 12) + Z * m->femIfoCols + 2 * Z + 2 * V);
-    b += 8 * (V * D * T + V * T + 2 * GN * E + (1 + NS) * E + 3 * Z + 2 * V + 9 * V * D + (D + 1) * D * E + 5 * V);
-    // vertex-ordered force slots; boundary facets, their adjacency and forces
-    b += 4 * ((D + 1) * E + V + 1) + 4 * ((long long)m->bdrElmNum * D + V * m->vtxBdrCols) +
-         8 * D * D * (long long)m->bdrElmNum;
-    b += 8 * (7 * sa::MAXP + p->cgItrMax + 200010);
-    if (t) {
-        long long F = m->vfdElmNum, Ft = t->tgtElmNum;
-        b += 8 * (Ft * (2 * D + 1) + F * (4 * D + 1) + F + Ft + V * D) + 4 * (F * D + V * 9);
-    }
-    if (m->ctlNum > 0) {
-        long long C = m->ctlNum;
-        // rows padded to whole panels of 256 (FP32) / 128 (FP64) rows
-        auto pad = [](long long n, long long pr) { return (n + pr - 1) / pr * pr; };
-        // K_cc, rocBLAS output, preconditioner, vectors, slice partials, cross partials
-        b += 8 * (2 * C * C + pad(C, 128) * C + 14 * C * D + 512 * C * D + 128 * D * V + 32 * C * D);
-        if (p->precision == SA_PREC_MIXED) {
-            const long long a32 = 4 * (pad(V, 256) * C + pad(C, 256) * V);
-            b += a32 + 8 * pad(C, 128) * C;
-            if (SA_SPLIT_RESID && a32 <= ((long long)SA_SPLIT_MAX_GB << 30)) b += a32;   // FP32 remainders
-        }
-        else if ((pad(V, 128) * C + pad(C, 128) * V) * 8 <= (48LL << 30)) b += 8 * (pad(V, 128) * C + pad(C, 128) * V);
-        // spectral preconditioner: recorded Krylov basis, Ritz vectors, projected matrices
-        b += 8 * ((3 * SA_SPEC_M + SA_SPEC_K) * C * D + (6 + 65) * SA_SPEC_M * SA_SPEC_M + 2 * SA_SPEC_M +
-                  2 * SA_SPEC_K);
-    }
-    return b;
+    // a session constructor run in sizing mode: it validates the tables and
+    // counts every allocation a session of this mesh makes, touching no device
+    long long bytes = -1;
+    const int rc = sa::guarded([&] {
+        sa::SizingScope sizing;
+        std::unique_ptr<sa::EngineBase> eng(sa::make_engine(m, t, p, nullptr));
+        bytes = eng->bytes;
+    });
+    return rc == SA_OK ? bytes : -1;
 }
 
 int sa_deformation(const sa_mesh *mesh, const sa_params *prm, const double *tauVarMat, int tauVarNum, int candNum,

@@ -72,6 +72,9 @@ struct MeshCopy {
         p.modVec = mod.data(); p.ynkActPrmVec = ynk.data(); p.reaActPrmVec = rea.data();
         if (hasT) { t.cenPosMat = tcen.data(); t.uniDirMat = tdir.data(); t.elmVolVec = tvol.data(); }
         footprint = sa_session_footprint(&m, hasT ? &t : nullptr, &p);
+        // the sizing pass validates the tables: a mesh no session could be
+        // built for is refused at registration
+        if (footprint < 0) throw SaError(SA_E_ARG, sa_last_error());
     }
 };
 
@@ -168,21 +171,32 @@ class Server {
   private:
     double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0_).count(); }
 
-    // under the lock: choose a session for the head request; -1 = not now
-    int place(const Request &r, bool &create) {
+    // under the lock: choose a session for the head request.  Returns a
+    // slot, -1 (not now: wait for a request to finish or an eviction to
+    // complete), -2 (can never fit) or -3 (idle sessions must go first: they
+    // are moved to `evict`, their memory still counted in memUsed_ until the
+    // caller has destroyed them outside the lock and released it)
+    int place(const Request &r, bool &create, std::vector<Slot> &evict) {
         create = false;
         for (int i = 0; i < (int)slots_.size(); ++i)
             if (slots_[i].s && !slots_[i].busy && slots_[i].mesh == r.mesh) return i;
         long long need = meshes_[r.mesh]->footprint;
         if (need > budget_) return -2;   // can never fit
-        while (memUsed_ + need > budget_) {
-            int lru = -1;
-            for (int i = 0; i < (int)slots_.size(); ++i)
-                if (slots_[i].s && !slots_[i].busy && (lru < 0 || slots_[i].lastUse < slots_[lru].lastUse)) lru = i;
-            if (lru < 0) return -1;      // wait for a running request to finish
-            sa_session_destroy(slots_[lru].s);
-            memUsed_ -= slots_[lru].bytes;
-            slots_[lru] = Slot{};
+        if (memUsed_ + need > budget_) {
+            if (evicting_) return -1;    // another worker is releasing memory
+            long long freed = 0;
+            while (memUsed_ - freed + need > budget_) {
+                int lru = -1;
+                for (int i = 0; i < (int)slots_.size(); ++i)
+                    if (slots_[i].s && !slots_[i].busy && (lru < 0 || slots_[i].lastUse < slots_[lru].lastUse))
+                        lru = i;
+                if (lru < 0) break;      // not enough idle sessions: wait for running requests
+                freed += slots_[lru].bytes;
+                evict.push_back(slots_[lru]);
+                slots_[lru] = Slot{};
+            }
+            if (evict.empty()) return -1;
+            return -3;
         }
         int idx = -1;
         for (int i = 0; i < (int)slots_.size(); ++i)
@@ -201,14 +215,37 @@ class Server {
             int slot = -1;
             bool create = false;
             long long id = 0;
+            std::vector<Slot> evict;
             cv_.wait(g, [&] {
                 if (stop_) return true;
                 if (queue_.empty()) return false;
                 id = queue_.front();
-                slot = place(reqs_[id], create);
+                slot = place(reqs_[id], create, evict);
                 return slot != -1;
             });
-            if (stop_) return;
+            if (stop_) {
+                // (sessions taken for eviction are still destroyed)
+                g.unlock();
+                for (auto &sl : evict) sa_session_destroy(sl.s);
+                return;
+            }
+            if (slot == -3) {
+                // destroy the evicted idle sessions without holding the lock
+                // (their hipFree waits for the device), then release their
+                // memory and place the head request again
+                evicting_ = true;
+                g.unlock();
+                long long freed = 0;
+                for (auto &sl : evict) {
+                    sa_session_destroy(sl.s);
+                    freed += sl.bytes;
+                }
+                g.lock();
+                memUsed_ -= freed;
+                evicting_ = false;
+                cv_.notify_all();
+                continue;
+            }
             queue_.pop_front();
             Request &r = reqs_[id];
             if (slot == -2) {
@@ -280,6 +317,7 @@ class Server {
     long long budget_, memUsed_ = 0, lastId_ = 0;
     int running_ = 0;
     bool stop_ = false;
+    bool evicting_ = false;   // a worker is destroying evicted sessions
     std::chrono::steady_clock::time_point t0_;
 };
 

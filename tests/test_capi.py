@@ -62,20 +62,53 @@ def test_unsorted_fem_table_rejected():
 
 
 def test_server_failure_path_without_device():
-    """A request whose session cannot be created reports its error instead
-    of hanging (here: no GPU in this container, or an invalid mesh)."""
+    """An invalid mesh is refused when it is registered (the sizing pass
+    validates its tables); a request whose session cannot be created reports
+    its error instead of hanging (here: no GPU in this container)."""
     p = W.ring_problem(n_rings=4, n_per_ring=12)
     elm = dict(p["elm"])
     bad = elm["elmVtxMat"].copy()
     bad[0, 0] = 99999
     elm["elmVtxMat"] = bad
     srv = api.Server(workers=1, mem_budget=1 << 30)
-    mid = srv.add_mesh(dict(p, elm=elm))
+    with pytest.raises(api.SaError, match="range"):
+        srv.add_mesh(dict(p, elm=elm))
+    mid = srv.add_mesh(p)
     g = srv.wait(srv.submit(mid, p["tauVar"][:, 0], p["tauPrm"]))
-    assert g["rc"] in (-1, -3) and g["error"], g
-    if g["rc"] == -1:
-        assert "range" in g["error"]
+    if api.lib().sa_device_count() == 0:
+        assert g["rc"] == -3 and g["error"], g
     srv.close()
+
+
+def test_footprint_sizing_without_device():
+    """sa_session_footprint sizes a session without touching the device: it
+    grows with the time levels and the control points, counts the stored
+    all-landmark kernel matrix, and rejects an invalid mesh with -1."""
+    keep = api._Keep()
+    p = W.ring_problem(n_rings=4, n_per_ring=12)
+    def fp(prob, precision="fp64", T=None):
+        k = api._Keep()
+        elm = dict(prob["elm"])
+        if prob.get("ctlIdx") is not None:
+            elm["ctlIdxVec"] = prob["ctlIdx"]
+        m = api.make_mesh(elm, np.ascontiguousarray(prob["q0"]), k, with_vfd=True)
+        t = api.make_target(prob["target"], prob["vfd"], k)
+        prm = api.params_struct(prob["params"], k, precision, T)
+        return api.lib().sa_session_footprint(C.byref(m), C.byref(t), C.byref(prm))
+    V = p["q0"].shape[1]
+    a, b = fp(p, T=5), fp(p, T=9)
+    # four more time levels: positions and concentration of each (8 B x (D + 1) x V)
+    assert b - a == 4 * 8 * 3 * V
+    # the all-landmark form stores K (V rows padded to 128-row panels) and its slice partials
+    assert a > 8 * 128 * V
+    pc = dict(p, ctlIdx=np.arange(0, V, 4, dtype=np.int32))
+    assert fp(pc, "mixed") > 0 and fp(pc, "fp64") > 0
+    elm = dict(p["elm"])
+    bad = elm["elmVtxMat"].copy()
+    bad[0, 0] = 99999
+    elm["elmVtxMat"] = bad
+    assert fp(dict(p, elm=elm)) == -1
+    del keep
 
 
 @pytest.mark.parametrize("cname,pyname", [("sa_mesh", "SaMesh"), ("sa_params", "SaParams"),
