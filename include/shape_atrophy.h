@@ -198,7 +198,10 @@ int sa_session_set_params(sa_session *s, const sa_params *prm);
      SA_PROBE_GEOMETRY  out = per-element [Q^-1 (dim^2), det Q, frame 1 (dim), frame 2 (dim)] x elmNum
      SA_PROBE_VARIFOLD  in = landmarks (lmkNum x dim); out = [objective, gradient (lmkNum x dim)]
                         (the objective by the kernel a run uses, the gradient by the FP64 VALU kernel)
-     SA_PROBE_PENALTY   out = P(q0) in, the bottom penalty without btmWgt (lmkNum x dim) */
+     SA_PROBE_PENALTY   out = P(q0) in, the bottom penalty without btmWgt (lmkNum x dim)
+     SA_PROBE_KXC       out = K_xc(q0) in: in ctlNum x dim, out lmkNum x dim  (control-point
+     SA_PROBE_KCX       out = K_cx(q0) in: in lmkNum x dim, out ctlNum x dim   kernel products
+                        as the session's inner solve forms them, without knlEps) */
 #define SA_PROBE_ELASTIC 1
 #define SA_PROBE_KMATVEC 2
 #define SA_PROBE_YANK 3
@@ -206,6 +209,8 @@ int sa_session_set_params(sa_session *s, const sa_params *prm);
 #define SA_PROBE_GEOMETRY 5
 #define SA_PROBE_VARIFOLD 6
 #define SA_PROBE_PENALTY 7
+#define SA_PROBE_KXC 8
+#define SA_PROBE_KCX 9
 int sa_session_probe(sa_session *s, int what, const double *tauVarVec, int tauVarNum, const double *tauPrmVec,
                      int tauPrmNum, const double *in, double *out);
 /* bytes of device memory a session of this mesh holds */

@@ -555,40 +555,101 @@ void kernel_matvec(int order, double width, double eps, const double *a, int na,
 }
 
 // dense Cholesky (lower, column-major L(i,k) = A[k*n+i]) and solve: the
-// role of cusolverDnDpotrf / potrs (cholesky.cu:4-17, deform.cu:99-104)
+// role of cusolverDnDpotrf / potrs (cholesky.cu:4-17, deform.cu:99-104).
+// Left-looking by panels of CHB columns: every entry subtracts its terms in
+// ascending k, so the factor does not depend on the thread count.
+constexpr int CHB = 64;
 bool cholesky(std::vector<double> &A, int n) {
-    for (int j = 0; j < n; ++j) {
-        double *cj = A.data() + (size_t)j * n;
-        const int chunk = 256;
-        int nblk = (n - j + chunk - 1) / chunk;
-#pragma omp parallel for schedule(static)
-        for (int blk = 0; blk < nblk; ++blk) {
-            int i0 = j + blk * chunk, i1 = std::min(n, i0 + chunk);
-            for (int k = 0; k < j; ++k) {
-                const double *ck = A.data() + (size_t)k * n;
-                double ljk = ck[j];
-                for (int i = i0; i < i1; ++i) cj[i] -= ljk * ck[i];
+    double *a = A.data();
+    for (int j0 = 0; j0 < n; j0 += CHB) {
+        const int j1 = std::min(n, j0 + CHB);
+        // panel update A(j0:n, j0:j1) -= A(j0:n, 0:j0) A(j0:j1, 0:j0)^T
+        if (j0 > 0) {
+            const int nch = (n - j0 + CHB - 1) / CHB;
+#pragma omp parallel for schedule(dynamic)
+            for (int ch = 0; ch < nch; ++ch) {
+                const int i0 = j0 + ch * CHB, i1 = std::min(n, i0 + CHB);
+                for (int k = 0; k < j0; ++k) {
+                    const double *ck = a + (size_t)k * n;
+                    for (int jj = j0; jj < j1; ++jj) {
+                        const double ljk = ck[jj];
+                        double *cj = a + (size_t)jj * n;
+                        for (int i = std::max(i0, jj); i < i1; ++i) cj[i] -= ljk * ck[i];
+                    }
+                }
             }
         }
-        if (!(cj[j] > 0)) return false;
-        double l = std::sqrt(cj[j]);
-        cj[j] = l;
-        for (int i = j + 1; i < n; ++i) cj[i] /= l;
+        // the panel itself, column by column
+        for (int jj = j0; jj < j1; ++jj) {
+            double *cj = a + (size_t)jj * n;
+            const int nch = (n - jj + 255) / 256;
+#pragma omp parallel for schedule(static) if (nch > 4)
+            for (int ch = 0; ch < nch; ++ch) {
+                const int i0 = jj + ch * 256, i1 = std::min(n, i0 + 256);
+                for (int k = j0; k < jj; ++k) {
+                    const double *ck = a + (size_t)k * n;
+                    const double ljk = ck[jj];
+                    for (int i = i0; i < i1; ++i) cj[i] -= ljk * ck[i];
+                }
+            }
+            if (!(cj[jj] > 0)) return false;
+            const double l = std::sqrt(cj[jj]);
+            cj[jj] = l;
+            for (int i = jj + 1; i < n; ++i) cj[i] /= l;
+        }
     }
     return true;
 }
-void chol_solve(const std::vector<double> &L, int n, double *b) {
-    for (int k = 0; k < n; ++k) {
-        const double *ck = L.data() + (size_t)k * n;
-        b[k] /= ck[k];
-        double bk = b[k];
-        for (int i = k + 1; i < n; ++i) b[i] -= ck[i] * bk;
+
+// L L^T x = b for m right-hand sides (b: n x m, column-major), by panels of
+// CHB; the per-entry summation order is fixed (ascending k)
+void chol_solve(const std::vector<double> &L, int n, double *b, int m = 1) {
+    const double *l = L.data();
+    for (int j0 = 0; j0 < n; j0 += CHB) {
+        const int j1 = std::min(n, j0 + CHB);
+        for (int c = 0; c < m; ++c) {
+            double *bc = b + (size_t)c * n;
+            for (int k = j0; k < j1; ++k) {
+                const double *ck = l + (size_t)k * n;
+                bc[k] /= ck[k];
+                for (int i = k + 1; i < j1; ++i) bc[i] -= ck[i] * bc[k];
+            }
+        }
+        const int nch = (n - j1 + 255) / 256;
+#pragma omp parallel for schedule(static) if (nch > 4)
+        for (int ch = 0; ch < nch; ++ch) {
+            const int i0 = j1 + ch * 256, i1 = std::min(n, i0 + 256);
+            for (int c = 0; c < m; ++c) {
+                double *bc = b + (size_t)c * n;
+                for (int k = j0; k < j1; ++k) {
+                    const double *ck = l + (size_t)k * n;
+                    const double bk = bc[k];
+                    for (int i = i0; i < i1; ++i) bc[i] -= ck[i] * bk;
+                }
+            }
+        }
     }
-    for (int i = n - 1; i >= 0; --i) {
-        const double *ci = L.data() + (size_t)i * n;
-        double s = b[i];
-        for (int k = i + 1; k < n; ++k) s -= ci[k] * b[k];
-        b[i] = s / ci[i];
+    for (int j1 = n; j1 > 0; j1 -= CHB) {
+        const int j0 = std::max(0, j1 - CHB);
+#pragma omp parallel for schedule(static) if (n - j1 > 1024)
+        for (int k = j0; k < j1; ++k) {
+            const double *ck = l + (size_t)k * n;
+            for (int c = 0; c < m; ++c) {
+                double *bc = b + (size_t)c * n;
+                double s = 0.0;
+                for (int i = j1; i < n; ++i) s += ck[i] * bc[i];
+                bc[k] -= s;
+            }
+        }
+        for (int c = 0; c < m; ++c) {
+            double *bc = b + (size_t)c * n;
+            for (int k = j1 - 1; k >= j0; --k) {
+                const double *ck = l + (size_t)k * n;
+                double s = bc[k];
+                for (int i = k + 1; i < j1; ++i) s -= ck[i] * bc[i];
+                bc[k] = s / ck[k];
+            }
+        }
     }
 }
 
@@ -931,6 +992,7 @@ struct Solver {
     // pays the V^2 kernel evaluations once per step, as the reference does)
     std::vector<double> Kd;
     static constexpr int kDenseMax = 8000;
+    static constexpr double kStoreMax = 6e9;   // bytes of a stored K_xc (control-point form)
     void K_build(const double *q) {
         Kd.clear();
         if (V > kDenseMax) return;
@@ -1021,6 +1083,7 @@ struct Solver {
             for (int k = 0; k < C; ++k) qc[c * C + k] = q[c * V + m.ctlIdx[k]];
         // K_cc Cholesky
         std::vector<double> Kcc((size_t)C * C);
+#pragma omp parallel for schedule(static)
         for (int j = 0; j < C; ++j)
             for (int i = 0; i < C; ++i) {
                 double d2 = 0;
@@ -1029,21 +1092,66 @@ struct Solver {
             }
         std::vector<double> Lcc(Kcc);
         if (!cholesky(Lcc, C)) return 1;
+        // K_xc stored row by row (K[i C + j] = k(|q_i - qc_j|)) when it fits
+        // in kStoreMax bytes, else both products matrix-free
+        std::vector<double> Kst;
+        if ((double)V * C * sizeof(double) <= kStoreMax) {
+            Kst.resize((size_t)V * C);
+#pragma omp parallel for schedule(static)
+            for (int i = 0; i < V; ++i)
+                for (int j = 0; j < C; ++j) {
+                    double d2 = 0;
+                    for (int c = 0; c < D; ++c) { double d = q[c * V + i] - qc[c * C + j]; d2 += d * d; }
+                    Kst[(size_t)i * C + j] = kernel_value(D, p.knlOrder, d2, p.knlWidth);
+                }
+        }
         auto Kxc = [&](const double *al, double *out) {
-            kernel_matvec<D>(p.knlOrder, p.knlWidth, 0.0, q, V, qc.data(), C, al, out, false);
+            if (Kst.empty()) { kernel_matvec<D>(p.knlOrder, p.knlWidth, 0.0, q, V, qc.data(), C, al, out, false); return; }
+#pragma omp parallel for schedule(static)
+            for (int i = 0; i < V; ++i) {
+                const double *ki = Kst.data() + (size_t)i * C;
+                for (int c = 0; c < D; ++c) {
+                    double acc = 0.0;
+                    for (int j = 0; j < C; ++j) acc += ki[j] * al[c * C + j];
+                    out[c * V + i] = acc;
+                }
+            }
         };
         auto Kcx = [&](const double *y, double *out) {
-            kernel_matvec<D>(p.knlOrder, p.knlWidth, 0.0, qc.data(), C, q, V, y, out, false);
+            if (Kst.empty()) { kernel_matvec<D>(p.knlOrder, p.knlWidth, 0.0, qc.data(), C, q, V, y, out, false); return; }
+            // fixed row chunks, each summed in row order, the chunks added in
+            // chunk order: the result does not depend on the thread count
+            const int nch = std::max(1, std::min(64, V / 256));
+            std::vector<double> part((size_t)nch * nc, 0.0);
+#pragma omp parallel for schedule(static)
+            for (int ch = 0; ch < nch; ++ch) {
+                double *pc = part.data() + (size_t)ch * nc;
+                const int i0 = (int)((long long)V * ch / nch), i1 = (int)((long long)V * (ch + 1) / nch);
+                for (int i = i0; i < i1; ++i) {
+                    const double *ki = Kst.data() + (size_t)i * C;
+                    for (int c = 0; c < D; ++c) {
+                        const double yi = y[c * V + i];
+                        double *pcc = pc + (size_t)c * C;
+                        for (int j = 0; j < C; ++j) pcc[j] += ki[j] * yi;
+                    }
+                }
+            }
+            for (size_t k = 0; k < nc; ++k) {
+                double acc = part[k];
+                for (int ch = 1; ch < nch; ++ch) acc += part[(size_t)ch * nc + k];
+                out[k] = acc;
+            }
         };
         auto S = [&](const double *al, double *out) {   // (ldmWgt K_cc + K_cx B K_xc) al
-            std::vector<double> w(nv), bw(nv), t2(nc);
+            std::vector<double> w(nv), bw(nv);
             Kxc(al, w.data());
             B_apply(q, tau, w.data(), bw.data());
             Kcx(bw.data(), out);
-            for (int c = 0; c < D; ++c)
-                for (int i = 0; i < C; ++i) {
+#pragma omp parallel for schedule(static)
+            for (int i = 0; i < C; ++i)
+                for (int c = 0; c < D; ++c) {
                     double s = 0;
-                    for (int j = 0; j < C; ++j) s += Kcc[(size_t)j * C + i] * al[c * C + j];
+                    for (int j = 0; j < C; ++j) s += Kcc[(size_t)i * C + j] * al[c * C + j];   // symmetric
                     out[c * C + i] += p.ldmWgt * s;
                 }
         };
@@ -1058,7 +1166,7 @@ struct Solver {
         double tol = bb * p.cgTolSqu;
         auto prec = [&](const double *in, double *out) {
             std::memcpy(out, in, sizeof(double) * nc);
-            for (int c = 0; c < D; ++c) chol_solve(Lcc, C, out + (size_t)c * C);
+            chol_solve(Lcc, C, out, D);
         };
         int it = 0;
         if (rr >= tol) {

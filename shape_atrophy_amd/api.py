@@ -355,7 +355,8 @@ class Session:
     def set_profiling(self, on: bool = True):
         _check(lib().sa_session_set_profiling(self._h, 1 if on else 0))
 
-    PROBES = {"elastic": 1, "kmatvec": 2, "yank": 3, "fem": 4, "geometry": 5, "varifold": 6, "penalty": 7}
+    PROBES = {"elastic": 1, "kmatvec": 2, "yank": 3, "fem": 4, "geometry": 5, "varifold": 6, "penalty": 7,
+              "kxc": 8, "kcx": 9}
 
     def probe(self, what: str, tauVar, tauPrm, x=None) -> np.ndarray:
         """One operator at the initial state (sa_session_probe)."""
@@ -363,7 +364,9 @@ class Session:
         tp = np.ascontiguousarray(np.asarray(tauPrm, dtype=np.float64).reshape(-1))
         d, V = self.dim, self.V
         E, Z = self.mesh.elmNum, self.mesh.nzrNum
-        size = {"elastic": d * V, "kmatvec": d * V, "penalty": d * V, "yank": d * V + V, "fem": 2 * Z,
+        nc = self.mesh.ctlNum
+        size = {"elastic": d * V, "kmatvec": d * V, "penalty": d * V, "kxc": d * V, "kcx": d * nc,
+                "yank": d * V + V, "fem": 2 * Z,
                 "geometry": (d * d + 1 + 2 * d) * E, "varifold": 1 + d * V}[what]
         out = np.zeros(size)
         xin = None if x is None else np.ascontiguousarray(x, dtype=np.float64)

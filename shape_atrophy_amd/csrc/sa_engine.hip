@@ -214,6 +214,38 @@ std::vector<int> incidence(const std::vector<int> &items, int nItems, int k, int
     return tbl;
 }
 
+// positions (SoA, n points, D coordinates) -> their indices in Morton
+// (Z-curve) order of the coordinates quantised to 2^(63/D) cells per
+// dimension over the bounding box; ties keep index order
+std::vector<int> morton_order(const double *p, int n, int D) {
+    std::vector<int> idx(n);
+    for (int i = 0; i < n; ++i) idx[i] = i;
+    if (n < 2) return idx;
+    const int bits = D == 3 ? 21 : 31;
+    double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    for (int c = 0; c < D; ++c) {
+        lo[c] = hi[c] = p[(size_t)c * n];
+        for (int i = 1; i < n; ++i) {
+            lo[c] = std::min(lo[c], p[(size_t)c * n + i]);
+            hi[c] = std::max(hi[c], p[(size_t)c * n + i]);
+        }
+    }
+    double ext = 0.0;
+    for (int c = 0; c < D; ++c) ext = std::max(ext, hi[c] - lo[c]);
+    const double scale = ext > 0.0 ? (double)((1ull << bits) - 1) / ext : 0.0;
+    std::vector<unsigned long long> code(n, 0ull);
+    for (int i = 0; i < n; ++i) {
+        unsigned long long q[3] = {0, 0, 0};
+        for (int c = 0; c < D; ++c) q[c] = (unsigned long long)((p[(size_t)c * n + i] - lo[c]) * scale);
+        unsigned long long m = 0;
+        for (int b = bits - 1; b >= 0; --b)
+            for (int c = 0; c < D; ++c) m = (m << 1) | ((q[c] >> b) & 1ull);
+        code[i] = m;
+    }
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return code[a] < code[b]; });
+    return idx;
+}
+
 struct Stats {
     int status = 0, steps = 0, pcgIters = 0, rdIters = 0, refine = 0, fp64Steps = 0, failReason = 0;
     double kmvMs = 0.0, opMs = 0.0;   // summed durations of the timed kernels (profiling on)
@@ -285,6 +317,12 @@ struct Engine : EngineBase {
     // vtxStart[i] where vertex i's list starts (absent when an (e, l) is
     // listed twice: the element-ordered gather then runs instead)
     DBuf<int> elmSlot, vtxStart;
+    // mixed form on the matrix cores (kx_mfma_k): the kernel products of the
+    // inner solve evaluated matrix-free, nothing stored; landmarks and
+    // control points visited in Morton order (position -> index)
+    bool kxMfma = false;
+    DBuf<int> lmkPerm, ctlPerm;
+    int slicesAm = 1, slicesTm = 1;
     // boundary yank (sa_params.bdrYank): boundary facets, their vertex
     // adjacency, the facet forces; Bn = 0 when the tables are absent or unusable
     DBuf<int> bdrVtx, vtxBdr;
@@ -585,10 +623,10 @@ struct Engine : EngineBase {
         if (!hLyr.empty()) { elmLyr.alloc(E, tot); elmLyr.upload(hLyr.data(), E, s); }
         dmod.alloc(mod.size(), tot); dmod.upload(mod.data(), mod.size(), s);
         q0.alloc((size_t)V * D, tot);
+        std::vector<double> hq0;
         if (!dryRun) {
-            auto hq0 = to_host(lmkIni, (size_t)V * D, dev);
+            hq0 = to_host(lmkIni, (size_t)V * D, dev);
             q0.upload(hq0.data(), hq0.size(), s);
-            CK(hipStreamSynchronize(s));   // hq0 goes out of scope
         }
 
         if (tg) {
@@ -687,7 +725,27 @@ struct Engine : EngineBase {
             nt32 = padded_rows<float>(NC) * (size_t)V;
             na64 = padded_rows<double>(V) * (size_t)NC;
             nt64 = padded_rows<double>(NC) * (size_t)V;
-            if (precision == SA_PREC_MIXED) {
+            {
+                // the matrix-core products (3D and Gaussian kernels) replace the
+                // stored FP32 matrices when those would pass 48 GB, or with
+                // SA_KX_MFMA=1.  Measured on the 51,210-landmark shell with
+                // 5,000 control points: 175 us per product against 179 us for the
+                // stored FP32 product, but 3e-7 relative error against 2.5e-8,
+                // which doubles the inner CG iterations; so the store is the default
+                const char *e = std::getenv("SA_KX_MFMA");
+                const bool big = (na32 + nt32) * sizeof(float) > ((size_t)48 << 30);
+                kxMfma = precision == SA_PREC_MIXED && (D == 3 || knlOrder == -1) &&
+                         (e && (e[0] == '0' || e[0] == '1') ? e[0] == '1' : big);
+            }
+            if (kxMfma) {
+                // nothing stored: the inner products run on the matrix cores,
+                // the refinement residuals and fallback steps are matrix-free FP64
+                lmkPerm.alloc(V, tot);
+                ctlPerm.alloc(NC, tot);
+                const int rbA = gridOf(V, KXM_ROWS), rbT = gridOf(NC, KXM_ROWS);
+                slicesAm = std::max(1, std::min(gridOf(NC, 256), (KXM_BLOCKS + rbA - 1) / rbA));
+                slicesTm = std::max(1, std::min(gridOf(V, 256), (KXM_BLOCKS + rbT - 1) / rbT));
+            } else if (precision == SA_PREC_MIXED) {
                 // FP32 pair, plus the FP32 remainders when both fit in
                 // SA_SPLIT_MAX_GB; the quartet then also holds the FP64 pair
                 // of fallback steps (padded to 256 rows it is at least as
@@ -716,7 +774,8 @@ struct Engine : EngineBase {
             slicesM = std::max(gridOf(NC, KS_MAX_COLS), std::min(256, (KS_BLOCKS + rbM - 1) / rbM));
             slicesM32 = std::max(gridOf(NC, KS_MAX_COLS), std::min(256, (KS_BLOCKS + rbM32 - 1) / rbM32));
             // (twice the product slices for the split residual products)
-            partS.alloc(std::max({(size_t)2 * slicesA * D * V, (size_t)2 * slicesT * nc, (size_t)slicesM * nc,
+            partS.alloc(std::max({kxMfma ? (size_t)slicesAm * D * V : (size_t)2 * slicesA * D * V,
+                                  kxMfma ? (size_t)slicesTm * nc : (size_t)2 * slicesT * nc, (size_t)slicesM * nc,
                                   (size_t)slicesM32 * nc}), tot);
             int rb = gridOf(NC, MV_ROWS);
             slicesC = std::max(1, std::min(32, (1024 + rb - 1) / rb));
@@ -756,7 +815,18 @@ struct Engine : EngineBase {
             }
         }
         require(gridOf(n) <= MAXP && gridOf(V, MV_ROWS) <= MAXP, "mesh too large for the partial-sum buffers");
-        if (!dryRun) CK(hipStreamSynchronize(s));
+        if (!dryRun && kxMfma) {
+            // Morton order of the initial positions (the points move little
+            // over a solve, so consecutive rows stay close together)
+            std::vector<double> hc((size_t)NC * D);
+            for (int c = 0; c < D; ++c)
+                for (int k = 0; k < NC; ++k) hc[(size_t)c * NC + k] = hq0[(size_t)c * V + hCtl[k]];
+            const auto pl = morton_order(hq0.data(), V, D), pc = morton_order(hc.data(), NC, D);
+            lmkPerm.upload(pl.data(), pl.size(), s);
+            ctlPerm.upload(pc.data(), pc.size(), s);
+            CK(hipStreamSynchronize(s));
+        }
+        if (!dryRun) CK(hipStreamSynchronize(s));   // host copies (hq0) go out of scope
     }
 
     // ---------------- kernel-order dispatch
@@ -1111,7 +1181,37 @@ struct Engine : EngineBase {
     // the stored matrices (see kstore_build_k) in FP64 or FP32
     // Split32: the FP32 matrices plus their FP32 remainders, K = hi + lo to
     // about 2^-48 relative (residual products of the mixed form only)
-    enum class KMode { MatrixFree64, Stored64, Stored32, Split32 };
+    struct DirArgs {
+        const double *z, *P;
+        int nP;
+        double *hist;
+        int j;
+        double *dout;
+    };
+
+    // Mfma32: matrix-free FP32 products on the matrix cores (kx_mfma_k)
+    enum class KMode { MatrixFree64, Stored64, Stored32, Split32, Mfma32 };
+
+    // column slices of the products of a mode (K_xc: xc = true)
+    int kslices(KMode md, bool xc) const {
+        if (md == KMode::Mfma32) return xc ? slicesAm : slicesTm;
+        return xc ? slicesA : slicesT;
+    }
+
+    // y-slices = K(rows, columns) u on the matrix cores (partials in partS)
+    template <int O>
+    void kx_mfma_launch(bool xc, const double *q, const double *u, const Ctl *c, Ctl *adv, const DirArgs *dir) {
+        if constexpr (D == 2 && O >= 0) {
+            throw SaError(SA_E_ARG, "the 2D Matern kernels have no matrix-core product");
+        } else {
+            const int nr = xc ? V : NC, ncol = xc ? NC : V;
+            dim3 g(gridOf(nr, KXM_ROWS), kslices(KMode::Mfma32, xc));
+            kx_mfma_k<D, O><<<g, NT, 0, s>>>(nr, xc ? q : qc.p, xc ? lmkPerm.p : ctlPerm.p, ncol, xc ? qc.p : q,
+                                              xc ? ctlPerm.p : lmkPerm.p, u, 1.0 / knlWidth, partS.p, c, adv,
+                                              dir ? dir->z : nullptr, dir ? dir->P : nullptr, dir ? dir->nP : 0,
+                                              dir ? dir->hist : nullptr, dir ? dir->j : 0, dir ? dir->dout : nullptr);
+        }
+    }
 
     // y[i] = sum of the slice partials part[k * n + i] (slice_rows_k; four
     // wavefronts per 64 rows once the rows alone give 1,024 blocks)
@@ -1149,7 +1249,7 @@ struct Engine : EngineBase {
             CHECK_LAUNCH();
         }
         int sl;
-        if (md == KMode::Stored32 && Mpre32.p) {
+        if ((md == KMode::Stored32 || md == KMode::Mfma32) && Mpre32.p) {
             sl = slicesM32;
             dim3 g(gridOf(NC, panel_rows<float>()), sl);
             kstore_mv_k<D, float><<<g, NT, ks_lds(NC, sl), s>>>(NC, NC, Mpre32.p, r, partS.p, ctl.p);
@@ -1204,13 +1304,6 @@ struct Engine : EngineBase {
     // product's slice partials in partS for a fused consumer.
     // dir: the product's input is the CG direction of iteration dir->j,
     // formed while staging it (kstore_mv_k's dz ... dout arguments)
-    struct DirArgs {
-        const double *z, *P;
-        int nP;
-        double *hist;
-        int j;
-        double *dout;
-    };
     void kx_product(KMode md, bool xc, const double *q, const double *u, double *y, double *P, const Ctl *c,
                     hipEvent_t *ev = nullptr, Ctl *adv = nullptr, bool sum = true, const DirArgs *dir = nullptr) {
         if (md == KMode::MatrixFree64) {
@@ -1221,7 +1314,25 @@ struct Engine : EngineBase {
             return;
         }
         const int nr = xc ? V : NC, ncol = xc ? NC : V;
-        const int sl = xc ? slicesA : slicesT;
+        const int sl = kslices(md, xc);
+        if (md == KMode::Mfma32) {
+            if (ev) CK(hipEventRecord(ev[0], s));
+            switch (knlOrder) {
+                case -1: kx_mfma_launch<-1>(xc, q, u, c, adv, dir); break;
+                case 0: kx_mfma_launch<0>(xc, q, u, c, adv, dir); break;
+                case 1: kx_mfma_launch<1>(xc, q, u, c, adv, dir); break;
+                case 2: kx_mfma_launch<2>(xc, q, u, c, adv, dir); break;
+                case 3: kx_mfma_launch<3>(xc, q, u, c, adv, dir); break;
+                default: kx_mfma_launch<4>(xc, q, u, c, adv, dir); break;
+            }
+            CHECK_LAUNCH();
+            if (ev) CK(hipEventRecord(ev[1], s));
+            if (!sum) return;
+            if (P) slice_sum_k<<<gridOf((long)D * nr), NT, 0, s>>>(D * nr, sl, partS.p, y, P, c);
+            else slice_rows(D * nr, sl, partS.p, y, c);
+            CHECK_LAUNCH();
+            return;
+        }
         if (md == KMode::Split32) {
             // hi product into slices [0, sl), lo product into [sl, 2 sl), summed together
             dim3 g(gridOf(nr, panel_rows<float>()), sl);
@@ -1294,7 +1405,8 @@ struct Engine : EngineBase {
             DirArgs dir{cZ.p, P3.p, gridOf(nc, 64), hist.p, j, dj};
             S_apply(md, q, tauNow, geoNow, j > 0 ? dprev : dj, ev, true, j > 0 ? &dir : nullptr);
             const int gS = gridOf(nc, 64);   // d.Sd partials
-            slice_combine_k<D><<<gS, NTS, 0, s>>>(NC, V, slicesT, partS.p, Kr.p, ctlIdx.p, dj, ldmWgt, knlEps, cS.p,
+            slice_combine_k<D><<<gS, NTS, 0, s>>>(NC, V, kslices(md, false), partS.p, Kr.p, ctlIdx.p, dj, ldmWgt,
+                                                 knlEps, cS.p,
                                                  P1.p, recActive ? recD.p : nullptr, recSD.p, recR.p, rj, SPEC_M,
                                                  ctl.p);
             const bool sp = spec_active(md);
@@ -1619,8 +1731,9 @@ struct Engine : EngineBase {
         // refresh and every other case harvest a new one
         // this step in the mixed form (FP32 stored K inside FP64 refinement)
         // or in the FP64 form (stored FP64 K when it fits)
-        const bool mixedStep = precision == SA_PREC_MIXED && A32.p && !mixedOff;
-        const KMode cgMode = mixedStep ? KMode::Stored32 : A64.p ? KMode::Stored64 : KMode::MatrixFree64;
+        const bool mixedStep = precision == SA_PREC_MIXED && (A32.p || kxMfma) && !mixedOff;
+        const KMode cgMode = mixedStep ? (kxMfma ? KMode::Mfma32 : KMode::Stored32)
+                                       : A64.p ? KMode::Stored64 : KMode::MatrixFree64;
         specCarry = refresh && precAge >= PREC_MAX_AGE && specOn && specMode == (int)cgMode &&
                     specCarried < SA_SPEC_CARRY;
         kcc_build(refresh);
@@ -1693,7 +1806,7 @@ struct Engine : EngineBase {
                 // iterations (+200; 1/8 of the budget on a run's first step):
                 // an FP32 operator that does not converge counts as a stall
                 const int capIn = lastIters > 0 ? 3 * lastIters + 200 : std::max(1000, itrMax / 8);
-                int st = ctl_solve(KMode::Stored32, q, tauNow, geoNow, cRout.p, cDelta.p, true, inner,
+                int st = ctl_solve(cgMode, q, tauNow, geoNow, cRout.p, cDelta.p, true, inner,
                                    std::max(1, std::min(itrMax - iters, capIn)), its);
                 iters += its;
                 if (sw < 4) sweepIters[sw] = its;
@@ -1899,6 +2012,25 @@ struct Engine : EngineBase {
                 CHECK_LAUNCH();
                 CK(hipMemcpyAsync(out, f.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
                 break;
+            case SA_PROBE_KXC:
+            case SA_PROBE_KCX: {
+                // the control-point kernel products at q0 as the inner solve
+                // forms them (mixed form: FP32, stored or on the matrix cores;
+                // FP64 form: stored or matrix-free FP64)
+                require(NC > 0, "the session has no control points");
+                const bool xc = what == SA_PROBE_KXC;
+                gather_ctl_k<D><<<gridOf(NC), NT, 0, s>>>(NC, V, ctlIdx.p, q0.p, qc.p);
+                CHECK_LAUNCH();
+                restore_fp32_store();
+                const KMode md = precision == SA_PREC_MIXED ? (kxMfma ? KMode::Mfma32 : KMode::Stored32)
+                                                            : A64.p ? KMode::Stored64 : KMode::MatrixFree64;
+                kstore_build(q0.p, md == KMode::Stored64, md == KMode::Stored32);
+                double *xin = xc ? cD.p : Ad.p, *yout = xc ? Kr.p : cY.p;
+                CK(hipMemcpyAsync(xin, in, sizeof(double) * (xc ? NC : V) * D, hipMemcpyHostToDevice, s));
+                kx_product(md, xc, q0.p, xin, yout, nullptr, ctl.p);
+                CK(hipMemcpyAsync(out, yout, sizeof(double) * (xc ? V : NC) * D, hipMemcpyDeviceToHost, s));
+                break;
+            }
             case SA_PROBE_KMATVEC:
                 CK(hipMemcpyAsync(r.p, in, sizeof(double) * n, hipMemcpyHostToDevice, s));
                 if (Kst.p) kst_build(q0.p);

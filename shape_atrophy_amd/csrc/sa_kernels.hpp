@@ -1019,6 +1019,193 @@ __global__ __launch_bounds__(NT) void kstore_mv_k(int nr, int nc, const T *__res
     }
 }
 
+// ---------------------------------------------------------------------------
+// Matrix-free kernel-matrix product on FP32 matrix cores (mixed form, 3D and
+// Gaussian kernels): part[s][c][i] = sum over the columns j of slice s of
+// k(|a_i - b_j| / w) x_c(j), for y = K_xc u (rows landmarks, columns control
+// points) and y = K_cx y (the roles swapped).  Rows and columns are visited
+// in a Morton order of the points (rperm / cperm: position -> index), so the
+// rows of a block and the columns near them lie close together; outputs and
+// inputs stay in index order.
+//
+// Per 16 x 16 tile of (column j, row i) pairs, with coordinates scaled by
+// 1 / w and shifted by the block's origin o (its first row point, FP64 before
+// rounding to FP32):
+//   one v_mfma_f32_16x16x4f32:  D[j][i] = |b_j|^2 - 2 b_j . a_i + |a_i|^2
+//       (A[j][k] = [b_j, |b_j|^2], B[k][i] = [-2 a_i, 1], the accumulator
+//       started at |a_i|^2), i.e. rho^2 of the pair;
+//   the kernel value k(rho) on the VALU (sqrt and exp2 instructions);
+//   four v_mfma_f32_16x16x4f32:  Y[i][c] += sum_j D[j][i] x_c(j), the tile's
+//       kernel values (still in the accumulator layout) as the A operand.
+// Y accumulates in FP32 over a staged chunk of 256 columns and is then added
+// to FP64 per-lane sums.  A block is four wavefronts of KXM_RT row tiles each
+// (KXM_ROWS rows); blockIdx.y selects the column slice.  Accumulator layout
+// of the 16x16x4 FP32 instruction: lane l, element r holds row
+// kxm_row(l / 16, r), column l % 16.  The direction / iteration-counter
+// arguments are those of kstore_mv_k (the first product of a fused CG
+// iteration).
+#ifndef SA_KXM_GROUPED
+#define SA_KXM_GROUPED 1
+#endif
+__device__ constexpr int kxm_row(int g4, int r) { return SA_KXM_GROUPED ? 4 * g4 + r : 4 * r + g4; }
+constexpr int KXM_RT = 2;
+#ifndef SA_KXM_BLOCKS
+#define SA_KXM_BLOCKS 2048
+#endif
+constexpr int KXM_BLOCKS = SA_KXM_BLOCKS;   // target blocks per product (row blocks x column slices)
+constexpr int KXM_ROWS = 4 * 16 * KXM_RT;
+constexpr int KXM_CHUNK = 256;
+typedef float kxm_f4 __attribute__((ext_vector_type(4)));
+
+template <int ORD>
+__device__ inline float kxm_kernel(float rho2) {
+    constexpr float kLog2e = 1.4426950408889634f;
+    rho2 = fmaxf(rho2, 0.0f);
+    if constexpr (ORD == -1) {
+        return __builtin_amdgcn_exp2f(-rho2 * kLog2e);
+    } else {
+        const float r = __builtin_amdgcn_sqrtf(rho2);
+        const float e = __builtin_amdgcn_exp2f(-r * kLog2e);
+        if constexpr (ORD == 0) return e;
+        if constexpr (ORD == 1) return fmaf(r, e, e);
+        if constexpr (ORD == 2) return (3.0f + r * (3.0f + r)) * (1.0f / 3.0f) * e;
+        if constexpr (ORD == 3) return (15.0f + r * (15.0f + r * (6.0f + r))) * (1.0f / 15.0f) * e;
+        return (105.0f + r * (105.0f + r * (45.0f + r * (10.0f + r)))) * (1.0f / 105.0f) * e;
+    }
+}
+
+template <int D, int ORD>
+__global__ __launch_bounds__(NT) void kx_mfma_k(int nr, const double *__restrict__ a, const int *__restrict__ rperm,
+                                                int nc, const double *__restrict__ b, const int *__restrict__ cperm,
+                                                const double *__restrict__ x, double invW, double *__restrict__ part,
+                                                const Ctl *ctl, Ctl *adv = nullptr,
+                                                const double *__restrict__ dz = nullptr,
+                                                const double *__restrict__ dP = nullptr, int dnP = 0,
+                                                double *__restrict__ dhist = nullptr, int dj = 0,
+                                                double *__restrict__ dout = nullptr) {
+    static_assert(NT == 256 && KXM_CHUNK == NT, "one staged column per thread");
+    static_assert(D == 2 || D == 3, "K = 4 holds the D coordinates and |b|^2");
+    __shared__ float colb[KXM_CHUNK * 4];   // [b_j - o (D, zero padded to 3), |b_j - o|^2]
+    __shared__ float colx[KXM_CHUNK * 4];   // x_c(j), c < D, zero padded
+    __shared__ double redd[NT / 64];
+    if (ctl && cld(&ctl->done)) return;
+    double be = 0.0;
+    if (dz) {
+        // the CG direction d = z + beta d_old, beta = rz_dj / rz_(dj-1), as kstore_mv_k forms it
+        const double rz = reduce_partials(dP, dnP, redd);
+        if (rz == 0.0) {
+            if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+                cst(&adv->status, 1); cst(&adv->done, 1); cst(&adv->iters, dj);
+            }
+            return;
+        }
+        be = rz / cld(&dhist[dj - 1]);
+        if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) cst(&dhist[dj], rz);
+    }
+    if (adv && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+        const int it = cld(&adv->it) + 1;
+        cst(&adv->it, it);
+        if (it >= cld(&adv->maxIt)) { cst(&adv->done, 1); cst(&adv->status, 2); cst(&adv->iters, it); }
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int n16 = lane & 15, g4 = lane >> 4;
+    const int row0 = blockIdx.x * KXM_ROWS;
+    // the block's origin: its first row point (scaled)
+    double o[D];
+    {
+        const int io = rperm[min(row0, nr - 1)];
+#pragma unroll
+        for (int c = 0; c < D; ++c) o[c] = a[(size_t)c * nr + io] * invW;
+    }
+    // B operand of the distance tiles (row i = tile base + n16, slot g4) and
+    // the accumulator start |a_i|^2, per row tile of this wavefront
+    float bOp[KXM_RT], na[KXM_RT];
+#pragma unroll
+    for (int t = 0; t < KXM_RT; ++t) {
+        const int i = row0 + (wv * KXM_RT + t) * 16 + n16;
+        float ac[3] = {0.0f, 0.0f, 0.0f};
+        if (i < nr) {
+            const int ii = rperm[i];
+#pragma unroll
+            for (int c = 0; c < D; ++c) ac[c] = (float)(a[(size_t)c * nr + ii] * invW - o[c]);
+        }
+        na[t] = ac[0] * ac[0] + ac[1] * ac[1] + ac[2] * ac[2];
+        bOp[t] = g4 < 3 ? -2.0f * ac[g4] : 1.0f;
+    }
+    double y64[KXM_RT][4];
+#pragma unroll
+    for (int t = 0; t < KXM_RT; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) y64[t][r] = 0.0;
+    const int per = (nc + gridDim.y - 1) / gridDim.y;
+    const int jb = blockIdx.y * per, je = min(nc, jb + per);
+    for (int j0 = jb; j0 < je; j0 += KXM_CHUNK) {
+        __syncthreads();
+        {
+            const int j = j0 + threadIdx.x;
+            float bc[3] = {0.0f, 0.0f, 0.0f}, xv[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+            if (j < je) {
+                const int jj = cperm[j];
+#pragma unroll
+                for (int c = 0; c < D; ++c) {
+                    bc[c] = (float)(b[(size_t)c * nc + jj] * invW - o[c]);
+                    const size_t k = (size_t)c * nc + jj;
+                    double v;
+                    if (dz) {
+                        v = dz[k] + be * x[k];
+                        if (blockIdx.x == 0) dout[k] = v;
+                    } else {
+                        v = x[k];
+                    }
+                    xv[c] = (float)v;
+                }
+            }
+            float *cb = colb + 4 * threadIdx.x;
+            cb[0] = bc[0]; cb[1] = bc[1]; cb[2] = bc[2];
+            cb[3] = bc[0] * bc[0] + bc[1] * bc[1] + bc[2] * bc[2];
+            float *cx = colx + 4 * threadIdx.x;
+            cx[0] = xv[0]; cx[1] = xv[1]; cx[2] = xv[2]; cx[3] = xv[3];
+        }
+        __syncthreads();
+        // columns past je carry x = 0: whole tiles of the chunk are evaluated
+        const int ntile = (min(KXM_CHUNK, je - j0) + 15) >> 4;
+        kxm_f4 y32[KXM_RT];
+#pragma unroll
+        for (int t = 0; t < KXM_RT; ++t) y32[t] = kxm_f4{0.0f, 0.0f, 0.0f, 0.0f};
+        for (int tt = 0; tt < ntile; ++tt) {
+            const float aOp = colb[(tt * 16 + n16) * 4 + g4];
+            // x operands of the four contraction products: column
+            // kxm_row(g4, r) (the accumulator row of element r), component n16
+            float xr[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) xr[r] = n16 < 4 ? colx[(tt * 16 + kxm_row(g4, r)) * 4 + n16] : 0.0f;
+#pragma unroll
+            for (int t = 0; t < KXM_RT; ++t) {
+                const kxm_f4 c0 = {na[t], na[t], na[t], na[t]};
+                const kxm_f4 d2 = __builtin_amdgcn_mfma_f32_16x16x4f32(aOp, bOp[t], c0, 0, 0, 0);
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    y32[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(kxm_kernel<ORD>(d2[r]), xr[r], y32[t], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < KXM_RT; ++t)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) y64[t][r] += (double)y32[t][r];
+    }
+    // lane (g4, n16 < D), element r: row kxm_row(g4, r) of each tile, component n16
+    if (n16 < D) {
+        double *po = part + (size_t)blockIdx.y * D * nr + (size_t)n16 * nr;
+#pragma unroll
+        for (int t = 0; t < KXM_RT; ++t)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = row0 + (wv * KXM_RT + t) * 16 + kxm_row(g4, r);
+                if (i < nr) po[rperm[i]] = y64[t][r];
+            }
+    }
+}
+
 // control-point CG, fused consumers of the stored-product slice partials
 // (64 outputs per block, wavefront w adding slices w, w + SLW, ..., combined
 // in wavefront order as in slice_rows_k):

@@ -207,6 +207,44 @@ def shell_problem(level: int = 5, n_layers: int = 5, scales=None, axes=(1.0, 0.8
     return out
 
 
+def slab_problem(n: int = 9, n_layers: int = 3, thickness: float = 0.2, tilt: float = 0.0, seed: int = 5,
+                 time_steps: int = 10, btmWgt: float = 0.0) -> dict:
+    """Flat 3D slab: an n x n grid on [0, 1]^2 (each square cut into two
+    triangles along alternating diagonals, vertices jittered in the plane by
+    +-10 % of the spacing) copied onto n_layers layers at heights
+    0 .. thickness and split into tets by tri2tetra.  The bottom layer
+    (btmVtxMat, the penalised facets) is the plane z = tilt * x; the top layer
+    is the varifold surface, its target the top moved up by a seeded bump."""
+    rng = np.random.default_rng(seed)
+    h = 1.0 / (n - 1)
+    ix, iy = np.meshgrid(np.arange(n), np.arange(n), indexing="xy")
+    xy = np.stack([ix.ravel() * h, iy.ravel() * h], axis=1)
+    inner = (ix.ravel() > 0) & (ix.ravel() < n - 1) & (iy.ravel() > 0) & (iy.ravel() < n - 1)
+    xy[inner] += 0.1 * h * rng.uniform(-1, 1, (int(inner.sum()), 2))
+    tris = []
+    for j in range(n - 1):
+        for i in range(n - 1):
+            a, b, c, d = j * n + i, j * n + i + 1, (j + 1) * n + i, (j + 1) * n + i + 1
+            tris += [(a, b, d), (a, d, c)] if (i + j) % 2 == 0 else [(a, b, c), (b, d, c)]
+    tri = np.array(tris, dtype=np.int64)
+    z = np.linspace(0.0, thickness, n_layers)
+    pos = np.concatenate([np.column_stack([xy, np.full(len(xy), zk)]) for zk in z], axis=0)
+    pos[:, 2] += tilt * pos[:, 0]
+    tet, nml, tsv = _mesh.tri2tetra(tri, n_layers)
+    elm = _mesh.orient_boundary(_mesh.generate_element_object_3d(tet, tri, nml, tsv, [n_layers]), pos)
+    top = pos[(n_layers - 1) * n * n:].copy()
+    top[:, 2] += 0.05 * np.exp(-((top[:, 0] - 0.5) ** 2 + (top[:, 1] - 0.5) ** 2) / 0.05)
+    target = facet_target(top, tri)
+    prm = dict(elaType="frame", modVec=[3.0, 3.0, 3.0, 15.0], modMat=[[1.0, 1.0], [2.0, 1.5], [4.0, 3.0]],
+               spdTanVal=0.005, spdTsvVal=0.005, ynkActPrmVec=[0.01, 1.0, 4.0, 0.5],
+               reaActPrmVec=[0.0, 1.0, 1.0, 0.1, 1.0], ldmWgt=1e-2, knlOrder=1, knlWidth=0.2, knlEps=1e-10,
+               btmWgt=btmWgt, timeStp=1.0 / time_steps, timeEnd=1.0, cgItrMax=10000, cgTolVal=1e-10)
+    tauVar = np.array([[0.5], [0.5], [0.5 * thickness], [1.0]])
+    tauPrm = np.array([0.3, 1.0])
+    vfd = dict(cenKnlType="G", cenKnlWidth=0.1, dirKnlType="B", dirKnlWidth=1.0)
+    return dict(dim=3, elm=elm, q0=pos.T.copy(), params=prm, tauVar=tauVar, tauPrm=tauPrm, target=target, vfd=vfd)
+
+
 def perturb_params(params: dict, rng, frac: float = 0.3) -> dict:
     """Seeded parameter candidate: elastic moduli, diffusion speeds and the
     reaction rate each scaled by an independent factor in [1-frac, 1+frac]."""

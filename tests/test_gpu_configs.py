@@ -1,0 +1,203 @@
+"""Every benchmark configuration at its own size, on the GPU.
+
+C1  2D concentric-ring disk, 2,000 landmarks, 20 steps, kernel on every
+    landmark, FP64: frame (reference) and per-layer isotropic Lame models,
+    against the FP64 oracle run live.
+C3  C1 in the mixed form with the kernel on every landmark as control
+    points, against the oracle's FP64 control-point solve run live.
+C2  the 51,210-landmark shell, 245,760 tets, 5,000 control points: its first
+    two steps in the FP64 and mixed forms against the oracle's (kept in
+    tests/golden/c2_oracle.npz, written by make_c2_oracle.py); the full
+    50-step solve in both forms, whose objectives must agree to 1e-6.
+C4  batched meshes drawn from the batch configuration's size ranges (2D
+    1k-20k, 3D 5k-60k landmarks, seeded parameters), 20 steps.
+C5  a Poisson stream over a pool drawn from the served configuration's
+    range (1k-200k landmarks, 2D and 3D), per-request parameters, 10 steps.
+For C4 and C5 (property checks): every request succeeds; each result is
+the same alone and in the mix within 1e-9 relative in the objective (the
+FP64 form is bitwise equal in most runs; intermittent differences of
+~1e-11 under concurrency have been seen, see the design notes); and the
+mixed form, served and alone, matches the FP64 form to 1e-6 in the
+objective (the headline tolerance).
+
+Stated tolerances against the oracle (relative): C1 displacement,
+concentration, objective and gradient 1e-9; C3 1e-7 (the control-point
+system on every landmark is far worse conditioned than the reference
+system, so the CG tolerance 1e-10 leaves ~1e-8 in the solution); C2 1e-7 for
+displacement, concentration and gradient, 1e-8 for the objective.
+"""
+import importlib.util
+import os
+import time
+
+import numpy as np
+import pytest
+
+from shape_atrophy_amd import api, workloads as W
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def rel(a, b):
+    a = np.asarray(a, dtype=float)
+    b = np.asarray(b, dtype=float)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def say(msg):
+    print(f"[gpu] {msg}", flush=True)
+
+
+def errors(r, o, q0):
+    disp = np.linalg.norm(o["lmkStk"][-1] - q0)
+    return {"displacement": float(np.linalg.norm(r["lmkStk"][-1] - o["lmkStk"][-1]) / disp),
+            "concentration": rel(r["tauMat"], o["tauMat"]),
+            "objective": abs(r["obj"] - o["obj"]) / abs(o["obj"]),
+            "gradient": rel(r["grad"], o["grad"])}
+
+
+@pytest.mark.parametrize("ela", ["frame", "isotropic"])
+def test_c1_full_size(ela, oracle):
+    p = W.ring_problem(elaType=ela)
+    assert p["q0"].shape[1] == 2000 and api.time_num(p["params"]["timeStp"], p["params"]["timeEnd"]) == 21
+    say(f"C1 {ela}: V=2000, 20 steps")
+    t0 = time.perf_counter()
+    r = api.Session(p).run(p["tauVar"][:, 0], p["tauPrm"], want_traj=True, want_grad=True)
+    tg = time.perf_counter() - t0
+    o = oracle.objfcn(p, with_grad=True)
+    assert r["status"] == 0 and o["status"] == 0
+    e = errors(r, o, p["q0"])
+    say(f"C1 {ela}: gpu {tg:.2f}s, pcg {r['pcg_iters']} (oracle {int(o['pcg_iters'].sum())}), errors {e}")
+    for k, v in e.items():
+        assert v < 1e-9, (k, v)
+
+
+def test_c3_mixed_full_size(oracle):
+    p = W.ring_problem()
+    V = p["q0"].shape[1]
+    p["ctlIdx"] = np.arange(V, dtype=np.int32)
+    say("C3: V=2000, control points = landmarks, mixed")
+    r = api.Session(p, precision="mixed").run(p["tauVar"][:, 0], p["tauPrm"], want_traj=True, want_grad=True)
+    o = oracle.objfcn(p, with_grad=True)
+    assert r["status"] == 0 and o["status"] == 0
+    e = errors(r, o, p["q0"])
+    say(f"C3: pcg {r['pcg_iters']} sweeps {r['refine']} fp64 steps {r['fp64_steps']}, errors {e}")
+    for k, v in e.items():
+        assert v < 1e-7, (k, v)
+
+
+def c2_fixture():
+    spec = importlib.util.spec_from_file_location("make_c2_oracle", os.path.join(HERE, "golden", "make_c2_oracle.py"))
+    mk = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mk)
+    fx = np.load(os.path.join(HERE, "golden", "c2_oracle.npz"))
+    p = mk.workload()
+    assert str(fx["fingerprint"]) == mk.fingerprint(p), "c2_oracle.npz was written for other inputs"
+    return mk, fx, p
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_c2_full_size_first_steps(precision):
+    mk, fx, p = c2_fixture()
+    say(f"C2 {precision}: V=51210, E=245760, C=5000, {mk.TIME_NUM - 1} steps")
+    s = api.Session(p, precision=precision)
+    r = s.run(p["tauVar"][:, 0], p["tauPrm"], want_traj=True, want_grad=True)
+    s.close()
+    assert r["status"] == 0
+    idx = fx["idx"]
+    q0, qT, tauT = p["q0"], r["lmkStk"][-1], r["tauMat"][-1]
+    e = {"displacement": float(np.linalg.norm(qT[:, idx] - fx["qT"]) / np.linalg.norm(fx["qT"] - q0[:, idx])),
+         "concentration": rel(tauT[idx], fx["tauT"]),
+         "objective": abs(r["obj"] - float(fx["obj"])) / abs(float(fx["obj"])),
+         "gradient": rel(r["grad"][:, idx], fx["grad"])}
+    ck = mk.checks(q0, qT, tauT, r["grad"])
+    ckrel = np.abs(ck - fx["checks"]) / np.abs(fx["checks"])
+    say(f"C2 {precision}: pcg {r['pcg_iters']} (oracle {fx['pcg_iters'].tolist()}), sweeps {r['refine']}, "
+        f"errors {e}, full-vector checks {ckrel.tolist()}")
+    assert e["objective"] < 1e-8, e
+    for k in ("displacement", "concentration", "gradient"):
+        assert e[k] < 1e-7, (k, e[k])
+    assert np.all(ckrel < 1e-7), ckrel
+
+
+def test_c2_fifty_steps_mixed_matches_fp64():
+    p = W.shell_problem(level=5, n_layers=5, time_steps=50, n_ctl=5000)
+    out = {}
+    for prec in ("fp64", "mixed"):
+        say(f"C2 {prec}: 50 steps")
+        s = api.Session(p, precision=prec)
+        out[prec] = s.run(p["tauVar"][:, 0], p["tauPrm"])
+        s.close()
+        say(f"C2 {prec}: {out[prec]['steps']} steps {out[prec]['seconds']:.2f}s obj {out[prec]['obj']:.15e}")
+        assert out[prec]["status"] == 0 and out[prec]["steps"] == 50
+    d = abs(out["mixed"]["obj"] - out["fp64"]["obj"]) / abs(out["fp64"]["obj"])
+    say(f"C2: mixed vs FP64 objective {d:.2e}")
+    assert d <= 1e-6
+
+
+def served_property_check(meshes, reqs, rate=None, workers=4):
+    """FP64 alone, FP64 served, mixed served and mixed alone: all succeed, the
+    FP64 served results equal the FP64 alone ones bitwise, the mixed ones are
+    within 1e-6 of FP64."""
+    def alone(precision):
+        res = []
+        for m, tv, prm in reqs:
+            s = api.Session(meshes[m], precision=precision)
+            s.set_params(prm)
+            r = s.run(tv, meshes[m]["tauPrm"])
+            s.close()
+            assert r["status"] == 0, (precision, m, r["fail_reason"])
+            res.append(r["obj"])
+        return res
+
+    def served(precision):
+        srv = api.Server(workers=workers)
+        ids = [srv.add_mesh(p, precision) for p in meshes]
+        rng = np.random.default_rng(5)
+        t_next = time.perf_counter()
+        rids = []
+        for m, tv, prm in reqs:
+            if rate:
+                t_next += rng.exponential(1.0 / rate)
+                time.sleep(max(0.0, t_next - time.perf_counter()))
+            rids.append(srv.submit(ids[m], tv, meshes[m]["tauPrm"], prm))
+        got = [srv.wait(r) for r in rids]
+        srv.close()
+        for g in got:
+            assert g["rc"] == 0 and g["status"] == 0, g
+        return [g["obj"] for g in got], got
+
+    a64 = alone("fp64")
+    s64, g64 = served("fp64")
+    smx, gmx = served("mixed")
+    amx = alone("mixed")
+    d64 = max(abs(x - y) / abs(y) for x, y in zip(s64, a64))
+    dev = max(max(abs(x - y) / abs(y) for x, y in zip(smx, a64)), max(abs(x - y) / abs(y) for x, y in zip(amx, a64)))
+    mix_alone = max(abs(x - y) / abs(y) for x, y in zip(smx, amx))
+    say(f"served: FP64 served vs alone {d64:.2e} (bitwise equal: {s64 == a64}), mixed vs FP64 {dev:.2e}, "
+        f"mixed served vs alone {mix_alone:.2e}, fp64 steps {[g['fp64_steps'] for g in gmx]}, "
+        f"factor retries {[g['factor_retries'] for g in g64 + gmx]}")
+    assert d64 <= 1e-9 and mix_alone <= 1e-9
+    assert dev <= 1e-6
+
+
+def test_c4_batched_meshes():
+    meshes = W.batch_problems(n=8, time_steps=20)
+    say(f"C4: {[(p['dim'], p['q0'].shape[1]) for p in meshes]}, 20 steps")
+    reqs = [(k, p["tauVar"][:, 0], p["params"]) for k, p in enumerate(meshes)]
+    served_property_check(meshes, reqs)
+
+
+def test_c5_served_stream():
+    pool = W.batch_problems(n=4, seed=5, time_steps=10, v2d=(1000, 200000), v3d=(1000, 200000), mixed_dims=True)
+    rng = np.random.default_rng(55)
+    reqs = []
+    for k in range(8):
+        m = int(rng.integers(len(pool)))
+        prm = W.perturb_params(pool[m]["params"], rng)
+        tv = pool[m]["tauVar"][:, 0] * (1.0 + 0.05 * rng.standard_normal(pool[m]["tauVar"].shape[0]))
+        reqs.append((m, tv, prm))
+    say(f"C5: pool {[(p['dim'], p['q0'].shape[1]) for p in pool]}, 8 requests, 10 steps, Poisson 4 req/s")
+    served_property_check(pool, reqs, rate=4.0)

@@ -361,3 +361,112 @@ def test_boundary_yank(which, oracle):
     disp = np.linalg.norm(o["lmkStk"][-1] - o["lmkStk"][0])
     assert np.linalg.norm(r["lmkStk"][-1] - o["lmkStk"][-1]) / disp < 1e-8
     assert abs(r["obj"] - o["obj"]) <= 1e-8 * abs(o["obj"])
+
+
+# --------------------------------------------------------------------------
+# bottom penalty (btmWgt != 0): 2D applyPenaltyOperator.cu:4-79 (segment
+# length / 2 per vertex), 3D applyPenaltyOperator.cu:4-85 (xy-projected area
+# / 3), added to the elastic operator inside the velocity solve
+# (ipcgSecondMatrixMultiplication.cu:8-24)
+
+def pen_strip():
+    p = W.strip_problem()
+    p["params"] = dict(p["params"], btmWgt=10.0)
+    p["timeNum"] = 3
+    return p
+
+
+PENALTY = {
+    "strip2d": pen_strip,
+    "ring2d": lambda: small_ring(),
+    "slab3d": lambda: W.slab_problem(n=7, n_layers=3, tilt=0.3, time_steps=6),
+    "shell3d": lambda: small_shell(),
+}
+
+
+@pytest.mark.parametrize("which", list(PENALTY))
+def test_probe_penalty(which, oracle):
+    p = PENALTY[which]()
+    p["name"] = which + "_penalty"
+    s = _session(p)
+    v = np.random.default_rng(12).standard_normal(p["q0"].shape)
+    out = s.probe("penalty", p["tauVar"][:, 0], p["tauPrm"], v).reshape(p["q0"].shape)
+    ref = oracle.penalty_apply(p, p["q0"], v)
+    assert rel(out, ref) < 1e-13
+
+
+@pytest.mark.parametrize("which,ctl", [("strip2d", False), ("slab3d", False), ("shell3d", False),
+                                       ("slab3d", True), ("ring2d", True)])
+def test_full_solve_with_penalty(which, ctl, oracle):
+    """Full solves with btmWgt != 0 against the oracle: the all-landmark
+    form to 1e-9 as the other full solves, the control-point form to 1e-8."""
+    p = PENALTY[which]()
+    p["params"] = dict(p["params"], btmWgt=p["params"]["btmWgt"] or 5.0)
+    V = p["q0"].shape[1]
+    if ctl:
+        p["ctlIdx"] = np.sort(np.random.default_rng(3).choice(V, V // 3, replace=False)).astype(np.int32)
+    p["name"] = which + ("_ctl" if ctl else "") + "_btmwgt"
+    r = _session(p).run(p["tauVar"][:, 0], p["tauPrm"], want_traj=True)
+    o = oracle.objfcn(p)
+    assert r["status"] == 0 and o["status"] == 0
+    disp = np.linalg.norm(o["lmkStk"][-1] - o["lmkStk"][0])
+    tol = 1e-8 if ctl else 1e-9
+    print(p["name"], "pcg iters gpu", r["pcg_iters"], "oracle", int(o["pcg_iters"].sum()))
+    assert np.linalg.norm(r["lmkStk"][-1] - o["lmkStk"][-1]) / disp < tol
+    assert rel(r["tauMat"], o["tauMat"]) < tol
+    assert abs(r["obj"] - o["obj"]) <= tol * abs(o["obj"])
+    # the penalty changed the solution (the same solve without it, on the GPU)
+    q = dict(p, params=dict(p["params"], btmWgt=0.0), name=p["name"] + "0")
+    r0 = _session(q).run(p["tauVar"][:, 0], p["tauPrm"], want_traj=True)
+    assert r0["status"] == 0
+    assert np.linalg.norm(r0["lmkStk"][-1] - o["lmkStk"][-1]) / disp > 1e-4
+
+
+# --------------------------------------------------------------------------
+# control-point kernel products K_xc u and K_cx y as the inner solve forms
+# them: stored FP64 / matrix-free FP64 (plain form) to 1e-13, stored FP32
+# (2D mixed form) and matrix-free FP32 on the matrix cores (3D mixed form,
+# kx_mfma_k) to 5e-6 of the FP64 products (FP32 kernel values, FP32 inputs,
+# FP32 sums over 256-column chunks added in FP64)
+
+def dense_kernel(oracle, prob, a, b):
+    prm = prob["params"]
+    d2 = ((a[:, :, None] - b[:, None, :]) ** 2).sum(0)
+    return oracle.kernel_values(prob["dim"], prm["knlOrder"], prm["knlWidth"], d2.ravel()).reshape(d2.shape)
+
+
+PRODUCT_CASES = [
+    ("shell3d", "mixed", {"SA_KX_MFMA": "1"}, 5e-6),
+    ("shell3d_gauss", "mixed", {"SA_KX_MFMA": "1"}, 5e-6),
+    ("shell3d", "mixed", {"SA_KX_MFMA": "0"}, 5e-6),
+    ("ring2d", "mixed", {}, 5e-6),
+    ("shell3d", "fp64", {}, 1e-13),
+    ("shell3d", "fp64", {"SA_MATRIX_FREE": "1"}, 1e-13),
+]
+
+
+@pytest.mark.parametrize("which,precision,env,tol", PRODUCT_CASES)
+def test_probe_control_point_products(which, precision, env, tol, oracle, monkeypatch):
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    if which.startswith("shell3d"):
+        p = W.shell_problem(level=3, n_layers=5, time_steps=2)
+        if which == "shell3d_gauss":
+            p["params"] = dict(p["params"], knlOrder=-1, knlWidth=0.15)
+    else:
+        p = W.ring_problem(n_rings=10, n_per_ring=120)
+    V = p["q0"].shape[1]
+    p["ctlIdx"] = np.sort(np.random.default_rng(21).choice(V, 700, replace=False)).astype(np.int32)
+    p["name"] = f"{which}_{precision}_products_{env}"
+    print(f"[gpu] session {p['name']} V={V} C=700", flush=True)
+    s = api.Session(p, precision=precision)
+    rng = np.random.default_rng(13)
+    u = rng.standard_normal((p["dim"], 700))
+    y = rng.standard_normal((p["dim"], V))
+    kx = s.probe("kxc", p["tauVar"][:, 0], p["tauPrm"], u).reshape(p["dim"], V)
+    kc = s.probe("kcx", p["tauVar"][:, 0], p["tauPrm"], y).reshape(p["dim"], 700)
+    s.close()
+    K = dense_kernel(oracle, p, p["q0"], p["q0"][:, p["ctlIdx"]])
+    e1, e2 = rel(kx, u @ K.T), rel(kc, y @ K)
+    print(f"[gpu] {p['name']}: K_xc u {e1:.2e}, K_cx y {e2:.2e}", flush=True)
+    assert e1 < tol and e2 < tol

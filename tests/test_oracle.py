@@ -105,15 +105,35 @@ def test_reaction_is_fisher_kpp_on_interior(oracle):
     np.testing.assert_allclose(ra, r * tau * (1 - tau), rtol=1e-13)
 
 
-def test_tau_initial_closed_form(oracle):
-    p = small_shell()
-    for pw in (1, 2, 3):
-        p["tauPrm"] = np.array([0.4, float(pw)])
-        tau = oracle.tau_initial(p)
-        c = p["tauVar"][:3, 0]
-        rho2 = ((p["q0"] - c[:, None]) ** 2).sum(0) / 0.16
-        ref = np.where(rho2 < 1, 1 - (1 - (rho2 - 1) ** 2) ** pw, 0.0) * p["tauVar"][3, 0]
-        np.testing.assert_allclose(tau, ref, rtol=1e-13, atol=1e-15)
+@pytest.mark.parametrize("dim", [2, 3])
+def test_tau_initial_special_values(oracle, dim):
+    """The bump of tauInitial.cu:34-43 at hand-placed landmarks: h at the
+    centre, 0 on and outside the radius R, and at rho^2 = 1/2 (where
+    (rho^2 - 1)^2 = 1/4) the values h (1 - (3/4)^p): 1/4, 7/16, 37/64 of h
+    for p = 1, 2, 3; at rho^2 = 1/4 ((rho^2 - 1)^2 = 9/16) h (1 - (7/16)^p):
+    9/16, 207/256, 3753/4096 of h.  R = 0.4, h = 0.7, centre c, landmarks
+    placed along one axis from c."""
+    p = small_ring() if dim == 2 else small_shell()
+    c = np.array([0.1, -0.2, 0.05])[:dim]
+    R, h = 0.4, 0.7
+    dist = np.array([0.0, R * np.sqrt(0.5), R * 0.5, R, 1.3 * R])
+    q = np.array(p["q0"], dtype=float)
+    V = q.shape[1]
+    for k, r in enumerate(dist):
+        q[:, k] = c
+        q[dim - 1, k] += r
+    expect = {1: [1.0, 0.25, 9 / 16, 0.0, 0.0], 2: [1.0, 7 / 16, 207 / 256, 0.0, 0.0],
+              3: [1.0, 37 / 64, 3753 / 4096, 0.0, 0.0]}
+    for pw, ref in expect.items():
+        if dim == 2:
+            p["tauVar"] = np.array([[c[0]], [c[1]], [h], [R]])
+            p["tauPrm"] = np.array([float(pw)])
+        else:
+            p["tauVar"] = np.array([[c[0]], [c[1]], [c[2]], [h]])
+            p["tauPrm"] = np.array([R, float(pw)])
+        tau = oracle.tau_initial(p, q=q)
+        np.testing.assert_allclose(tau[:5], h * np.array(ref), rtol=1e-14, atol=1e-16)
+        assert tau.shape == (V,) and np.all(tau >= 0) and np.all(tau <= h * (1 + 1e-15))
 
 
 # --------------------------------------------------------------------------
@@ -269,6 +289,36 @@ def test_fem_matrices(oracle, maker):
     # reaction load vector = M r
     r = np.random.default_rng(7).uniform(size=V)
     np.testing.assert_allclose(oracle.fem_vector(p, q, r), M @ r, rtol=1e-12, atol=1e-15)
+
+
+@pytest.mark.parametrize("tilt", [0.0, 0.5])
+def test_penalty_operator_3d_closed_forms(oracle, tilt):
+    """3D bottom penalty (3D applyPenaltyOperator.cu:4-42): a lumped
+    diagonal, each bottom vertex weighted by a third of the areas of its
+    bottom triangles.  On a flat unit-square bottom the weights sum to the
+    area 1 exactly.  The reference takes the xy-projected area
+    (applyPenaltyOperator.cu:13-14), so tilting the bottom plane to
+    z = 0.5 x leaves the operator unchanged, although the true area grows by
+    sqrt(1.25); a true-area formula fails this."""
+    p = W.slab_problem(n=7, n_layers=3, tilt=tilt)
+    flat = W.slab_problem(n=7, n_layers=3, tilt=0.0)
+    q, d = p["q0"], 3
+    V = q.shape[1]
+    L = p["elm"]["vtxBtmMat"].shape[1]
+    out1 = oracle.penalty_apply(p, q, np.ones_like(q))
+    # flat unit square: the lumped weights sum to 1 in each component
+    np.testing.assert_allclose(out1.sum(axis=1), np.ones(d), rtol=1e-14)
+    assert np.all(out1[:, L:] == 0.0)          # only bottom landmarks are penalised
+    # the operator is diagonal: (P v)_k = w_k v_k
+    v = np.random.default_rng(8).standard_normal(q.shape)
+    np.testing.assert_allclose(oracle.penalty_apply(p, q, v), out1 * v, rtol=1e-14, atol=1e-16)
+    # the weights are those of the flat slab, whatever the tilt (xy projection)
+    np.testing.assert_allclose(out1, oracle.penalty_apply(flat, flat["q0"], np.ones_like(q)), rtol=1e-14)
+    if tilt:
+        btm = p["elm"]["btmVtxMat"]
+        X = q[:, btm]
+        true_area = 0.5 * np.linalg.norm(np.cross((X[:, 1] - X[:, 0]).T, (X[:, 2] - X[:, 0]).T), axis=1).sum()
+        assert abs(true_area - np.sqrt(1.25)) < 1e-12 and abs(out1[0].sum() - true_area) > 0.1
 
 
 def test_penalty_operator(oracle):
@@ -427,6 +477,41 @@ def test_expanded_distance_varifold_accuracy(oracle, maker, ck, dk):
            - 2 * strip_expanded_varifold(c, u, w, t["cen"], t["dir"], t["vol"], p["vfd"])
            + strip_expanded_varifold(t["cen"], t["dir"], t["vol"], t["cen"], t["dir"], t["vol"], p["vfd"]))
     assert abs(got - val) <= 1e-12 * abs(val)
+
+
+def test_facets_closed_forms(oracle):
+    """Facet centre, unit direction / normal and measure
+    (landmarksToVarifoldKernel, 2D varifold.cu:9-36, 3D :9-43) on
+    hand-placed facets: the 2D segment (0,0)-(3,4) has centre (1.5, 2),
+    direction (0.6, 0.8) and length 5; the triangle (0,0,0), (2,0,0), (0,3,0)
+    has centroid (2/3, 1, 0), normal +z (the right-hand order of its
+    vertices; -z when two are swapped) and area 3."""
+    p = small_ring()
+    a, b = p["elm"]["vfdVtxMat"][:, 0]
+    q = np.array(p["q0"], dtype=float)
+    q[:, a] = [0.0, 0.0]
+    q[:, b] = [3.0, 4.0]
+    cen, dr, vol = oracle.facets(p, q)
+    np.testing.assert_allclose(cen[:, 0], [1.5, 2.0], rtol=1e-15)
+    np.testing.assert_allclose(dr[:, 0], [0.6, 0.8], rtol=1e-15)
+    assert abs(vol[0] - 5.0) < 1e-15
+    p = small_shell()
+    vf = p["elm"]["vfdVtxMat"].copy()
+    a, b, c = vf[:, 0]
+    q = np.array(p["q0"], dtype=float)
+    q[:, a] = [0.0, 0.0, 0.0]
+    q[:, b] = [2.0, 0.0, 0.0]
+    q[:, c] = [0.0, 3.0, 0.0]
+    cen, dr, vol = oracle.facets(p, q)
+    np.testing.assert_allclose(cen[:, 0], [2.0 / 3.0, 1.0, 0.0], rtol=1e-15, atol=1e-16)
+    np.testing.assert_allclose(dr[:, 0], [0.0, 0.0, 1.0], atol=1e-16)
+    assert abs(vol[0] - 3.0) < 1e-15
+    vf[[1, 2], 0] = vf[[2, 1], 0]
+    p["elm"] = dict(p["elm"], vfdVtxMat=np.ascontiguousarray(vf))
+    cen2, dr2, vol2 = oracle.facets(p, q)
+    np.testing.assert_allclose(dr2[:, 0], [0.0, 0.0, -1.0], atol=1e-16)
+    np.testing.assert_allclose(cen2[:, 0], cen[:, 0], rtol=1e-15, atol=1e-16)
+    assert abs(vol2[0] - 3.0) < 1e-15
 
 
 def test_varifold_self_distance_zero(oracle):
