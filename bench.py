@@ -38,6 +38,8 @@ PEAK_HBM_GBS = 8000.0
 # WRITE_SIZE x 1.0 on MI355X.  Keyed by (config, precision); bytes per launch,
 # mean of the K_xc u and K_cx y launches (profiles/kstore_mv_k_*_c2_*.md).
 TRAFFIC: dict = {("c2", "mixed"): 0.9758e9, ("c2", "fp64"): 2.0645e9}
+TRAFFIC_SOURCE = ("not measured in this run: the counter runs of profiles/kstore_mv_k_float_c2_mixed.md and "
+                  "profiles/kstore_mv_k_double_c2_fp64.md (rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE)")
 
 
 def workload(name: str, steps: int):
@@ -295,19 +297,69 @@ def mode_fit(args, api, W):
     }
 
 
+def mode_products(args, api, W):
+    """The control-point kernel products of the mixed form's inner solve in
+    their two implementations on one workload: stored FP32 matrices
+    (kstore_mv_k) and matrix-free on the matrix cores (kx_mfma_k,
+    SA_KX_MFMA=1).  Relative error of K_xc u and K_cx y against the FP64
+    products, and a solve of --steps steps in each (CG iterations, the
+    average product time from the kernel timer, steps/s)."""
+    steps = args.steps if args.steps is not None else 10
+    prob, desc = workload(args.config, steps)
+    d, V = prob["q0"].shape
+    nc = len(prob["ctlIdx"])
+    rng = np.random.default_rng(1)
+    u, y = rng.standard_normal((d, nc)), rng.standard_normal((d, V))
+    tv, tp = prob["tauVar"][:, 0], prob["tauPrm"]
+    s64 = api.Session(prob, precision="fp64")
+    ref = (s64.probe("kxc", tv, tp, u).reshape(d, V), s64.probe("kcx", tv, tp, y).reshape(d, nc))
+    r64 = s64.run(tv, tp)
+    s64.close()
+    rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    out = {}
+    for name, env in (("stored_fp32", "0"), ("matrix_cores", "1")):
+        os.environ["SA_KX_MFMA"] = env
+        s = api.Session(prob, precision="mixed")
+        kx = s.probe("kxc", tv, tp, u).reshape(d, V)
+        kc = s.probe("kcx", tv, tp, y).reshape(d, nc)
+        s.run(tv, tp)                      # warm-up
+        s.set_profiling(True)
+        r = s.run(tv, tp)
+        s.close()
+        out[name] = {"kxc_rel_err": rel(kx, ref[0]), "kcx_rel_err": rel(kc, ref[1]), "steps": r["steps"],
+                     "seconds": round(r["seconds"], 4), "steps_per_s": round(r["steps"] / r["seconds"], 3),
+                     "pcg_iters": r["pcg_iters"], "refine_sweeps": r["refine"],
+                     "product_avg_us": round(1e3 * r["kmatvec_ms"] / max(r["kmatvec_count"], 1), 1),
+                     "obj_rel_err_vs_fp64": abs(r["obj"] - r64["obj"]) / abs(r64["obj"])}
+        print(f"[bench] {name}: {out[name]}", file=sys.stderr, flush=True)
+    os.environ.pop("SA_KX_MFMA", None)
+    return {"metric": "mixed-form inner kernel products: stored FP32 vs matrix cores",
+            "value": out["matrix_cores"]["product_avg_us"], "unit": "us per product (matrix cores)",
+            "n_gpus": 1, "steps": steps, "warmup": 1, "ms_per_step": None, "higher_is_better": False,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64+f32", "data": "synthetic",
+            "config": {"workload": desc, "landmarks": V, "control_points": nc}, "fp64": {"pcg_iters": r64["pcg_iters"]},
+            **out}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--repeats", type=int, default=1)
+    ap.add_argument("--repeats", type=int, default=1, help="solves of --steps steps in the timed region")
+    ap.add_argument("--latency-solves", type=int, default=10,
+                    help="further solves (kernel timer off) whose wall times give the p50/p99 solve latency")
+    ap.add_argument("--serve-requests", type=int, default=4,
+                    help="mixed solves of the workload with perturbed tau variables served by 4 workers beside "
+                         "each alone (0: skip)")
+    ap.add_argument("--cpu-c1-solves", type=int, default=5, help="C1 oracle solves whose median the CPU leg reports")
     ap.add_argument("--config", default="c2")
     ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true",
                     help="time without the per-kernel events (graph-captured CG iterations); no roofline")
     ap.add_argument("--cpu-iters", type=int, default=0, help="CG iterations of the CPU sample (0: sized to the workload)")
-    ap.add_argument("--mode", default="single", choices=["single", "batch", "serve", "precision", "fit"],
+    ap.add_argument("--mode", default="single", choices=["single", "batch", "serve", "precision", "fit", "products"],
                     help="single: one solve of --config; batch: 64 uneven meshes concurrently; serve: Poisson stream")
     ap.add_argument("--workers", type=int, default=8)
     ap.add_argument("--batch-n", type=int, default=64)
@@ -336,7 +388,7 @@ def main():
     if args.mode != "single":
         from shape_atrophy_amd import workloads as W
         res = {"batch": mode_batch, "serve": mode_serve, "precision": mode_precision,
-               "fit": mode_fit}[args.mode](args, api, W)
+               "fit": mode_fit, "products": mode_products}[args.mode](args, api, W)
         print(json.dumps(res))
         return
     steps = args.steps if args.steps is not None else 50   # the configured solve: 50 time steps
@@ -356,17 +408,28 @@ def main():
               flush=True)
     sess.set_time_num(steps + 1)
     sess.set_profiling(not args.no_profile)
-    lat, runs = [], []
+    runs = []
     t0 = time.perf_counter()
     for r in range(args.repeats):
         out = sess.run(tv, tp)
-        lat.append(out["seconds"])
         runs.append(out)
         print(f"[bench] solve {r}: {out['steps']} steps {out['seconds']:.3f}s obj={out['obj']:.12e} "
               f"pcg={out['pcg_iters']} rd={out['rd_iters']} status={out['status']}", file=sys.stderr, flush=True)
     wall = time.perf_counter() - t0
     total_steps = sum(o["steps"] for o in runs)
     value = total_steps / wall
+    # solve latency: further solves without the kernel timer, each timed alone
+    sess.set_profiling(False)
+    lat, lat_runs = [], []
+    for r in range(args.latency_solves):
+        out = sess.run(tv, tp)
+        lat.append(out["seconds"])
+        lat_runs.append(out)
+    if lat:
+        print(f"[bench] latency: {len(lat)} solves, p50 {percentile(lat, 50):.3f}s p99 {percentile(lat, 99):.3f}s",
+              file=sys.stderr, flush=True)
+    else:
+        lat = [o["seconds"] for o in runs]
 
     # roofline of the dominant kernel: the kernel-matrix product
     kmv_ms = sum(o["kmatvec_ms"] for o in runs)
@@ -388,6 +451,7 @@ def main():
             roof = {"bound": "hbm", "kernel": f"kstore_mv_k<{d}, {'double' if esz == 8 else 'float'}>",
                     "achieved": round(ach, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                     "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": TRAFFIC.get((args.config, args.precision)),
+                    "traffic_source": TRAFFIC_SOURCE,
                     "avg_ms": round(avg, 4), "launches": kmv_n,
                     "work": f"{esz} B x {V} x {nc} stored kernel entries + 8 B x {d} x ({V} + {nc}) vectors per launch"}
         elif 8 * ((V + 127) // 128 * 128) * V <= (48 << 30):
@@ -398,6 +462,7 @@ def main():
             roof = {"bound": "hbm", "kernel": f"kstore_mv_k<{d}, double> + kfin_k (stored K)",
                     "achieved": round(ach, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                     "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": TRAFFIC.get((args.config, args.precision)),
+                    "traffic_source": TRAFFIC_SOURCE,
                     "avg_ms": round(avg, 4), "launches": kmv_n,
                     "work": f"8 B x {V} x {V} stored kernel entries + 8 B x {d} x 3 x {V} vectors per product"}
         else:
@@ -436,32 +501,18 @@ def main():
                         "mfma_frac": round(ach / PEAK_FP64_TFLOPS, 4),
                         "work": "24 MFMA flop per pair (3 x 16x16x4 FP64 per 256 pairs)"})
 
-    cpu = None
-    if not args.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle as O
-        O.build()
-        threads = min(os.cpu_count() or 1, 16)
-        pairs = V * (nc if nc else V)
-        n_it = args.cpu_iters or int(np.clip(5e8 / pairs, 1, 200))
-        smp = O.pcg_sample(prob, n_it, threads=threads)
-        its_per_step = sum(o["pcg_iters"] for o in runs) / max(total_steps, 1)
-        cpu_step = smp["setup_s"] + smp["cg_start_s"] + its_per_step * smp["per_iter_s"]
-        cpu = {"value": round(1.0 / cpu_step, 6), "unit": "steps/s", "cores": threads, "kind": "port",
-               "sample": (f"FP64 oracle on the first step of the same workload: geometry/FEM/yank set-up "
-                          f"{smp['setup_s']:.2f}s, CG start {smp['cg_start_s']:.2f}s, {smp['iters']} CG "
-                          f"iterations at {smp['per_iter_s']:.2f}s each, scaled to the GPU-measured "
-                          f"{its_per_step:.1f} CG iterations per step (reaction-diffusion not included)")}
-        # the operator alone: median of 20 applies on one core, then of 100 on
-        # all the cores above (the second call leaves that thread count set)
-        one = O.time_elastic_apply(prob, reps=20, threads=1)
-        allc = O.time_elastic_apply(prob, reps=100, threads=threads)
-        cpu["operator_apply"] = {"median_ms": round(1e3 * allc, 3), "cores": threads, "reps": 100,
-                                 "single_core_median_ms": round(1e3 * one, 3), "single_core_reps": 20,
-                                 "gpu_avg_ms": op["avg_ms"] if op else None}
+    # served pass: mixed solves of this workload with perturbed tau variables
+    # through the request server (4 workers, one session per request in
+    # flight), each also run alone on the bench session
+    served = None
+    if args.serve_requests > 0:
+        served = serve_pass(api, prob, args, steps, sess)
+
+    cpu = cpu_leg(prob, args, op) if not args.no_cpu else None
 
     # accuracy of the timed form against the plain FP64 GPU form on the same
-    # solve (the FP64 form itself is held to the FP64 oracle by the parity tests)
+    # solve (the FP64 form is held to the FP64 oracle on this workload's first
+    # two steps by tests/test_gpu_configs.py)
     ref64 = None
     if args.precision != "fp64":
         s64 = api.Session(prob, precision="fp64")
@@ -470,15 +521,22 @@ def main():
         o64 = s64.run(tv, tp)
         w64 = time.perf_counter() - t1
         s64.close()
-        ref64 = {"value": round(o64["steps"] / w64, 4), "objective": o64["obj"],
+        ref64 = {"value": round(o64["steps"] / w64, 4), "objective": o64["obj"], "status": o64["status"],
                  "obj_rel_err": abs(runs[-1]["obj"] - o64["obj"]) / abs(o64["obj"]),
                  "pcg_iters_per_step": round(o64["pcg_iters"] / max(o64["steps"], 1), 1)}
         print(f"[bench] fp64 form: {o64['steps']} steps {w64:.3f}s obj={o64['obj']:.12e}", file=sys.stderr,
               flush=True)
+    sess.close()
 
+    # the metric holds only at <= 1e-6 relative error of the objective against
+    # FP64, with every solve completed: otherwise value is 0 (value_unchecked
+    # keeps the throughput)
+    all_runs = runs + lat_runs
+    failed = sum(1 for o in all_runs if o["status"] != 0 or o["steps"] != steps)
+    acc_ok = failed == 0 and (ref64 is None or (ref64["status"] == 0 and ref64["obj_rel_err"] <= 1e-6))
     res = {
         "metric": METRIC,
-        "value": round(value, 4),
+        "value": round(value, 4) if acc_ok else 0.0,
         "unit": "steps/s",
         "n_gpus": 1,
         "steps": total_steps,
@@ -491,8 +549,16 @@ def main():
         "data": "synthetic",
         "config": {"workload": desc, "config": args.config, "landmarks": V, "tets": E, "varifold_facets": F,
                    "control_points": nc, "time_steps_per_solve": steps, "solves": args.repeats,
-                   "precision": args.precision},
-        "latency_s": {"p50": percentile(lat, 50), "p99": percentile(lat, 99)},
+                   "precision": args.precision,
+                   "velocity_preconditioner": preconditioner_name(prob, args.precision),
+                   "elasticity": prob["params"]["elaType"]},
+        "accuracy_ok": acc_ok,
+        "value_unchecked": round(value, 4),
+        "failed_solves": failed,
+        "latency_s": {"p50": percentile(lat, 50), "p99": percentile(lat, 99), "solves": len(lat),
+                      "steps_per_solve": steps},
+        "steps_per_s_without_kernel_timer": (round(sum(o["steps"] for o in lat_runs) / sum(lat), 4)
+                                             if lat_runs else None),
         "pcg_iters_per_step": round(sum(o["pcg_iters"] for o in runs) / max(total_steps, 1), 1),
         "refine_sweeps_per_step": round(sum(o["refine"] for o in runs) / max(total_steps, 1), 2),
         "objective": runs[-1]["obj"],
@@ -501,8 +567,106 @@ def main():
         "varifold": vfd,
         "cpu_baseline": cpu,
         "fp64_form": ref64,
+        "served": served,
     }
     print(json.dumps(res))
+
+
+def preconditioner_name(prob, precision):
+    if prob.get("ctlIdx") is not None and len(prob["ctlIdx"]):
+        return ("control-point CG preconditioned by X^T X = (K_cc + eps I)^-1 (inverted Cholesky factor) "
+                "plus a spectral term of Ritz vectors" + ("; FP32 inner solves in FP64 refinement"
+                                                          if precision == "mixed" else ""))
+    return "K-preconditioned CG on ldmWgt K^-1 + E + btmWgt P (ipcg.cu)"
+
+
+def serve_pass(api, prob, args, steps, sess):
+    """Mixed solves with perturbed tau variables: each alone on the bench
+    session, then all served at once by 4 workers; served steps/s, latency,
+    failures and the largest alone-vs-mix difference of the objective."""
+    rng = np.random.default_rng(77)
+    tp = prob["tauPrm"]
+    tvs = [prob["tauVar"][:, 0] * (1.0 + 0.02 * rng.standard_normal(prob["tauVar"].shape[0]))
+           for _ in range(args.serve_requests)]
+    alone = [sess.run(tv, tp) for tv in tvs]
+    pr = dict(prob, timeNum=steps + 1)
+    srv = api.Server(workers=min(4, args.serve_requests))
+    mid = srv.add_mesh(pr, args.precision)
+    t0 = time.perf_counter()
+    rids = [srv.submit(mid, tv, tp) for tv in tvs]
+    got = [srv.wait(r) for r in rids]
+    wall = time.perf_counter() - t0
+    info = srv.info()
+    srv.close()
+    ok = [k for k, g in enumerate(got) if g["rc"] == 0 and g["status"] == 0 and alone[k]["status"] == 0]
+    diff = max((abs(got[k]["obj"] - alone[k]["obj"]) / abs(alone[k]["obj"]) for k in ok), default=None)
+    lat = [g["t_end"] - g["t_submit"] for g in got]
+    out = {"requests": len(tvs), "workers": min(4, args.serve_requests), "precision": args.precision,
+           "steps_per_s": round(sum(g["steps"] for g in got) / wall, 4),
+           "serial_alone_steps_per_s": round(sum(a["steps"] for a in alone) / sum(a["seconds"] for a in alone), 4),
+           "latency_s": {"p50": percentile(lat, 50), "p99": percentile(lat, 99)},
+           "failed_requests": len(tvs) - len(ok), "max_rel_diff_alone_vs_mix": diff,
+           "factor_retries": sum(g["factor_retries"] for g in got), "device_bytes_reserved": info["mem_used"]}
+    print(f"[bench] served: {out}", file=sys.stderr, flush=True)
+    return out
+
+
+def cpu_info():
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)),
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS"), "model": model}
+
+
+def cpu_leg(prob, args, op):
+    """The FP64 oracle on the host: one complete step of the workload (the
+    first: geometry, kernel matrices and their factorisation, the velocity CG
+    from a zero start to 1e-10, the landmark update and the
+    reaction-diffusion solve) on all the cores this process may use; the C1
+    full solve (20 steps and the objective), median of --cpu-c1-solves; the
+    operator apply alone, median of 100 on all cores and of 20 on one."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    from shape_atrophy_amd import workloads as W
+    O.build()
+    # every core this process may run on, within the OpenMP share the host sets
+    threads = len(os.sched_getaffinity(0))
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+        threads = min(threads, int(os.environ["OMP_NUM_THREADS"]))
+    t0 = time.perf_counter()
+    o = O.objfcn(dict(prob, timeNum=2), with_target=False, threads=threads)
+    t_step = time.perf_counter() - t0
+    print(f"[bench] cpu: one step {t_step:.2f}s on {threads} threads (status {o['status']}, "
+          f"{int(o['pcg_iters'][0])} CG iterations)", file=sys.stderr, flush=True)
+    cpu = {"value": round(1.0 / t_step, 6), "unit": "steps/s", "cores": threads, "kind": "port",
+           "sample": (f"FP64 oracle: one complete first step of the same workload ({int(o['pcg_iters'][0])} CG "
+                      f"iterations to 1e-10 from a zero start, reaction-diffusion included), {t_step:.2f}s "
+                      f"on {threads} threads"),
+           "host": cpu_info(), "step_s": round(t_step, 3), "status": o["status"]}
+    c1 = W.ring_problem()
+    ts = []
+    for _ in range(args.cpu_c1_solves):
+        t1 = time.perf_counter()
+        r1 = O.objfcn(c1, threads=threads)
+        ts.append(time.perf_counter() - t1)
+    if ts:
+        med = float(np.median(ts))
+        cpu["c1_full_solve"] = {"median_s": round(med, 3), "solves": len(ts), "steps_per_s": round(20 / med, 4),
+                                "cores": threads, "status": r1["status"]}
+        print(f"[bench] cpu: C1 solve median {med:.2f}s of {len(ts)}", file=sys.stderr, flush=True)
+    one = O.time_elastic_apply(prob, reps=20, threads=1)
+    allc = O.time_elastic_apply(prob, reps=100, threads=threads)
+    cpu["operator_apply"] = {"median_ms": round(1e3 * allc, 3), "cores": threads, "reps": 100,
+                             "single_core_median_ms": round(1e3 * one, 3), "single_core_reps": 20,
+                             "gpu_avg_ms": op["avg_ms"] if op else None}
+    return cpu
 
 
 if __name__ == "__main__":
