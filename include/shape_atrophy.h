@@ -235,6 +235,27 @@ int sa_server_submit(sa_server *sv, int meshId, const double *tauVarVec, int tau
 int sa_server_wait(sa_server *sv, long long reqId, double *obj, sa_stats *stats, double *times);
 int sa_server_info(sa_server *sv, long long *memUsed, int *sessions, int *queued, int *running);
 
+/* The varifold objective of computeObjfcn (varifold.cu value kernels, dsum.cu)
+   for landmarks, facets and target already in device memory, allocating
+   nothing.  No counterpart in the reference: it is for callers that cannot
+   allocate device memory (code under HIP graph capture, tight serving or
+   fitting loops).  It computes what sa_session_probe(SA_PROBE_VARIFOLD)
+   returns as its objective, with the same kernels, but keeps its
+   intermediate values (facet centres, directions and measures, pair-sum
+   rows) in a fixed device array of the library (8 MiB per device, held
+   until the process ends) instead of a session.
+     lmkMat     device, lmkNum x dim (column-major, as lmkIniMat)
+     vfdVtxMat  device, vfdElmNum x dim int32 (computeObjfcn arg 13)
+     tgt        host struct whose cenPosMat / uniDirMat / elmVolVec are device pointers
+     objDev     device, 1 double: the objective (NaN when a facet index is outside [0, lmkNum))
+     stream     hipStream_t of the launches (NULL: the null stream); the call only enqueues work
+   Limit: (2 dim + 2) vfdElmNum + tgtElmNum + 3 <= 1,048,576, else SA_E_ARG.
+   Every call on a device shares that device's array: calls that overlap on
+   one device (other streams or threads) race and must be serialised by the
+   caller. */
+int sa_varifold_objective_noalloc(int dim, int lmkNum, const double *lmkMat, int vfdElmNum, const int *vfdVtxMat,
+                                  const sa_target *tgt, double *objDev, void *stream);
+
 const char *sa_last_error(void);
 int sa_device_count(void);
 

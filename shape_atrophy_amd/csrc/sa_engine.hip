@@ -2168,6 +2168,63 @@ int guarded(Fn &&fn) {
 }  // namespace
 }  // namespace sa
 
+// Device scratch of sa_varifold_objective_noalloc (facet centres,
+// directions and measures, the pair-sum rows, the two partial sums and the
+// index flag): SA_NOALLOC_SCRATCH doubles, 8 MiB, held on every device the
+// library runs on until the process ends.  In host code `scratch` names a
+// host-side stand-in: kernels get the device address from
+// hipGetSymbolAddress, looked up once per device.
+#define SA_NOALLOC_SCRATCH (1 << 20)
+static __device__ __align__(256) double scratch[SA_NOALLOC_SCRATCH];
+
+namespace sa {
+namespace {
+std::mutex g_scratchMu;
+constexpr int kMaxDevices = 64;
+double *g_scratchDev[kMaxDevices] = {};
+
+// device address of `scratch` on the current device (looked up on the
+// device's first call, with the device current)
+double *scratch_on_device() {
+    int dev = -1;
+    CK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) throw SaError(SA_E_DEVICE, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratchMu);
+    if (!g_scratchDev[dev]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            throw SaError(SA_E_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratchDev[dev] = static_cast<double *>(ptr);
+    }
+    return g_scratchDev[dev];
+}
+
+template <int D>
+void varifold_noalloc(int V, const double *q, int F, const int *vfd, const sa_target &t, double *obj,
+                      hipStream_t st) {
+    double *base = scratch_on_device();
+    double *cen = base, *dir = cen + (size_t)D * F, *vol = dir + (size_t)D * F, *rows = vol + F;
+    double *scal = rows + F + t.tgtElmNum;
+    int *bad = reinterpret_cast<int *>(scal + 2);
+    VfdPrm vp{};
+    vp.cenType = t.cenKnlType;
+    vp.dirType = t.dirKnlType;
+    vp.cenInvW2 = 1.0 / (t.cenKnlWidth * t.cenKnlWidth);
+    vp.dirInvW2 = 1.0 / (t.dirKnlWidth * t.dirKnlWidth);
+    const int nbD = gridOf(F, MV_ROWS), nbT = gridOf(t.tgtElmNum, MV_ROWS);
+    zero_ints_k<<<1, 64, 0, st>>>(bad, 1);
+    facets_k<D><<<gridOf(F), NT, 0, st>>>(F, V, vfd, q, cen, dir, vol, bad);
+    vfd_rows_k<D, false><<<nbD + nbT, NT, 0, st>>>(F, t.tgtElmNum, nbD, cen, dir, vol, t.cenPosMat, t.uniDirMat,
+                                                   t.elmVolVec, vp, rows, nullptr, nullptr);
+    sum_k<<<1, NT, 0, st>>>(rows, F, scal);
+    sum_k<<<1, NT, 0, st>>>(rows + F, t.tgtElmNum, scal + 1);
+    vfd_total_k<<<1, 64, 0, st>>>(scal, bad, obj);
+    CHECK_LAUNCH();
+}
+}  // namespace
+}  // namespace sa
+
 struct sa_session {
     std::unique_ptr<sa::EngineBase> eng;
 };
@@ -2313,6 +2370,30 @@ int sa_compute_objfcn(const sa_mesh *mesh, const sa_target *tgt, const sa_params
                      last ? dqObjMat : nullptr, st);
             if (objVec) objVec[c] = o;
         }
+    });
+}
+
+int sa_varifold_objective_noalloc(int dim, int lmkNum, const double *lmkMat, int vfdElmNum, const int *vfdVtxMat,
+                                  const sa_target *tgt, double *objDev, void *stream) {
+    return sa::guarded([&] {
+        if (dim != 2 && dim != 3) throw sa::SaError(SA_E_ARG, "dim must be 2 or 3");
+        if (!tgt || !lmkMat || !vfdVtxMat || !objDev || !tgt->cenPosMat || !tgt->uniDirMat || !tgt->elmVolVec)
+            throw sa::SaError(SA_E_ARG, "null argument");
+        if (lmkNum <= 0 || vfdElmNum <= 0 || tgt->tgtElmNum <= 0) throw sa::SaError(SA_E_ARG, "empty input");
+        if (tgt->cenKnlType != 'G' && tgt->cenKnlType != 'C')
+            throw sa::SaError(SA_E_ARG, "cenKnlType must be 'G' or 'C'");
+        if (tgt->dirKnlType != 'B' && tgt->dirKnlType != 'L' && tgt->dirKnlType != 'O' && tgt->dirKnlType != 'U')
+            throw sa::SaError(SA_E_ARG, "dirKnlType must be 'B', 'L', 'O' or 'U'");
+        if (!(tgt->cenKnlWidth > 0) || !(tgt->dirKnlWidth > 0)) throw sa::SaError(SA_E_ARG, "kernel widths must be positive");
+        // centres, directions, measures, rows of both sets, 2 sums, the flag
+        const long long need = (2LL * dim + 2) * vfdElmNum + (long long)tgt->tgtElmNum + 3;
+        if (need > SA_NOALLOC_SCRATCH)
+            throw sa::SaError(SA_E_ARG, "facet sets too large for the fixed scratch: (2 dim + 2) vfdElmNum + "
+                                        "tgtElmNum + 3 = " + std::to_string(need) + " doubles > " +
+                                        std::to_string(SA_NOALLOC_SCRATCH));
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        if (dim == 2) sa::varifold_noalloc<2>(lmkNum, lmkMat, vfdElmNum, vfdVtxMat, *tgt, objDev, st);
+        else sa::varifold_noalloc<3>(lmkNum, lmkMat, vfdElmNum, vfdVtxMat, *tgt, objDev, st);
     });
 }
 

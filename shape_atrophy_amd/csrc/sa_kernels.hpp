@@ -1731,11 +1731,30 @@ __global__ __launch_bounds__(NT) void rd_dir_k(int V, const double *__restrict__
 // ---------------------------------------------------------------------------
 // varifold (2D varifold.cu:9-357, 3D :9-370)
 
+// bad (optional): facets whose vertex indices lie outside [0, V) set
+// *bad = 1 and are given zero measure instead of being read (callers whose
+// tables the host never checked)
 template <int D>
 __global__ void facets_k(int F, int V, const int *__restrict__ vfdVtx, const double *__restrict__ q,
-                         double *__restrict__ cen, double *__restrict__ dir, double *__restrict__ vol) {
+                         double *__restrict__ cen, double *__restrict__ dir, double *__restrict__ vol,
+                         int *__restrict__ bad = nullptr) {
     int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= F) return;
+    if (bad) {
+        bool ok = true;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            const int v = vfdVtx[k * F + f];
+            ok = ok && v >= 0 && v < V;
+        }
+        if (!ok) {
+            cst(bad, 1);
+#pragma unroll
+            for (int c = 0; c < D; ++c) { cen[c * F + f] = 0.0; dir[c * F + f] = 0.0; }
+            vol[f] = 0.0;
+            return;
+        }
+    }
     if constexpr (D == 2) {
         int a = vfdVtx[f], b = vfdVtx[F + f];
         double d0 = q[b] - q[a], d1 = q[V + b] - q[V + a];
@@ -2021,6 +2040,17 @@ __global__ void vfd_grad_k(int V, int F, const int *__restrict__ adjTbl, const i
     }
 #pragma unroll
     for (int c = 0; c < D; ++c) grad[c * V + i] = g[c];
+}
+
+// objective = scal[0] + scal[1] (NaN when a facet index was out of range)
+__global__ void vfd_total_k(const double *__restrict__ scal, const int *__restrict__ bad, double *__restrict__ obj) {
+    if (threadIdx.x == 0) obj[0] = cld(bad) ? (double)NAN : scal[0] + scal[1];
+}
+
+// zero n ints (a fill kernel: usable under graph capture, unlike a memset node)
+__global__ void zero_ints_k(int *__restrict__ p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) cst(&p[i], 0);
 }
 
 // single-block fixed-order sum of n values into out[0]
