@@ -355,6 +355,8 @@ def main():
     ap.add_argument("--cpu-c1-solves", type=int, default=5, help="C1 oracle solves whose median the CPU leg reports")
     ap.add_argument("--config", default="c2")
     ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"])
+    ap.add_argument("--vel-prec", default="default", choices=["default", "jacobi"],
+                    help="preconditioner of the velocity CG (jacobi: control-point form only)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true",
                     help="time without the per-kernel events (graph-captured CG iterations); no roofline")
@@ -393,6 +395,7 @@ def main():
         return
     steps = args.steps if args.steps is not None else 50   # the configured solve: 50 time steps
     prob, desc = workload(args.config, steps)
+    prob["params"] = dict(prob["params"], velPrec=args.vel_prec)
     d, V = prob["q0"].shape
     E = prob["elm"]["elmVtxMat"].shape[1]
     F = prob["elm"]["vfdVtxMat"].shape[1]
@@ -573,6 +576,9 @@ def main():
 
 
 def preconditioner_name(prob, precision):
+    if prob["params"].get("velPrec", "default") == "jacobi":
+        return "control-point CG preconditioned by the diagonal of the control-point system (Jacobi)" + (
+            "; FP32 inner solves in FP64 refinement" if precision == "mixed" else "")
     if prob.get("ctlIdx") is not None and len(prob["ctlIdx"]):
         return ("control-point CG preconditioned by X^T X = (K_cc + eps I)^-1 (inverted Cholesky factor) "
                 "plus a spectral term of Ritz vectors" + ("; FP32 inner solves in FP64 refinement"

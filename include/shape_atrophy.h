@@ -106,7 +106,14 @@ typedef struct sa_params {
                                     (computeExternalYank.cu:49-84 | 3D :58-99, gathered through
                                     vtxBdrMat :106-116 | :125-135); needs bdrVtxMat/vtxBdrMat.
                                     0: as the reference runs. */
+    int velPrec;                 /* preconditioner of the velocity CG: SA_VELPREC_DEFAULT (the
+                                    reference's K preconditioning; control points: (K_cc + eps I)^-1
+                                    with a spectral term) | SA_VELPREC_JACOBI (control-point form
+                                    only: the diagonal of the control-point system) */
 } sa_params;
+
+#define SA_VELPREC_DEFAULT 0
+#define SA_VELPREC_JACOBI 1
 
 typedef struct sa_target {
     int tgtElmNum;               /* rows of the target matrices */
@@ -201,7 +208,9 @@ int sa_session_set_params(sa_session *s, const sa_params *prm);
      SA_PROBE_PENALTY   out = P(q0) in, the bottom penalty without btmWgt (lmkNum x dim)
      SA_PROBE_KXC       out = K_xc(q0) in: in ctlNum x dim, out lmkNum x dim  (control-point
      SA_PROBE_KCX       out = K_cx(q0) in: in lmkNum x dim, out ctlNum x dim   kernel products
-                        as the session's inner solve forms them, without knlEps) */
+                        as the session's inner solve forms them, without knlEps)
+     SA_PROBE_JACOBI    out = the diagonal of the control-point system at q0, tau0 (ctlNum x dim;
+                        needs a session with velPrec = SA_VELPREC_JACOBI) */
 #define SA_PROBE_ELASTIC 1
 #define SA_PROBE_KMATVEC 2
 #define SA_PROBE_YANK 3
@@ -211,6 +220,7 @@ int sa_session_set_params(sa_session *s, const sa_params *prm);
 #define SA_PROBE_PENALTY 7
 #define SA_PROBE_KXC 8
 #define SA_PROBE_KCX 9
+#define SA_PROBE_JACOBI 10
 int sa_session_probe(sa_session *s, int what, const double *tauVarVec, int tauVarNum, const double *tauPrmVec,
                      int tauPrmNum, const double *in, double *out);
 /* bytes of device memory a session of this mesh holds */

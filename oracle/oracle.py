@@ -37,7 +37,8 @@ class OParams(C.Structure):
     _fields_ = [("ldmWgt", _D), ("knlOrder", _I), ("knlWidth", _D), ("knlEps", _D), ("modVec", _D * 4),
                 ("elaType", _I), ("nLyr", _I), ("modMat", _PD), ("spdTan", _D), ("spdTsv", _D),
                 ("ynk", _D * 4), ("rea", _D * 5), ("btmWgt", _D), ("timeStp", _D), ("timeNum", _I),
-                ("cgItrMax", _I), ("cgTolSqu", _D), ("pcgInit", _I), ("rdSolve", _I), ("rdTolSqu", _D)]
+                ("cgItrMax", _I), ("cgTolSqu", _D), ("pcgInit", _I), ("rdSolve", _I), ("rdTolSqu", _D),
+                ("velPrec", _I)]
 
 
 class OTarget(C.Structure):
@@ -153,6 +154,7 @@ def make_params(prob: dict, keep: _Keep, pcg_init: int = 1, rd_solve: int = 1, r
     o.pcgInit = pcg_init
     o.rdSolve = rd_solve
     o.rdTolSqu = rd_tol ** 2
+    o.velPrec = 1 if p.get("velPrec", "default") == "jacobi" else 0
     return o
 
 
@@ -329,6 +331,18 @@ def varifold(prob: dict, q, with_grad: bool = False):
     grad = np.zeros_like(q) if with_grad else None
     v = lib().or_varifold(C.byref(m), C.byref(t), _dp(q), _dp(grad) if grad is not None else None)
     return (v, grad) if with_grad else v
+
+
+def jacobi_diag(prob: dict, q, tau) -> np.ndarray:
+    """diag(S) of the control-point system (the Jacobi preconditioner) at
+    positions q and concentration tau, dim x ctlNum."""
+    keep, m = _mesh_call(prob)
+    p = make_params(prob, keep)
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    tau = np.ascontiguousarray(tau, dtype=np.float64)
+    out = np.zeros((prob["dim"], len(prob["ctlIdx"])))
+    lib().or_jacobi_diag(C.byref(m), C.byref(p), _dp(q), _dp(tau), _dp(out))
+    return out
 
 
 def facets(prob: dict, q):

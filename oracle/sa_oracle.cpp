@@ -51,6 +51,7 @@ struct OParams {
     double spdTan, spdTsv; double ynk[4]; double rea[5];
     double btmWgt, timeStp; int timeNum, cgItrMax; double cgTolSqu;
     int pcgInit, rdSolve; double rdTolSqu;
+    int velPrec;   // 1: the control-point CG preconditioned by diag(S) (Jacobi), else (K_cc + eps I)^-1
 };
 
 struct OTarget {
@@ -1075,7 +1076,8 @@ struct Solver {
     // preconditioned with K_cc^{-1} (applied through a dense Cholesky of
     // K_cc).  With the control points = all landmarks this is the reference
     // system multiplied on both sides by K.
-    int ctl_pcg(const double *q, const double *tau, const double *f, double *v, double *a, int *iters) {
+    int ctl_pcg(const double *q, const double *tau, const double *f, double *v, double *a, int *iters,
+                double *jdOut = nullptr) {
         int C = m.ctlNum;
         size_t nc = (size_t)D * C, nv = (size_t)D * V;
         std::vector<double> qc(nc);
@@ -1164,8 +1166,26 @@ struct Solver {
         for (size_t i = 0; i < nc; ++i) r[i] = rhs[i] - Sd[i];
         double rr = dot(r.data(), r.data(), nc);
         double tol = bb * p.cgTolSqu;
+        // Jacobi: diag(S) from S applied to every unit vector (brute force,
+        // small meshes only), no knowledge of how the GPU assembles it
+        std::vector<double> jd;
+        if (p.velPrec == 1) {
+            jd.resize(nc);
+            std::vector<double> e(nc, 0.0), Se(nc);
+            for (size_t k = 0; k < nc; ++k) {
+                e[k] = 1.0;
+                S(e.data(), Se.data());
+                jd[k] = Se[k];
+                e[k] = 0.0;
+            }
+            if (jdOut) std::memcpy(jdOut, jd.data(), sizeof(double) * nc);
+        }
         auto prec = [&](const double *in, double *out) {
             std::memcpy(out, in, sizeof(double) * nc);
+            if (p.velPrec == 1) {
+                for (size_t k = 0; k < nc; ++k) out[k] /= jd[k];
+                return;
+            }
             chol_solve(Lcc, C, out, D);
         };
         int it = 0;
@@ -1441,6 +1461,27 @@ double or_varifold(const OMesh *m, const OTarget *t, const double *q, double *gr
         if (grad) varifold_grad<3>(*m, q, cen, dir, vol, *t, grad);
     }
     return v;
+}
+
+// diag(S) of the control-point system at positions q and concentration tau
+// (the Jacobi preconditioner), by S applied to unit vectors
+void or_jacobi_diag(const OMesh *m, const OParams *p, const double *q, const double *tau, double *out) {
+    OParams pp = *p;
+    pp.velPrec = 1;
+    pp.cgItrMax = 0;
+    const int V = m->lmkNum, D = m->dim;
+    // any nonzero right-hand side: the diagonal is formed before the (empty) iteration
+    std::vector<double> f((size_t)D * V, 1.0), v((size_t)D * V), a((size_t)D * m->ctlNum, 0.0);
+    int its = 0;
+    if (D == 2) {
+        Solver<2> s(*m, pp);
+        compute_frames<2>(*m, q, s.ws.f1, s.ws.f2);
+        s.ctl_pcg(q, tau, f.data(), v.data(), a.data(), &its, out);
+    } else {
+        Solver<3> s(*m, pp);
+        compute_frames<3>(*m, q, s.ws.f1, s.ws.f2);
+        s.ctl_pcg(q, tau, f.data(), v.data(), a.data(), &its, out);
+    }
 }
 
 void or_facets(const OMesh *m, const double *q, double *cen, double *dir, double *vol) {

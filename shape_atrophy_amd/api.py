@@ -52,7 +52,7 @@ class SaParams(C.Structure):
                 ("knlEps", C.c_double), ("modVec", _PD), ("modNum", C.c_int), ("spdTanVal", C.c_double),
                 ("spdTsvVal", C.c_double), ("ynkActPrmVec", _PD), ("reaActPrmVec", _PD), ("btmWgt", C.c_double),
                 ("timeStp", C.c_double), ("timeNum", C.c_int), ("cgItrMax", C.c_int), ("cgTolSqu", C.c_double),
-                ("precision", C.c_int), ("rdTolVal", C.c_double), ("bdrYank", C.c_int)]
+                ("precision", C.c_int), ("rdTolVal", C.c_double), ("bdrYank", C.c_int), ("velPrec", C.c_int)]
 
 
 class SaTarget(C.Structure):
@@ -189,7 +189,7 @@ def make_mesh(elmObj: dict, lmkIni: np.ndarray, keep: _Keep, with_vfd: bool) -> 
 
 def make_params(elaType, ldmWgt, knlOrder, knlWidth, knlEps, modVec, ynkActPrmVec, spdTanVal, spdTsvVal,
                 reaActPrmVec, btmWgt, timeStp, timeEnd, cgItrMax, cgTolVal, keep, precision="fp64",
-                rdTolVal=1e-13, bdrYank=0) -> SaParams:
+                rdTolVal=1e-13, bdrYank=0, velPrec="default") -> SaParams:
     et = str(elaType).lower()
     if et not in ("frame", "isotropic"):
         raise ValueError(f"Unknown elasticity type {elaType}.")
@@ -214,6 +214,7 @@ def make_params(elaType, ldmWgt, knlOrder, knlWidth, knlEps, modVec, ynkActPrmVe
     p.precision = {"fp64": SA_PREC_FP64, "mixed": SA_PREC_MIXED}[precision]
     p.rdTolVal = float(rdTolVal)
     p.bdrYank = int(bool(bdrYank))
+    p.velPrec = {"default": 0, "jacobi": 1}[velPrec]
     return p
 
 
@@ -309,7 +310,7 @@ class Session:
                                p["modMat"] if p["elaType"] == "isotropic" else p["modVec"], p["ynkActPrmVec"],
                                p["spdTanVal"], p["spdTsvVal"], p["reaActPrmVec"], p["btmWgt"], p["timeStp"],
                                p["timeEnd"], p["cgItrMax"], p["cgTolVal"], keep, precision,
-                               p.get("rdTolVal", 1e-13), p.get("bdrYank", 0))
+                               p.get("rdTolVal", 1e-13), p.get("bdrYank", 0), p.get("velPrec", "default"))
         if prob.get("timeNum"):
             self.prm.timeNum = int(prob["timeNum"])
         with_target = with_target and "target" in prob
@@ -356,7 +357,7 @@ class Session:
         _check(lib().sa_session_set_profiling(self._h, 1 if on else 0))
 
     PROBES = {"elastic": 1, "kmatvec": 2, "yank": 3, "fem": 4, "geometry": 5, "varifold": 6, "penalty": 7,
-              "kxc": 8, "kcx": 9}
+              "kxc": 8, "kcx": 9, "jacobi": 10}
 
     def probe(self, what: str, tauVar, tauPrm, x=None) -> np.ndarray:
         """One operator at the initial state (sa_session_probe)."""
@@ -365,7 +366,7 @@ class Session:
         d, V = self.dim, self.V
         E, Z = self.mesh.elmNum, self.mesh.nzrNum
         nc = self.mesh.ctlNum
-        size = {"elastic": d * V, "kmatvec": d * V, "penalty": d * V, "kxc": d * V, "kcx": d * nc,
+        size = {"elastic": d * V, "kmatvec": d * V, "penalty": d * V, "kxc": d * V, "kcx": d * nc, "jacobi": d * nc,
                 "yank": d * V + V, "fem": 2 * Z,
                 "geometry": (d * d + 1 + 2 * d) * E, "varifold": 1 + d * V}[what]
         out = np.zeros(size)
@@ -392,7 +393,7 @@ def params_struct(p: dict, keep, precision: str = "fp64", timeNum: int | None = 
                       p["modMat"] if p["elaType"] == "isotropic" else p["modVec"], p["ynkActPrmVec"],
                       p["spdTanVal"], p["spdTsvVal"], p["reaActPrmVec"], p["btmWgt"], p["timeStp"], p["timeEnd"],
                       p["cgItrMax"], p["cgTolVal"], keep, precision, p.get("rdTolVal", 1e-13),
-                      p.get("bdrYank", 0))
+                      p.get("bdrYank", 0), p.get("velPrec", "default"))
     if timeNum:
         prm.timeNum = int(timeNum)
     return prm

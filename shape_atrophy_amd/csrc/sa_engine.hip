@@ -322,6 +322,11 @@ struct Engine : EngineBase {
     // control points visited in Morton order (position -> index)
     bool kxMfma = false;
     DBuf<int> lmkPerm, ctlPerm;
+    // Jacobi preconditioning of the control-point CG (sa_params.velPrec):
+    // per-element stiffness blocks, their per-nonzero gather, the lumped
+    // penalty weights and diag(S), rebuilt every step; no factorisation
+    bool jacobi = false;
+    DBuf<double> stiffBlk, bcc, pw, dS;
     int slicesAm = 1, slicesTm = 1;
     // boundary yank (sa_params.bdrYank): boundary facets, their vertex
     // adjacency, the facet forces; Bn = 0 when the tables are absent or unusable
@@ -438,6 +443,7 @@ struct Engine : EngineBase {
         require(p.timeNum >= 1 && p.timeNum <= T, "timeNum exceeds the session's allocation");
         require(p.cgItrMax >= 1 && p.cgItrMax <= itrMax0, "cgItrMax exceeds the session's allocation");
         require(!p.bdrYank || Bn > 0, "bdrYank needs consistent bdrVtxMat / vtxBdrMat");
+        require((p.velPrec == SA_VELPREC_JACOBI) == jacobi, "set_params cannot change the velocity preconditioner");
         bdrYank = p.bdrYank != 0;
         mod.assign(p.modVec, p.modVec + p.modNum);
         CK(hipMemcpyAsync(dmod.p, mod.data(), sizeof(double) * mod.size(), hipMemcpyHostToDevice, s));
@@ -518,6 +524,10 @@ struct Engine : EngineBase {
         }
         bdrYank = p.bdrYank != 0;
         require(!bdrYank || Bn > 0, "bdrYank needs consistent bdrVtxMat / vtxBdrMat");
+        require(p.velPrec == SA_VELPREC_DEFAULT || p.velPrec == SA_VELPREC_JACOBI, "unknown velPrec");
+        require(p.velPrec != SA_VELPREC_JACOBI || m.ctlNum > 0,
+                "Jacobi preconditioning applies to the control-point form (ctlNum > 0)");
+        jacobi = p.velPrec == SA_VELPREC_JACOBI;
         for (int v : hElm) require(v >= 0 && v < V, "elmVtxMat index out of range");
         for (int v : hFrm) require(v >= 0 && v < V, "frame stencil index out of range");
         for (int v : hTsv) require(v >= 0 && v < V, "tsvVtxMat index out of range");
@@ -681,7 +691,8 @@ struct Engine : EngineBase {
         if (NC > 0) {
             size_t nc = (size_t)NC * D;
             ctlIdx.alloc(NC, tot); ctlIdx.upload(hCtl.data(), NC, s);
-            qc.alloc(nc, tot); Kcc.alloc((size_t)NC * NC, tot);
+            qc.alloc(nc, tot);
+            if (!jacobi) Kcc.alloc((size_t)NC * NC, tot);
             cA.alloc(nc, tot); cA0.alloc(nc, tot); cAm1.alloc(nc, tot); cAm2.alloc(nc, tot); cAm3.alloc(nc, tot);
             cG.alloc(nc, tot); cR.alloc(nc, tot); cZ.alloc(nc, tot); cD.alloc(nc, tot);
             cR2.alloc(nc, tot); cD2.alloc(nc, tot);
@@ -689,8 +700,9 @@ struct Engine : EngineBase {
             cRout.alloc(nc, tot); cDelta.alloc(nc, tot); cZero.alloc(nc, tot);
             cZero.zero(s);
             {
+                // (the spectral term corrects the X^T X preconditioner, not the Jacobi one)
                 const char *e = std::getenv("SA_SPECTRAL");
-                specEnabled = !(e && e[0] == '0');
+                specEnabled = !(e && e[0] == '0') && !jacobi;
             }
             if (specEnabled) {
                 recD.alloc(nc * SPEC_M, tot); recSD.alloc(nc * SPEC_M, tot); recR.alloc(nc * SPEC_M, tot);
@@ -705,16 +717,23 @@ struct Engine : EngineBase {
             // (zero padding rows); the preconditioner leaves rocBLAS
             // column-major (Mcol) and is repacked
             const size_t pc = padded_rows<double>(NC) * (size_t)NC;
-            Mcol.alloc((size_t)NC * NC, tot);
-            Mpre.alloc(pc, tot);
-            Mpre.zero(s);
-            if (precision == SA_PREC_MIXED && !SA_MIXED_CC32) {
+            if (jacobi) {
+                stiffBlk.alloc((size_t)D * NS * E, tot);
+                bcc.alloc((size_t)D * Z, tot);
+                pw.alloc(std::max(BL, 1), tot);
+                dS.alloc(nc, tot);
+            } else {
+                Mcol.alloc((size_t)NC * NC, tot);
+                Mpre.alloc(pc, tot);
+                Mpre.zero(s);
+            }
+            if (precision == SA_PREC_MIXED && !SA_MIXED_CC32 && !jacobi) {
                 // FP64 K_cc for the FP32 inner solve: its FP32 rounding need
                 // not be positive definite when K_cc is ill conditioned
                 Kc.alloc(pc, tot);
                 Kc.zero(s);
             }
-            if (precision == SA_PREC_MIXED && SA_MIXED_CC32) {
+            if (precision == SA_PREC_MIXED && SA_MIXED_CC32 && !jacobi) {
                 // FP32 copy of the preconditioner for the FP32 inner solve (a
                 // step whose FP32 operator is unfit falls back to FP64)
                 const size_t pf = padded_rows<float>(NC) * (size_t)NC;
@@ -781,8 +800,9 @@ struct Engine : EngineBase {
             slicesC = std::max(1, std::min(32, (1024 + rb - 1) / rb));
             part.alloc((size_t)slicesC * nc, tot);
             potrfInfo.alloc(2, tot);
+            potrfInfo.zero(s);
             require(gridOf((long)nc, 64) <= MAXP, "too many control points for the partial-sum buffers");
-            if (!dryRun) {
+            if (!dryRun && !jacobi) {
                 std::lock_guard<std::mutex> lk(g_libMu);
                 if (rocblas_create_handle(&blas) != rocblas_status_success)
                     throw SaError(SA_E_DEVICE, "rocblas_create_handle failed");
@@ -1048,6 +1068,33 @@ struct Engine : EngineBase {
         if (specCarry) spec_carry_finish();
     }
 
+    // diag(S) of the control-point system at positions q (the control points
+    // already gathered into qc), concentration tauNow and geometry geoNow:
+    // the Jacobi preconditioner, rebuilt every step
+    void jacobi_build(const double *q, const double *tauNow, const double *geoNow) {
+        if (elaType == SA_ELA_FRAME)
+            stiff_blocks_k<D, false><<<gridOf(E), NT, 0, s>>>(E, elmVtx.p, geoNow, tauNow, elmLyr.p, dmod.p, stiffBlk.p);
+        else
+            stiff_blocks_k<D, true><<<gridOf(E), NT, 0, s>>>(E, elmVtx.p, geoNow, tauNow, elmLyr.p, dmod.p, stiffBlk.p);
+        stiff_gather_k<D><<<gridOf(Z), NT, 0, s>>>(Z, E, femIfo.p, stiffBlk.p, bcc.p);
+        const bool pen = btmWgt != 0.0 && B > 0;
+        if (pen) pen_weight_k<D><<<gridOf(BL), NT, 0, s>>>(BL, vtxBtm.p, meas.p, pw.p);
+        CHECK_LAUNCH();
+        const double invW = 1.0 / knlWidth;
+#define SA_JD(O) jacobi_diag_k<D, O><<<NC, NT, 0, s>>>(V, NC, q, qc.p, rowPtr.p, colIdx.p, Z, bcc.p, pen ? BL : 0, \
+                                                       pw.p, btmWgt, ldmWgt, knlEps, invW, dS.p)
+        switch (knlOrder) {
+            case -1: SA_JD(-1); break;
+            case 0: SA_JD(0); break;
+            case 1: SA_JD(1); break;
+            case 2: SA_JD(2); break;
+            case 3: SA_JD(3); break;
+            default: SA_JD(4); break;
+        }
+#undef SA_JD
+        CHECK_LAUNCH();
+    }
+
     // U <- U G^-1/2 (G = Q g Q^T, the Gram matrix kcc_build_once formed):
     // the carried Ritz vectors become orthonormal in the new M^-1 inner
     // product, so M + U diag(cf) U^T stays positive definite (each cf in
@@ -1241,6 +1288,11 @@ struct Engine : EngineBase {
     // (ctl_update_k wrote them); else spec_wp_k computes them here
     void prec_dot(KMode md, const double *r, double *z, int wpReady = 0) {
         const int nc = NC * D;
+        if (jacobi) {
+            jacobi_dot_k<<<gridOf(nc, 64), 64, 0, s>>>(nc, P2.p, gridOf(nc), r, dS.p, z, P3.p, ctl.p);
+            CHECK_LAUNCH();
+            return;
+        }
         const bool sp = spec_active(md);
         int nPw = wpReady;
         if (sp && !wpReady) {
@@ -1694,7 +1746,7 @@ struct Engine : EngineBase {
     // its iterate with the rest of the budget
     int ctl_solve(KMode md, const double *q, const double *tauNow, const double *geoNow, const double *rhs, double *xs,
                   bool zeroStart, double tolSqu, int maxIt, int &iters) {
-        const int cap = precAge > 0 ? std::min(std::max(1, maxIt / 2), 4 * precIters + 100) : maxIt;
+        const int cap = precAge > 0 && !jacobi ? std::min(std::max(1, maxIt / 2), 4 * precIters + 100) : maxIt;
         const bool spUsed = specOn && specMode == (int)md && !(specRecord && fused_iter(md));
         int st = ctl_pcg(md, q, tauNow, geoNow, rhs, xs, zeroStart, tolSqu, cap, iters);
         if (st == 1 && spUsed && iters < cap) {
@@ -1736,7 +1788,8 @@ struct Engine : EngineBase {
                                        : A64.p ? KMode::Stored64 : KMode::MatrixFree64;
         specCarry = refresh && precAge >= PREC_MAX_AGE && specOn && specMode == (int)cgMode &&
                     specCarried < SA_SPEC_CARRY;
-        kcc_build(refresh);
+        if (jacobi) jacobi_build(q, tauNow, geoNow);
+        else kcc_build(refresh);
         if (refresh) { precAge = 0; forceRefresh = false; }
         if (!mixedStep && precision == SA_PREC_MIXED) use_fp64_store();
         kstore_build(q, !mixedStep, mixedStep);
@@ -2031,6 +2084,14 @@ struct Engine : EngineBase {
                 CK(hipMemcpyAsync(out, yout, sizeof(double) * (xc ? V : NC) * D, hipMemcpyDeviceToHost, s));
                 break;
             }
+            case SA_PROBE_JACOBI:
+                require(jacobi, "the session does not use Jacobi preconditioning");
+                gather_ctl_k<D><<<gridOf(NC), NT, 0, s>>>(NC, V, ctlIdx.p, q0.p, qc.p);
+                if (btmWgt != 0.0 && B > 0) btm_meas_k<D><<<gridOf(B), NT, 0, s>>>(B, V, btmVtx.p, q0.p, meas.p);
+                CHECK_LAUNCH();
+                jacobi_build(q0.p, tau.p, geo[0].p);
+                CK(hipMemcpyAsync(out, dS.p, sizeof(double) * NC * D, hipMemcpyDeviceToHost, s));
+                break;
             case SA_PROBE_KMATVEC:
                 CK(hipMemcpyAsync(r.p, in, sizeof(double) * n, hipMemcpyHostToDevice, s));
                 if (Kst.p) kst_build(q0.p);

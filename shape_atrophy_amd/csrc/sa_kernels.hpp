@@ -390,6 +390,199 @@ __global__ __launch_bounds__(NT) void elastic_elem_k(int E, int V, const int *__
     }
 }
 
+// ---------------------------------------------------------------------------
+// Jacobi preconditioner of the control-point system S = ldmWgt (K_cc + eps I)
+// + K_cx (E + btmWgt P) K_xc: diag(S) for every control point j and
+// component c.  With k_j = K(., c_j) over the landmarks and B_cc the
+// (c, c) block of the elastic stiffness,
+//   S_(jc,jc) = ldmWgt (k(0) + eps) + k_j^T B_cc k_j + btmWgt sum_v P_v k_j(v)^2.
+//
+// stiff_blocks_k: per element the (c, c) blocks of its stiffness,
+// B_e[(l, c), (m, c)] = |T| g_l . sigma(sym(e_c g_m^T)) e_c with g_m the P1
+// gradients (rows of Q^{-1}, g_0 = -sum), i.e. the force on vertex l,
+// component c, of the element operator for a unit displacement of vertex m
+// along c; symmetric, kept for l <= m in the femIfoMat slot order
+// (00, 01, .. 0D, 11, ..): blk[(c NS + slot) E + e].
+template <int D, bool ISO>
+__global__ __launch_bounds__(NT) void stiff_blocks_k(int E, const int *__restrict__ elmVtx,
+                                                     const double *__restrict__ geo, const double *__restrict__ tau,
+                                                     const int *__restrict__ elmLyr, const double *__restrict__ mod,
+                                                     double *__restrict__ blk) {
+    using G = GeoLayout<D>;
+    constexpr int NS = G::NS;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    double g[D + 1][D];
+#pragma unroll
+    for (int b = 0; b < D; ++b) {
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            g[k + 1][b] = geo[(G::INV + k * D + b) * (size_t)E + e];
+            s -= g[k + 1][b];
+        }
+        g[0][b] = s;
+    }
+    const double vol = elem_vol<D>(geo[G::DET * (size_t)E + e]);
+    double f1[D], f2[D], md[D + 1], lam = 0.0, mu = 0.0;
+    if constexpr (ISO) {
+        const int l = elmLyr ? elmLyr[e] : 0;
+        double ub = 0.0;
+#pragma unroll
+        for (int k = 0; k <= D; ++k) ub += tau[elmVtx[k * E + e]];
+        ub /= (D + 1);
+        const double sc = 1.0 - 0.5 * ub;
+        lam = mod[2 * l] * sc;
+        mu = mod[2 * l + 1] * sc;
+    } else {
+#pragma unroll
+        for (int c = 0; c < D; ++c) { f1[c] = geo[(G::F1 + c) * (size_t)E + e]; f2[c] = geo[(G::F2 + c) * (size_t)E + e]; }
+#pragma unroll
+        for (int m = 0; m <= D; ++m) md[m] = cld(&mod[m]);
+    }
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+        int sl = 0;
+#pragma unroll
+        for (int l = 0; l <= D; ++l)
+#pragma unroll
+            for (int m = l; m <= D; ++m, ++sl) {
+                double eps[D][D], sig[D][D];
+#pragma unroll
+                for (int a = 0; a < D; ++a)
+#pragma unroll
+                    for (int b = 0; b < D; ++b)
+                        eps[a][b] = 0.5 * ((a == c ? g[m][b] : 0.0) + (b == c ? g[m][a] : 0.0));
+                if constexpr (ISO) {
+                    const double tr = eps[c][c];
+#pragma unroll
+                    for (int a = 0; a < D; ++a)
+#pragma unroll
+                        for (int b = 0; b < D; ++b) sig[a][b] = (a == b ? lam * tr : 0.0) + 2.0 * mu * eps[a][b];
+                } else {
+                    frame_stress<D>(eps, f1, f2, md, sig);
+                }
+                double v = 0.0;
+#pragma unroll
+                for (int j = 0; j < D; ++j) v += g[l][j] * sig[j][c];
+                blk[((size_t)c * NS + sl) * E + e] = vol * v;
+            }
+    }
+}
+
+// per-nonzero gather of the (c, c) stiffness blocks onto the femVtxMat
+// pattern (femIfoMat lists (element, slot E + element) per nonzero, as for
+// the FEM matrices): bcc[c Z + z]
+template <int D>
+__global__ __launch_bounds__(NT) void stiff_gather_k(int Z, int E, const int *__restrict__ femIfo,
+                                                     const double *__restrict__ blk, double *__restrict__ bcc) {
+    constexpr int NS = GeoLayout<D>::NS;
+    const int z = blockIdx.x * blockDim.x + threadIdx.x;
+    if (z >= Z) return;
+    const int adj = femIfo[z];
+    double acc[D];
+#pragma unroll
+    for (int c = 0; c < D; ++c) acc[c] = 0.0;
+    for (int a = 0; a < adj; ++a) {
+        const int gi = femIfo[(3 + 2 * a) * Z + z];
+#pragma unroll
+        for (int c = 0; c < D; ++c) acc[c] += blk[(size_t)c * NS * E + gi];
+    }
+#pragma unroll
+    for (int c = 0; c < D; ++c) bcc[(size_t)c * Z + z] = acc[c];
+}
+
+// lumped penalty weight of each bottom landmark: P_v = sum over its bottom
+// facets of |F| / D (the diagonal of the penalty operator, pen_elem_k)
+template <int D>
+__global__ void pen_weight_k(int BL, const int *__restrict__ vtxBtm, const double *__restrict__ meas,
+                             double *__restrict__ pw) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= BL) return;
+    const int adj = vtxBtm[v];
+    double s = 0.0;
+    for (int a = 0; a < adj; ++a) s += meas[vtxBtm[(1 + 2 * a) * BL + v]] / (double)D;
+    pw[v] = s;
+}
+
+// one block per control point j: diag(S) for its D components, the kernel
+// values k_j(v) evaluated in FP64 from the positions; the threads' partial
+// sums are added in a fixed order
+template <int D, int ORD>
+__global__ __launch_bounds__(NT) void jacobi_diag_k(int V, int C, const double *__restrict__ q,
+                                                    const double *__restrict__ qc, const int *__restrict__ rowPtr,
+                                                    const int *__restrict__ colIdx, int Z,
+                                                    const double *__restrict__ bcc, int BL,
+                                                    const double *__restrict__ pw, double btmWgt, double ldmWgt,
+                                                    double eps, double invW, double *__restrict__ dS) {
+    __shared__ double etab[64];
+    __shared__ double red[NT / 64];
+    init_exp_table(etab);
+    __syncthreads();
+    const int j = blockIdx.x;
+    double cj[D];
+#pragma unroll
+    for (int c = 0; c < D; ++c) cj[c] = qc[(size_t)c * C + j];
+    auto kval = [&](int v) {
+        double d2 = 0.0;
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            const double dd = q[(size_t)c * V + v] - cj[c];
+            d2 += dd * dd;
+        }
+        return rkhs_kernel<D, ORD>(d2, invW, etab);
+    };
+    double acc[D];
+#pragma unroll
+    for (int c = 0; c < D; ++c) acc[c] = 0.0;
+    for (int v = threadIdx.x; v < V; v += NT) {
+        const double kv = kval(v);
+        double t[D];
+#pragma unroll
+        for (int c = 0; c < D; ++c) t[c] = 0.0;
+        for (int z = rowPtr[v]; z < rowPtr[v + 1]; ++z) {
+            const int w = colIdx[z];
+            const double kw = w == v ? kv : kval(w);
+#pragma unroll
+            for (int c = 0; c < D; ++c) t[c] += bcc[(size_t)c * Z + z] * kw;
+        }
+        const double p = v < BL ? btmWgt * pw[v] * kv * kv : 0.0;
+#pragma unroll
+        for (int c = 0; c < D; ++c) acc[c] += kv * t[c] + p;
+    }
+    const double k0 = rkhs_kernel<D, ORD>(0.0, invW, etab);
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+        const double s = block_sum(acc[c], red);
+        if (threadIdx.x == 0) dS[(size_t)c * C + j] = ldmWgt * (k0 + eps) + s;
+    }
+}
+
+// Jacobi step of the control-point CG: stop on |r|^2 (partials Prr) below
+// the tolerance, else z = r / diag(S) and the partial of r.z for each block
+// of 64 entries (the count the direction update reads)
+__global__ __launch_bounds__(64) void jacobi_dot_k(int n, const double *__restrict__ Prr, int nPrr,
+                                                   const double *__restrict__ r, const double *__restrict__ dS,
+                                                   double *__restrict__ z, double *__restrict__ P3, Ctl *ctl) {
+    __shared__ double red[1];
+    if (cld(&ctl->done)) return;
+    const int it = cld(&ctl->it);
+    const double rr = reduce_partials(Prr, nPrr, red);
+    if (rr < cld(&ctl->tol)) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) { cst(&ctl->done, 1); cst(&ctl->iters, it + 1); }
+        return;
+    }
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    double rz = 0.0;
+    if (i < n) {
+        const double zi = r[i] / dS[i];
+        z[i] = zi;
+        rz = r[i] * zi;
+    }
+    const double t = block_sum(rz, red);
+    if (threadIdx.x == 0) P3[blockIdx.x] = t;
+}
+
 // bottom facet measure: 2D segment length, 3D xy-projected area
 // (applyPenaltyOperator.cu:13-14 | 3D :12-14)
 template <int D>

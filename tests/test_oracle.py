@@ -599,3 +599,29 @@ def test_operator_apply_timing_sample(oracle):
     t2 = oracle.time_elastic_apply(p, reps=5, threads=2)
     assert 0.0 < t1 < 10.0 and 0.0 < t2 < 10.0
     np.testing.assert_array_equal(oracle.elastic_apply(p, p["q0"], tau0, v), before)
+
+
+# --------------------------------------------------------------------------
+# Jacobi preconditioning of the control-point CG (build-defined option)
+
+def test_jacobi_preconditioned_solve_matches(oracle):
+    """diag(S) by unit vectors: with zero elastic moduli and no penalty it is
+    ldmWgt (k(0) + eps) exactly; the Jacobi-preconditioned control-point CG
+    reaches the solution of the (K_cc + eps I)^-1-preconditioned one to the
+    CG tolerance."""
+    p = small_shell()
+    V = p["q0"].shape[1]
+    p["ctlIdx"] = np.sort(np.random.default_rng(3).choice(V, V // 3, replace=False)).astype(np.int32)
+    p["timeNum"] = 4
+    tau0 = oracle.tau_initial(p)
+    z = dict(p, params=dict(p["params"], modVec=[0.0, 0.0, 0.0, 0.0]))
+    dz = oracle.jacobi_diag(z, p["q0"], tau0)
+    np.testing.assert_allclose(dz, p["params"]["ldmWgt"] * (1.0 + p["params"]["knlEps"]), rtol=1e-15)
+    d = oracle.jacobi_diag(p, p["q0"], tau0)
+    assert np.all(d > dz)
+    ref = oracle.objfcn(p)
+    jac = oracle.objfcn(dict(p, params=dict(p["params"], velPrec="jacobi")))
+    assert ref["status"] == 0 and jac["status"] == 0
+    disp = np.linalg.norm(ref["lmkStk"][-1] - ref["lmkStk"][0])
+    assert np.linalg.norm(jac["lmkStk"][-1] - ref["lmkStk"][-1]) / disp < 1e-7
+    assert abs(jac["obj"] - ref["obj"]) <= 1e-7 * abs(ref["obj"])
