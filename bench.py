@@ -341,6 +341,87 @@ def mode_products(args, api, W):
             **out}
 
 
+TUNE_TARGETS = (32, 64, 128, 256, 512, 1024, 2048, 4096, 8192, 16384)
+# the library's built-in targets (KS_BLOCKS = 1024; twice that for FP32 K_xc /
+# K_cx; 2048 for the all-landmark product), kept unless a target is more than
+# TUNE_KEEP faster: two sweeps on different boxes differed by up to 5% at the
+# same setting on the large products
+TUNE_DEFAULT = {("xc", "f32"): 2048, ("cx", "f32"): 2048, ("xc", "f64"): 1024, ("cx", "f64"): 1024,
+                ("cc", "f32"): 1024, ("cc", "f64"): 1024, ("kk", "f64"): 2048}
+TUNE_KEEP = 1.05
+
+
+def tune_pick(prod: str, ty: str, times: dict) -> int:
+    """The target the table keeps: the built-in one unless another is more
+    than TUNE_KEEP faster, then the smallest within 1% of the fastest."""
+    best = min(times.values())
+    dflt = TUNE_DEFAULT[(prod, ty)]
+    if dflt in times and times[dflt] <= TUNE_KEEP * best:
+        return dflt
+    return min(t for t in times if times[t] <= 1.01 * best)
+
+
+def tune_shapes(W):
+    """Meshes of the tile sweep: control-point shells from the batch range to
+    the largest served size, and all-landmark meshes."""
+    ctl = [("shell 3x4, 256 control points", dict(level=3, n_layers=4, n_ctl=256)),
+           ("shell 4x3, 500 control points", dict(level=4, n_layers=3, n_ctl=500)),
+           ("shell 4x5, 1281 control points", dict(level=4, n_layers=5, n_ctl=1281)),
+           ("shell 5x3, 2000 control points", dict(level=5, n_layers=3, n_ctl=2000)),
+           ("shell 5x5, 5000 control points", dict(level=5, n_layers=5, n_ctl=5000)),
+           ("shell 6x5, 2000 control points", dict(level=6, n_layers=5, n_ctl=2000))]
+    for desc, kw in ctl:
+        yield desc, W.shell_problem(time_steps=2, **kw)
+    yield "2D ring disk, 2000 landmarks", W.ring_problem()
+    yield "2D ring disk, 6000 landmarks", W.ring_problem(n_rings=20, n_per_ring=300)
+    yield "shell 4x5, all landmarks", W.shell_problem(level=4, n_layers=5, time_steps=2)
+    yield "shell 5x5, all landmarks", W.shell_problem(level=5, n_layers=5, time_steps=2)
+
+
+def mode_tune(args, api, W):
+    """Offline sweep of the tile table: for each mesh, precision and stored
+    product, the device time of one product (kernel and slice sum, the mean
+    of --tune-reps launches, best of three) at each block target; the table
+    keeps the target tune_pick chooses.  Writes the table to
+    gpurun_out/tiles.txt (copy it beside the library to use it)."""
+    rows, table = [], []
+    names = {"xc": "kxc", "cx": "kcx", "cc": "precond", "kk": "kmatvec"}
+    for desc, p in tune_shapes(W):
+        d, V = p["q0"].shape
+        nc = len(p["ctlIdx"]) if p.get("ctlIdx") is not None else 0
+        precs = ("mixed", "fp64") if nc else ("fp64",)
+        for prec in precs:
+            f32 = prec == "mixed"
+            prods = ("xc", "cx", "cc") if nc else ("kk",)
+            for prod in prods:
+                shape = {"xc": (V, nc), "cx": (nc, V), "cc": (nc, nc), "kk": (V, V)}[prod]
+                times = {}
+                for t in TUNE_TARGETS:
+                    os.environ["SA_KS_TARGET"] = f"{prod}={t}"
+                    s = api.Session(p, precision=prec)
+                    try:
+                        times[t] = min(s.time_product(names[prod], args.tune_reps) for _ in range(3))
+                    finally:
+                        s.close()
+                ty = "f32" if f32 and prod != "kk" else "f64"
+                pick = tune_pick(prod, ty, times)
+                table.append(f"{prod} {ty} {shape[0]} {shape[1]} {pick}")
+                rows.append({"mesh": desc, "precision": prec, "product": prod, "rows": shape[0], "cols": shape[1],
+                             "us": {str(t): round(v, 2) for t, v in times.items()}, "target": pick})
+                print(f"[tune] {desc} {prec} {prod} {shape}: " +
+                      " ".join(f"{t}:{v:.1f}" for t, v in times.items()) + f" -> {pick}", file=sys.stderr, flush=True)
+    os.environ.pop("SA_KS_TARGET", None)
+    os.makedirs("gpurun_out", exist_ok=True)
+    with open(os.path.join("gpurun_out", "tiles.txt"), "w") as f:
+        f.write("# product type rows cols target-blocks  (python bench.py --mode tune)\n")
+        f.write("\n".join(table) + "\n")
+    return {"metric": "tile sweep of the stored kernel products", "value": None, "unit": "us per product",
+            "n_gpus": 1, "steps": None, "warmup": None, "ms_per_step": None, "higher_is_better": False,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64+f32", "data": "synthetic",
+            "config": {"workload": "tile sweep", "targets": list(TUNE_TARGETS), "reps": args.tune_reps},
+            "sweep": rows, "table": table}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -361,8 +442,10 @@ def main():
     ap.add_argument("--no-profile", action="store_true",
                     help="time without the per-kernel events (graph-captured CG iterations); no roofline")
     ap.add_argument("--cpu-iters", type=int, default=0, help="CG iterations of the CPU sample (0: sized to the workload)")
-    ap.add_argument("--mode", default="single", choices=["single", "batch", "serve", "precision", "fit", "products"],
+    ap.add_argument("--mode", default="single",
+                    choices=["single", "batch", "serve", "precision", "fit", "products", "tune"],
                     help="single: one solve of --config; batch: 64 uneven meshes concurrently; serve: Poisson stream")
+    ap.add_argument("--tune-reps", type=int, default=30, help="product launches per timing of --mode tune")
     ap.add_argument("--workers", type=int, default=8)
     ap.add_argument("--batch-n", type=int, default=64)
     ap.add_argument("--pool", type=int, default=12)
@@ -390,7 +473,7 @@ def main():
     if args.mode != "single":
         from shape_atrophy_amd import workloads as W
         res = {"batch": mode_batch, "serve": mode_serve, "precision": mode_precision,
-               "fit": mode_fit, "products": mode_products}[args.mode](args, api, W)
+               "fit": mode_fit, "products": mode_products, "tune": mode_tune}[args.mode](args, api, W)
         print(json.dumps(res))
         return
     steps = args.steps if args.steps is not None else 50   # the configured solve: 50 time steps

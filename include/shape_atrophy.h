@@ -226,6 +226,25 @@ int sa_session_probe(sa_session *s, int what, const double *tauVarVec, int tauVa
 /* bytes of device memory a session of this mesh holds */
 long long sa_session_footprint(const sa_mesh *mesh, const sa_target *tgt, const sa_params *prm);
 
+/* Tile table (no counterpart in the reference; for the offline sweep that
+   writes it).  The stored kernel products launch about `target` blocks (row
+   panels x column slices); the targets come from tiles.txt beside the
+   library, one line per product, element type and measured shape:
+     <xc|cx|cc|kk> <f32|f64> <rows> <cols> <target>
+   (xc: K_xc, landmarks x control points; cx: K_cx; cc: the control-point
+   preconditioner product; kk: K over all landmarks).  A session takes the
+   entry of its product and type nearest its shape.  SA_TILE_TABLE names
+   another file; SA_KS_TARGET ("N" or "xc=N,cx=M,...") overrides the table.
+   sa_tile_target returns the target a new session would take (0: no entry,
+   the built-in default applies; -1: bad argument).
+   sa_session_time_product times one product as a solve launches it (product
+   kernel and slice sum) at q0: the mean over reps launches after a warm-up,
+   in microseconds; what = SA_PROBE_KXC, SA_PROBE_KCX, SA_PROBE_KMATVEC (the
+   all-landmark form) or SA_TIME_PRECOND. */
+#define SA_TIME_PRECOND 11
+int sa_tile_target(const char *product, int f32, long long rows, long long cols);
+int sa_session_time_product(sa_session *s, int what, int reps, double *usPerProduct);
+
 /* Request server (no counterpart in the reference): solves of registered
    meshes run on worker threads, each request on a device-resident session of
    its mesh with its own HIP stream, so requests overlap on the device.

@@ -92,6 +92,8 @@ def lib():
         L.sa_session_set_profiling.argtypes = [C.c_void_p, C.c_int]
         L.sa_session_set_time_num.argtypes = [C.c_void_p, C.c_int]
         L.sa_session_set_params.argtypes = [C.c_void_p, _P(SaParams)]
+        L.sa_session_time_product.argtypes = [C.c_void_p, C.c_int, C.c_int, _PD]
+        L.sa_tile_target.argtypes = [C.c_char_p, C.c_int, C.c_longlong, C.c_longlong]
         L.sa_server_create.argtypes = [C.c_int, C.c_longlong, _P(C.c_void_p)]
         L.sa_server_destroy.argtypes = [C.c_void_p]
         L.sa_server_add_mesh.argtypes = [C.c_void_p, _P(SaMesh), _P(SaTarget), _P(SaParams), _PD, _PI]
@@ -104,6 +106,12 @@ def lib():
                                         _PD, _PD, _PD]
         _lib = L
     return _lib
+
+
+def tile_target(product: str, f32: bool, rows: int, cols: int) -> int:
+    """Block target a new session takes for a stored product of this shape
+    from the tile table (0: none, the built-in default)."""
+    return lib().sa_tile_target(product.encode(), 1 if f32 else 0, int(rows), int(cols))
 
 
 def _check(code: int):
@@ -374,6 +382,15 @@ class Session:
         _check(lib().sa_session_probe(self._h, self.PROBES[what], _dp(tv), tv.size, _dp(tp), tp.size, _dp(xin),
                                       _dp(out)))
         return out
+
+    TIMED = {"kxc": 8, "kcx": 9, "kmatvec": 2, "precond": 11}
+
+    def time_product(self, what: str, reps: int = 20) -> float:
+        """Device microseconds of one kernel product as a solve launches it
+        (sa_session_time_product): "kxc", "kcx", "kmatvec" or "precond"."""
+        us = C.c_double()
+        _check(lib().sa_session_time_product(self._h, self.TIMED[what], int(reps), C.byref(us)))
+        return us.value
 
     def close(self):
         if getattr(self, "_h", None):

@@ -26,6 +26,7 @@
 #include <vector>
 #include <type_traits>
 #include <cstdlib>
+#include <dlfcn.h>
 
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
@@ -41,6 +42,96 @@ thread_local std::string g_err;
 // rocBLAS / rocSOLVER calls of all sessions run one at a time (each to
 // completion on its stream before the next starts)
 std::mutex g_libMu;
+
+// Block-count targets of the stored kernel products (row panels x column
+// slices), per product and matrix shape, from an offline sweep on the device
+// kept in tiles.txt beside the library.  A line
+//   <product> <f32|f64> <rows> <cols> <target blocks>
+// names the product xc (K_xc: landmarks x control points), cx (K_cx), cc
+// (the C x C preconditioner M) or kk (K over all landmarks), the stored
+// element type and the shape it was measured on; '#' starts a comment.  A
+// session takes the entry of its product and type nearest its shape (sum of
+// the log distances of rows and columns), else the built-in default.
+// SA_TILE_TABLE names another file ("" reads none); SA_KS_TARGET overrides
+// every entry ("2048", or per product "xc=1024,cx=4096"), for the sweep.
+// The slice counts are a function of the mesh and the table, so the partial
+// sums of one mesh are added in one fixed order.
+struct TileEntry {
+    char prod[4];
+    bool f32;
+    double rows, cols;
+    int target;
+};
+
+std::vector<TileEntry> load_tile_table() {
+    std::vector<TileEntry> t;
+    std::string path;
+    if (const char *e = std::getenv("SA_TILE_TABLE")) {
+        path = e;
+    } else {
+        Dl_info info;
+        if (dladdr(reinterpret_cast<void *>(&load_tile_table), &info) && info.dli_fname) {
+            path = info.dli_fname;
+            const size_t sl = path.rfind('/');
+            path = (sl == std::string::npos ? std::string(".") : path.substr(0, sl)) + "/tiles.txt";
+        }
+    }
+    if (path.empty()) return t;
+    FILE *f = std::fopen(path.c_str(), "r");
+    if (!f) return t;
+    char line[256];
+    while (std::fgets(line, sizeof line, f)) {
+        if (char *h = std::strchr(line, '#')) *h = 0;
+        TileEntry en{};
+        char prod[8], ty[8];
+        long rows, cols;
+        int tg;
+        if (std::sscanf(line, "%7s %7s %ld %ld %d", prod, ty, &rows, &cols, &tg) != 5) continue;
+        const std::string p(prod), y(ty);
+        if ((p != "xc" && p != "cx" && p != "cc" && p != "kk") || (y != "f32" && y != "f64") || rows < 1 ||
+            cols < 1 || tg < 16 || tg > 65536)
+            continue;
+        std::memcpy(en.prod, prod, 3);
+        en.f32 = y == "f32";
+        en.rows = (double)rows;
+        en.cols = (double)cols;
+        en.target = tg;
+        t.push_back(en);
+    }
+    std::fclose(f);
+    return t;
+}
+
+const std::vector<TileEntry> &tile_table() {
+    static const std::vector<TileEntry> t = load_tile_table();
+    return t;
+}
+
+// target block count of one stored product (see TileEntry)
+int ks_target(const char *prod, bool f32, long rows, long cols, int dflt) {
+    if (const char *e = std::getenv("SA_KS_TARGET")) {
+        // "N" for every product, or "xc=N,cx=M,..."
+        const std::string s(e);
+        if (!s.empty() && s.find('=') == std::string::npos) {
+            const int v = std::atoi(s.c_str());
+            if (v >= 16 && v <= 65536) return v;
+        }
+        const std::string key = std::string(prod) + "=";
+        const size_t k = s.find(key);
+        if (k != std::string::npos && (k == 0 || s[k - 1] == ',')) {
+            const int v = std::atoi(s.c_str() + k + key.size());
+            if (v >= 16 && v <= 65536) return v;
+        }
+    }
+    const TileEntry *best = nullptr;
+    double bd = 0.0;
+    for (const auto &en : tile_table()) {
+        if (std::strcmp(en.prod, prod) || en.f32 != f32) continue;
+        const double d = std::fabs(std::log(en.rows / (double)rows)) + std::fabs(std::log(en.cols / (double)cols));
+        if (!best || d < bd) { best = &en; bd = d; }
+    }
+    return best ? best->target : dflt;
+}
 
 #ifndef SA_MIXED_CC32
 #define SA_MIXED_CC32 1   // mixed form: K_cc from the FP32 stored rows, FP32 preconditioner
@@ -290,6 +381,7 @@ struct EngineBase {
     virtual void probe(int what, const double *tauVar, int nVar, const double *tauPrm, int nPrm, const double *in,
                        double *out) = 0;
     virtual void set_params(const sa_params &p) = 0;
+    virtual double time_product(int what, int reps) = 0;
     KernelTimer timer;
 };
 
@@ -781,17 +873,21 @@ struct Engine : EngineBase {
                 A64.p = kmem.p;
                 At64.p = kmem.p ? kmem.p + na64 : nullptr;
             }
-            // about KS_BLOCKS blocks per stored product (row blocks x column slices)
-            const int prS = precision == SA_PREC_MIXED ? panel_rows<float>() : panel_rows<double>();
+            // blocks per stored product (row blocks x column slices) from the
+            // tile table; by default KS_BLOCKS, twice that for FP32 products
+            // (half the bytes per column), and at least enough slices for the
+            // LDS copy of x (KS_MAX_COLS columns)
+            const bool f32 = precision == SA_PREC_MIXED;
+            const int prS = f32 ? panel_rows<float>() : panel_rows<double>();
             const int rbA = gridOf(V, prS), rbT = gridOf(NC, prS), rbM = gridOf(NC, panel_rows<double>());
-            // (and at least enough slices for the LDS copy of x: KS_MAX_COLS columns)
-            // FP32 products stream half the bytes per column: twice the blocks
-            const int tgt = precision == SA_PREC_MIXED ? 2 * KS_BLOCKS : KS_BLOCKS;
+            const int dflt = f32 ? 2 * KS_BLOCKS : KS_BLOCKS;
+            const int tgA = ks_target("xc", f32, V, NC, dflt), tgT = ks_target("cx", f32, NC, V, dflt);
+            const int tgM = ks_target("cc", false, NC, NC, KS_BLOCKS), tgM32 = ks_target("cc", true, NC, NC, KS_BLOCKS);
             const int rbM32 = gridOf(NC, panel_rows<float>());
-            slicesA = std::max(gridOf(NC, KS_MAX_COLS), std::min(64, (tgt + rbA - 1) / rbA));
-            slicesT = std::max(gridOf(V, KS_MAX_COLS), std::min(256, (tgt + rbT - 1) / rbT));
-            slicesM = std::max(gridOf(NC, KS_MAX_COLS), std::min(256, (KS_BLOCKS + rbM - 1) / rbM));
-            slicesM32 = std::max(gridOf(NC, KS_MAX_COLS), std::min(256, (KS_BLOCKS + rbM32 - 1) / rbM32));
+            slicesA = std::max(gridOf(NC, KS_MAX_COLS), std::min(64, (tgA + rbA - 1) / rbA));
+            slicesT = std::max(gridOf(V, KS_MAX_COLS), std::min(256, (tgT + rbT - 1) / rbT));
+            slicesM = std::max(gridOf(NC, KS_MAX_COLS), std::min(256, (tgM + rbM - 1) / rbM));
+            slicesM32 = std::max(gridOf(NC, KS_MAX_COLS), std::min(256, (tgM32 + rbM32 - 1) / rbM32));
             // (twice the product slices for the split residual products)
             partS.alloc(std::max({kxMfma ? (size_t)slicesAm * D * V : (size_t)2 * slicesA * D * V,
                                   kxMfma ? (size_t)slicesTm * nc : (size_t)2 * slicesT * nc, (size_t)slicesM * nc,
@@ -829,7 +925,8 @@ struct Engine : EngineBase {
                 if (Kst.n) {
                     Kst.zero(s);
                     const int rbK = gridOf(V, panel_rows<double>());
-                    slicesK = std::max(gridOf(V, KS_MAX_COLS), std::min(256, (2048 + rbK - 1) / rbK));
+                    const int tgK = ks_target("kk", false, V, V, 2048);
+                    slicesK = std::max(gridOf(V, KS_MAX_COLS), std::min(256, (tgK + rbK - 1) / rbK));
                     partS.alloc((size_t)slicesK * D * V, tot);
                 }
             }
@@ -2127,6 +2224,70 @@ struct Engine : EngineBase {
         CK(hipStreamSynchronize(s));
     }
 
+    // Device time of one kernel product as a solve launches it (the product
+    // kernel and its slice sum) at q0 on zero input: the mean over reps
+    // launches after a warm-up one, in microseconds.
+    double time_product(int what, int reps) override {
+        require(reps >= 1 && reps <= 100000, "reps must be in [1, 100000]");
+        struct Ev {
+            hipEvent_t e = nullptr;
+            ~Ev() { if (e) (void)hipEventDestroy(e); }
+        } e0, e1;
+        CK(hipEventCreate(&e0.e));
+        CK(hipEventCreate(&e1.e));
+        CK(hipMemsetAsync(ctl.p, 0, sizeof(Ctl), s));
+        auto timed = [&](auto &&one) {
+            one();
+            CK(hipEventRecord(e0.e, s));
+            for (int k = 0; k < reps; ++k) one();
+            CK(hipEventRecord(e1.e, s));
+            CK(hipEventSynchronize(e1.e));
+            float ms = 0.0f;
+            CK(hipEventElapsedTime(&ms, e0.e, e1.e));
+            return 1e3 * (double)ms / reps;
+        };
+        switch (what) {
+            case SA_PROBE_KXC:
+            case SA_PROBE_KCX: {
+                require(NC > 0, "the session has no control points");
+                const bool xc = what == SA_PROBE_KXC;
+                gather_ctl_k<D><<<gridOf(NC), NT, 0, s>>>(NC, V, ctlIdx.p, q0.p, qc.p);
+                CHECK_LAUNCH();
+                restore_fp32_store();
+                const KMode md = precision == SA_PREC_MIXED ? (kxMfma ? KMode::Mfma32 : KMode::Stored32)
+                                                            : A64.p ? KMode::Stored64 : KMode::MatrixFree64;
+                kstore_build(q0.p, md == KMode::Stored64, md == KMode::Stored32);
+                double *xin = xc ? cD.p : Ad.p, *yout = xc ? Kr.p : cY.p;
+                CK(hipMemsetAsync(xin, 0, sizeof(double) * (xc ? NC : V) * D, s));
+                return timed([&] { kx_product(md, xc, q0.p, xin, yout, nullptr, ctl.p); });
+            }
+            case SA_PROBE_KMATVEC:
+                require(NC == 0, "the session has control points (time SA_PROBE_KXC / SA_PROBE_KCX)");
+                if (Kst.p) kst_build(q0.p);
+                CK(hipMemsetAsync(r.p, 0, sizeof(double) * V * D, s));
+                return timed([&] { kmatvec<false>(q0.p, r.p, Kr.p, nullptr, 0, -1); });
+            case SA_TIME_PRECOND: {
+                require(NC > 0 && !jacobi, "the session has no control-point preconditioner product");
+                CK(hipMemsetAsync(cR.p, 0, sizeof(double) * NC * D, s));
+                const bool f32 = precision == SA_PREC_MIXED && Mpre32.p;
+                const int sl = f32 ? slicesM32 : slicesM;
+                return timed([&] {
+                    if (f32) {
+                        dim3 g(gridOf(NC, panel_rows<float>()), sl);
+                        kstore_mv_k<D, float><<<g, NT, ks_lds(NC, sl), s>>>(NC, NC, Mpre32.p, cR.p, partS.p, ctl.p);
+                    } else {
+                        dim3 g(gridOf(NC, panel_rows<double>()), sl);
+                        kstore_mv_k<D, double><<<g, NT, ks_lds(NC, sl), s>>>(NC, NC, Mpre.p, cR.p, partS.p, ctl.p);
+                    }
+                    CHECK_LAUNCH();
+                    slice_rows(D * NC, sl, partS.p, cZ.p, ctl.p);
+                });
+            }
+            default:
+                throw SaError(SA_E_ARG, "unknown product");
+        }
+    }
+
     void run(const double *tauVar, int nVar, const double *tauPrm, int nPrm, double *obj, double *lmkOut,
              double *tauOut, double *gradOut, Stats &st) override {
         const size_t n = (size_t)V * D;
@@ -2385,6 +2546,21 @@ int sa_session_probe(sa_session *ss, int what, const double *tauVarVec, int tauV
         if (!out) throw sa::SaError(SA_E_ARG, "null output");
         ss->eng->probe(what, tauVarVec, tauVarNum, tauPrmVec, tauPrmNum, in, out);
     });
+}
+
+int sa_session_time_product(sa_session *ss, int what, int reps, double *usPerProduct) {
+    return sa::guarded([&] {
+        if (!ss || !ss->eng) throw sa::SaError(SA_E_NOTFOUND, "null session");
+        if (!usPerProduct) throw sa::SaError(SA_E_ARG, "null output");
+        *usPerProduct = ss->eng->time_product(what, reps);
+    });
+}
+
+int sa_tile_target(const char *product, int f32, long long rows, long long cols) {
+    if (!product || rows < 1 || cols < 1) return -1;
+    const std::string p(product);
+    if (p != "xc" && p != "cx" && p != "cc" && p != "kk") return -1;
+    return sa::ks_target(product, f32 != 0, (long)rows, (long)cols, 0);
 }
 
 long long sa_session_footprint(const sa_mesh *m, const sa_target *t, const sa_params *p) {

@@ -133,3 +133,60 @@ def test_struct_layout_matches_header(cname, pyname, tmp_path):
     assert int(out["size"]) == C.sizeof(py), (cname, out["size"], C.sizeof(py))
     for f in fields:
         assert int(out[f]) == getattr(py, f).offset, (cname, f)
+
+
+def test_tile_table_lookup(tmp_path):
+    """The tile table: the entry of the product and element type nearest the
+    shape (log distance of rows and columns), comments and malformed lines
+    skipped, no entry -> 0 (the built-in default), SA_KS_TARGET over all.
+    The table is read once per process, so each case runs in its own."""
+    import subprocess
+    import sys
+    tbl = tmp_path / "tiles.txt"
+    tbl.write_text("# product type rows cols target\n"
+                   "xc f32 51210 5000 2048   # measured\n"
+                   "xc f32 2568 256 512\n"
+                   "xc f64 51210 5000 1024\n"
+                   "cx f32 5000 51210 4096\n"
+                   "kk f64 2000 2000 256\n"
+                   "xc f32 100 100 7\n"          # target out of range: skipped
+                   "zz f32 100 100 512\n"        # unknown product: skipped
+                   "garbage\n")
+    code = ("from shape_atrophy_amd import api; import json; print(json.dumps(["
+            "api.tile_target('xc', True, 51210, 5000), api.tile_target('xc', True, 40000, 4000),"
+            "api.tile_target('xc', True, 3000, 300), api.tile_target('xc', True, 100, 100),"
+            "api.tile_target('xc', False, 2568, 256), api.tile_target('cx', True, 2000, 20000),"
+            "api.tile_target('cx', False, 5000, 51210), api.tile_target('kk', False, 50000, 50000),"
+            "api.tile_target('cc', True, 100, 100), api.tile_target('qq', True, 10, 10)]))")
+
+    def run(**env):
+        e = {k: v for k, v in os.environ.items() if k != "SA_KS_TARGET"}
+        e.update(SA_TILE_TABLE=str(tbl), **env)
+        out = subprocess.run([sys.executable, "-c", code], cwd=ROOT, env=e, check=True, capture_output=True,
+                             text=True).stdout
+        return __import__("json").loads(out.strip().splitlines()[-1])
+
+    assert run() == [2048, 2048, 512, 512, 1024, 4096, 0, 256, 0, -1]
+    assert run(SA_KS_TARGET="300") == [300, 300, 300, 300, 300, 300, 300, 300, 300, -1]
+    assert run(SA_KS_TARGET="cx=640,kk=128") == [2048, 2048, 512, 512, 1024, 640, 640, 128, 0, -1]
+
+
+def test_committed_tile_table():
+    """Every line of the tile table beside the library is well formed, and
+    the library takes its entries: a shape in the table gets its own
+    target."""
+    path = os.path.join(ROOT, "shape_atrophy_amd", "tiles.txt")
+    entries = []
+    for line in open(path):
+        line = line.split("#")[0].split()
+        if not line:
+            continue
+        prod, ty, rows, cols, tgt = line
+        assert prod in ("xc", "cx", "cc", "kk") and ty in ("f32", "f64"), line
+        assert int(rows) > 0 and int(cols) > 0 and 16 <= int(tgt) <= 65536, line
+        entries.append((prod, ty == "f32", int(rows), int(cols), int(tgt)))
+    assert entries
+    if os.environ.get("SA_TILE_TABLE") is None and os.environ.get("SA_KS_TARGET") is None:
+        for prod, f32, rows, cols, tgt in entries:
+            same = [e[4] for e in entries if e[:4] == (prod, f32, rows, cols)]
+            assert api.tile_target(prod, f32, rows, cols) == same[0], (prod, f32, rows, cols)

@@ -472,6 +472,55 @@ def test_probe_control_point_products(which, precision, env, tol, oracle, monkey
     assert e1 < tol and e2 < tol
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_tile_targets_change_only_the_summation_order(precision, monkeypatch):
+    """The block target of the stored products (the tile table's entry,
+    forced here with SA_KS_TARGET) sets their column slices, so it changes
+    only the order of the FP64 partial sums: products agree to 1e-12 across
+    targets, a solve's objective to 1e-8; the product timer returns a
+    positive time for every product."""
+    p = W.shell_problem(level=3, n_layers=5, time_steps=4)
+    V = p["q0"].shape[1]
+    p["ctlIdx"] = np.sort(np.random.default_rng(21).choice(V, 700, replace=False)).astype(np.int32)
+    rng = np.random.default_rng(13)
+    u, y = rng.standard_normal((3, 700)), rng.standard_normal((3, V))
+    tv, tp = p["tauVar"][:, 0], p["tauPrm"]
+    out = {}
+    for tgt in ("64", "16384"):
+        monkeypatch.setenv("SA_KS_TARGET", tgt)
+        print(f"[gpu] tile target {tgt}, {precision}, V={V} C=700", flush=True)
+        s = api.Session(p, precision=precision)
+        kx, kc = s.probe("kxc", tv, tp, u), s.probe("kcx", tv, tp, y)
+        us = {w: s.time_product(w, 5) for w in ("kxc", "kcx", "precond")}
+        r = s.run(tv, tp)
+        s.close()
+        print(f"[gpu] tile target {tgt}: {us} us, obj {r['obj']:.15e}", flush=True)
+        assert r["status"] == 0
+        assert all(np.isfinite(v) and v > 0 for v in us.values()), us
+        out[tgt] = (kx, kc, r["obj"])
+    a, b = out["64"], out["16384"]
+    assert rel(a[0], b[0]) < 1e-12 and rel(a[1], b[1]) < 1e-12
+    assert abs(a[2] - b[2]) <= 1e-8 * abs(b[2])
+
+
+def test_tile_targets_all_landmark_product(oracle, monkeypatch):
+    """The same for the stored all-landmark product: (K + eps I) x at two
+    block targets against the oracle's (1e-12), and its timer."""
+    p = small_shell()
+    V = p["q0"].shape[1]
+    x = np.random.default_rng(4).standard_normal(p["q0"].shape)
+    prm = p["params"]
+    ref = oracle.kernel_matvec(3, prm["knlOrder"], prm["knlWidth"], prm["knlEps"], p["q0"], x)
+    for tgt in ("16", "4096"):
+        monkeypatch.setenv("SA_KS_TARGET", f"kk={tgt}")
+        print(f"[gpu] all-landmark product, target {tgt}, V={V}", flush=True)
+        s = api.Session(p)
+        got = s.probe("kmatvec", p["tauVar"][:, 0], p["tauPrm"], x).reshape(x.shape)
+        us = s.time_product("kmatvec", 5)
+        s.close()
+        assert rel(got, ref) < 1e-12 and np.isfinite(us) and us > 0
+
+
 # --------------------------------------------------------------------------
 # Jacobi preconditioning of the control-point CG (sa_params.velPrec): the
 # diagonal of the control-point system against the oracle's, formed there by
