@@ -347,13 +347,16 @@ TUNE_TARGETS = (32, 64, 128, 256, 512, 1024, 2048, 4096, 8192, 16384)
 # TUNE_KEEP faster: two sweeps on different boxes differed by up to 5% at the
 # same setting on the large products
 TUNE_DEFAULT = {("xc", "f32"): 2048, ("cx", "f32"): 2048, ("xc", "f64"): 1024, ("cx", "f64"): 1024,
-                ("cc", "f32"): 1024, ("cc", "f64"): 1024, ("kk", "f64"): 2048}
+                ("cc", "f32"): 1024, ("cc", "f64"): 1024, ("kk", "f64"): 2048,
+                ("op", "f64"): 1}   # the fused operator apply's launch shape (index, 0-3)
+OP_SHAPES = (0, 1, 2, 3)
 TUNE_KEEP = 1.05
 
 
 def tune_pick(prod: str, ty: str, times: dict) -> int:
     """The target the table keeps: the built-in one unless another is more
-    than TUNE_KEEP faster, then the smallest within 1% of the fastest."""
+    than TUNE_KEEP faster, then the smallest within 1% of the fastest (for
+    the operator, the launch shape)."""
     best = min(times.values())
     dflt = TUNE_DEFAULT[(prod, ty)]
     if dflt in times and times[dflt] <= TUNE_KEEP * best:
@@ -381,14 +384,36 @@ def tune_shapes(W):
 def mode_tune(args, api, W):
     """Offline sweep of the tile table: for each mesh, precision and stored
     product, the device time of one product (kernel and slice sum, the mean
-    of --tune-reps launches, best of three) at each block target; the table
-    keeps the target tune_pick chooses.  Writes the table to
+    of --tune-reps launches, best of three) at each block target, and for
+    each mesh the fused operator apply at each launch shape; the table keeps
+    what tune_pick chooses.  Writes the table to
     gpurun_out/tiles.txt (copy it beside the library to use it)."""
     rows, table = [], []
     names = {"xc": "kxc", "cx": "kcx", "cc": "precond", "kk": "kmatvec"}
+    op_done = set()
     for desc, p in tune_shapes(W):
         d, V = p["q0"].shape
         nc = len(p["ctlIdx"]) if p.get("ctlIdx") is not None else 0
+        E = p["elm"]["elmVtxMat"].shape[1]
+        if (E, V) not in op_done:
+            # the fused operator apply: one launch shape per session
+            op_done.add((E, V))
+            times = {}
+            for c in OP_SHAPES:
+                os.environ["SA_OPF_CFG"] = str(c)
+                s = api.Session(p)
+                try:
+                    s.probe("elastic", p["tauVar"][:, 0], p["tauPrm"], np.zeros((d, V)))
+                    times[c] = min(s.time_product("operator", args.tune_reps) for _ in range(3))
+                finally:
+                    s.close()
+            os.environ.pop("SA_OPF_CFG", None)
+            pick = tune_pick("op", "f64", times)
+            table.append(f"op f64 {E} {V} {pick}")
+            rows.append({"mesh": desc, "precision": "fp64", "product": "op", "rows": E, "cols": V,
+                         "us": {str(t): round(v, 2) for t, v in times.items()}, "target": pick})
+            print(f"[tune] {desc} operator (E {E}, V {V}): " + " ".join(f"{t}:{v:.1f}" for t, v in times.items())
+                  + f" -> {pick}", file=sys.stderr, flush=True)
         precs = ("mixed", "fp64") if nc else ("fp64",)
         for prec in precs:
             f32 = prec == "mixed"
@@ -413,7 +438,7 @@ def mode_tune(args, api, W):
     os.environ.pop("SA_KS_TARGET", None)
     os.makedirs("gpurun_out", exist_ok=True)
     with open(os.path.join("gpurun_out", "tiles.txt"), "w") as f:
-        f.write("# product type rows cols target-blocks  (python bench.py --mode tune)\n")
+        f.write("# product type rows cols target-blocks; op: elements landmarks launch-shape  (python bench.py --mode tune)\n")
         f.write("\n".join(table) + "\n")
     return {"metric": "tile sweep of the stored kernel products", "value": None, "unit": "us per product",
             "n_gpus": 1, "steps": None, "warmup": None, "ms_per_step": None, "higher_is_better": False,

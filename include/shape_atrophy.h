@@ -235,12 +235,26 @@ long long sa_session_footprint(const sa_mesh *mesh, const sa_target *tgt, const 
    preconditioner product; kk: K over all landmarks).  A session takes the
    entry of its product and type nearest its shape.  SA_TILE_TABLE names
    another file; SA_KS_TARGET ("N" or "xc=N,cx=M,...") overrides the table.
+   A line "op f64 <elmNum> <lmkNum> <shape>" sets the launch shape (0-3) of
+   the fused operator apply for meshes near that size (SA_OPF_CFG overrides).
    sa_tile_target returns the target a new session would take (0: no entry,
-   the built-in default applies; -1: bad argument).
+   the built-in default applies; -1: bad argument); for product "op" (rows =
+   elmNum, cols = lmkNum) the launch shape a session would take.
    sa_session_time_product times one product as a solve launches it (product
    kernel and slice sum) at q0: the mean over reps launches after a warm-up,
    in microseconds; what = SA_PROBE_KXC, SA_PROBE_KCX, SA_PROBE_KMATVEC (the
-   all-landmark form) or SA_TIME_PRECOND. */
+   all-landmark form), SA_TIME_PRECOND, or SA_PROBE_ELASTIC (the operator
+   apply (E + btmWgt P) x at the geometry of the last run's or probe's start). */
+/* The vertex chunks of the fused operator apply (one launch: element forces
+   in LDS, summed per vertex in vtxElmMat order), built on the host from a
+   host mesh and positions as a session builds them, and checked against the
+   invariants the kernel's indexing relies on (no counterpart in the
+   reference; for tests).  chunks: launch blocks; elementEntries: element
+   evaluations per apply (at least elmNum); haloEntries: vertices staged in
+   LDS per apply.  SA_E_ARG when a check fails or a vertex's elements pass
+   a chunk's capacity (such meshes keep the two-launch apply). */
+int sa_op_chunk_check(const sa_mesh *mesh, const double *lmkIniMat, int *chunks, long long *elementEntries,
+                      long long *haloEntries);
 #define SA_TIME_PRECOND 11
 int sa_tile_target(const char *product, int f32, long long rows, long long cols);
 int sa_session_time_product(sa_session *s, int what, int reps, double *usPerProduct);

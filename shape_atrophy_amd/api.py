@@ -383,11 +383,12 @@ class Session:
                                       _dp(out)))
         return out
 
-    TIMED = {"kxc": 8, "kcx": 9, "kmatvec": 2, "precond": 11}
+    TIMED = {"kxc": 8, "kcx": 9, "kmatvec": 2, "precond": 11, "operator": 1}
 
     def time_product(self, what: str, reps: int = 20) -> float:
         """Device microseconds of one kernel product as a solve launches it
-        (sa_session_time_product): "kxc", "kcx", "kmatvec" or "precond"."""
+        (sa_session_time_product): "kxc", "kcx", "kmatvec", "precond" or
+        "operator" (the operator apply at the last start's geometry)."""
         us = C.c_double()
         _check(lib().sa_session_time_product(self._h, self.TIMED[what], int(reps), C.byref(us)))
         return us.value

@@ -11,7 +11,7 @@
 namespace sa {
 
 constexpr int NT = 256;          // threads per block of the streaming kernels
-constexpr int MAXP = 8192;       // capacity of a partial-sum buffer
+constexpr int MAXP = 32768;      // capacity of a partial-sum buffer
 
 // geometry record per element, SoA over elements: [Q^{-1} (D*D, row-major),
 // det Q, frame 1 (D), frame 2 (D)]

@@ -50,6 +50,8 @@ std::mutex g_libMu;
 // names the product xc (K_xc: landmarks x control points), cx (K_cx), cc
 // (the C x C preconditioner M) or kk (K over all landmarks), the stored
 // element type and the shape it was measured on; '#' starts a comment.  A
+// line "op f64 <elements> <landmarks> <shape>" gives instead the launch
+// shape (index into OPF_SHAPES) of the fused operator apply.  A
 // session takes the entry of its product and type nearest its shape (sum of
 // the log distances of rows and columns), else the built-in default.
 // SA_TILE_TABLE names another file ("" reads none); SA_KS_TARGET overrides
@@ -88,8 +90,9 @@ std::vector<TileEntry> load_tile_table() {
         int tg;
         if (std::sscanf(line, "%7s %7s %ld %ld %d", prod, ty, &rows, &cols, &tg) != 5) continue;
         const std::string p(prod), y(ty);
-        if ((p != "xc" && p != "cx" && p != "cc" && p != "kk") || (y != "f32" && y != "f64") || rows < 1 ||
-            cols < 1 || tg < 16 || tg > 65536)
+        const bool op = p == "op";
+        if ((p != "xc" && p != "cx" && p != "cc" && p != "kk" && !op) || (y != "f32" && y != "f64") || rows < 1 ||
+            cols < 1 || (op ? (tg < 0 || tg >= OPF_NSHAPES) : (tg < 16 || tg > 65536)))
             continue;
         std::memcpy(en.prod, prod, 3);
         en.f32 = y == "f32";
@@ -105,6 +108,18 @@ std::vector<TileEntry> load_tile_table() {
 const std::vector<TileEntry> &tile_table() {
     static const std::vector<TileEntry> t = load_tile_table();
     return t;
+}
+
+// the table's entry of a product and type nearest a shape (dflt: none)
+int tile_lookup(const char *prod, bool f32, long rows, long cols, int dflt) {
+    const TileEntry *best = nullptr;
+    double bd = 0.0;
+    for (const auto &en : tile_table()) {
+        if (std::strcmp(en.prod, prod) || en.f32 != f32) continue;
+        const double d = std::fabs(std::log(en.rows / (double)rows)) + std::fabs(std::log(en.cols / (double)cols));
+        if (!best || d < bd) { best = &en; bd = d; }
+    }
+    return best ? best->target : dflt;
 }
 
 // target block count of one stored product (see TileEntry)
@@ -123,14 +138,7 @@ int ks_target(const char *prod, bool f32, long rows, long cols, int dflt) {
             if (v >= 16 && v <= 65536) return v;
         }
     }
-    const TileEntry *best = nullptr;
-    double bd = 0.0;
-    for (const auto &en : tile_table()) {
-        if (std::strcmp(en.prod, prod) || en.f32 != f32) continue;
-        const double d = std::fabs(std::log(en.rows / (double)rows)) + std::fabs(std::log(en.cols / (double)cols));
-        if (!best || d < bd) { best = &en; bd = d; }
-    }
-    return best ? best->target : dflt;
+    return tile_lookup(prod, f32, rows, cols, dflt);
 }
 
 #ifndef SA_MIXED_CC32
@@ -337,6 +345,169 @@ std::vector<int> morton_order(const double *p, int n, int D) {
     return idx;
 }
 
+// Chunk tables of the fused operator apply (op_fused_k), built from the
+// count-prefixed vertex -> element lists hVE (vtxElmMat layout), the element
+// table hElm (elmVtxMat layout) and a vertex order ord: vertices taken in
+// that order while a chunk has at most nt vertices, maxs list entries and
+// hm halo vertices (the vertices of the elements its lists name); per chunk
+// each such element once (ascending element id) with the halo position of
+// each of its vertices and the LDS slot of each listed (element, local
+// vertex) pair.  loc and slot are SoA by local vertex: [l NU + u].  An empty
+// result: a vertex whose elements alone pass hm halo vertices.
+struct OpTables {
+    std::vector<int> chV{0}, chE{0}, chH{0}, chVtx, chLs, chHv, chElm, loc, slot;
+};
+
+OpTables op_tables(const std::vector<int> &hVE, const std::vector<int> &hElm, int V, int E, int D,
+                   const std::vector<int> &ord, int nt, int maxs, int hm) {
+    OpTables t;
+    std::vector<int> aos, laos, mark(E, -1), hmark(V, -1), stamp(V, -1), perm, tmpE, tmpS, tmpL;
+    int nv = 0, ns = 0, nh = 0, pass = 0;
+    auto close_chunk = [&] {
+        // the chunk's element entries in ascending element order
+        const int u0 = t.chE.back(), u1 = (int)t.chElm.size();
+        perm.resize(u1 - u0);
+        for (int u = u0; u < u1; ++u) perm[u - u0] = u;
+        std::sort(perm.begin(), perm.end(), [&](int a, int b) { return t.chElm[a] < t.chElm[b]; });
+        tmpE.assign(perm.size(), 0);
+        tmpS.assign(perm.size() * (D + 1), 0);
+        tmpL.assign(perm.size() * (D + 1), 0);
+        for (size_t r = 0; r < perm.size(); ++r) {
+            tmpE[r] = t.chElm[perm[r]];
+            for (int l = 0; l <= D; ++l) {
+                tmpS[r * (D + 1) + l] = aos[(size_t)perm[r] * (D + 1) + l];
+                tmpL[r * (D + 1) + l] = laos[(size_t)perm[r] * (D + 1) + l];
+            }
+        }
+        for (size_t r = 0; r < perm.size(); ++r) {
+            t.chElm[u0 + r] = tmpE[r];
+            mark[tmpE[r]] = -1;
+            for (int l = 0; l <= D; ++l) {
+                aos[(size_t)(u0 + r) * (D + 1) + l] = tmpS[r * (D + 1) + l];
+                laos[(size_t)(u0 + r) * (D + 1) + l] = tmpL[r * (D + 1) + l];
+            }
+        }
+        for (int h = t.chH.back(); h < (int)t.chHv.size(); ++h) hmark[t.chHv[h]] = -1;
+        t.chV.push_back((int)t.chVtx.size());
+        t.chE.push_back(u1);
+        t.chH.push_back((int)t.chHv.size());
+        nv = ns = nh = 0;
+    };
+    // halo vertices vertex i would add to the open chunk
+    auto growth = [&](int i) {
+        ++pass;
+        int g = 0;
+        for (int a = 0; a < hVE[i]; ++a) {
+            const int e = hVE[(size_t)(1 + 2 * a) * V + i];
+            if (mark[e] >= 0) continue;
+            for (int l = 0; l <= D; ++l) {
+                const int w = hElm[(size_t)l * E + e];
+                if (hmark[w] < 0 && stamp[w] != pass) { stamp[w] = pass; ++g; }
+            }
+        }
+        return g;
+    };
+    for (int i : ord) {
+        const int adj = hVE[i];
+        int g = growth(i);
+        if (nv > 0 && (nv == nt || ns + adj > maxs || nh + g > hm)) {
+            close_chunk();
+            g = growth(i);
+        }
+        if (adj > maxs || g > hm) return OpTables{};
+        t.chVtx.push_back(i);
+        t.chLs.push_back(ns);
+        for (int a = 0; a < adj; ++a) {
+            const int e = hVE[(size_t)(1 + 2 * a) * V + i], l = hVE[(size_t)(2 + 2 * a) * V + i];
+            if (mark[e] < 0) {
+                mark[e] = (int)t.chElm.size();
+                t.chElm.push_back(e);
+                aos.insert(aos.end(), D + 1, -1);
+                for (int l2 = 0; l2 <= D; ++l2) {
+                    const int w = hElm[(size_t)l2 * E + e];
+                    if (hmark[w] < 0) {
+                        hmark[w] = nh++;
+                        t.chHv.push_back(w);
+                    }
+                    laos.push_back(hmark[w]);
+                }
+            }
+            aos[(size_t)mark[e] * (D + 1) + l] = ns + a;
+        }
+        ns += adj;
+        ++nv;
+    }
+    if (nv > 0) close_chunk();
+    const size_t NU = t.chElm.size();
+    t.slot.assign((size_t)(D + 1) * NU, -1);
+    t.loc.assign((size_t)(D + 1) * NU, -1);
+    for (size_t u = 0; u < NU; ++u)
+        for (int l = 0; l <= D; ++l) {
+            t.slot[(size_t)l * NU + u] = aos[u * (D + 1) + l];
+            t.loc[(size_t)l * NU + u] = laos[u * (D + 1) + l];
+        }
+    return t;
+}
+
+// the invariants op_fused_k's indexing relies on: every vertex in exactly
+// one chunk of at most nt vertices; a chunk's list entries (at most maxs)
+// each own one LDS slot, written by exactly one (element, local vertex)
+// pair of that chunk; its halo (at most hm vertices) holds each element
+// entry's vertices at the positions loc names; ids in range
+void check_op_tables(const OpTables &t, const std::vector<int> &hVE, const std::vector<int> &hElm, int V, int E,
+                     int D, int nt, int maxs, int hm) {
+    const int K = (int)t.chV.size() - 1;
+    const int NU = (int)t.chElm.size();
+    require(K >= 1 && t.chV[0] == 0 && t.chV[K] == V && (int)t.chVtx.size() == V && (int)t.chLs.size() == V &&
+                (int)t.chE.size() == K + 1 && t.chE[0] == 0 && t.chE[K] == NU && (int)t.chH.size() == K + 1 &&
+                t.chH[0] == 0 && t.chH[K] == (int)t.chHv.size() && t.slot.size() == (size_t)(D + 1) * NU &&
+                t.loc.size() == (size_t)(D + 1) * NU,
+            "operator chunk tables: bad shape");
+    std::vector<char> seen(V, 0), wr;
+    for (int k = 0; k < K; ++k) {
+        const int v0 = t.chV[k], v1 = t.chV[k + 1], h0 = t.chH[k], nh = t.chH[k + 1] - h0;
+        require(v1 > v0 && v1 - v0 <= nt && t.chE[k + 1] >= t.chE[k] && nh >= 0 && nh <= hm,
+                "operator chunk tables: bad chunk");
+        for (int h = h0; h < h0 + nh; ++h) require(t.chHv[h] >= 0 && t.chHv[h] < V, "operator chunk tables: bad halo");
+        int ns = 0;
+        for (int j = v0; j < v1; ++j) {
+            const int i = t.chVtx[j];
+            require(i >= 0 && i < V && !seen[i] && t.chLs[j] == ns, "operator chunk tables: bad vertex entry");
+            seen[i] = 1;
+            ns += hVE[i];
+        }
+        require(ns <= maxs, "operator chunk tables: chunk over its slots");
+        wr.assign(ns, 0);
+        for (int u = t.chE[k]; u < t.chE[k + 1]; ++u) {
+            const int e = t.chElm[u];
+            require(e >= 0 && e < E, "operator chunk tables: element out of range");
+            for (int l = 0; l <= D; ++l) {
+                const int lc = t.loc[(size_t)l * NU + u];
+                require(lc >= 0 && lc < nh && t.chHv[h0 + lc] == hElm[(size_t)l * E + e],
+                        "operator chunk tables: halo position wrong");
+                const int sl = t.slot[(size_t)l * NU + u];
+                if (sl < 0) continue;
+                require(sl < ns && !wr[sl], "operator chunk tables: slot out of range or written twice");
+                wr[sl] = 1;
+            }
+        }
+        for (int a = 0; a < ns; ++a) require(wr[a], "operator chunk tables: slot never written");
+    }
+}
+
+// the launch shape of the fused operator apply for a mesh of E elements
+// and V vertices: SA_OPF_CFG (index into OPF_SHAPES), else the tile table's
+// "op" entry nearest the mesh, else OPF_DEFAULT
+constexpr int OPF_DEFAULT = 1;
+int opf_cfg(long E, long V) {
+    if (const char *e = std::getenv("SA_OPF_CFG")) {
+        const int c = std::atoi(e);
+        if (c >= 0 && c < OPF_NSHAPES) return c;
+    }
+    const int c = tile_lookup("op", false, std::max(E, 1L), std::max(V, 1L), OPF_DEFAULT);
+    return c >= 0 && c < OPF_NSHAPES ? c : OPF_DEFAULT;
+}
+
 struct Stats {
     int status = 0, steps = 0, pcgIters = 0, rdIters = 0, refine = 0, fp64Steps = 0, failReason = 0;
     double kmvMs = 0.0, opMs = 0.0;   // summed durations of the timed kernels (profiling on)
@@ -409,6 +580,12 @@ struct Engine : EngineBase {
     // vtxStart[i] where vertex i's list starts (absent when an (e, l) is
     // listed twice: the element-ordered gather then runs instead)
     DBuf<int> elmSlot, vtxStart;
+    // the operator apply in one launch (op_fused_k): vertex chunks and the
+    // elements their lists name; SA_OP_FUSED=0 keeps the two-launch apply
+    // (elastic_elem_k + gather_op_k)
+    DBuf<int> opChV, opChVtx, opChLs, opChH, opChHv, opChE, opChElm, opChLoc, opChSlot;
+    int opChunks = 0, opNU = 0, opCapU = 0, opCfg = 0;
+    bool opFused = false;
     // mixed form on the matrix cores (kx_mfma_k): the kernel products of the
     // inner solve evaluated matrix-free, nothing stored; landmarks and
     // control points visited in Morton order (position -> index)
@@ -730,6 +907,7 @@ struct Engine : EngineBase {
             hq0 = to_host(lmkIni, (size_t)V * D, dev);
             q0.upload(hq0.data(), hq0.size(), s);
         }
+        if (vtxStart.n) build_op_chunks(hVtxElm, hElm, hq0, dryRun);
 
         if (tg) {
             hasTarget = true;
@@ -1009,9 +1187,103 @@ struct Engine : EngineBase {
         CHECK_LAUNCH();
     }
 
+    // Vertex chunks of the fused operator apply: vertices in Morton order of
+    // q0, taken greedily while a chunk has at most nt vertices and maxs list
+    // entries (the launch shape); per chunk, each element its vertices' lists name once,
+    // with the LDS slot of each of its listed vertices.  The buffers are
+    // sized from the tables alone (at most one element entry per list entry;
+    // chunk count bounded below), so the sizing pass, which has no positions,
+    // counts the same bytes.
+    void build_op_chunks(const std::vector<int> &hVE, const std::vector<int> &hElm, const std::vector<double> &hq,
+                         bool dryRun) {
+        const char *env = std::getenv("SA_OP_FUSED");
+        if (env && env[0] == '0') return;
+        opCfg = opf_cfg(E, V);
+        const int nt = OPF_SHAPES[opCfg].nt, maxs = OPF_SHAPES[opCfg].maxs, hm = OPF_SHAPES[opCfg].halo;
+        int maxAdj = 0;
+        long S = 0;
+        for (int i = 0; i < V; ++i) { maxAdj = std::max(maxAdj, hVE[i]); S += hVE[i]; }
+        // (a vertex's elements then have at most (D + 1) maxAdj <= hm vertices)
+        if (V == 0 || maxAdj > maxs || (long)(D + 1) * maxAdj > hm || S > INT32_MAX / (D + 1)) return;
+        // a chunk is closed only when the next vertex v would pass nt
+        // vertices, maxs entries or hm halo vertices.  A chunk's halo has at
+        // most D + 1 vertices per list entry, so in the last two cases the
+        // closed chunk and the next (which starts with v) hold together more
+        // than min(maxs, hm / (D + 1)) entries; each chunk is in at most two
+        // such pairs, so there are fewer than 2 S / min(...) + V / nt + 1
+        // chunks
+        const long capCh = V / nt + 2 * (S / maxs) + 2 * ((long)(D + 1) * S / hm) + 6;
+        // the all-landmark CG consumes one partial per chunk
+        if (NC == 0 && capCh > MAXP) return;
+        opCapU = (int)std::max(S, 1L);
+        const long capH = std::min((long)(D + 1) * S, capCh * (long)hm);
+        long long &tot = bytes;
+        opChV.alloc(capCh + 1, tot); opChE.alloc(capCh + 1, tot); opChH.alloc(capCh + 1, tot);
+        opChVtx.alloc(V, tot); opChLs.alloc(V, tot); opChHv.alloc(std::max(capH, 1L), tot);
+        opChElm.alloc(opCapU, tot);
+        opChSlot.alloc((size_t)(D + 1) * opCapU, tot); opChLoc.alloc((size_t)(D + 1) * opCapU, tot);
+        if (dryRun) { opFused = true; return; }
+        const OpTables t = op_tables(hVE, hElm, V, E, D, morton_order(hq.data(), V, D), nt, maxs, hm);
+        if (t.chVtx.empty()) {
+            // not reached (the halo bound above); keeps the two-launch apply
+            for (DBuf<int> *b : {&opChV, &opChE, &opChH, &opChVtx, &opChLs, &opChHv, &opChElm, &opChSlot, &opChLoc})
+                b->alloc(0, tot);
+            return;
+        }
+        check_op_tables(t, hVE, hElm, V, E, D, nt, maxs, hm);
+        opChunks = (int)t.chV.size() - 1;
+        opNU = (int)t.chElm.size();
+        require(opChunks <= capCh && opNU <= opCapU && (long)t.chHv.size() <= capH,
+                "operator chunk tables exceed their bound");
+        opChV.upload(t.chV.data(), t.chV.size(), s); opChE.upload(t.chE.data(), t.chE.size(), s);
+        opChH.upload(t.chH.data(), t.chH.size(), s);
+        opChVtx.upload(t.chVtx.data(), t.chVtx.size(), s); opChLs.upload(t.chLs.data(), t.chLs.size(), s);
+        opChHv.upload(t.chHv.data(), t.chHv.size(), s);
+        opChElm.upload(t.chElm.data(), t.chElm.size(), s); opChSlot.upload(t.slot.data(), t.slot.size(), s);
+        opChLoc.upload(t.loc.data(), t.loc.size(), s);
+        CK(hipStreamSynchronize(s));   // the host vectors go out of scope
+        opFused = true;
+    }
+
+    // partials the operator apply leaves for the all-landmark CG (MODE 0, 1)
+    int op_parts() const { return opFused ? opChunks : gridOf(V); }
+
+    // y = (E + btmWgt P) x in the CG mode MODE of gather_op_k (positions q,
+    // concentration tauNow, geometry geoNow of q; pen = false: E x alone)
+    template <int MODE>
+    void op_apply(const double *q, const double *xin, const double *tauNow, const double *geoNow, const double *u,
+                  const double *w, double *out, double *out2, double *P, bool pen = true) {
+        if (!opFused) {
+            elastic(xin, tauNow, geoNow, ctl.p, pen);
+            gather_op<MODE>(u, w, out, out2, P, pen);
+            return;
+        }
+        const bool pn = pen && btmWgt != 0.0 && B > 0;
+        if (pn) {
+            pen_elem_k<D><<<gridOf(B), NT, 0, s>>>(B, V, btmVtx.p, meas.p, xin, pfrc.p, ctl.p);
+            CHECK_LAUNCH();
+        }
+#define SA_OPF(ISO, BT, MS, HM) op_fused_k<D, ISO, MODE, BT, MS, HM><<<opChunks, BT, 0, s>>>(                 \
+            opChV.p, opChVtx.p, opChLs.p, opChH.p, opChHv.p, opChE.p, opChElm.p, opChLoc.p, opChSlot.p, opNU, E,  \
+            V, vtxStart.p, q, geoNow, xin, tauNow, elmLyr.p, dmod.p, BL, B, vtxBtm.p, pn ? pfrc.p : nullptr,      \
+            btmWgt, ldmWgt, u, w, out, out2, P, ctl.p)
+#define SA_OPF_SHAPE(BT, MS, HM) \
+        if (elaType == SA_ELA_FRAME) SA_OPF(false, BT, MS, HM); else SA_OPF(true, BT, MS, HM)
+        static_assert(OPF_NSHAPES == 4, "one case per launch shape");
+        switch (opCfg) {
+            case 0: SA_OPF_SHAPE(256, 1024, 512); break;
+            case 1: SA_OPF_SHAPE(512, 1024, 512); break;
+            case 2: SA_OPF_SHAPE(512, 2048, 1024); break;
+            default: SA_OPF_SHAPE(1024, 2048, 1024); break;
+        }
+#undef SA_OPF_SHAPE
+#undef SA_OPF
+        CHECK_LAUNCH();
+    }
+
     // element pass of E x with the geometry geoNow; the forces land in vertex
     // order when the slot map exists
-    void elastic(const double *xin, const double *tauNow, const double *geoNow, const Ctl *c) {
+    void elastic(const double *xin, const double *tauNow, const double *geoNow, const Ctl *c, bool pen = true) {
         int g = gridOf(E);
         if (elaType == SA_ELA_FRAME)
             elastic_elem_k<D, false, double><<<g, NT, 0, s>>>(E, V, elmVtx.p, geoNow, xin, tauNow, elmLyr.p, dmod.p,
@@ -1020,15 +1292,15 @@ struct Engine : EngineBase {
             elastic_elem_k<D, true, double><<<g, NT, 0, s>>>(E, V, elmVtx.p, geoNow, xin, tauNow, elmLyr.p, dmod.p,
                                                              frc.p, c, elmSlot.p);
         CHECK_LAUNCH();
-        if (btmWgt != 0.0 && B > 0) {
+        if (pen && btmWgt != 0.0 && B > 0) {
             pen_elem_k<D><<<gridOf(B), NT, 0, s>>>(B, V, btmVtx.p, meas.p, xin, pfrc.p, c);
             CHECK_LAUNCH();
         }
     }
 
     template <int MODE>
-    void gather_op(const double *u, const double *w, double *out, double *out2, double *P) {
-        const double *pf = (btmWgt != 0.0 && B > 0) ? pfrc.p : nullptr;
+    void gather_op(const double *u, const double *w, double *out, double *out2, double *P, bool pen = true) {
+        const double *pf = (pen && btmWgt != 0.0 && B > 0) ? pfrc.p : nullptr;
         gather_op_k<D, MODE><<<gridOf(V), NT, 0, s>>>(V, E, vtxElm.p, frc.p, BL, B, vtxBtm.p, pf, btmWgt, ldmWgt, u, w,
                                                      out, out2, P, ctl.p, vtxStart.p);
         CHECK_LAUNCH();
@@ -1052,9 +1324,8 @@ struct Engine : EngineBase {
         CHECK_LAUNCH();
         if (warm) kmatvec<false>(q, a.p, x.p, nullptr, 0, -1);   // x0 = K alpha_{t-1}
         // r = b - A x0, Kid = r
-        elastic(x.p, tauNow, geoNow, ctl.p);
-        gather_op<1>(a.p, f.p, r.p, Kid.p, P2.p);
-        kmatvec<true>(q, r.p, Kr.p, P2.p, gV, -1);
+        op_apply<1>(q, x.p, tauNow, geoNow, a.p, f.p, r.p, Kid.p, P2.p);
+        kmatvec<true>(q, r.p, Kr.p, P2.p, op_parts(), -1);
         pcg_dir_k<<<gN, NT, 0, s>>>(n, P3.p, gM, hist.p, -1, r.p, Kr.p, Kid.p, d.p, ctl.p);
         CHECK_LAUNCH();
         int it0 = 0, chunk = 4;
@@ -1065,10 +1336,10 @@ struct Engine : EngineBase {
             for (int it = it0; it < itEnd; ++it) {
                 hipEvent_t *ev = timer.on ? &timer.ev[6 * (it - it0)] : nullptr;
                 if (ev) CK(hipEventRecord(ev[2], s));
-                elastic(d.p, tauNow, geoNow, ctl.p);
-                gather_op<0>(Kid.p, d.p, Ad.p, nullptr, P1.p);
+                op_apply<0>(q, d.p, tauNow, geoNow, Kid.p, d.p, Ad.p, nullptr, P1.p);
                 if (ev) CK(hipEventRecord(ev[3], s));
-                pcg_update_k<<<gN, NT, 0, s>>>(n, P1.p, gV, hist.p, it, x.p, a.p, r.p, d.p, Kid.p, Ad.p, P2.p, ctl.p);
+                pcg_update_k<<<gN, NT, 0, s>>>(n, P1.p, op_parts(), hist.p, it, x.p, a.p, r.p, d.p, Kid.p, Ad.p,
+                                               P2.p, ctl.p);
                 CHECK_LAUNCH();
                 if (ev) CK(hipEventRecord(ev[0], s));
                 kmatvec<true>(q, r.p, Kr.p, P2.p, gN, it);
@@ -1526,8 +1797,7 @@ struct Engine : EngineBase {
                  hipEvent_t *ev, bool fused = false, const DirArgs *dir = nullptr) {
         kx_product(md, true, q, u, Kr.p, nullptr, ctl.p, ev, fused ? ctl.p : nullptr, true, dir);   // w = K_xc u
         if (ev) CK(hipEventRecord(ev[2], s));
-        elastic(Kr.p, tauNow, geoNow, ctl.p);
-        gather_op<2>(nullptr, nullptr, Ad.p, nullptr, nullptr);                 // B w
+        op_apply<2>(q, Kr.p, tauNow, geoNow, nullptr, nullptr, Ad.p, nullptr, nullptr);   // B w
         if (ev) CK(hipEventRecord(ev[3], s));
         kx_product(md, false, q, Ad.p, cY.p, nullptr, ctl.p, ev ? ev + 4 : nullptr, nullptr, !fused);   // K_cx B w
         if (md == KMode::Stored32 && Kc.p) cc_product(Kc.p, u, cK.p);
@@ -2142,11 +2412,8 @@ struct Engine : EngineBase {
         switch (what) {
             case SA_PROBE_ELASTIC:
                 CK(hipMemcpyAsync(x.p, in, sizeof(double) * n, hipMemcpyHostToDevice, s));
-                elastic(x.p, tau.p, geo[0].p, ctl.p);
-                // E x alone (no penalty term)
-                gather_op_k<D, 2><<<gridOf(V), NT, 0, s>>>(V, E, vtxElm.p, frc.p, BL, B, vtxBtm.p, nullptr, 0.0, 0.0,
-                                                           nullptr, nullptr, f.p, nullptr, nullptr, ctl.p, vtxStart.p);
-                CHECK_LAUNCH();
+                // E x alone (no penalty term), as a solve applies it
+                op_apply<2>(q0.p, x.p, tau.p, geo[0].p, nullptr, nullptr, f.p, nullptr, nullptr, false);
                 CK(hipMemcpyAsync(out, f.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
                 break;
             case SA_PROBE_PENALTY:
@@ -2266,6 +2533,15 @@ struct Engine : EngineBase {
                 if (Kst.p) kst_build(q0.p);
                 CK(hipMemsetAsync(r.p, 0, sizeof(double) * V * D, s));
                 return timed([&] { kmatvec<false>(q0.p, r.p, Kr.p, nullptr, 0, -1); });
+            case SA_PROBE_ELASTIC:
+                // the operator apply as the control-point CG runs it
+                // ((E + btmWgt P) x at q0, tau0 of the session's last start)
+                CK(hipMemsetAsync(x.p, 0, sizeof(double) * V * D, s));
+                if (btmWgt != 0.0 && B > 0) {
+                    btm_meas_k<D><<<gridOf(B), NT, 0, s>>>(B, V, btmVtx.p, q0.p, meas.p);
+                    CHECK_LAUNCH();
+                }
+                return timed([&] { op_apply<2>(q0.p, x.p, tau.p, geo[0].p, nullptr, nullptr, f.p, nullptr, nullptr); });
             case SA_TIME_PRECOND: {
                 require(NC > 0 && !jacobi, "the session has no control-point preconditioner product");
                 CK(hipMemsetAsync(cR.p, 0, sizeof(double) * NC * D, s));
@@ -2556,9 +2832,39 @@ int sa_session_time_product(sa_session *ss, int what, int reps, double *usPerPro
     });
 }
 
+int sa_op_chunk_check(const sa_mesh *m, const double *lmkIniMat, int *chunks, long long *elementEntries,
+                      long long *haloEntries) {
+    return sa::guarded([&] {
+        if (!m || !lmkIniMat || m->on_device) throw sa::SaError(SA_E_ARG, "host mesh and positions required");
+        const int D = m->dim, V = m->lmkNum, E = m->elmNum, cols = m->vtxElmCols;
+        if ((D != 2 && D != 3) || V < 1 || E < 1 || cols < 1 || !m->vtxElmMat)
+            throw sa::SaError(SA_E_ARG, "bad mesh sizes");
+        std::vector<int> hVE(m->vtxElmMat, m->vtxElmMat + (size_t)V * cols);
+        const int cfg = sa::opf_cfg(E, V), nt = sa::OPF_SHAPES[cfg].nt, maxs = sa::OPF_SHAPES[cfg].maxs,
+                  hm = sa::OPF_SHAPES[cfg].halo;
+        if (!m->elmVtxMat) throw sa::SaError(SA_E_ARG, "bad mesh sizes");
+        std::vector<int> hElm(m->elmVtxMat, m->elmVtxMat + (size_t)E * (D + 1));
+        for (int v : hElm) sa::require(v >= 0 && v < V, "elmVtxMat index out of range");
+        for (int i = 0; i < V; ++i) {
+            sa::require(hVE[i] >= 0 && 1 + 2 * hVE[i] <= cols && hVE[i] <= maxs, "vtxElmMat count out of range");
+            for (int a = 0; a < hVE[i]; ++a) {
+                const int e = hVE[(size_t)(1 + 2 * a) * V + i], l = hVE[(size_t)(2 + 2 * a) * V + i];
+                sa::require(e >= 0 && e < E && l >= 0 && l <= D, "vtxElmMat entry out of range");
+            }
+        }
+        const sa::OpTables t = sa::op_tables(hVE, hElm, V, E, D, sa::morton_order(lmkIniMat, V, D), nt, maxs, hm);
+        sa::require(!t.chVtx.empty(), "a vertex's elements pass the halo capacity");
+        sa::check_op_tables(t, hVE, hElm, V, E, D, nt, maxs, hm);
+        if (chunks) *chunks = (int)t.chV.size() - 1;
+        if (elementEntries) *elementEntries = (long long)t.chElm.size();
+        if (haloEntries) *haloEntries = (long long)t.chHv.size();
+    });
+}
+
 int sa_tile_target(const char *product, int f32, long long rows, long long cols) {
     if (!product || rows < 1 || cols < 1) return -1;
     const std::string p(product);
+    if (p == "op") return sa::opf_cfg((long)rows, (long)cols);
     if (p != "xc" && p != "cx" && p != "cc" && p != "kk") return -1;
     return sa::ks_target(product, f32 != 0, (long)rows, (long)cols, 0);
 }

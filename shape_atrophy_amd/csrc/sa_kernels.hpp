@@ -289,36 +289,19 @@ __global__ __launch_bounds__(NT) void gather_k(int V, int E, const int *__restri
 // computeStrainTensor.cu:4-53|4-61 + frmElasticComputeKernel
 // (applyFrameElasticOperator.cu:41-85|113-164).  ISO selects the per-layer
 // isotropic law sigma = (1 - 0.5 u_bar)(lambda tr(eps) I + 2 mu eps).
-// slot non-null: the force on local
-// vertex l goes to frc[slot[l E + e] * D + c], the position of (e, l) in the
-// vertex-ordered list gather_op_k reads contiguously (slot < 0: not listed).
-template <int D, bool ISO, typename T>
-__global__ __launch_bounds__(NT) void elastic_elem_k(int E, int V, const int *__restrict__ elmVtx,
-                                                     const double *__restrict__ geo, const T *__restrict__ x,
-                                                     const double *__restrict__ tau, const int *__restrict__ elmLyr,
-                                                     const double *__restrict__ mod, T *__restrict__ frc,
-                                                     const Ctl *ctl, const int *__restrict__ slot = nullptr) {
+// Element e with vertex velocities xv, Q^{-1} inv, volume vol and (ISO)
+// mean concentration ub of its vertices; the force on its local vertex l
+// lands in fo[l].
+template <int D, bool ISO>
+__device__ inline void elem_forces(int e, int E, const double (&xv)[D + 1][D], double ub, const double (&inv)[D][D],
+                                   double vol, const double *__restrict__ geo, const int *__restrict__ elmLyr,
+                                   const double *__restrict__ mod, double (&fo)[D + 1][D]) {
     using G = GeoLayout<D>;
-    if (ctl && cld(&ctl->done)) return;
-    int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= E) return;
-    int vtx[D + 1];
-#pragma unroll
-    for (int k = 0; k <= D; ++k) vtx[k] = elmVtx[k * E + e];
-    double inv[D][D];
-#pragma unroll
-    for (int a = 0; a < D; ++a)
-#pragma unroll
-        for (int b = 0; b < D; ++b) inv[a][b] = geo[(G::INV + a * D + b) * (size_t)E + e];
-    const double vol = elem_vol<D>(geo[G::DET * (size_t)E + e]);
-    double x0[D];
-#pragma unroll
-    for (int c = 0; c < D; ++c) x0[c] = (double)x[c * V + vtx[0]];
     double Vm[D][D];
 #pragma unroll
     for (int k = 0; k < D; ++k)
 #pragma unroll
-        for (int c = 0; c < D; ++c) Vm[c][k] = (double)x[c * V + vtx[k + 1]] - x0[c];
+        for (int c = 0; c < D; ++c) Vm[c][k] = xv[k + 1][c] - xv[0][c];
     double Dv[D][D];
 #pragma unroll
     for (int a = 0; a < D; ++a)
@@ -337,10 +320,6 @@ __global__ __launch_bounds__(NT) void elastic_elem_k(int E, int V, const int *__
     double sig[D][D];
     if constexpr (ISO) {
         int l = elmLyr ? elmLyr[e] : 0;
-        double ub = 0.0;
-#pragma unroll
-        for (int k = 0; k <= D; ++k) ub += tau[vtx[k]];
-        ub /= (D + 1);
         double sc = 1.0 - 0.5 * ub;
         double lam = mod[2 * l] * sc, mu = mod[2 * l + 1] * sc;
         double tr = 0.0;
@@ -360,7 +339,6 @@ __global__ __launch_bounds__(NT) void elastic_elem_k(int E, int V, const int *__
         for (int m = 0; m <= D; ++m) md[m] = cld(&mod[m]);
         frame_stress<D>(eps, f1, f2, md, sig);
     }
-    double fo[D + 1][D];
 #pragma unroll
     for (int c = 0; c < D; ++c) {
         double s0 = 0.0;
@@ -374,6 +352,44 @@ __global__ __launch_bounds__(NT) void elastic_elem_k(int E, int V, const int *__
         }
         fo[0][c] = s0 * vol;
     }
+}
+
+// elem_forces over every element, Q^{-1} and det Q from the stored geometry.
+// slot non-null: the force on local vertex l goes to frc[slot[l E + e] * D + c],
+// the position of (e, l) in the vertex-ordered list gather_op_k reads
+// contiguously (slot < 0: not listed).
+template <int D, bool ISO, typename T>
+__global__ __launch_bounds__(NT) void elastic_elem_k(int E, int V, const int *__restrict__ elmVtx,
+                                                     const double *__restrict__ geo, const T *__restrict__ x,
+                                                     const double *__restrict__ tau, const int *__restrict__ elmLyr,
+                                                     const double *__restrict__ mod, T *__restrict__ frc,
+                                                     const Ctl *ctl, const int *__restrict__ slot = nullptr) {
+    using G = GeoLayout<D>;
+    if (ctl && cld(&ctl->done)) return;
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    int vtx[D + 1];
+#pragma unroll
+    for (int k = 0; k <= D; ++k) vtx[k] = elmVtx[k * E + e];
+    double inv[D][D];
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+#pragma unroll
+        for (int b = 0; b < D; ++b) inv[a][b] = geo[(G::INV + a * D + b) * (size_t)E + e];
+    const double vol = elem_vol<D>(geo[G::DET * (size_t)E + e]);
+    double xv[D + 1][D];
+#pragma unroll
+    for (int k = 0; k <= D; ++k)
+#pragma unroll
+        for (int c = 0; c < D; ++c) xv[k][c] = (double)x[c * V + vtx[k]];
+    double ub = 0.0;
+    if constexpr (ISO) {
+#pragma unroll
+        for (int k = 0; k <= D; ++k) ub += tau[vtx[k]];
+        ub /= (D + 1);
+    }
+    double fo[D + 1][D];
+    elem_forces<D, ISO>(e, E, xv, ub, inv, vol, geo, elmLyr, mod, fo);
     if (slot) {
 #pragma unroll
         for (int l = 0; l <= D; ++l) {
@@ -686,6 +702,145 @@ __global__ __launch_bounds__(NT) void gather_op_k(int V, int E, const int *__res
     if constexpr (MODE != 2) {
         double s = block_sum(ss, red);
         if (threadIdx.x == 0) P[blockIdx.x] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// The operator apply in one launch: elastic_elem_k and gather_op_k fused, the
+// element forces kept in LDS.  The vertices are grouped into chunks (built on
+// the host from the vertex -> element lists: spatially close vertices, at
+// most BT vertices, MS list entries and HM halo vertices per chunk).  A
+// block takes one chunk.  It stages the positions and the input field x (and
+// the concentration, ISO) of the chunk's halo (every vertex of an element
+// its vertices' lists name) in LDS, computes each such element once (Q^{-1}
+// and det Q from the positions, as geometry_k forms them), writes the force
+// on each listed vertex into that list entry's LDS slot, and then each vertex
+// sums its own entries in list order (ascending element, as gather_op_k does
+// from HBM), adds the bottom penalty and applies the CG mode of gather_op_k.
+// Tables (one chunk k):
+//   chV[k] .. chV[k+1]   its vertex entries: vertex chVtx[j], whose list
+//                        entries take LDS slots chLs[j] .. chLs[j] + count
+//   chH[k] .. chH[k+1]   its halo: vertex chHv[h]
+//   chE[k] .. chE[k+1]   its element entries: element chElm[u], local vertex l
+//                        at halo position chLoc[l NU + u]; the force on it goes
+//                        to LDS slot chSlot[l NU + u] (-1: not in the chunk)
+// P[k]: the block's partial of the mode's dot product (MODE 0 and 1).
+struct OpfShape {
+    int nt, maxs, halo;   // BT, MS, HM
+};
+constexpr OpfShape OPF_SHAPES[] = {{256, 1024, 512}, {512, 1024, 512}, {512, 2048, 1024}, {1024, 2048, 1024}};
+constexpr int OPF_NSHAPES = 4;
+
+template <int D, bool ISO, int MODE, int BT, int MS, int HM>
+__global__ __launch_bounds__(BT) void op_fused_k(const int *__restrict__ chV, const int *__restrict__ chVtx,
+                                                 const int *__restrict__ chLs, const int *__restrict__ chH,
+                                                 const int *__restrict__ chHv, const int *__restrict__ chE,
+                                                 const int *__restrict__ chElm, const int *__restrict__ chLoc,
+                                                 const int *__restrict__ chSlot, int NU, int E, int V,
+                                                 const int *__restrict__ vstart, const double *__restrict__ q,
+                                                 const double *__restrict__ geo, const double *__restrict__ x,
+                                                 const double *__restrict__ tau, const int *__restrict__ elmLyr,
+                                                 const double *__restrict__ mod, int BL, int B,
+                                                 const int *__restrict__ vtxBtm, const double *__restrict__ pfrc,
+                                                 double btmWgt, double ldmWgt, const double *__restrict__ u,
+                                                 const double *__restrict__ w, double *__restrict__ out,
+                                                 double *__restrict__ out2, double *__restrict__ P, const Ctl *ctl) {
+    __shared__ double f[MS * D];
+    __shared__ double hq[D * HM], hx[D * HM];
+    __shared__ double ht[ISO ? HM : 1];
+    __shared__ double red[BT / 64];
+    if (cld(&ctl->done)) return;
+    const int k = blockIdx.x;
+    const int h0 = chH[k], nh = chH[k + 1] - h0;
+    for (int h = threadIdx.x; h < nh; h += BT) {
+        const int v = chHv[h0 + h];
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            hq[c * HM + h] = q[c * V + v];
+            hx[c * HM + h] = x[c * V + v];
+        }
+        if constexpr (ISO) ht[h] = tau[v];
+    }
+    __syncthreads();
+    const int e0 = chE[k], e1 = chE[k + 1];
+    for (int ue = e0 + (int)threadIdx.x; ue < e1; ue += BT) {
+        const int e = chElm[ue];
+        int loc[D + 1];
+#pragma unroll
+        for (int l = 0; l <= D; ++l) loc[l] = chLoc[(size_t)l * NU + ue];
+        double ed[D][D], xv[D + 1][D];
+#pragma unroll
+        for (int kk = 0; kk < D; ++kk)
+#pragma unroll
+            for (int c = 0; c < D; ++c) ed[kk][c] = hq[c * HM + loc[kk + 1]] - hq[c * HM + loc[0]];
+#pragma unroll
+        for (int l = 0; l <= D; ++l)
+#pragma unroll
+            for (int c = 0; c < D; ++c) xv[l][c] = hx[c * HM + loc[l]];
+        double ub = 0.0;
+        if constexpr (ISO) {
+#pragma unroll
+            for (int l = 0; l <= D; ++l) ub += ht[loc[l]];
+            ub /= (D + 1);
+        }
+        double inv[D][D], det;
+        mat_inv<D>(ed, inv, det);
+        double fo[D + 1][D];
+        elem_forces<D, ISO>(e, E, xv, ub, inv, elem_vol<D>(det), geo, elmLyr, mod, fo);
+#pragma unroll
+        for (int l = 0; l <= D; ++l) {
+            const int sl = chSlot[(size_t)l * NU + ue];
+            if (sl < 0) continue;
+#pragma unroll
+            for (int c = 0; c < D; ++c) f[sl * D + c] = fo[l][c];
+        }
+    }
+    __syncthreads();
+    const int j = chV[k] + (int)threadIdx.x;
+    double ss = 0.0;
+    if (j < chV[k + 1]) {
+        const int i = chVtx[j];
+        const int a0 = chLs[j], a1 = a0 + (vstart[i + 1] - vstart[i]);
+        double acc[D];
+#pragma unroll
+        for (int c = 0; c < D; ++c) acc[c] = 0.0;
+        for (int a = a0; a < a1; ++a)
+#pragma unroll
+            for (int c = 0; c < D; ++c) acc[c] += f[a * D + c];
+        if (pfrc && i < BL) {
+            double pa[D];
+#pragma unroll
+            for (int c = 0; c < D; ++c) pa[c] = 0.0;
+            const int adj2 = vtxBtm[i];
+            for (int a = 0; a < adj2; ++a) {
+                const int b = vtxBtm[(1 + 2 * a) * BL + i], l = vtxBtm[(2 + 2 * a) * BL + i];
+#pragma unroll
+                for (int c = 0; c < D; ++c) pa[c] += pfrc[(l * D + c) * (size_t)B + b];
+            }
+#pragma unroll
+            for (int c = 0; c < D; ++c) acc[c] = 1.0 * acc[c] + btmWgt * pa[c];
+        }
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            const int kx = c * V + i;
+            if constexpr (MODE == 0) {
+                const double ad = ldmWgt * u[kx] + 1.0 * acc[c];
+                out[kx] = ad;
+                ss += w[kx] * ad;
+            } else if constexpr (MODE == 2) {
+                out[kx] = acc[c];
+            } else {
+                const double ax = ldmWgt * u[kx] + acc[c];
+                const double r = 1.0 * w[kx] - 1.0 * ax;
+                out[kx] = r;
+                out2[kx] = r;
+                ss += r * r;
+            }
+        }
+    }
+    if constexpr (MODE != 2) {
+        const double sb = block_sum(ss, red);
+        if (threadIdx.x == 0) P[k] = sb;
     }
 }
 

@@ -149,6 +149,9 @@ def test_tile_table_lookup(tmp_path):
                    "xc f64 51210 5000 1024\n"
                    "cx f32 5000 51210 4096\n"
                    "kk f64 2000 2000 256\n"
+                   "op f64 245760 51210 3\n"
+                   "op f64 11520 2568 0\n"
+                   "op f64 100 100 4\n"          # no such launch shape: skipped
                    "xc f32 100 100 7\n"          # target out of range: skipped
                    "zz f32 100 100 512\n"        # unknown product: skipped
                    "garbage\n")
@@ -157,7 +160,8 @@ def test_tile_table_lookup(tmp_path):
             "api.tile_target('xc', True, 3000, 300), api.tile_target('xc', True, 100, 100),"
             "api.tile_target('xc', False, 2568, 256), api.tile_target('cx', True, 2000, 20000),"
             "api.tile_target('cx', False, 5000, 51210), api.tile_target('kk', False, 50000, 50000),"
-            "api.tile_target('cc', True, 100, 100), api.tile_target('qq', True, 10, 10)]))")
+            "api.tile_target('cc', True, 100, 100), api.tile_target('qq', True, 10, 10),"
+            "api.tile_target('op', False, 983040, 204810), api.tile_target('op', False, 3600, 2000)]))")
 
     def run(**env):
         e = {k: v for k, v in os.environ.items() if k != "SA_KS_TARGET"}
@@ -166,9 +170,11 @@ def test_tile_table_lookup(tmp_path):
                              text=True).stdout
         return __import__("json").loads(out.strip().splitlines()[-1])
 
-    assert run() == [2048, 2048, 512, 512, 1024, 4096, 0, 256, 0, -1]
-    assert run(SA_KS_TARGET="300") == [300, 300, 300, 300, 300, 300, 300, 300, 300, -1]
-    assert run(SA_KS_TARGET="cx=640,kk=128") == [2048, 2048, 512, 512, 1024, 640, 640, 128, 0, -1]
+    assert run() == [2048, 2048, 512, 512, 1024, 4096, 0, 256, 0, -1, 3, 0]
+    # the block-target override leaves the operator's launch shape to the table
+    assert run(SA_KS_TARGET="300") == [300, 300, 300, 300, 300, 300, 300, 300, 300, -1, 3, 0]
+    assert run(SA_KS_TARGET="cx=640,kk=128") == [2048, 2048, 512, 512, 1024, 640, 640, 128, 0, -1, 3, 0]
+    assert run(SA_OPF_CFG="2")[-2:] == [2, 2]
 
 
 def test_committed_tile_table():
@@ -182,11 +188,44 @@ def test_committed_tile_table():
         if not line:
             continue
         prod, ty, rows, cols, tgt = line
-        assert prod in ("xc", "cx", "cc", "kk") and ty in ("f32", "f64"), line
-        assert int(rows) > 0 and int(cols) > 0 and 16 <= int(tgt) <= 65536, line
+        assert prod in ("xc", "cx", "cc", "kk", "op") and ty in ("f32", "f64"), line
+        assert int(rows) > 0 and int(cols) > 0, line
+        assert (0 <= int(tgt) < 4) if prod == "op" else (16 <= int(tgt) <= 65536), line
         entries.append((prod, ty == "f32", int(rows), int(cols), int(tgt)))
     assert entries
-    if os.environ.get("SA_TILE_TABLE") is None and os.environ.get("SA_KS_TARGET") is None:
+    if all(os.environ.get(k) is None for k in ("SA_TILE_TABLE", "SA_KS_TARGET", "SA_OPF_CFG")):
         for prod, f32, rows, cols, tgt in entries:
             same = [e[4] for e in entries if e[:4] == (prod, f32, rows, cols)]
             assert api.tile_target(prod, f32, rows, cols) == same[0], (prod, f32, rows, cols)
+
+
+def op_chunks(p):
+    keep = api._Keep()
+    q = np.ascontiguousarray(p["q0"], dtype=np.float64)   # dim x lmkNum row-major = lmkNum x dim column-major
+    m = api.make_mesh(dict(p["elm"]), q, keep, with_vfd=False)
+    n, u, h = C.c_int(), C.c_longlong(), C.c_longlong()
+    rc = api.lib().sa_op_chunk_check(C.byref(m), q.ctypes.data_as(C.POINTER(C.c_double)), C.byref(n), C.byref(u),
+                                     C.byref(h))
+    return rc, n.value, u.value, h.value
+
+
+@pytest.mark.parametrize("cfg", ["0", "1", "2", "3"])
+@pytest.mark.parametrize("which", ["ring", "shell", "slab", "c2"])
+def test_operator_chunk_tables(which, cfg, monkeypatch):
+    """The vertex chunks of the fused operator apply, built on the host from
+    the mesh as a session builds them, pass the checks the kernel's indexing
+    relies on (every vertex in one chunk; every list entry one LDS slot,
+    written once), and each element is evaluated at most as often as it is
+    listed, at least once."""
+    p = {"ring": lambda: W.ring_problem(),
+         "shell": lambda: W.shell_problem(level=3, n_layers=4, time_steps=2),
+         "slab": lambda: W.slab_problem(n=9, n_layers=3, tilt=0.3, time_steps=2),
+         "c2": lambda: W.shell_problem(level=5, n_layers=5, time_steps=2)}[which]()
+    monkeypatch.setenv("SA_OPF_CFG", cfg)
+    rc, n, u, h = op_chunks(p)
+    E = p["elm"]["elmVtxMat"].shape[1]
+    S = int(p["elm"]["vtxElmMat"][0].sum())
+    assert rc == 0, api.lib().sa_last_error()
+    assert n >= 1 and E <= u <= S and p["q0"].shape[1] <= h, (n, u, E, S, h)
+    print(f"{which} launch shape {cfg}: {n} chunks, {u / E:.2f} evaluations per element, "
+          f"{h / p['q0'].shape[1]:.2f} halo loads per vertex")

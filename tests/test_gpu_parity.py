@@ -423,6 +423,38 @@ def test_full_solve_with_penalty(which, ctl, oracle):
 
 
 # --------------------------------------------------------------------------
+# the fused operator apply (op_fused_k, one launch, forces in LDS) against the
+# two-launch apply (elastic_elem_k + gather_op_k, SA_OP_FUSED=0): the same
+# forces summed in the same order, Q^{-1} recomputed by the same function
+
+@pytest.mark.parametrize("which", list(PROBLEMS) + ["slab3d_btm"])
+def test_fused_operator_apply_matches_two_launch(which, monkeypatch):
+    p = W.slab_problem(n=7, n_layers=3, tilt=0.3, time_steps=6, btmWgt=5.0) if which == "slab3d_btm" \
+        else PROBLEMS[which]()
+    p["name"] = which + "_fused"
+    v = np.random.default_rng(5).standard_normal(p["q0"].shape)
+    V = p["q0"].shape[1]
+    out = {}
+    for fused in ("1", "0"):
+        monkeypatch.setenv("SA_OP_FUSED", fused)
+        s = _session(p)
+        e = s.probe("elastic", p["tauVar"][:, 0], p["tauPrm"], v)
+        r = s.run(p["tauVar"][:, 0], p["tauPrm"])
+        s.close()
+        pc = dict(p, ctlIdx=np.sort(np.random.default_rng(3).choice(V, V // 3, replace=False)).astype(np.int32))
+        sc = api.Session(pc, precision="mixed")
+        rc = sc.run(p["tauVar"][:, 0], p["tauPrm"])
+        sc.close()
+        assert r["status"] == 0 and rc["status"] == 0
+        out[fused] = (e, r["obj"], rc["obj"])
+    a, b = out["1"], out["0"]
+    print(f"[gpu] {which}: fused vs two-launch apply {rel(a[0], b[0]):.1e}, objectives {a[1]!r} {b[1]!r}, "
+          f"control points {a[2]!r} {b[2]!r}", flush=True)
+    assert rel(a[0], b[0]) < 1e-14
+    assert abs(a[1] - b[1]) <= 1e-11 * abs(b[1]) and abs(a[2] - b[2]) <= 1e-9 * abs(b[2])
+
+
+# --------------------------------------------------------------------------
 # control-point kernel products K_xc u and K_cx y as the inner solve forms
 # them: stored FP64 / matrix-free FP64 (plain form) to 1e-13, stored FP32
 # (2D mixed form) and matrix-free FP32 on the matrix cores (3D mixed form,
