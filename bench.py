@@ -341,6 +341,35 @@ def mode_products(args, api, W):
             **out}
 
 
+def mode_operator(args, api, W):
+    """The operator apply (E + btmWgt P) x of one workload, as the CG runs it:
+    the fused launch (op_fused_k, the default) against the two-launch apply
+    (elastic_elem_k + gather_op_k, SA_OP_FUSED=0).  Device time per apply
+    from sa_session_time_product (mean of --tune-reps launches, best of
+    three), against the algorithmic bytes 64 E + 72 V (connectivity and
+    frames per element; positions, input and output per vertex, each once)."""
+    prob, desc = workload(args.config, 1)
+    d, V = prob["q0"].shape
+    E = prob["elm"]["elmVtxMat"].shape[1]
+    alg = 64 * E + 72 * V if d == 3 else 40 * E + 48 * V
+    out = {}
+    for name, env in (("fused", "1"), ("two_launch", "0")):
+        os.environ["SA_OP_FUSED"] = env
+        s = api.Session(prob)
+        s.probe("elastic", prob["tauVar"][:, 0], prob["tauPrm"], np.zeros((d, V)))
+        us = min(s.time_product("operator", args.tune_reps) for _ in range(3))
+        s.close()
+        out[name] = {"us": round(us, 2), "achieved_GBs": round(alg / us * 1e-3, 1),
+                     "frac": round(alg / us * 1e-3 / PEAK_HBM_GBS, 4)}
+        print(f"[bench] operator {name}: {out[name]}", file=sys.stderr, flush=True)
+    os.environ.pop("SA_OP_FUSED", None)
+    return {"metric": "operator apply (E + btmWgt P) x, device time per apply", "value": out["fused"]["us"],
+            "unit": "us", "n_gpus": 1, "steps": None, "warmup": None, "ms_per_step": None,
+            "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": desc, "landmarks": V, "elements": E, "reps": args.tune_reps},
+            "algorithmic_bytes": alg, **out}
+
+
 TUNE_TARGETS = (32, 64, 128, 256, 512, 1024, 2048, 4096, 8192, 16384)
 # the library's built-in targets (KS_BLOCKS = 1024; twice that for FP32 K_xc /
 # K_cx; 2048 for the all-landmark product), kept unless a target is more than
@@ -468,7 +497,7 @@ def main():
                     help="time without the per-kernel events (graph-captured CG iterations); no roofline")
     ap.add_argument("--cpu-iters", type=int, default=0, help="CG iterations of the CPU sample (0: sized to the workload)")
     ap.add_argument("--mode", default="single",
-                    choices=["single", "batch", "serve", "precision", "fit", "products", "tune"],
+                    choices=["single", "batch", "serve", "precision", "fit", "products", "tune", "operator"],
                     help="single: one solve of --config; batch: 64 uneven meshes concurrently; serve: Poisson stream")
     ap.add_argument("--tune-reps", type=int, default=30, help="product launches per timing of --mode tune")
     ap.add_argument("--workers", type=int, default=8)
@@ -498,7 +527,8 @@ def main():
     if args.mode != "single":
         from shape_atrophy_amd import workloads as W
         res = {"batch": mode_batch, "serve": mode_serve, "precision": mode_precision,
-               "fit": mode_fit, "products": mode_products, "tune": mode_tune}[args.mode](args, api, W)
+               "fit": mode_fit, "products": mode_products, "tune": mode_tune,
+               "operator": mode_operator}[args.mode](args, api, W)
         print(json.dumps(res))
         return
     steps = args.steps if args.steps is not None else 50   # the configured solve: 50 time steps
@@ -589,7 +619,8 @@ def main():
     if op_n:
         avg = op_ms / op_n
         B = (64 * E + 72 * V) if d == 3 else (40 * E + 48 * V)
-        op = {"bound": "hbm", "kernel": "elastic_elem_k + gather_op_k", "achieved": round(B / (avg * 1e-3) / 1e9, 1),
+        kern = "elastic_elem_k + gather_op_k" if os.environ.get("SA_OP_FUSED", "1")[:1] == "0" else "op_fused_k"
+        op = {"bound": "hbm", "kernel": kern, "achieved": round(B / (avg * 1e-3) / 1e9, 1),
               "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(B / (avg * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
               "avg_ms": round(avg, 4), "algorithmic_bytes": B}
 

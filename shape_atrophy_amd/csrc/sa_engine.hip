@@ -583,7 +583,8 @@ struct Engine : EngineBase {
     // the operator apply in one launch (op_fused_k): vertex chunks and the
     // elements their lists name; SA_OP_FUSED=0 keeps the two-launch apply
     // (elastic_elem_k + gather_op_k)
-    DBuf<int> opChV, opChVtx, opChLs, opChH, opChHv, opChE, opChElm, opChLoc, opChSlot;
+    DBuf<int> opChV, opChVtx, opChLs, opChH, opChHv, opChE, opChElm, opChLS;
+    DBuf<double> opFr[2];        // element-entry frames of geo[0], geo[1] (frame model)
     int opChunks = 0, opNU = 0, opCapU = 0, opCfg = 0;
     bool opFused = false;
     // mixed form on the matrix cores (kx_mfma_k): the kernel products of the
@@ -1200,6 +1201,7 @@ struct Engine : EngineBase {
         if (env && env[0] == '0') return;
         opCfg = opf_cfg(E, V);
         const int nt = OPF_SHAPES[opCfg].nt, maxs = OPF_SHAPES[opCfg].maxs, hm = OPF_SHAPES[opCfg].halo;
+        require(hm <= 0xffff && maxs < 0x7fff, "launch shape too large for the packed halo/slot words");
         int maxAdj = 0;
         long S = 0;
         for (int i = 0; i < V; ++i) { maxAdj = std::max(maxAdj, hVE[i]); S += hVE[i]; }
@@ -1221,13 +1223,16 @@ struct Engine : EngineBase {
         opChV.alloc(capCh + 1, tot); opChE.alloc(capCh + 1, tot); opChH.alloc(capCh + 1, tot);
         opChVtx.alloc(V, tot); opChLs.alloc(V, tot); opChHv.alloc(std::max(capH, 1L), tot);
         opChElm.alloc(opCapU, tot);
-        opChSlot.alloc((size_t)(D + 1) * opCapU, tot); opChLoc.alloc((size_t)(D + 1) * opCapU, tot);
+        opChLS.alloc((size_t)(D + 1) * opCapU, tot);
+        if (elaType == SA_ELA_FRAME)
+            for (auto &b : opFr) b.alloc((size_t)2 * D * opCapU, tot);
         if (dryRun) { opFused = true; return; }
         const OpTables t = op_tables(hVE, hElm, V, E, D, morton_order(hq.data(), V, D), nt, maxs, hm);
         if (t.chVtx.empty()) {
             // not reached (the halo bound above); keeps the two-launch apply
-            for (DBuf<int> *b : {&opChV, &opChE, &opChH, &opChVtx, &opChLs, &opChHv, &opChElm, &opChSlot, &opChLoc})
+            for (DBuf<int> *b : {&opChV, &opChE, &opChH, &opChVtx, &opChLs, &opChHv, &opChElm, &opChLS})
                 b->alloc(0, tot);
+            for (auto &b : opFr) b.alloc(0, tot);
             return;
         }
         check_op_tables(t, hVE, hElm, V, E, D, nt, maxs, hm);
@@ -1239,14 +1244,24 @@ struct Engine : EngineBase {
         opChH.upload(t.chH.data(), t.chH.size(), s);
         opChVtx.upload(t.chVtx.data(), t.chVtx.size(), s); opChLs.upload(t.chLs.data(), t.chLs.size(), s);
         opChHv.upload(t.chHv.data(), t.chHv.size(), s);
-        opChElm.upload(t.chElm.data(), t.chElm.size(), s); opChSlot.upload(t.slot.data(), t.slot.size(), s);
-        opChLoc.upload(t.loc.data(), t.loc.size(), s);
+        opChElm.upload(t.chElm.data(), t.chElm.size(), s);
+        // halo position | (LDS slot + 1) << 16 (halo < 2^16, slot + 1 < 2^15)
+        std::vector<int> ls(t.slot.size());
+        for (size_t a = 0; a < ls.size(); ++a) ls[a] = t.loc[a] | ((t.slot[a] + 1) << 16);
+        opChLS.upload(ls.data(), ls.size(), s);
         CK(hipStreamSynchronize(s));   // the host vectors go out of scope
         opFused = true;
     }
 
     // partials the operator apply leaves for the all-landmark CG (MODE 0, 1)
     int op_parts() const { return opFused ? opChunks : gridOf(V); }
+
+    // after geometry_k wrote geo[g]: its frames in element-entry order
+    void op_frames(int g) {
+        if (!opFused || !opFr[g].p || opNU == 0) return;
+        op_frames_k<D><<<gridOf(opNU), NT, 0, s>>>(opNU, E, opChElm.p, geo[g].p, opFr[g].p);
+        CHECK_LAUNCH();
+    }
 
     // y = (E + btmWgt P) x in the CG mode MODE of gather_op_k (positions q,
     // concentration tauNow, geometry geoNow of q; pen = false: E x alone)
@@ -1263,10 +1278,12 @@ struct Engine : EngineBase {
             pen_elem_k<D><<<gridOf(B), NT, 0, s>>>(B, V, btmVtx.p, meas.p, xin, pfrc.p, ctl.p);
             CHECK_LAUNCH();
         }
+        require(geoNow == geo[0].p || geoNow == geo[1].p, "operator apply on a geometry without entry frames");
+        const double *frE = geoNow == geo[1].p ? opFr[1].p : opFr[0].p;
 #define SA_OPF(ISO, BT, MS, HM) op_fused_k<D, ISO, MODE, BT, MS, HM><<<opChunks, BT, 0, s>>>(                 \
-            opChV.p, opChVtx.p, opChLs.p, opChH.p, opChHv.p, opChE.p, opChElm.p, opChLoc.p, opChSlot.p, opNU, E,  \
-            V, vtxStart.p, q, geoNow, xin, tauNow, elmLyr.p, dmod.p, BL, B, vtxBtm.p, pn ? pfrc.p : nullptr,      \
-            btmWgt, ldmWgt, u, w, out, out2, P, ctl.p)
+            opChV.p, opChVtx.p, opChLs.p, opChH.p, opChHv.p, opChE.p, opChElm.p, opChLS.p, opNU, V, vtxStart.p,   \
+            q, frE, xin, tauNow, elmLyr.p, dmod.p, BL, B, vtxBtm.p, pn ? pfrc.p : nullptr, btmWgt, ldmWgt, u, w,  \
+            out, out2, P, ctl.p)
 #define SA_OPF_SHAPE(BT, MS, HM) \
         if (elaType == SA_ELA_FRAME) SA_OPF(false, BT, MS, HM); else SA_OPF(true, BT, MS, HM)
         static_assert(OPF_NSHAPES == 4, "one case per launch shape");
@@ -2399,6 +2416,7 @@ struct Engine : EngineBase {
         geometry_k<D><<<gridOf(E), NT, 0, s>>>(E, V, elmVtx.p, frmVtx.p, tsvVtx.p, q0.p, spdTan, spdTsv, geo[0].p,
                                                femElm.p);
         CHECK_LAUNCH();
+        op_frames(0);
         fem_gather_k<<<gridOf(Z), NT, 0, s>>>(Z, E, femIfo.p, femElm.p, dt, ppd[0].p, lhs.p);
         CHECK_LAUNCH();
     }
@@ -2621,6 +2639,7 @@ struct Engine : EngineBase {
             geometry_k<D><<<gridOf(E), NT, 0, s>>>(E, V, elmVtx.p, frmVtx.p, tsvVtx.p, qn, spdTan, spdTsv, geo[nx].p,
                                                    femElm.p);
             CHECK_LAUNCH();
+            op_frames(nx);
             fem_gather_k<<<gridOf(Z), NT, 0, s>>>(Z, E, femIfo.p, femElm.p, dt, ppd[nx].p, lhs.p);
             CHECK_LAUNCH();
             int rits = 0;

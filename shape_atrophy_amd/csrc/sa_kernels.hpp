@@ -289,14 +289,14 @@ __global__ __launch_bounds__(NT) void gather_k(int V, int E, const int *__restri
 // computeStrainTensor.cu:4-53|4-61 + frmElasticComputeKernel
 // (applyFrameElasticOperator.cu:41-85|113-164).  ISO selects the per-layer
 // isotropic law sigma = (1 - 0.5 u_bar)(lambda tr(eps) I + 2 mu eps).
-// Element e with vertex velocities xv, Q^{-1} inv, volume vol and (ISO)
-// mean concentration ub of its vertices; the force on its local vertex l
-// lands in fo[l].
+// Element e with vertex velocities xv, Q^{-1} inv, volume vol, frames f1,
+// f2 (frame model) or the mean concentration ub of its vertices (ISO); the
+// force on its local vertex l lands in fo[l].
 template <int D, bool ISO>
-__device__ inline void elem_forces(int e, int E, const double (&xv)[D + 1][D], double ub, const double (&inv)[D][D],
-                                   double vol, const double *__restrict__ geo, const int *__restrict__ elmLyr,
-                                   const double *__restrict__ mod, double (&fo)[D + 1][D]) {
-    using G = GeoLayout<D>;
+__device__ inline void elem_forces(int e, const double (&xv)[D + 1][D], double ub, const double (&inv)[D][D],
+                                   double vol, const double (&f1)[D], const double (&f2)[D],
+                                   const int *__restrict__ elmLyr, const double *__restrict__ mod,
+                                   double (&fo)[D + 1][D]) {
     double Vm[D][D];
 #pragma unroll
     for (int k = 0; k < D; ++k)
@@ -330,9 +330,6 @@ __device__ inline void elem_forces(int e, int E, const double (&xv)[D + 1][D], d
 #pragma unroll
             for (int b = 0; b < D; ++b) sig[a][b] = (a == b ? lam * tr : 0.0) + 2.0 * mu * eps[a][b];
     } else {
-        double f1[D], f2[D];
-#pragma unroll
-        for (int c = 0; c < D; ++c) { f1[c] = geo[(G::F1 + c) * (size_t)E + e]; f2[c] = geo[(G::F2 + c) * (size_t)E + e]; }
         // moduli change between runs (host copies): read coherently
         double md[D + 1];
 #pragma unroll
@@ -388,8 +385,13 @@ __global__ __launch_bounds__(NT) void elastic_elem_k(int E, int V, const int *__
         for (int k = 0; k <= D; ++k) ub += tau[vtx[k]];
         ub /= (D + 1);
     }
+    double f1[D], f2[D];
+    if constexpr (!ISO) {
+#pragma unroll
+        for (int c = 0; c < D; ++c) { f1[c] = geo[(G::F1 + c) * (size_t)E + e]; f2[c] = geo[(G::F2 + c) * (size_t)E + e]; }
+    }
     double fo[D + 1][D];
-    elem_forces<D, ISO>(e, E, xv, ub, inv, vol, geo, elmLyr, mod, fo);
+    elem_forces<D, ISO>(e, xv, ub, inv, vol, f1, f2, elmLyr, mod, fo);
     if (slot) {
 #pragma unroll
         for (int l = 0; l <= D; ++l) {
@@ -721,9 +723,12 @@ __global__ __launch_bounds__(NT) void gather_op_k(int V, int E, const int *__res
 //   chV[k] .. chV[k+1]   its vertex entries: vertex chVtx[j], whose list
 //                        entries take LDS slots chLs[j] .. chLs[j] + count
 //   chH[k] .. chH[k+1]   its halo: vertex chHv[h]
-//   chE[k] .. chE[k+1]   its element entries: element chElm[u], local vertex l
-//                        at halo position chLoc[l NU + u]; the force on it goes
-//                        to LDS slot chSlot[l NU + u] (-1: not in the chunk)
+//   chE[k] .. chE[k+1]   its element entries: element chElm[u] (read for
+//                        its layer, ISO), frames frE[c NU + u] (f1) and
+//                        frE[(D + c) NU + u] (f2) in entry order (frame
+//                        model, op_frames_k), and per local vertex l the word
+//                        chLS[l NU + u] = halo position | (LDS slot + 1) << 16
+//                        (slot -1: that vertex is not in the chunk)
 // P[k]: the block's partial of the mode's dot product (MODE 0 and 1).
 struct OpfShape {
     int nt, maxs, halo;   // BT, MS, HM
@@ -731,14 +736,30 @@ struct OpfShape {
 constexpr OpfShape OPF_SHAPES[] = {{256, 1024, 512}, {512, 1024, 512}, {512, 2048, 1024}, {1024, 2048, 1024}};
 constexpr int OPF_NSHAPES = 4;
 
+// frames of the chunks' element entries in entry order, from the geometry
+// (once per geometry: the fused apply then reads them coalesced)
+template <int D>
+__global__ __launch_bounds__(NT) void op_frames_k(int NU, int E, const int *__restrict__ chElm,
+                                                  const double *__restrict__ geo, double *__restrict__ frE) {
+    using G = GeoLayout<D>;
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= NU) return;
+    const int e = chElm[u];
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+        frE[(size_t)c * NU + u] = geo[(G::F1 + c) * (size_t)E + e];
+        frE[(size_t)(D + c) * NU + u] = geo[(G::F2 + c) * (size_t)E + e];
+    }
+}
+
 template <int D, bool ISO, int MODE, int BT, int MS, int HM>
 __global__ __launch_bounds__(BT) void op_fused_k(const int *__restrict__ chV, const int *__restrict__ chVtx,
                                                  const int *__restrict__ chLs, const int *__restrict__ chH,
                                                  const int *__restrict__ chHv, const int *__restrict__ chE,
-                                                 const int *__restrict__ chElm, const int *__restrict__ chLoc,
-                                                 const int *__restrict__ chSlot, int NU, int E, int V,
-                                                 const int *__restrict__ vstart, const double *__restrict__ q,
-                                                 const double *__restrict__ geo, const double *__restrict__ x,
+                                                 const int *__restrict__ chElm, const int *__restrict__ chLS,
+                                                 int NU, int V, const int *__restrict__ vstart,
+                                                 const double *__restrict__ q, const double *__restrict__ frE,
+                                                 const double *__restrict__ x,
                                                  const double *__restrict__ tau, const int *__restrict__ elmLyr,
                                                  const double *__restrict__ mod, int BL, int B,
                                                  const int *__restrict__ vtxBtm, const double *__restrict__ pfrc,
@@ -764,10 +785,13 @@ __global__ __launch_bounds__(BT) void op_fused_k(const int *__restrict__ chV, co
     __syncthreads();
     const int e0 = chE[k], e1 = chE[k + 1];
     for (int ue = e0 + (int)threadIdx.x; ue < e1; ue += BT) {
-        const int e = chElm[ue];
-        int loc[D + 1];
+        int loc[D + 1], sl[D + 1];
 #pragma unroll
-        for (int l = 0; l <= D; ++l) loc[l] = chLoc[(size_t)l * NU + ue];
+        for (int l = 0; l <= D; ++l) {
+            const int w = chLS[(size_t)l * NU + ue];
+            loc[l] = w & 0xffff;
+            sl[l] = (w >> 16) - 1;
+        }
         double ed[D][D], xv[D + 1][D];
 #pragma unroll
         for (int kk = 0; kk < D; ++kk)
@@ -783,16 +807,20 @@ __global__ __launch_bounds__(BT) void op_fused_k(const int *__restrict__ chV, co
             for (int l = 0; l <= D; ++l) ub += ht[loc[l]];
             ub /= (D + 1);
         }
+        double f1[D], f2[D];
+        if constexpr (!ISO) {
+#pragma unroll
+            for (int c = 0; c < D; ++c) { f1[c] = frE[(size_t)c * NU + ue]; f2[c] = frE[(size_t)(D + c) * NU + ue]; }
+        }
         double inv[D][D], det;
         mat_inv<D>(ed, inv, det);
         double fo[D + 1][D];
-        elem_forces<D, ISO>(e, E, xv, ub, inv, elem_vol<D>(det), geo, elmLyr, mod, fo);
+        elem_forces<D, ISO>(ISO ? chElm[ue] : 0, xv, ub, inv, elem_vol<D>(det), f1, f2, elmLyr, mod, fo);
 #pragma unroll
         for (int l = 0; l <= D; ++l) {
-            const int sl = chSlot[(size_t)l * NU + ue];
-            if (sl < 0) continue;
+            if (sl[l] < 0) continue;
 #pragma unroll
-            for (int c = 0; c < D; ++c) f[sl * D + c] = fo[l][c];
+            for (int c = 0; c < D; ++c) f[sl[l] * D + c] = fo[l][c];
         }
     }
     __syncthreads();
