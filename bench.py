@@ -751,7 +751,7 @@ def serve_pass(api, prob, args, steps, sess):
            "serial_alone_steps_per_s": round(sum(a["steps"] for a in alone) / sum(a["seconds"] for a in alone), 4),
            "latency_s": {"p50": percentile(lat, 50), "p99": percentile(lat, 99)},
            "failed_requests": len(tvs) - len(ok), "max_rel_diff_alone_vs_mix": diff,
-           "factor_retries": sum(g["factor_retries"] for g in got), "device_bytes_reserved": info["mem_used"]}
+           "factorizations": sum(g["factorizations"] for g in got), "device_bytes_reserved": info["mem_used"]}
     print(f"[bench] served: {out}", file=sys.stderr, flush=True)
     return out
 

@@ -144,7 +144,7 @@ typedef struct sa_stats {
     long long varifoldPairs;     /*   ... the facet pairs it summed (F^2 + F Ft, + Ft^2 on a session's first) */
     int varifoldMfma;            /*   ... 1 when the pair contractions ran on the matrix cores */
     long long deviceBytes;       /* device memory the session holds (= sa_session_footprint of its mesh) */
-    int factorRetries;           /* control-point K_cc factorisations repeated after a reported failure */
+    int factorizations;          /* control-point K_cc factorisations (preconditioner rebuilds) of the run */
 } sa_stats;
 
 /* sa_stats.failReason */

@@ -67,7 +67,7 @@ class SaStats(C.Structure):
                 ("kmatvecCount", C.c_int), ("operatorMs", C.c_double), ("operatorCount", C.c_int),
                 ("fp64Steps", C.c_int), ("failReason", C.c_int), ("varifoldMs", C.c_double),
                 ("varifoldPairs", C.c_longlong), ("varifoldMfma", C.c_int), ("deviceBytes", C.c_longlong),
-                ("factorRetries", C.c_int)]
+                ("factorizations", C.c_int)]
 
 
 class SaError(RuntimeError):
@@ -349,7 +349,7 @@ class Session:
                     fail_reason=st.failReason, seconds=st.seconds,
                     kmatvec_ms=st.kmatvecMs, kmatvec_count=st.kmatvecCount, operator_ms=st.operatorMs,
                     operator_count=st.operatorCount, varifold_ms=st.varifoldMs, varifold_pairs=st.varifoldPairs,
-                    varifold_mfma=st.varifoldMfma, device_bytes=st.deviceBytes, factor_retries=st.factorRetries)
+                    varifold_mfma=st.varifoldMfma, device_bytes=st.deviceBytes, factorizations=st.factorizations)
 
     def set_params(self, params: dict):
         """New scalar parameters for the following runs (sa_session_set_params)."""
@@ -465,7 +465,7 @@ class Server:
         rc = lib().sa_server_wait(self._h, C.c_longlong(req_id), _dp(obj), C.byref(st), _dp(times))
         return dict(rc=rc, error=lib().sa_last_error().decode() if rc else "", obj=float(obj[0]),
                     status=st.status, steps=st.steps, pcg_iters=st.pcgIters, refine=st.refineSweeps,
-                    fp64_steps=st.fp64Steps, fail_reason=st.failReason, factor_retries=st.factorRetries,
+                    fp64_steps=st.fp64Steps, fail_reason=st.failReason, factorizations=st.factorizations,
                     seconds=st.seconds, t_submit=times[0], t_start=times[1], t_end=times[2])
 
     def info(self) -> dict:

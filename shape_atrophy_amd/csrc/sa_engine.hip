@@ -22,6 +22,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <condition_variable>
 #include <mutex>
 #include <vector>
 #include <type_traits>
@@ -39,9 +40,101 @@ namespace sa {
 namespace {
 
 thread_local std::string g_err;
-// rocBLAS / rocSOLVER calls of all sessions run one at a time (each to
-// completion on its stream before the next starts)
-std::mutex g_libMu;
+
+// Device gate.  rocSOLVER's Cholesky factorisation (ROCm 7.2, gfx950) returns
+// wrong factors when kernels of another hardware queue run beside it:
+// rocsolver_dpotrf of a 2,000 x 2,000 kernel matrix gave a different factor
+// in 16 of 20 calls (and reported it indefinite in 7) while a memory-bound
+// kernel ran on another stream, and the same factor in every call alone, with
+// the kernels serialised (AMD_SERIALIZE_KERNEL=3, HIP_LAUNCH_BLOCKING=1) or
+// with one hardware queue (GPU_MAX_HW_QUEUES=1).  dtrtri, dsyrk, dgemm and
+// dsyevd gave bitwise the same results under the same load.  So the
+// factorisation runs with no other session's work on the device: a session
+// holds the gate shared while it has work queued and lets it go at its host
+// synchronisations when a factorisation waits (gate_yield); the
+// factorisation holds it exclusive, waiting for the others to drain.
+class DeviceGate {
+    std::mutex mu_;
+    std::condition_variable cv_;
+    int shared_ = 0, waiting_ = 0;
+    bool excl_ = false;
+
+  public:
+    void lock_shared() {
+        std::unique_lock<std::mutex> l(mu_);
+        cv_.wait(l, [&] { return !excl_ && waiting_ == 0; });
+        ++shared_;
+    }
+    void unlock_shared() {
+        std::lock_guard<std::mutex> l(mu_);
+        --shared_;
+        cv_.notify_all();
+    }
+    void lock() {
+        std::unique_lock<std::mutex> l(mu_);
+        ++waiting_;
+        cv_.wait(l, [&] { return !excl_ && shared_ == 0; });
+        --waiting_;
+        excl_ = true;
+    }
+    void unlock() {
+        std::lock_guard<std::mutex> l(mu_);
+        excl_ = false;
+        cv_.notify_all();
+    }
+    bool exclusive_waiting() {
+        std::lock_guard<std::mutex> l(mu_);
+        return waiting_ > 0;
+    }
+};
+DeviceGate g_gate;
+std::mutex g_handleMu;              // rocBLAS handle creation and destruction
+thread_local int g_gateDepth = 0;   // shared holds of this thread (nested scopes count once)
+// SA_DEVICE_GATE=0 switches the gate off (to measure what it prevents)
+const bool g_gateOn = [] {
+    const char *e = std::getenv("SA_DEVICE_GATE");
+    return !(e && e[0] == '0');
+}();
+
+// shared hold for the span of an entry point that queues device work (on =
+// false: no hold, for the sizing pass)
+struct GateShared {
+    bool on;
+    explicit GateShared(bool use = true) : on(use && g_gateOn) {
+        if (on && g_gateDepth++ == 0) g_gate.lock_shared();
+    }
+    ~GateShared() {
+        if (on && --g_gateDepth == 0) g_gate.unlock_shared();
+    }
+    GateShared(const GateShared &) = delete;
+    GateShared &operator=(const GateShared &) = delete;
+};
+
+// at a host synchronisation (this thread's stream idle): step aside for a
+// waiting library call
+void gate_yield() {
+    if (g_gateDepth > 0 && g_gate.exclusive_waiting()) {
+        g_gate.unlock_shared();
+        g_gate.lock_shared();
+    }
+}
+
+// exclusive hold for the factorisation (its stream synchronised inside)
+struct GateExclusive {
+    bool held;
+    GateExclusive() : held(g_gateDepth > 0) {
+        if (!g_gateOn) return;
+        if (held) g_gate.unlock_shared();
+        g_gate.lock();
+    }
+    ~GateExclusive() {
+        if (!g_gateOn) return;
+        g_gate.unlock();
+        if (held) g_gate.lock_shared();
+    }
+    GateExclusive(const GateExclusive &) = delete;
+    GateExclusive &operator=(const GateExclusive &) = delete;
+};
 
 // Block-count targets of the stored kernel products (row panels x column
 // slices), per product and matrix shape, from an offline sweep on the device
@@ -515,7 +608,7 @@ struct Stats {
     double vfdMs = 0.0;               // the varifold row-sum launch of the objective (profiling on)
     long long vfdPairs = 0;           //   ... the facet pairs it summed
     int vfdMfma = 0;                  //   ... 1 when it ran on the matrix cores (vfd_rows_mfma_k)
-    int factorRetries = 0;            // K_cc factorisations repeated after a reported failure
+    int factorizations = 0;           // K_cc factorisations (preconditioner rebuilds) of the run
 };
 
 // hipEvent pairs around the kernel-matrix product and the elastic operator
@@ -730,8 +823,7 @@ struct Engine : EngineBase {
 
     ~Engine() override {
         if (blas) {
-            // handle creation and every library call hold the library lock
-            std::lock_guard<std::mutex> lk(g_libMu);
+            std::lock_guard<std::mutex> lk(g_handleMu);
             (void)rocblas_destroy_handle(blas);
         }
         if (hctl) (void)hipHostFree(hctl);
@@ -739,6 +831,7 @@ struct Engine : EngineBase {
     }
 
     Engine(const sa_mesh &m, const sa_target *tg, const sa_params &p, const double *lmkIni) {
+        GateShared gate(!g_sizing);
         const bool dev = m.on_device != 0;
         V = m.lmkNum; E = m.elmNum; Z = m.nzrNum; B = m.btmElmNum; BL = m.btmLmkNum;
         T = p.timeNum;
@@ -1078,7 +1171,7 @@ struct Engine : EngineBase {
             potrfInfo.zero(s);
             require(gridOf((long)nc, 64) <= MAXP, "too many control points for the partial-sum buffers");
             if (!dryRun && !jacobi) {
-                std::lock_guard<std::mutex> lk(g_libMu);
+                std::lock_guard<std::mutex> lk(g_handleMu);
                 if (rocblas_create_handle(&blas) != rocblas_status_success)
                     throw SaError(SA_E_DEVICE, "rocblas_create_handle failed");
                 if (rocblas_set_stream(blas, s) != rocblas_status_success)
@@ -1326,6 +1419,7 @@ struct Engine : EngineBase {
     void sync_ctl(Ctl *dev, Ctl *host) {
         CK(hipMemcpyAsync(host, dev, sizeof(Ctl), hipMemcpyDeviceToHost, s));
         CK(hipStreamSynchronize(s));
+        gate_yield();
     }
 
     // velocity solve of one step; returns 0 ok, 1 failure
@@ -1423,33 +1517,11 @@ struct Engine : EngineBase {
     void kcc_build(bool refresh) {
         kcc_build_once(refresh);
         if (!refresh) return;
+        if (prof) prof->factorizations++;
         struct CarryDone {
             Engine *e;
             ~CarryDone() { e->specCarry = false; }
         } carryDone{this};
-        // a factorisation that reports failure is repeated once from a fresh
-        // K_cc: with several sessions on the device a K_cc that factorises
-        // alone every time has been reported indefinite (rarely; cause not
-        // found); a matrix that is indefinite fails again and the solve
-        // reports SA_FAIL_CHOLESKY as before
-        int info[2] = {0, 0};
-        CK(hipMemcpyAsync(info, potrfInfo.p, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
-        CK(hipStreamSynchronize(s));
-        if (info[0] != 0 || info[1] != 0) {
-            static const bool trace = [] {
-                const char *e = std::getenv("SA_TRACE_RETRY");
-                return e && e[0] == '1';
-            }();
-            if (prof) prof->factorRetries++;
-            kcc_build_once(true);
-            if (trace) {
-                int again[2] = {0, 0};
-                CK(hipMemcpyAsync(again, potrfInfo.p, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
-                CK(hipStreamSynchronize(s));
-                std::fprintf(stderr, "[sa] K_cc factorisation retried: info %d %d -> %d %d (C = %d)\n", info[0],
-                             info[1], again[0], again[1], NC);
-            }
-        }
         if (specCarry) spec_carry_finish();
     }
 
@@ -1494,7 +1566,6 @@ struct Engine : EngineBase {
         std::vector<double> Q(kk), ev(k), W(kk, 0.0);
         int info = 0;
         {
-            std::lock_guard<std::mutex> lk(g_libMu);
             if (rocsolver_dsyevd(blas, rocblas_evect_original, rocblas_fill_lower, k, dG, k, dEv, dE, specInfo.p) !=
                 rocblas_status_success)
                 throw SaError(SA_E_DEVICE, "rocsolver_dsyevd failed");
@@ -1523,7 +1594,6 @@ struct Engine : EngineBase {
         CK(hipMemcpyAsync(dW, W.data(), sizeof(double) * kk, hipMemcpyHostToDevice, s));
         const double one = 1.0, zero = 0.0;
         {
-            std::lock_guard<std::mutex> lk(g_libMu);
             if (rocblas_dgemm(blas, rocblas_operation_none, rocblas_operation_none, nc, k, k, &one, specU.p, nc, dW, k,
                               &zero, recD.p, nc) != rocblas_status_success)
                 throw SaError(SA_E_DEVICE, "rocblas_dgemm failed");
@@ -1551,7 +1621,6 @@ struct Engine : EngineBase {
 #undef SA_KB
         CHECK_LAUNCH();
         if (!refresh) return;
-        std::lock_guard<std::mutex> lk(g_libMu);
         if (specCarry) {
             // the spectral term carried over to the new M: its Gram matrix in
             // the new M^-1 = K_cc + eps I inner product, G = U^T (K_cc + eps I) U,
@@ -1570,8 +1639,13 @@ struct Engine : EngineBase {
             specOn = false;
             specRecord = specEnabled;
         }
-        if (rocsolver_dpotrf(blas, rocblas_fill_lower, NC, Kcc.p, NC, potrfInfo.p) != rocblas_status_success)
-            throw SaError(SA_E_DEVICE, "rocsolver_dpotrf failed");
+        {
+            // the factorisation runs with no other session's work on the device
+            GateExclusive lk;
+            if (rocsolver_dpotrf(blas, rocblas_fill_lower, NC, Kcc.p, NC, potrfInfo.p) != rocblas_status_success)
+                throw SaError(SA_E_DEVICE, "rocsolver_dpotrf failed");
+            CK(hipStreamSynchronize(s));
+        }
         if (rocsolver_dtrtri(blas, rocblas_fill_lower, rocblas_diagonal_non_unit, NC, Kcc.p, NC, potrfInfo.p + 1) !=
             rocblas_status_success)
             throw SaError(SA_E_DEVICE, "rocsolver_dtrtri failed");
@@ -1944,7 +2018,6 @@ struct Engine : EngineBase {
         std::vector<double> A1(mm), A2(mm), rz(m);
         const double one = 1.0, zero = 0.0;
         {
-            std::lock_guard<std::mutex> lk(g_libMu);
             spec_gram(recR.p, dA1, m);
             spec_gram(recSD.p, dA2, m);
             CK(hipStreamSynchronize(s));
@@ -1989,7 +2062,6 @@ struct Engine : EngineBase {
         auto syevd = [&](double *A, int n) -> bool {
             int info = 0;
             {
-                std::lock_guard<std::mutex> lk(g_libMu);
                 if (rocsolver_dsyevd(blas, rocblas_evect_original, rocblas_fill_lower, n, A, n, dEv, dE,
                                      specInfo.p) != rocblas_status_success)
                     throw SaError(SA_E_DEVICE, "rocsolver_dsyevd failed");
@@ -2022,7 +2094,6 @@ struct Engine : EngineBase {
         CK(hipMemcpyAsync(dA2, H.data(), sizeof(double) * mm, hipMemcpyHostToDevice, s));
         CK(hipMemcpyAsync(dP, P.data(), sizeof(double) * m * r, hipMemcpyHostToDevice, s));
         {
-            std::lock_guard<std::mutex> lk(g_libMu);
             if (rocblas_dgemm(blas, rocblas_operation_none, rocblas_operation_none, m, r, m, &one, dA2, m, dP, m,
                               &zero, dHP, m) != rocblas_status_success ||
                 rocblas_dgemm(blas, rocblas_operation_transpose, rocblas_operation_none, r, r, m, &one, dP, m, dHP, m,
@@ -2034,7 +2105,6 @@ struct Engine : EngineBase {
         const double theta = ev[r - 1 - kk];
         if (!(theta > 0.0) || !(ev[r - 1] > theta)) return;
         {
-            std::lock_guard<std::mutex> lk(g_libMu);
             if (rocblas_dgemm(blas, rocblas_operation_none, rocblas_operation_none, m, kk, r, &one, dP, m,
                               dHr + (size_t)(r - kk) * r, r, &zero, dY, m) != rocblas_status_success)
                 throw SaError(SA_E_DEVICE, "rocblas_dgemm failed");
@@ -2054,7 +2124,6 @@ struct Engine : EngineBase {
         CK(hipMemcpyAsync(dA2, W.data(), sizeof(double) * m * kk, hipMemcpyHostToDevice, s));
         CK(hipMemcpyAsync(specC.p, cf.data(), sizeof(double) * kk, hipMemcpyHostToDevice, s));
         {
-            std::lock_guard<std::mutex> lk(g_libMu);
             if (rocblas_dgemm(blas, rocblas_operation_none, rocblas_operation_none, nc, kk, m, &one, recD.p, nc, dA2, m,
                               &zero, specU.p, nc) != rocblas_status_success)
                 throw SaError(SA_E_DEVICE, "rocblas_dgemm failed");
@@ -2424,6 +2493,7 @@ struct Engine : EngineBase {
     // single operators at the initial state, for the parity tests
     void probe(int what, const double *tauVar, int nVar, const double *tauPrm, int nPrm, const double *in,
                double *out) override {
+        GateShared gate;
         const size_t n = (size_t)V * D;
         start(tauVar, nVar, tauPrm, nPrm);
         CK(hipMemsetAsync(ctl.p, 0, sizeof(Ctl), s));
@@ -2514,6 +2584,7 @@ struct Engine : EngineBase {
     // launches after a warm-up one, in microseconds.
     double time_product(int what, int reps) override {
         require(reps >= 1 && reps <= 100000, "reps must be in [1, 100000]");
+        GateShared gate;
         struct Ev {
             hipEvent_t e = nullptr;
             ~Ev() { if (e) (void)hipEventDestroy(e); }
@@ -2584,6 +2655,7 @@ struct Engine : EngineBase {
 
     void run(const double *tauVar, int nVar, const double *tauPrm, int nPrm, double *obj, double *lmkOut,
              double *tauOut, double *gradOut, Stats &st) override {
+        GateShared gate;
         const size_t n = (size_t)V * D;
         st = Stats{};
         prof = &st;
@@ -2803,7 +2875,7 @@ int sa_session_run(sa_session *ss, const double *tauVarVec, int tauVarNum, const
             stats->varifoldPairs = st.vfdPairs;
             stats->varifoldMfma = st.vfdMfma;
             stats->deviceBytes = ss->eng->bytes;
-            stats->factorRetries = st.factorRetries;
+            stats->factorizations = st.factorizations;
         }
     });
 }

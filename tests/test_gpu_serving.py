@@ -43,12 +43,13 @@ def alone(ms, reqs, precision):
     return res
 
 
-@pytest.mark.parametrize("precision", [
-    "fp64",
-    pytest.param("mixed", marks=pytest.mark.skip(
-        reason="ended in a host segmentation fault while CG iterations were graph-captured in worker threads; "
-               "graph capture is now off, and the case stays off the GPU until the cause is shown")),
-])
+# The mixed case once ended in a host segmentation fault while CG iterations
+# were graph-captured in worker threads; that code path is gone (no graph
+# capture remains in the library), and the served mixed form has run since
+# in the C4 and C5 configuration tests.  FP64: bitwise equal.  Mixed: bitwise
+# equal in every run since the K_cc factorisation runs with no other
+# session's work on the device, asserted to 1e-9 relative.
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_alone_equals_mix(precision):
     ms = meshes()
     reqs = requests(ms)
@@ -58,9 +59,16 @@ def test_alone_equals_mix(precision):
     rids = [srv.submit(ids[m], tv, ms[m]["tauPrm"], prm) for m, tv, prm in reqs]
     got = [srv.wait(r) for r in rids]
     srv.close()
-    for g, r in zip(got, ref):
+    for g in got:
         assert g["rc"] == 0 and g["status"] == 0, g
-        assert g["obj"] == r, (g["obj"], r)
+    objs = [g["obj"] for g in got]
+    d = max(abs(a - b) / abs(b) for a, b in zip(objs, ref))
+    print(f"[gpu] served {precision}: max relative difference to alone {d:.2e}, bitwise equal {objs == ref}",
+          flush=True)
+    if precision == "fp64":
+        assert objs == ref
+    else:
+        assert d <= 1e-9
     # candidates differ, so the objectives differ
     assert len(set(ref)) == len(ref)
 
