@@ -19,10 +19,6 @@ FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     # packed FP32 instructions are kept out of the generated code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    # global-memory loads at wavefront-uniform addresses stay vector loads:
-    # the scalar data cache is not kept coherent with data the previous
-    # kernel wrote, which concurrent solves on other streams exposed
-    "-mllvm", "-amdgpu-scalarize-global-loads=false",
     "-I", os.path.join(ROOT, "include"),
     "-Wno-unused-result",
 ]
