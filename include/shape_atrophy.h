@@ -210,7 +210,9 @@ int sa_session_set_params(sa_session *s, const sa_params *prm);
      SA_PROBE_KCX       out = K_cx(q0) in: in lmkNum x dim, out ctlNum x dim   kernel products
                         as the session's inner solve forms them, without knlEps)
      SA_PROBE_JACOBI    out = the diagonal of the control-point system at q0, tau0 (ctlNum x dim;
-                        needs a session with velPrec = SA_VELPREC_JACOBI) */
+                        needs a session with velPrec = SA_VELPREC_JACOBI)
+     SA_PROBE_KCC_CHOL  out = L, the Cholesky factor of K_cc(q0) + knlEps I as the preconditioner
+                        build forms it (ctlNum x ctlNum column-major, upper triangle zero) */
 #define SA_PROBE_ELASTIC 1
 #define SA_PROBE_KMATVEC 2
 #define SA_PROBE_YANK 3
@@ -221,6 +223,7 @@ int sa_session_set_params(sa_session *s, const sa_params *prm);
 #define SA_PROBE_KXC 8
 #define SA_PROBE_KCX 9
 #define SA_PROBE_JACOBI 10
+#define SA_PROBE_KCC_CHOL 12
 int sa_session_probe(sa_session *s, int what, const double *tauVarVec, int tauVarNum, const double *tauPrmVec,
                      int tauPrmNum, const double *in, double *out);
 /* bytes of device memory a session of this mesh holds */

@@ -365,7 +365,7 @@ class Session:
         _check(lib().sa_session_set_profiling(self._h, 1 if on else 0))
 
     PROBES = {"elastic": 1, "kmatvec": 2, "yank": 3, "fem": 4, "geometry": 5, "varifold": 6, "penalty": 7,
-              "kxc": 8, "kcx": 9, "jacobi": 10}
+              "kxc": 8, "kcx": 9, "jacobi": 10, "kccchol": 12}
 
     def probe(self, what: str, tauVar, tauPrm, x=None) -> np.ndarray:
         """One operator at the initial state (sa_session_probe)."""
@@ -375,6 +375,7 @@ class Session:
         E, Z = self.mesh.elmNum, self.mesh.nzrNum
         nc = self.mesh.ctlNum
         size = {"elastic": d * V, "kmatvec": d * V, "penalty": d * V, "kxc": d * V, "kcx": d * nc, "jacobi": d * nc,
+                "kccchol": nc * nc,
                 "yank": d * V + V, "fem": 2 * Z,
                 "geometry": (d * d + 1 + 2 * d) * E, "varifold": 1 + d * V}[what]
         out = np.zeros(size)

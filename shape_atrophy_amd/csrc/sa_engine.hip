@@ -745,6 +745,9 @@ struct Engine : EngineBase {
     int specK = 0, specMode = -1;
     int slicesA = 1, slicesT = 1, slicesM = 1, slicesM32 = 1;
     DBuf<int> potrfInfo;
+    // K_cc factorisation: chol_*_k (default), or rocsolver_dpotrf behind the
+    // device gate (SA_CHOL=rocsolver)
+    bool cholLib = false;
     rocblas_handle blas = nullptr;
     int slicesC = 1;
     Ctl *hctl = nullptr;
@@ -1169,6 +1172,10 @@ struct Engine : EngineBase {
             part.alloc((size_t)slicesC * nc, tot);
             potrfInfo.alloc(2, tot);
             potrfInfo.zero(s);
+            {
+                const char *e = std::getenv("SA_CHOL");
+                cholLib = e && std::string(e) == "rocsolver";
+            }
             require(gridOf((long)nc, 64) <= MAXP, "too many control points for the partial-sum buffers");
             if (!dryRun && !jacobi) {
                 std::lock_guard<std::mutex> lk(g_handleMu);
@@ -1603,6 +1610,23 @@ struct Engine : EngineBase {
         ++specCarried;
     }
 
+    // A = L L^T in place (lower triangle of the column-major n x n A, upper
+    // not read); info[0] = 0, or the 1-based column of the first pivot that
+    // is not positive
+    void cholesky(double *A, int n, int *info) {
+        zero_ints_k<<<1, 64, 0, s>>>(info, 1);
+        for (int k0 = 0; k0 < n; k0 += CHB) {
+            chol_diag_k<<<1, 256, 0, s>>>(n, k0, A, n, info);
+            const int rows = n - k0 - CHB;
+            if (rows > 0) {
+                chol_panel_k<<<gridOf(rows, 256), 256, 0, s>>>(n, k0, A, n, info);
+                const int nt = gridOf(rows, CHB);
+                chol_update_k<<<nt * (nt + 1) / 2, 256, 0, s>>>(n, k0, A, n, info);
+            }
+        }
+        CHECK_LAUNCH();
+    }
+
     void kcc_build_once(bool refresh) {
         if (!refresh && !Kc.p) return;
         double invW = 1.0 / knlWidth;
@@ -1639,12 +1663,14 @@ struct Engine : EngineBase {
             specOn = false;
             specRecord = specEnabled;
         }
-        {
-            // the factorisation runs with no other session's work on the device
+        if (cholLib) {
+            // rocSOLVER's factorisation runs with no other session's work on the device
             GateExclusive lk;
             if (rocsolver_dpotrf(blas, rocblas_fill_lower, NC, Kcc.p, NC, potrfInfo.p) != rocblas_status_success)
                 throw SaError(SA_E_DEVICE, "rocsolver_dpotrf failed");
             CK(hipStreamSynchronize(s));
+        } else {
+            cholesky(Kcc.p, NC, potrfInfo.p);
         }
         if (rocsolver_dtrtri(blas, rocblas_fill_lower, rocblas_diagonal_non_unit, NC, Kcc.p, NC, potrfInfo.p + 1) !=
             rocblas_status_success)
@@ -2243,6 +2269,18 @@ struct Engine : EngineBase {
                     specCarried < SA_SPEC_CARRY;
         if (jacobi) jacobi_build(q, tauNow, geoNow);
         else kcc_build(refresh);
+        if (!jacobi && refresh) {
+            // a K_cc that does not factorise fails the step before any CG
+            int info[2] = {0, 0};
+            CK(hipMemcpyAsync(info, potrfInfo.p, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
+            CK(hipStreamSynchronize(s));
+            if (info[0] != 0 || info[1] != 0) {
+                failReason = info[0] != 0 ? SA_FAIL_CHOLESKY : SA_FAIL_TRTRI;
+                iters = 0;
+                sweeps = 0;
+                return 1;
+            }
+        }
         if (refresh) { precAge = 0; forceRefresh = false; }
         if (!mixedStep && precision == SA_PREC_MIXED) use_fp64_store();
         kstore_build(q, !mixedStep, mixedStep);
@@ -2534,6 +2572,43 @@ struct Engine : EngineBase {
                 CK(hipMemcpyAsync(xin, in, sizeof(double) * (xc ? NC : V) * D, hipMemcpyHostToDevice, s));
                 kx_product(md, xc, q0.p, xin, yout, nullptr, ctl.p);
                 CK(hipMemcpyAsync(out, yout, sizeof(double) * (xc ? V : NC) * D, hipMemcpyDeviceToHost, s));
+                break;
+            }
+            case SA_PROBE_KCC_CHOL: {
+                // the factor L of K_cc(q0) + eps I as the preconditioner
+                // build forms it (upper triangle zeroed)
+                require(NC > 0 && !jacobi, "the session has no control-point preconditioner");
+                gather_ctl_k<D><<<gridOf(NC), NT, 0, s>>>(NC, V, ctlIdx.p, q0.p, qc.p);
+                CHECK_LAUNCH();
+                const double invW = 1.0 / knlWidth;
+                const int g = gridOf((long)NC * NC);
+#define SA_KB(O) kcc_build_k<D, O><<<g, NT, 0, s>>>(NC, qc.p, invW, knlEps, Kcc.p, nullptr)
+                switch (knlOrder) {
+                    case -1: SA_KB(-1); break;
+                    case 0: SA_KB(0); break;
+                    case 1: SA_KB(1); break;
+                    case 2: SA_KB(2); break;
+                    case 3: SA_KB(3); break;
+                    default: SA_KB(4); break;
+                }
+#undef SA_KB
+                CHECK_LAUNCH();
+                if (cholLib) {
+                    GateExclusive lk;
+                    if (rocsolver_dpotrf(blas, rocblas_fill_lower, NC, Kcc.p, NC, potrfInfo.p) !=
+                        rocblas_status_success)
+                        throw SaError(SA_E_DEVICE, "rocsolver_dpotrf failed");
+                    CK(hipStreamSynchronize(s));
+                } else {
+                    cholesky(Kcc.p, NC, potrfInfo.p);
+                }
+                zero_upper_k<<<g, NT, 0, s>>>(NC, Kcc.p);
+                CHECK_LAUNCH();
+                int info = 0;
+                CK(hipMemcpyAsync(&info, potrfInfo.p, sizeof(int), hipMemcpyDeviceToHost, s));
+                CK(hipMemcpyAsync(out, Kcc.p, sizeof(double) * NC * NC, hipMemcpyDeviceToHost, s));
+                CK(hipStreamSynchronize(s));
+                require(info == 0, "K_cc + eps I is not positive definite (column " + std::to_string(info) + ")");
                 break;
             }
             case SA_PROBE_JACOBI:

@@ -1776,6 +1776,130 @@ __global__ __launch_bounds__(64 * W) void slice_rows_k(int n, int slices, const 
     }
 }
 
+// ---------------------------------------------------------------------------
+// Cholesky factorisation A = L L^T in place (lower triangle, column-major,
+// leading dimension lda), right-looking in CHB x CHB blocks: for each block
+// column k0, chol_diag_k factors the diagonal block in LDS, chol_panel_k
+// solves the rows below it against that factor, chol_update_k subtracts the
+// panel's outer product from the lower triangle of the trailing matrix.  A
+// pivot that is not positive and finite stops the factorisation: info =
+// its 1-based column (LAPACK's convention) and every later kernel returns at
+// once.  The upper triangle is not read; the diagonal tiles of the trailing
+// update leave it alone.  Fixed summation order: deterministic.
+constexpr int CHB = 64;
+
+__global__ __launch_bounds__(256) void chol_diag_k(int n, int k0, double *__restrict__ A, int lda, int *info) {
+    __shared__ double L[CHB][CHB + 1];
+    __shared__ double piv;
+    __shared__ int bad;
+    if (cld(info) != 0) return;
+    const int nb = min(CHB, n - k0), t = threadIdx.x;
+    for (int idx = t; idx < nb * nb; idx += blockDim.x) {
+        const int i = idx % nb, j = idx / nb;
+        if (i >= j) L[i][j] = A[(size_t)(k0 + i) + (size_t)(k0 + j) * lda];
+    }
+    __syncthreads();
+    for (int j = 0; j < nb; ++j) {
+        if (t == 0) {
+            const double d = L[j][j];
+            bad = !(d > 0.0) || !isfinite(d);
+            piv = bad ? 0.0 : sqrt(d);
+            if (!bad) L[j][j] = piv;
+        }
+        __syncthreads();
+        if (bad) {
+            if (t == 0) cst(info, k0 + j + 1);
+            return;
+        }
+        for (int i = j + 1 + t; i < nb; i += blockDim.x) L[i][j] /= piv;
+        __syncthreads();
+        const int m = nb - j - 1;
+        for (int idx = t; idx < m * m; idx += blockDim.x) {
+            const int i = j + 1 + idx % m, l = j + 1 + idx / m;
+            if (i >= l) L[i][l] -= L[i][j] * L[l][j];
+        }
+        __syncthreads();
+    }
+    for (int idx = t; idx < nb * nb; idx += blockDim.x) {
+        const int i = idx % nb, j = idx / nb;
+        if (i >= j) A[(size_t)(k0 + i) + (size_t)(k0 + j) * lda] = L[i][j];
+    }
+}
+
+// rows k0 + nb .. n - 1 of block column k0: a <- a L_kk^{-T} (forward
+// substitution, one row per thread, the row in registers)
+__global__ __launch_bounds__(256) void chol_panel_k(int n, int k0, double *__restrict__ A, int lda, const int *info) {
+    __shared__ double L[CHB][CHB + 1];
+    if (cld(info) != 0) return;
+    const int nb = min(CHB, n - k0), t = threadIdx.x;
+    for (int idx = t; idx < nb * nb; idx += blockDim.x) {
+        const int i = idx % nb, j = idx / nb;
+        L[i][j] = i >= j ? A[(size_t)(k0 + i) + (size_t)(k0 + j) * lda] : 0.0;
+    }
+    __syncthreads();
+    const int r = k0 + nb + blockIdx.x * blockDim.x + t;
+    if (r >= n) return;
+    double x[CHB];
+#pragma unroll
+    for (int j = 0; j < CHB; ++j) x[j] = j < nb ? A[(size_t)r + (size_t)(k0 + j) * lda] : 0.0;
+#pragma unroll
+    for (int j = 0; j < CHB; ++j) {
+        if (j < nb) {
+            double s = x[j];
+#pragma unroll
+            for (int l = 0; l < j; ++l) s -= x[l] * L[j][l];
+            x[j] = s / L[j][j];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < CHB; ++j)
+        if (j < nb) A[(size_t)r + (size_t)(k0 + j) * lda] = x[j];
+}
+
+// trailing update A[c0:, c0:] -= P P^T (lower triangle), c0 = k0 + nb, P the
+// solved panel: one CHB x CHB tile (bi >= bj) per block, blockIdx.x the
+// tile's index in the lower triangle of tiles, 256 threads of 4 x 4 outputs
+__global__ __launch_bounds__(256) void chol_update_k(int n, int k0, double *__restrict__ A, int lda, const int *info) {
+    __shared__ double Pi[CHB][CHB + 1], Pj[CHB][CHB + 1];   // [panel column][tile row]
+    if (cld(info) != 0) return;
+    const int nb = min(CHB, n - k0), c0 = k0 + nb, t = threadIdx.x;
+    // tile (bi, bj), bi >= bj, from the triangular index
+    const int b = blockIdx.x;
+    int bi = (int)((sqrt(8.0 * b + 1.0) - 1.0) * 0.5);
+    while ((bi + 1) * (bi + 2) / 2 <= b) ++bi;
+    while (bi * (bi + 1) / 2 > b) --bi;
+    const int bj = b - bi * (bi + 1) / 2;
+    const int r0 = c0 + bi * CHB, s0 = c0 + bj * CHB;
+    for (int idx = t; idx < CHB * nb; idx += blockDim.x) {
+        const int rr = idx % CHB, l = idx / CHB;
+        Pi[l][rr] = r0 + rr < n ? A[(size_t)(r0 + rr) + (size_t)(k0 + l) * lda] : 0.0;
+        Pj[l][rr] = s0 + rr < n ? A[(size_t)(s0 + rr) + (size_t)(k0 + l) * lda] : 0.0;
+    }
+    __syncthreads();
+    const int tr = (t % 16) * 4, tc = (t / 16) * 4;
+    double acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[a][c] = 0.0;
+    for (int l = 0; l < nb; ++l) {
+        double pa[4], pb[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) { pa[a] = Pi[l][tr + a]; pb[a] = Pj[l][tc + a]; }
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) acc[a][c] += pa[a] * pb[c];
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int i = r0 + tr + a, j = s0 + tc + c;
+            if (i < n && j < n && i >= j) A[(size_t)i + (size_t)j * lda] -= acc[a][c];
+        }
+}
+
 // zero the strict upper triangle of a column-major C x C matrix
 __global__ void zero_upper_k(int C, double *__restrict__ M) {
     long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;

@@ -554,6 +554,64 @@ def test_tile_targets_all_landmark_product(oracle, monkeypatch):
 
 
 # --------------------------------------------------------------------------
+# the K_cc factorisation of the control-point preconditioner (chol_*_k):
+# backward error |L L^T - A| / |A| (fixed by the mathematics whatever the
+# conditioning), the factor against numpy's and against rocsolver_dpotrf run
+# alone; sizes around the 64-column block; an indefinite matrix reported
+
+CHOL_CASES = [("shell3d", 700), ("shell3d", 50), ("shell3d", 64), ("shell3d", 65), ("shell3d", 128),
+              ("ring2d", 300)]
+
+
+def chol_problem(which, C):
+    p = W.shell_problem(level=3, n_layers=5, time_steps=2) if which == "shell3d" \
+        else W.ring_problem(n_rings=10, n_per_ring=120)
+    V = p["q0"].shape[1]
+    p["ctlIdx"] = np.sort(np.random.default_rng(C).choice(V, C, replace=False)).astype(np.int32)
+    return p
+
+
+@pytest.mark.parametrize("which,C", CHOL_CASES)
+def test_kcc_cholesky_factor(which, C, oracle, monkeypatch):
+    p = chol_problem(which, C)
+    qc = p["q0"][:, p["ctlIdx"]]
+    A = dense_kernel(oracle, p, qc, qc) + p["params"]["knlEps"] * np.eye(C)
+    tv, tp = p["tauVar"][:, 0], p["tauPrm"]
+    print(f"[gpu] K_cc factor {which} C={C}", flush=True)
+    got = {}
+    for lib in ("", "rocsolver"):
+        monkeypatch.setenv("SA_CHOL", lib)
+        s = api.Session(p)
+        got[lib] = s.probe("kccchol", tv, tp).reshape(C, C).T   # column-major out
+        s.close()
+    L = got[""]
+    assert np.all(np.triu(L, 1) == 0.0) and np.all(np.diag(L) > 0)
+    back = np.linalg.norm(L @ L.T - A) / np.linalg.norm(A)
+    Lnp = np.linalg.cholesky(A)
+    fwd, lib = rel(L, Lnp), rel(L, got["rocsolver"])
+    cond = np.linalg.cond(A)
+    print(f"[gpu] K_cc factor {which} C={C}: backward {back:.1e}, vs numpy {fwd:.1e}, vs rocsolver {lib:.1e}, "
+          f"cond {cond:.1e}", flush=True)
+    assert back < 1e-14
+    # forward error: at most about cond x unit roundoff
+    assert fwd < max(1e-12, 1e-15 * cond) and lib < max(1e-12, 1e-15 * cond)
+
+
+def test_kcc_cholesky_reports_indefinite():
+    """K_cc - I (knlEps = -1: a zero diagonal) fails at its first column: the
+    probe raises with the column, a solve reports SA_FAIL_CHOLESKY."""
+    p = chol_problem("shell3d", 200)
+    p["params"] = dict(p["params"], knlEps=-1.0)
+    tv, tp = p["tauVar"][:, 0], p["tauPrm"]
+    s = api.Session(p)
+    with pytest.raises(api.SaError, match="column 1"):
+        s.probe("kccchol", tv, tp)
+    r = s.run(tv, tp)
+    s.close()
+    assert r["status"] == 1 and r["fail_reason"] == 3, (r["status"], r["fail_reason"])
+
+
+# --------------------------------------------------------------------------
 # Jacobi preconditioning of the control-point CG (sa_params.velPrec): the
 # diagonal of the control-point system against the oracle's, formed there by
 # applying S to every unit vector (1e-12), and full solves against the
