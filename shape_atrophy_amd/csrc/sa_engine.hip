@@ -1616,7 +1616,7 @@ struct Engine : EngineBase {
     void cholesky(double *A, int n, int *info) {
         zero_ints_k<<<1, 64, 0, s>>>(info, 1);
         for (int k0 = 0; k0 < n; k0 += CHB) {
-            chol_diag_k<<<1, 256, 0, s>>>(n, k0, A, n, info);
+            chol_diag_k<<<1, CHB, 0, s>>>(n, k0, A, n, info);
             const int rows = n - k0 - CHB;
             if (rows > 0) {
                 chol_panel_k<<<gridOf(rows, 256), 256, 0, s>>>(n, k0, A, n, info);

@@ -1788,42 +1788,47 @@ __global__ __launch_bounds__(64 * W) void slice_rows_k(int n, int slices, const 
 // update leave it alone.  Fixed summation order: deterministic.
 constexpr int CHB = 64;
 
-__global__ __launch_bounds__(256) void chol_diag_k(int n, int k0, double *__restrict__ A, int lda, int *info) {
-    __shared__ double L[CHB][CHB + 1];
+// one wavefront: lane i keeps row i of the diagonal block in registers
+// (r[l] = L[i][l]); step j publishes the pivot and then column j through
+// LDS, and every lane below updates its own row (the same operations, in the
+// same order, as a right-looking update of the whole block)
+__global__ __launch_bounds__(CHB) void chol_diag_k(int n, int k0, double *__restrict__ A, int lda, int *info) {
+    __shared__ double col[CHB];
     __shared__ double piv;
-    __shared__ int bad;
     if (cld(info) != 0) return;
-    const int nb = min(CHB, n - k0), t = threadIdx.x;
-    for (int idx = t; idx < nb * nb; idx += blockDim.x) {
-        const int i = idx % nb, j = idx / nb;
-        if (i >= j) L[i][j] = A[(size_t)(k0 + i) + (size_t)(k0 + j) * lda];
-    }
-    __syncthreads();
-    for (int j = 0; j < nb; ++j) {
-        if (t == 0) {
-            const double d = L[j][j];
-            bad = !(d > 0.0) || !isfinite(d);
-            piv = bad ? 0.0 : sqrt(d);
-            if (!bad) L[j][j] = piv;
+    const int nb = min(CHB, n - k0), i = threadIdx.x;
+    double r[CHB];
+#pragma unroll
+    for (int l = 0; l < CHB; ++l) r[l] = (i < nb && l <= i) ? A[(size_t)(k0 + i) + (size_t)(k0 + l) * lda] : 0.0;
+    bool ok = true;   // the same in every lane
+#pragma unroll
+    for (int j = 0; j < CHB; ++j) {
+        if (j < nb && ok) {
+            if (i == j) piv = r[j];
+            __syncthreads();
+            const double d = piv;
+            if (!(d > 0.0) || !isfinite(d)) {
+                if (i == 0) cst(info, k0 + j + 1);
+                ok = false;
+            } else {
+                const double p = sqrt(d);
+                if (i == j) r[j] = p;
+                else if (i > j) r[j] = r[j] / p;
+                if (i > j) col[i] = r[j];
+                __syncthreads();
+                if (i > j) {
+#pragma unroll
+                    for (int l = j + 1; l < CHB; ++l)
+                        if (l <= i) r[l] -= r[j] * col[l];
+                }
+                __syncthreads();
+            }
         }
-        __syncthreads();
-        if (bad) {
-            if (t == 0) cst(info, k0 + j + 1);
-            return;
-        }
-        for (int i = j + 1 + t; i < nb; i += blockDim.x) L[i][j] /= piv;
-        __syncthreads();
-        const int m = nb - j - 1;
-        for (int idx = t; idx < m * m; idx += blockDim.x) {
-            const int i = j + 1 + idx % m, l = j + 1 + idx / m;
-            if (i >= l) L[i][l] -= L[i][j] * L[l][j];
-        }
-        __syncthreads();
     }
-    for (int idx = t; idx < nb * nb; idx += blockDim.x) {
-        const int i = idx % nb, j = idx / nb;
-        if (i >= j) A[(size_t)(k0 + i) + (size_t)(k0 + j) * lda] = L[i][j];
-    }
+    if (!ok) return;
+#pragma unroll
+    for (int l = 0; l < CHB; ++l)
+        if (i < nb && l <= i) A[(size_t)(k0 + i) + (size_t)(k0 + l) * lda] = r[l];
 }
 
 // rows k0 + nb .. n - 1 of block column k0: a <- a L_kk^{-T} (forward
@@ -1845,10 +1850,11 @@ __global__ __launch_bounds__(256) void chol_panel_k(int n, int k0, double *__res
 #pragma unroll
     for (int j = 0; j < CHB; ++j) {
         if (j < nb) {
-            double s = x[j];
+            // four partial sums (shorter dependency chains), added in a fixed order
+            double p[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-            for (int l = 0; l < j; ++l) s -= x[l] * L[j][l];
-            x[j] = s / L[j][j];
+            for (int l = 0; l < j; ++l) p[l & 3] += x[l] * L[j][l];
+            x[j] = (x[j] - ((p[0] + p[1]) + (p[2] + p[3]))) / L[j][j];
         }
     }
 #pragma unroll
