@@ -494,7 +494,7 @@ def main():
                     help="preconditioner of the velocity CG (jacobi: control-point form only)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true",
-                    help="time without the per-kernel events (graph-captured CG iterations); no roofline")
+                    help="time without the kernel timer's events (no roofline, no operator or varifold timing)")
     ap.add_argument("--cpu-iters", type=int, default=0, help="CG iterations of the CPU sample (0: sized to the workload)")
     ap.add_argument("--mode", default="single",
                     choices=["single", "batch", "serve", "precision", "fit", "products", "tune", "operator"],
