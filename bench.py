@@ -371,11 +371,14 @@ def mode_operator(args, api, W):
 
 
 TUNE_TARGETS = (32, 64, 128, 256, 512, 1024, 2048, 4096, 8192, 16384)
-# the library's built-in targets (KS_BLOCKS = 1024; twice that for FP32 K_xc /
-# K_cx; 2048 for the all-landmark product), kept unless a target is more than
-# TUNE_KEEP faster: two sweeps on different boxes differed by up to 5% at the
-# same setting on the large products
-TUNE_DEFAULT = {("xc", "f32"): 2048, ("cx", "f32"): 2048, ("xc", "f64"): 1024, ("cx", "f64"): 1024,
+# FP32 products: a block needs at most 24 KiB of LDS, so the targets between
+# 256 and 2048 are resolved finer (one resident round is 4-6 blocks per CU)
+TUNE_TARGETS_F32 = (32, 64, 128, 256, 384, 512, 768, 1024, 1280, 1536, 2048, 4096)
+# the library's built-in targets (KS_BLOCKS = 1024; 2048 for the all-landmark
+# product), kept unless a target is more than TUNE_KEEP faster: two sweeps on
+# different boxes differed by up to 5% at the same setting on the large
+# products
+TUNE_DEFAULT = {("xc", "f32"): 1024, ("cx", "f32"): 1024, ("xc", "f64"): 1024, ("cx", "f64"): 1024,
                 ("cc", "f32"): 1024, ("cc", "f64"): 1024, ("kk", "f64"): 2048,
                 ("op", "f64"): 1}   # the fused operator apply's launch shape (index, 0-3)
 OP_SHAPES = (0, 1, 2, 3)
@@ -420,11 +423,14 @@ def mode_tune(args, api, W):
     rows, table = [], []
     names = {"xc": "kxc", "cx": "kcx", "cc": "precond", "kk": "kmatvec"}
     op_done = set()
+    f32_only = args.tune_only == "f32"
     for desc, p in tune_shapes(W):
         d, V = p["q0"].shape
         nc = len(p["ctlIdx"]) if p.get("ctlIdx") is not None else 0
         E = p["elm"]["elmVtxMat"].shape[1]
-        if (E, V) not in op_done:
+        if f32_only and not nc:
+            continue
+        if (E, V) not in op_done and not f32_only:
             # the fused operator apply: one launch shape per session
             op_done.add((E, V))
             times = {}
@@ -443,14 +449,14 @@ def mode_tune(args, api, W):
                          "us": {str(t): round(v, 2) for t, v in times.items()}, "target": pick})
             print(f"[tune] {desc} operator (E {E}, V {V}): " + " ".join(f"{t}:{v:.1f}" for t, v in times.items())
                   + f" -> {pick}", file=sys.stderr, flush=True)
-        precs = ("mixed", "fp64") if nc else ("fp64",)
+        precs = ("mixed",) if f32_only else ("mixed", "fp64") if nc else ("fp64",)
         for prec in precs:
             f32 = prec == "mixed"
             prods = ("xc", "cx", "cc") if nc else ("kk",)
             for prod in prods:
                 shape = {"xc": (V, nc), "cx": (nc, V), "cc": (nc, nc), "kk": (V, V)}[prod]
                 times = {}
-                for t in TUNE_TARGETS:
+                for t in (TUNE_TARGETS_F32 if f32 and prod != "kk" else TUNE_TARGETS):
                     os.environ["SA_KS_TARGET"] = f"{prod}={t}"
                     s = api.Session(p, precision=prec)
                     try:
@@ -500,6 +506,8 @@ def main():
                     choices=["single", "batch", "serve", "precision", "fit", "products", "tune", "operator"],
                     help="single: one solve of --config; batch: 64 uneven meshes concurrently; serve: Poisson stream")
     ap.add_argument("--tune-reps", type=int, default=30, help="product launches per timing of --mode tune")
+    ap.add_argument("--tune-only", default="all", choices=["all", "f32"],
+                    help="--mode tune: every product and the operator, or only the FP32 products of the mixed form")
     ap.add_argument("--workers", type=int, default=8)
     ap.add_argument("--batch-n", type=int, default=64)
     ap.add_argument("--pool", type=int, default=12)

@@ -1149,13 +1149,12 @@ struct Engine : EngineBase {
                 At64.p = kmem.p ? kmem.p + na64 : nullptr;
             }
             // blocks per stored product (row blocks x column slices) from the
-            // tile table; by default KS_BLOCKS, twice that for FP32 products
-            // (half the bytes per column), and at least enough slices for the
-            // LDS copy of x (KS_MAX_COLS columns)
+            // tile table, by default KS_BLOCKS, and at least enough slices
+            // for the LDS copy of x (KS_MAX_COLS columns)
             const bool f32 = precision == SA_PREC_MIXED;
             const int prS = f32 ? panel_rows<float>() : panel_rows<double>();
             const int rbA = gridOf(V, prS), rbT = gridOf(NC, prS), rbM = gridOf(NC, panel_rows<double>());
-            const int dflt = f32 ? 2 * KS_BLOCKS : KS_BLOCKS;
+            const int dflt = KS_BLOCKS;
             const int tgA = ks_target("xc", f32, V, NC, dflt), tgT = ks_target("cx", f32, NC, V, dflt);
             const int tgM = ks_target("cc", false, NC, NC, KS_BLOCKS), tgM32 = ks_target("cc", true, NC, NC, KS_BLOCKS);
             const int rbM32 = gridOf(NC, panel_rows<float>());
@@ -1231,7 +1230,7 @@ struct Engine : EngineBase {
         if (Kst.p) {
             // stored K (built from qq by kst_build this step): HBM-bound product
             dim3 g(gridOf(V, panel_rows<double>()), slicesK);
-            kstore_mv_k<D, double><<<g, NT, ks_lds(V, slicesK), s>>>(V, V, Kst.p, xin, partS.p, ctl.p);
+            kstore_mv_k<D, double><<<g, NT, ks_lds<double>(V, slicesK), s>>>(V, V, Kst.p, xin, partS.p, ctl.p);
             CHECK_LAUNCH();
             kfin_k<D, CHECK><<<gridOf(V), NT, 0, s>>>(V, slicesK, partS.p, xin, yout, knlEps, Prr, nPrr, P3.p, it,
                                                       ctl.p);
@@ -1703,8 +1702,9 @@ struct Engine : EngineBase {
         CK(hipStreamSynchronize(s));
     }
 
-    // dynamic LDS of kstore_mv_k: the D x slice copy of x
-    static size_t ks_lds(int ncol, int slices) { return sizeof(double) * D * (size_t)gridOf(ncol, slices); }
+    // dynamic LDS of kstore_mv_k
+    template <typename T>
+    static size_t ks_lds(int ncol, int slices) { return ks_lds_bytes<D, T>(ncol, slices); }
 
     template <typename T>
     static size_t padded_rows(int n) { return (size_t)gridOf(n, panel_rows<T>()) * panel_rows<T>(); }
@@ -1756,7 +1756,7 @@ struct Engine : EngineBase {
     // y = M x for a dense FP64 C x C panel-major matrix
     void cc_product(const double *M, const double *x, double *y) {
         dim3 g(gridOf(NC, panel_rows<double>()), slicesM);
-        kstore_mv_k<D, double><<<g, NT, ks_lds(NC, slicesM), s>>>(NC, NC, M, x, partS.p, ctl.p);
+        kstore_mv_k<D, double><<<g, NT, ks_lds<double>(NC, slicesM), s>>>(NC, NC, M, x, partS.p, ctl.p);
         CHECK_LAUNCH();
         slice_rows(D * NC, slicesM, partS.p, y, ctl.p);
     }
@@ -1789,11 +1789,11 @@ struct Engine : EngineBase {
         if ((md == KMode::Stored32 || md == KMode::Mfma32) && Mpre32.p) {
             sl = slicesM32;
             dim3 g(gridOf(NC, panel_rows<float>()), sl);
-            kstore_mv_k<D, float><<<g, NT, ks_lds(NC, sl), s>>>(NC, NC, Mpre32.p, r, partS.p, ctl.p);
+            kstore_mv_k<D, float><<<g, NT, ks_lds<float>(NC, sl), s>>>(NC, NC, Mpre32.p, r, partS.p, ctl.p);
         } else {
             sl = slicesM;
             dim3 g(gridOf(NC, panel_rows<double>()), sl);
-            kstore_mv_k<D, double><<<g, NT, ks_lds(NC, sl), s>>>(NC, NC, Mpre.p, r, partS.p, ctl.p);
+            kstore_mv_k<D, double><<<g, NT, ks_lds<double>(NC, sl), s>>>(NC, NC, Mpre.p, r, partS.p, ctl.p);
         }
         CHECK_LAUNCH();
         slice_dot_k<<<gridOf(nc, 64), NTS, 0, s>>>(nc, sl, partS.p, P2.p, gridOf(nc), r, z, P3.p,
@@ -1873,8 +1873,8 @@ struct Engine : EngineBase {
         if (md == KMode::Split32) {
             // hi product into slices [0, sl), lo product into [sl, 2 sl), summed together
             dim3 g(gridOf(nr, panel_rows<float>()), sl);
-            kstore_mv_k<D, float><<<g, NT, ks_lds(ncol, sl), s>>>(nr, ncol, xc ? A32.p : At32.p, u, partS.p, c);
-            kstore_mv_k<D, float><<<g, NT, ks_lds(ncol, sl), s>>>(nr, ncol, xc ? A32lo.p : At32lo.p, u,
+            kstore_mv_k<D, float><<<g, NT, ks_lds<float>(ncol, sl), s>>>(nr, ncol, xc ? A32.p : At32.p, u, partS.p, c);
+            kstore_mv_k<D, float><<<g, NT, ks_lds<float>(ncol, sl), s>>>(nr, ncol, xc ? A32lo.p : At32lo.p, u,
                                                                    partS.p + (size_t)sl * D * nr, c);
             CHECK_LAUNCH();
             if (P) slice_sum_k<<<gridOf((long)D * nr), NT, 0, s>>>(D * nr, 2 * sl, partS.p, y, P, c);
@@ -1885,12 +1885,12 @@ struct Engine : EngineBase {
         if (ev) CK(hipEventRecord(ev[0], s));
         if (md == KMode::Stored64) {
             dim3 g(gridOf(nr, panel_rows<double>()), sl);
-            kstore_mv_k<D, double><<<g, NT, ks_lds(ncol, sl), s>>>(
+            kstore_mv_k<D, double><<<g, NT, ks_lds<double>(ncol, sl), s>>>(
                 nr, ncol, xc ? A64.p : At64.p, u, partS.p, c, adv, dir ? dir->z : nullptr, dir ? dir->P : nullptr,
                 dir ? dir->nP : 0, dir ? dir->hist : nullptr, dir ? dir->j : 0, dir ? dir->dout : nullptr);
         } else {
             dim3 g(gridOf(nr, panel_rows<float>()), sl);
-            kstore_mv_k<D, float><<<g, NT, ks_lds(ncol, sl), s>>>(
+            kstore_mv_k<D, float><<<g, NT, ks_lds<float>(ncol, sl), s>>>(
                 nr, ncol, xc ? A32.p : At32.p, u, partS.p, c, adv, dir ? dir->z : nullptr, dir ? dir->P : nullptr,
                 dir ? dir->nP : 0, dir ? dir->hist : nullptr, dir ? dir->j : 0, dir ? dir->dout : nullptr);
         }
@@ -2714,10 +2714,10 @@ struct Engine : EngineBase {
                 return timed([&] {
                     if (f32) {
                         dim3 g(gridOf(NC, panel_rows<float>()), sl);
-                        kstore_mv_k<D, float><<<g, NT, ks_lds(NC, sl), s>>>(NC, NC, Mpre32.p, cR.p, partS.p, ctl.p);
+                        kstore_mv_k<D, float><<<g, NT, ks_lds<float>(NC, sl), s>>>(NC, NC, Mpre32.p, cR.p, partS.p, ctl.p);
                     } else {
                         dim3 g(gridOf(NC, panel_rows<double>()), sl);
-                        kstore_mv_k<D, double><<<g, NT, ks_lds(NC, sl), s>>>(NC, NC, Mpre.p, cR.p, partS.p, ctl.p);
+                        kstore_mv_k<D, double><<<g, NT, ks_lds<double>(NC, sl), s>>>(NC, NC, Mpre.p, cR.p, partS.p, ctl.p);
                     }
                     CHECK_LAUNCH();
                     slice_rows(D * NC, sl, partS.p, cZ.p, ctl.p);
