@@ -37,7 +37,7 @@ PEAK_HBM_GBS = 8000.0
 # plain device copy of known size (--copy-calibrate): FETCH_SIZE x 2.0,
 # WRITE_SIZE x 1.0 on MI355X.  Keyed by (config, precision); bytes per launch,
 # mean of the K_xc u and K_cx y launches (profiles/kstore_mv_k_*_c2_*.md).
-TRAFFIC: dict = {("c2", "mixed"): 0.9758e9, ("c2", "fp64"): 2.0645e9}
+TRAFFIC: dict = {("c2", "mixed"): 1.0517e9, ("c2", "fp64"): 2.0645e9}
 TRAFFIC_SOURCE = ("not measured in this run: the counter runs of profiles/kstore_mv_k_float_c2_mixed.md and "
                   "profiles/kstore_mv_k_double_c2_fp64.md (rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE)")
 
