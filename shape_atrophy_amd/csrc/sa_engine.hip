@@ -248,7 +248,6 @@ int ks_target(const char *prod, bool f32, long rows, long cols, int dflt) {
 #ifndef SA_MPRE_BUILD
 #define SA_MPRE_BUILD 1
 #endif
-// steps a control-point preconditioner is reused at most
 // mixed form: squared relative tolerance of an inner FP32 solve (at most;
 // looser when the outer target needs less): 1e-9, a residual cut of about
 // 3e-5 per sweep, measured best on the 51k-landmark shell (1e-10 and 1e-8
@@ -282,6 +281,8 @@ int ks_target(const char *prod, bool f32, long rows, long cols, int dflt) {
 #ifndef SA_SPEC_CARRY
 #define SA_SPEC_CARRY 1000000
 #endif
+// steps a control-point preconditioner is reused at most (measured at the
+// 51k shell, mixed form: 17.0 (8) / 16.6 (12) / 16.2 (16) steps/s)
 #ifndef SA_PREC_MAX_AGE
 #define SA_PREC_MAX_AGE 8
 #endif
