@@ -98,6 +98,43 @@ def test_probe_kmatvec(order, oracle):
         assert rel(out, ref) < 1e-12, order
 
 
+@pytest.mark.parametrize("order", [-1, 0, 1, 2, 3, 4])
+def test_probe_kmatvec_matrix_free(order, oracle, monkeypatch):
+    """The all-landmark product without the stored K (kmatvec_k: the form
+    meshes above about 77,000 landmarks take; SA_MATRIX_FREE=1 forces it
+    here) against the oracle, 2D and 3D, every kernel order."""
+    monkeypatch.setenv("SA_MATRIX_FREE", "1")
+    for p in (small_ring(), small_shell()):
+        p["name"] = f"kmatvec{p['dim']}d_o{order}_matrix_free"
+        p["params"]["knlOrder"] = order
+        s = _session(p)
+        rng = np.random.default_rng(7)
+        x = rng.standard_normal(p["q0"].shape)
+        out = s.probe("kmatvec", p["tauVar"][:, 0], p["tauPrm"], x).reshape(p["q0"].shape)
+        s.close()
+        prm = p["params"]
+        ref = oracle.kernel_matvec(p["dim"], order, prm["knlWidth"], prm["knlEps"], p["q0"], x)
+        assert rel(out, ref) < 1e-12, order
+
+
+@pytest.mark.parametrize("which", ["ring2d", "shell3d"])
+def test_full_solve_matrix_free_all_landmarks(which, oracle, monkeypatch):
+    """A full all-landmark solve with K applied matrix-free (kmatvec_k in
+    every CG iteration and for the warm start) against the oracle, at the
+    full-solve tolerance."""
+    monkeypatch.setenv("SA_MATRIX_FREE", "1")
+    p = PROBLEMS[which]()
+    p["name"] = which + "_matrix_free"
+    s = _session(p)
+    r = s.run(p["tauVar"][:, 0], p["tauPrm"], want_traj=True)
+    o = oracle.objfcn(p, pcg_init=1, rd_solve=1)
+    assert r["status"] == 0 and o["status"] == 0
+    disp = np.linalg.norm(o["lmkStk"][-1] - o["lmkStk"][0])
+    assert np.linalg.norm(r["lmkStk"] - o["lmkStk"]) / (disp * np.sqrt(len(o["lmkStk"]))) < 1e-9
+    assert rel(r["tauMat"], o["tauMat"]) < 1e-9
+    assert abs(r["obj"] - o["obj"]) <= 1e-9 * abs(o["obj"])
+
+
 def test_probe_varifold(prob, oracle):
     s = _session(prob)
     rng = np.random.default_rng(9)
