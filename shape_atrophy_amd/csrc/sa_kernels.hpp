@@ -1222,7 +1222,10 @@ __global__ __launch_bounds__(NT) void kstore_build_k(int V, const double *__rest
 // part + s * D * nr; slice_rows_k sums the slices in slice order.
 constexpr int KS_MAX_COLS = 1024;
 // most Ritz vectors in the spectral term of the control-point preconditioner
-constexpr int SPEC_KMAX = 128;
+#ifndef SA_SPEC_KMAX
+#define SA_SPEC_KMAX 128
+#endif
+constexpr int SPEC_KMAX = SA_SPEC_KMAX;
 // slice-sum consumers (slice_rows_k, slice_combine_k, slice_dot_k): a block
 // sums the slice partials of 64 rows with SLW wavefronts, wavefront w adding
 // slices w, w + SLW, ..., the wavefront sums then added in order.  Sixteen
