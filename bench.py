@@ -605,10 +605,10 @@ def main():
                     "work": f"{esz} B x {V} x {nc} stored kernel entries + 8 B x {d} x ({V} + {nc}) vectors per launch"}
         elif 8 * ((V + 127) // 128 * 128) * V <= (48 << 30):
             # all landmarks, K stored per step (FP64, panel-major): each launch
-            # (K r plus the slice sum) reads K once
+            # reads K once (the CG's slice sum is in the direction kernel)
             B = 8 * V * V + 8 * d * 3 * V
             ach = B / (avg * 1e-3) / 1e9
-            roof = {"bound": "hbm", "kernel": f"kstore_mv_k<{d}, double> + kfin_k (stored K)",
+            roof = {"bound": "hbm", "kernel": f"kstore_mv_k<{d}, double> (stored K)",
                     "achieved": round(ach, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                     "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": TRAFFIC.get((args.config, args.precision)),
                     "traffic_source": TRAFFIC_SOURCE,

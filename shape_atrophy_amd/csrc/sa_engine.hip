@@ -1260,6 +1260,32 @@ struct Engine : EngineBase {
     // partial sums the kernel product leaves in P3 (consumed by pcg_dir_k)
     int kmv_parts() const { return Kst.p ? gridOf(V) : gridOf(V, MV_ROWS); }
 
+    // the preconditioning half of an all-landmark CG iteration: K r (|r|^2
+    // partials in P2, nPrr of them, for the stopping test), then the new
+    // direction.  Stored K: the product leaves per-block partials of
+    // r.(K r) and pcg_dir_stored_k forms (K + eps I) r itself (no kfin_k
+    // launch); matrix-free: kmatvec_k, then pcg_dir_k.  ev: events around the
+    // product (kernel timer).
+    void precond_dir(const double *q, int nPrr, int it, hipEvent_t *ev) {
+        const int n = V * D, gN = gridOf(n);
+        if (ev) CK(hipEventRecord(ev[0], s));
+        const dim3 g(gridOf(V, panel_rows<double>()), Kst.p ? slicesK : 1);
+        if (Kst.p && (long)g.x * g.y <= 3L * MAXP) {
+            kstore_mv_k<D, double><<<g, NT, ks_lds<double>(V, slicesK), s>>>(
+                V, V, Kst.p, r.p, partS.p, ctl.p, nullptr, nullptr, nullptr, 0, nullptr, 0, nullptr, r.p, P3.p);
+            CHECK_LAUNCH();
+            if (ev) CK(hipEventRecord(ev[1], s));
+            pcg_dir_stored_k<<<gN, NT, 0, s>>>(n, slicesK, partS.p, knlEps, P2.p, nPrr, P3.p, (int)(g.x * g.y),
+                                               hist.p, it, r.p, Kid.p, d.p, ctl.p);
+            CHECK_LAUNCH();
+            return;
+        }
+        kmatvec<true>(q, r.p, Kr.p, P2.p, nPrr, it);
+        if (ev) CK(hipEventRecord(ev[1], s));
+        pcg_dir_k<<<gN, NT, 0, s>>>(n, P3.p, kmv_parts(), hist.p, it, r.p, Kr.p, Kid.p, d.p, ctl.p);
+        CHECK_LAUNCH();
+    }
+
     // stored K = k(|q_i - q_j|) (without eps) for the all-landmark form
     void kst_build(const double *qq) {
         const double invW = 1.0 / knlWidth;
@@ -1435,7 +1461,7 @@ struct Engine : EngineBase {
     // velocity solve of one step; returns 0 ok, 1 failure
     int velocity(const double *q, const double *tauNow, const double *geoNow, bool warm, int &iters) {
         const int n = V * D;
-        const int gV = gridOf(V), gN = gridOf(n), gM = kmv_parts();
+        const int gV = gridOf(V), gN = gridOf(n);
         if (Kst.p) kst_build(q);
         // f and |f|^2
         yank(q, geoNow, f.p, P0.p);
@@ -1446,9 +1472,7 @@ struct Engine : EngineBase {
         if (warm) kmatvec<false>(q, a.p, x.p, nullptr, 0, -1);   // x0 = K alpha_{t-1}
         // r = b - A x0, Kid = r
         op_apply<1>(q, x.p, tauNow, geoNow, a.p, f.p, r.p, Kid.p, P2.p);
-        kmatvec<true>(q, r.p, Kr.p, P2.p, op_parts(), -1);
-        pcg_dir_k<<<gN, NT, 0, s>>>(n, P3.p, gM, hist.p, -1, r.p, Kr.p, Kid.p, d.p, ctl.p);
-        CHECK_LAUNCH();
+        precond_dir(q, op_parts(), -1, nullptr);
         int it0 = 0, chunk = 4;
         Ctl &h = hctl[0];
         while (true) {
@@ -1462,11 +1486,7 @@ struct Engine : EngineBase {
                 pcg_update_k<<<gN, NT, 0, s>>>(n, P1.p, op_parts(), hist.p, it, x.p, a.p, r.p, d.p, Kid.p, Ad.p,
                                                P2.p, ctl.p);
                 CHECK_LAUNCH();
-                if (ev) CK(hipEventRecord(ev[0], s));
-                kmatvec<true>(q, r.p, Kr.p, P2.p, gN, it);
-                if (ev) CK(hipEventRecord(ev[1], s));
-                pcg_dir_k<<<gN, NT, 0, s>>>(n, P3.p, gM, hist.p, it, r.p, Kr.p, Kid.p, d.p, ctl.p);
-                CHECK_LAUNCH();
+                precond_dir(q, gN, it, ev);
             }
             sync_ctl(ctl.p, &h);
             if (timer.on) {

@@ -957,6 +957,48 @@ __global__ __launch_bounds__(NT) void pcg_dir_k(int n, const double *__restrict_
     }
 }
 
+// pcg_dir_k for the stored all-landmark K, with kfin_k's work folded in:
+// stop when |r|^2 (partials Prr) is below the tolerance; r.(K + eps I) r from
+// the product's per-block partials of r.(its part of K r) (Pdot, nPdot of
+// them) plus eps |r|^2; each thread forms its own entry of (K + eps I) r from
+// the slice partials (slice order) and updates d and Kid with it
+__global__ __launch_bounds__(NT) void pcg_dir_stored_k(int n, int slices, const double *__restrict__ part,
+                                                       double eps, const double *__restrict__ Prr, int nPrr,
+                                                       const double *__restrict__ Pdot, int nPdot,
+                                                       double *__restrict__ hist, int it,
+                                                       const double *__restrict__ r, double *__restrict__ Kid,
+                                                       double *__restrict__ d, Ctl *ctl) {
+    __shared__ double red[NT / 64];
+    if (cld(&ctl->done)) return;
+    const double rr = reduce_partials(Prr, nPrr, red);
+    if (rr < cld(&ctl->tol)) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) { cst(&ctl->done, 1); cst(&ctl->iters, it + 1); }
+        return;
+    }
+    const double rKr = reduce_partials(Pdot, nPdot, red) + eps * rr;
+    if (rKr == 0.0) {   // ipcg.cu:48-53, 83-88
+        if (blockIdx.x == 0 && threadIdx.x == 0) { cst(&ctl->status, 1); cst(&ctl->done, 1); cst(&ctl->iters, it + 1); }
+        return;
+    }
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double kr = 0.0;
+    if (i < n) {
+        for (int k = 0; k < slices; ++k) kr += part[(size_t)k * n + i];
+        kr += eps * r[i];
+    }
+    if (it < 0) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) cst(&hist[0], rKr);
+        if (i < n) d[i] = kr;
+        return;
+    }
+    const double be = rKr / cld(&hist[it]);
+    if (blockIdx.x == 0 && threadIdx.x == 0) cst(&hist[it + 1], rKr);
+    if (i < n) {
+        Kid[i] = 1.0 * r[i] + be * Kid[i];
+        d[i] = 1.0 * kr + be * d[i];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // y = (K + eps I) x, K_ij = k(|q_i - q_j| / width), matrix-free.
 // Block = 64 rows x 4 column phases; columns are staged through LDS 256 at a
@@ -1279,6 +1321,9 @@ __device__ inline void fma_rows(double (&a)[R][D], int r0, int c, const double2 
     a[r0][c] += m.x * xv; a[r0 + 1][c] += m.y * xv;
 }
 
+// dotv / Pdot: the block's partial of dotv . (its slice's part of M x) over
+// its rows, to Pdot[blockIdx.y * gridDim.x + blockIdx.x] (dotv has D columns
+// of stride nr)
 template <int D, typename T>
 __global__ __launch_bounds__(NT) void kstore_mv_k(int nr, int nc, const T *__restrict__ M,
                                                   const double *__restrict__ x, double *__restrict__ part,
@@ -1286,7 +1331,9 @@ __global__ __launch_bounds__(NT) void kstore_mv_k(int nr, int nc, const T *__res
                                                   const double *__restrict__ dz = nullptr,
                                                   const double *__restrict__ dP = nullptr, int dnP = 0,
                                                   double *__restrict__ dhist = nullptr, int dj = 0,
-                                                  double *__restrict__ dout = nullptr) {
+                                                  double *__restrict__ dout = nullptr,
+                                                  const double *__restrict__ dotv = nullptr,
+                                                  double *__restrict__ Pdot = nullptr) {
     constexpr int PR = panel_rows<T>();
     constexpr int R = PR / 64;             // rows per lane
     constexpr int W = NT / 64;
@@ -1437,6 +1484,7 @@ __global__ __launch_bounds__(NT) void kstore_mv_k(int nr, int nc, const T *__res
     __syncthreads();
     if (wv == 0) {
         double *po = part + (size_t)blockIdx.y * D * nr;
+        double dp = 0.0;
 #pragma unroll
         for (int c = 0; c < D; ++c)
 #pragma unroll
@@ -1444,8 +1492,15 @@ __global__ __launch_bounds__(NT) void kstore_mv_k(int nr, int nc, const T *__res
                 double sum = a[k][c];
 #pragma unroll
                 for (int w = 0; w < W - 1; ++w) sum += red[(w * D + c) * PR + lane * R + k];
-                if (r0 + k < nr) po[c * nr + r0 + k] = sum;
+                if (r0 + k < nr) {
+                    po[c * nr + r0 + k] = sum;
+                    if (dotv) dp += dotv[(size_t)c * nr + r0 + k] * sum;
+                }
             }
+        if (Pdot) {
+            dp = wave_sum(dp);
+            if (lane == 0) Pdot[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = dp;
+        }
     }
 }
 
