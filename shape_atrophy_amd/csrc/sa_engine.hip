@@ -704,6 +704,7 @@ struct Engine : EngineBase {
     DBuf<double> q0, dmod, tCen, tDir, tVol;
     // state
     DBuf<double> lmk, tau, geo[2], femElm, ppd[2], lhs, ya, ra, f, x, a, r, Kid, Kr, d, Ad, frc, pfrc, meas;
+    DBuf<double> r2;   // the all-landmark CG's residual of odd iterations (stepFused)
     DBuf<double> rdRhs, rdR, rdZ, rdD, rdAd;
     DBuf<double> P0, P1, P2, P3, hist, rdHist;
     DBuf<double> dCen, dDir, dVol, rowVal, gC, gN, grad, scal;
@@ -1050,6 +1051,7 @@ struct Engine : EngineBase {
         lhs.alloc(Z, tot);
         ya.alloc(V, tot); ra.alloc(V, tot);
         f.alloc(n, tot); x.alloc(n, tot); a.alloc(n, tot); r.alloc(n, tot); Kid.alloc(n, tot);
+        if (NC == 0) r2.alloc(n, tot);
         Kr.alloc(n, tot); d.alloc(n, tot); Ad.alloc(n, tot);
         frc.alloc((size_t)(D + 1) * D * E, tot);
         if (B > 0) { pfrc.alloc((size_t)D * D * B, tot); meas.alloc(B, tot); }
@@ -1266,6 +1268,34 @@ struct Engine : EngineBase {
     // r.(K r) and pcg_dir_stored_k forms (K + eps I) r itself (no kfin_k
     // launch); matrix-free: kmatvec_k, then pcg_dir_k.  ev: events around the
     // product (kernel timer).
+    // stored K whose per-block partials fit P3 (the fused iteration below)
+    bool step_fused() const {
+        return Kst.p && r2.p && (long)gridOf(V, panel_rows<double>()) * slicesK <= 3L * MAXP;
+    }
+
+    // iteration it of the all-landmark CG after the operator apply, in two
+    // launches: the stored-K product takes pcg_update_k's step while staging
+    // its input (r_(it+1) = r_it - st Ad, x and alpha updated; the residual
+    // alternates between r (even iterations) and r2 (odd)), then
+    // pcg_dir_stored_k
+    void step_product_dir(int it, hipEvent_t *ev) {
+        const int n = V * D, gN = gridOf(n);
+        double *rIn = (it & 1) ? r2.p : r.p, *rOut = (it & 1) ? r.p : r2.p;
+        const dim3 g(gridOf(V, panel_rows<double>()), slicesK);
+        KsUpd upd;
+        upd.P = P1.p; upd.nP = op_parts(); upd.hist = hist.p; upd.it = it;
+        upd.rIn = rIn; upd.Ad = Ad.p; upd.d = d.p; upd.Kid = Kid.p;
+        upd.rOut = rOut; upd.xs = x.p; upd.as = a.p; upd.Prr = P2.p; upd.stop = ctl.p;
+        if (ev) CK(hipEventRecord(ev[0], s));
+        kstore_mv_k<D, double><<<g, NT, ks_lds<double>(V, slicesK), s>>>(
+            V, V, Kst.p, rIn, partS.p, ctl.p, nullptr, nullptr, nullptr, 0, nullptr, 0, nullptr, nullptr, P3.p, upd);
+        CHECK_LAUNCH();
+        if (ev) CK(hipEventRecord(ev[1], s));
+        pcg_dir_stored_k<<<gN, NT, 0, s>>>(n, slicesK, partS.p, knlEps, P2.p, (int)g.y, P3.p, (int)(g.x * g.y),
+                                           hist.p, it, rOut, Kid.p, d.p, ctl.p);
+        CHECK_LAUNCH();
+    }
+
     void precond_dir(const double *q, int nPrr, int it, hipEvent_t *ev) {
         const int n = V * D, gN = gridOf(n);
         if (ev) CK(hipEventRecord(ev[0], s));
@@ -1483,10 +1513,14 @@ struct Engine : EngineBase {
                 if (ev) CK(hipEventRecord(ev[2], s));
                 op_apply<0>(q, d.p, tauNow, geoNow, Kid.p, d.p, Ad.p, nullptr, P1.p);
                 if (ev) CK(hipEventRecord(ev[3], s));
-                pcg_update_k<<<gN, NT, 0, s>>>(n, P1.p, op_parts(), hist.p, it, x.p, a.p, r.p, d.p, Kid.p, Ad.p,
-                                               P2.p, ctl.p);
-                CHECK_LAUNCH();
-                precond_dir(q, gN, it, ev);
+                if (step_fused()) {
+                    step_product_dir(it, ev);
+                } else {
+                    pcg_update_k<<<gN, NT, 0, s>>>(n, P1.p, op_parts(), hist.p, it, x.p, a.p, r.p, d.p, Kid.p,
+                                                   Ad.p, P2.p, ctl.p);
+                    CHECK_LAUNCH();
+                    precond_dir(q, gN, it, ev);
+                }
             }
             sync_ctl(ctl.p, &h);
             if (timer.on) {

@@ -1324,6 +1324,23 @@ __device__ inline void fma_rows(double (&a)[R][D], int r0, int c, const double2 
 // dotv / Pdot: the block's partial of dotv . (its slice's part of M x) over
 // its rows, to Pdot[blockIdx.y * gridDim.x + blockIdx.x] (dotv has D columns
 // of stride nr)
+//
+// upd (all-landmark CG, square M): the step of pcg_update_k taken while the
+// input is staged.  st = hist[it] / d.Ad (partials P); the product's input
+// is r' = rIn - st Ad, formed by every block for its columns (and, for the
+// x.y partials, for its rows: dotv is then ignored); the first row panel
+// writes r' to rOut (another buffer than rIn, which other blocks still
+// read), x += st d, a += st Kid, and one |r'|^2 partial per slice to Prr.
+struct KsUpd {
+    const double *P = nullptr;
+    int nP = 0;
+    const double *hist = nullptr;
+    int it = 0;
+    const double *rIn = nullptr, *Ad = nullptr, *d = nullptr, *Kid = nullptr;
+    double *rOut = nullptr, *xs = nullptr, *as = nullptr, *Prr = nullptr;
+    Ctl *stop = nullptr;
+};
+
 template <int D, typename T>
 __global__ __launch_bounds__(NT) void kstore_mv_k(int nr, int nc, const T *__restrict__ M,
                                                   const double *__restrict__ x, double *__restrict__ part,
@@ -1333,7 +1350,7 @@ __global__ __launch_bounds__(NT) void kstore_mv_k(int nr, int nc, const T *__res
                                                   double *__restrict__ dhist = nullptr, int dj = 0,
                                                   double *__restrict__ dout = nullptr,
                                                   const double *__restrict__ dotv = nullptr,
-                                                  double *__restrict__ Pdot = nullptr) {
+                                                  double *__restrict__ Pdot = nullptr, KsUpd upd = KsUpd()) {
     constexpr int PR = panel_rows<T>();
     constexpr int R = PR / 64;             // rows per lane
     constexpr int W = NT / 64;
@@ -1371,11 +1388,24 @@ __global__ __launch_bounds__(NT) void kstore_mv_k(int nr, int nc, const T *__res
         cst(&adv->it, it);
         if (it >= cld(&adv->maxIt)) { cst(&adv->done, 1); cst(&adv->status, 2); cst(&adv->iters, it); }
     }
+    double st = 0.0;
+    if (upd.P) {
+        // pcg_update_k's step (ipcg.cu:63-68 on zero curvature)
+        const double dAd = reduce_partials(upd.P, upd.nP, redd);
+        if (dAd == 0.0) {
+            if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+                cst(&upd.stop->status, 1); cst(&upd.stop->done, 1); cst(&upd.stop->iters, upd.it);
+            }
+            return;
+        }
+        st = cld(&upd.hist[upd.it]) / dAd;
+    }
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int r0 = blockIdx.x * PR + R * lane;
     const int per = (nc + gridDim.y - 1) / gridDim.y;
     const int jb = blockIdx.y * per, je = min(nc, jb + per);
     const int ns = max(0, je - jb);
+    double rrp = 0.0;
     for (int t = threadIdx.x; t < D * ns; t += NT) {
         const int c = t / ns, j = t - c * ns;
         const size_t o = (size_t)c * nc + jb + j;
@@ -1383,9 +1413,22 @@ __global__ __launch_bounds__(NT) void kstore_mv_k(int nr, int nc, const T *__res
             const double v = dz[o] + be * x[o];
             sx[t] = v;
             if (blockIdx.x == 0) dout[o] = v;
+        } else if (upd.P) {
+            const double v = 1.0 * upd.rIn[o] - st * upd.Ad[o];
+            sx[t] = v;
+            if (blockIdx.x == 0) {
+                upd.rOut[o] = v;
+                upd.xs[o] = 1.0 * upd.xs[o] + st * upd.d[o];
+                if (upd.as) upd.as[o] = upd.as[o] + st * upd.Kid[o];
+                rrp += v * v;
+            }
         } else {
             sx[t] = x[o];
         }
+    }
+    if (upd.P && blockIdx.x == 0) {
+        const double t = block_sum(rrp, redd);   // (its barriers also order the staging)
+        if (threadIdx.x == 0) upd.Prr[blockIdx.y] = t;
     }
     __syncthreads();
     double a[R][D];
@@ -1494,7 +1537,9 @@ __global__ __launch_bounds__(NT) void kstore_mv_k(int nr, int nc, const T *__res
                 for (int w = 0; w < W - 1; ++w) sum += red[(w * D + c) * PR + lane * R + k];
                 if (r0 + k < nr) {
                     po[c * nr + r0 + k] = sum;
-                    if (dotv) dp += dotv[(size_t)c * nr + r0 + k] * sum;
+                    const size_t o = (size_t)c * nr + r0 + k;
+                    if (upd.P) dp += (1.0 * upd.rIn[o] - st * upd.Ad[o]) * sum;
+                    else if (dotv) dp += dotv[o] * sum;
                 }
             }
         if (Pdot) {
