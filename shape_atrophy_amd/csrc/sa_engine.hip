@@ -705,7 +705,7 @@ struct Engine : EngineBase {
     // state
     DBuf<double> lmk, tau, geo[2], femElm, ppd[2], lhs, ya, ra, f, x, a, r, Kid, Kr, d, Ad, frc, pfrc, meas;
     DBuf<double> r2;   // the all-landmark CG's residual of odd iterations (stepFused)
-    DBuf<double> rdRhs, rdR, rdZ, rdD, rdAd;
+    DBuf<double> rdRhs, rdR, rdZ, rdD, rdAd, rdD2;
     DBuf<double> P0, P1, P2, P3, hist, rdHist;
     DBuf<double> dCen, dDir, dVol, rowVal, gC, gN, grad, scal;
     DBuf<Ctl> ctl, rdCtl;
@@ -1056,6 +1056,7 @@ struct Engine : EngineBase {
         frc.alloc((size_t)(D + 1) * D * E, tot);
         if (B > 0) { pfrc.alloc((size_t)D * D * B, tot); meas.alloc(B, tot); }
         rdRhs.alloc(V, tot); rdR.alloc(V, tot); rdZ.alloc(V, tot); rdD.alloc(V, tot); rdAd.alloc(V, tot);
+        rdD2.alloc(V, tot);
         P0.alloc(MAXP, tot); P1.alloc(MAXP, tot); P2.alloc(2 * MAXP, tot); P3.alloc(3 * MAXP, tot);
         hist.alloc((size_t)std::max(itrMax, 1) + 2, tot);
         rdHist.alloc(200002, tot);
@@ -2474,11 +2475,20 @@ struct Engine : EngineBase {
         Ctl &h = hctl[1];
         while (true) {
             int itEnd = std::min(rdMax, it0 + chunk);
+            // two launches per iteration: the direction of iteration it
+            // is formed inside the next iteration's product (rd_spmv_dir_k);
+            // the direction alternates between rdD (even) and rdD2 (odd)
             for (int it = it0; it < itEnd; ++it) {
-                rd_spmv_k<<<gV, NT, 0, s>>>(V, rowPtr.p, colIdx.p, lhs.p, rdD.p, rdAd.p, P1.p, rdCtl.p);
-                rd_update_k<<<gV, NT, 0, s>>>(V, P1.p, gV, rdHist.p, it, diagIdx.p, lhs.p, tauNxt, rdR.p, rdZ.p, rdD.p,
+                double *dIt = (it & 1) ? rdD2.p : rdD.p;
+                if (it == 0) {
+                    rd_spmv_k<<<gV, NT, 0, s>>>(V, rowPtr.p, colIdx.p, lhs.p, dIt, rdAd.p, P1.p, rdCtl.p);
+                } else {
+                    double *dPrev = (it & 1) ? rdD.p : rdD2.p;
+                    rd_spmv_dir_k<<<gV, NT, 0, s>>>(V, rowPtr.p, colIdx.p, lhs.p, P2.p, gV, rdHist.p, it - 1, rdZ.p,
+                                                   dPrev, dIt, rdAd.p, P1.p, rdCtl.p);
+                }
+                rd_update_k<<<gV, NT, 0, s>>>(V, P1.p, gV, rdHist.p, it, diagIdx.p, lhs.p, tauNxt, rdR.p, rdZ.p, dIt,
                                              rdAd.p, P2.p, rdCtl.p);
-                rd_dir_k<<<gV, NT, 0, s>>>(V, P2.p, gV, rdHist.p, it, rdZ.p, rdD.p, rdCtl.p);
                 CHECK_LAUNCH();
             }
             sync_ctl(rdCtl.p, &h);

@@ -2397,6 +2397,45 @@ __global__ __launch_bounds__(NT) void rd_dir_k(int V, const double *__restrict__
     if (i < V) d[i] = zv[i] + be * d[i];
 }
 
+// rd_dir_k of iteration it and rd_spmv_k of iteration it + 1 in one launch:
+// the stopping test and beta = rz / hist[it] from the partials of
+// rd_update_k (P: r.r, P + MAXP: r.z); each row forms the new direction at
+// its columns, zv + beta dOld (the same value every row that reads it
+// computes), writes its own entry to dNew (another buffer than dOld, which
+// other rows still read) and the product row with its d.Ad partial
+__global__ __launch_bounds__(NT) void rd_spmv_dir_k(int V, const int *__restrict__ rowPtr,
+                                                    const int *__restrict__ colIdx, const double *__restrict__ A,
+                                                    const double *__restrict__ P, int nP, double *__restrict__ hist,
+                                                    int it, const double *__restrict__ zv,
+                                                    const double *__restrict__ dOld, double *__restrict__ dNew,
+                                                    double *__restrict__ Ad, double *__restrict__ Pd, Ctl *ctl) {
+    __shared__ double red[NT / 64];
+    if (cld(&ctl->done)) return;
+    const double rr = reduce_partials(P, nP, red);
+    if (rr < cld(&ctl->tol)) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) { cst(&ctl->done, 1); cst(&ctl->iters, it + 1); }
+        return;
+    }
+    const double rz = reduce_partials(P + MAXP, nP, red);
+    const double be = rz / cld(&hist[it]);
+    if (blockIdx.x == 0 && threadIdx.x == 0) cst(&hist[it + 1], rz);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double ss = 0.0;
+    if (i < V) {
+        double s = 0.0;
+        for (int z = rowPtr[i]; z < rowPtr[i + 1]; ++z) {
+            const int c = colIdx[z];
+            s += A[z] * (zv[c] + be * dOld[c]);
+        }
+        const double di = zv[i] + be * dOld[i];
+        dNew[i] = di;
+        Ad[i] = s;
+        ss = di * s;
+    }
+    const double t = block_sum(ss, red);
+    if (threadIdx.x == 0) Pd[blockIdx.x] = t;
+}
+
 // ---------------------------------------------------------------------------
 // varifold (2D varifold.cu:9-357, 3D :9-370)
 
