@@ -704,7 +704,7 @@ struct Engine : EngineBase {
     DBuf<double> q0, dmod, tCen, tDir, tVol;
     // state
     DBuf<double> lmk, tau, geo[2], femElm, ppd[2], lhs, ya, ra, f, x, a, r, Kid, Kr, d, Ad, frc, pfrc, meas;
-    DBuf<double> r2;   // the all-landmark CG's residual of odd iterations (stepFused)
+    DBuf<double> r2;   // the all-landmark CG's residual after odd iterations (step_product_dir)
     DBuf<double> rdRhs, rdR, rdZ, rdD, rdAd, rdD2;
     DBuf<double> P0, P1, P2, P3, hist, rdHist;
     DBuf<double> dCen, dDir, dVol, rowVal, gC, gN, grad, scal;
@@ -1263,13 +1263,8 @@ struct Engine : EngineBase {
     // partial sums the kernel product leaves in P3 (consumed by pcg_dir_k)
     int kmv_parts() const { return Kst.p ? gridOf(V) : gridOf(V, MV_ROWS); }
 
-    // the preconditioning half of an all-landmark CG iteration: K r (|r|^2
-    // partials in P2, nPrr of them, for the stopping test), then the new
-    // direction.  Stored K: the product leaves per-block partials of
-    // r.(K r) and pcg_dir_stored_k forms (K + eps I) r itself (no kfin_k
-    // launch); matrix-free: kmatvec_k, then pcg_dir_k.  ev: events around the
-    // product (kernel timer).
-    // stored K whose per-block partials fit P3 (the fused iteration below)
+    // the all-landmark CG iteration in three launches (step_product_dir):
+    // stored K whose per-block partials fit P3
     bool step_fused() const {
         return Kst.p && r2.p && (long)gridOf(V, panel_rows<double>()) * slicesK <= 3L * MAXP;
     }
@@ -1297,6 +1292,12 @@ struct Engine : EngineBase {
         CHECK_LAUNCH();
     }
 
+    // the preconditioning half of an all-landmark CG iteration: K r (|r|^2
+    // partials in P2, nPrr of them, for the stopping test), then the new
+    // direction.  Stored K: the product leaves per-block partials of
+    // r.(K r) and pcg_dir_stored_k forms (K + eps I) r itself (no kfin_k
+    // launch); matrix-free: kmatvec_k, then pcg_dir_k.  ev: events around the
+    // product (kernel timer).
     void precond_dir(const double *q, int nPrr, int it, hipEvent_t *ev) {
         const int n = V * D, gN = gridOf(n);
         if (ev) CK(hipEventRecord(ev[0], s));
