@@ -404,7 +404,12 @@ def tune_shapes(W):
            ("shell 4x5, 1281 control points", dict(level=4, n_layers=5, n_ctl=1281)),
            ("shell 5x3, 2000 control points", dict(level=5, n_layers=3, n_ctl=2000)),
            ("shell 5x5, 5000 control points", dict(level=5, n_layers=5, n_ctl=5000)),
-           ("shell 6x5, 2000 control points", dict(level=6, n_layers=5, n_ctl=2000))]
+           ("shell 6x5, 2000 control points", dict(level=6, n_layers=5, n_ctl=2000)),
+           # the batch's and the served pool's larger shells (2,000 control points)
+           ("shell 4x8, 2000 control points", dict(level=4, n_layers=8, n_ctl=2000)),
+           ("shell 5x4, 2000 control points", dict(level=5, n_layers=4, n_ctl=2000)),
+           ("shell 5x8, 2000 control points", dict(level=5, n_layers=8, n_ctl=2000)),
+           ("shell 6x4, 2000 control points", dict(level=6, n_layers=4, n_ctl=2000))]
     for desc, kw in ctl:
         yield desc, W.shell_problem(time_steps=2, **kw)
     yield "2D ring disk, 2000 landmarks", W.ring_problem()
