@@ -1366,6 +1366,50 @@ __global__ __launch_bounds__(NT) void kstore_mv_k(int nr, int nc, const T *__res
     extern __shared__ double sx[];
     __shared__ double redd[W];
     if (ctl && cld(&ctl->done)) return;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int r0 = blockIdx.x * PR + R * lane;
+    const int per = (nc + gridDim.y - 1) / gridDim.y;
+    const int jb = blockIdx.y * per, je = min(nc, jb + per);
+    const int ns = max(0, je - jb);
+    const T *base = M + (size_t)blockIdx.x * PR * nc + (size_t)jb * PR + R * lane;
+    // FP32: each wavefront's first two columns are requested before the
+    // input is staged, so the stream starts while the staging's loads and
+    // reductions are in flight
+    V4 m0[NV], m1[NV];
+    if constexpr (sizeof(T) == 4) {
+        if (wv < ns)
+#pragma unroll
+            for (int v = 0; v < NV; ++v) m0[v] = ks_load(reinterpret_cast<const V4 *>(base + (size_t)wv * PR + v * EV));
+        if (wv + W < ns)
+#pragma unroll
+            for (int v = 0; v < NV; ++v)
+                m1[v] = ks_load(reinterpret_cast<const V4 *>(base + (size_t)(wv + W) * PR + v * EV));
+    }
+    // the staged input's operands (x; z and d_old; r and Ad), a thread's
+    // columns j0 + s NT + threadIdx.x (s < SJ) requested together: the
+    // first pass before the partials below are reduced
+    constexpr int SJ = 2;
+    double va[SJ][D], vb[SJ][D];
+    // (block-uniform column bases, so the loads take 32-bit lane offsets)
+    const double *__restrict__ srcA = dz ? dz : upd.P ? upd.rIn : x;
+    const double *__restrict__ srcB = dz ? x : upd.Ad;
+    const bool two = dz || upd.P;
+    auto fetch = [&](int j0) {
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            const double *__restrict__ a = srcA + (size_t)c * nc + jb;
+            const double *__restrict__ b = srcB + (size_t)c * nc + jb;
+#pragma unroll
+            for (int s = 0; s < SJ; ++s) {
+                const int j = j0 + s * NT + (int)threadIdx.x;
+                if (j < ns) {
+                    va[s][c] = a[j];
+                    if (two) vb[s][c] = b[j];
+                }
+            }
+        }
+    };
+    fetch(0);
     double be = 0.0;
     if (dz) {
         // the CG direction of iteration dj, d = z + beta d_old with beta =
@@ -1400,33 +1444,55 @@ __global__ __launch_bounds__(NT) void kstore_mv_k(int nr, int nc, const T *__res
         }
         st = cld(&upd.hist[upd.it]) / dAd;
     }
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int r0 = blockIdx.x * PR + R * lane;
-    const int per = (nc + gridDim.y - 1) / gridDim.y;
-    const int jb = blockIdx.y * per, je = min(nc, jb + per);
-    const int ns = max(0, je - jb);
     double rrp = 0.0;
-    for (int t = threadIdx.x; t < D * ns; t += NT) {
-        const int c = t / ns, j = t - c * ns;
-        const size_t o = (size_t)c * nc + jb + j;
-        if (dz) {
-            const double v = dz[o] + be * x[o];
-            sx[t] = v;
-            if (blockIdx.x == 0) dout[o] = v;
-        } else if (upd.P) {
-            const double v = 1.0 * upd.rIn[o] - st * upd.Ad[o];
-            sx[t] = v;
-            if (blockIdx.x == 0) {
-                upd.rOut[o] = v;
-                upd.xs[o] = 1.0 * upd.xs[o] + st * upd.d[o];
-                if (upd.as) upd.as[o] = upd.as[o] + st * upd.Kid[o];
-                rrp += v * v;
+    for (int j0 = 0; j0 < ns; j0 += SJ * NT) {
+        if (j0 > 0) fetch(j0);
+#pragma unroll
+        for (int c = 0; c < D; ++c)
+#pragma unroll
+            for (int s = 0; s < SJ; ++s) {
+                const int j = j0 + s * NT + (int)threadIdx.x;
+                if (j >= ns) continue;
+                double v = va[s][c];
+                if (dz) v = v + be * vb[s][c];
+                else if (upd.P) v = 1.0 * v - st * vb[s][c];
+                sx[c * ns + j] = v;
+                if (blockIdx.x == 0) {
+                    if (dz) dout[(size_t)c * nc + jb + j] = v;
+                    else if (upd.P) {
+                        upd.rOut[(size_t)c * nc + jb + j] = v;
+                        rrp += v * v;
+                    }
+                }
             }
-        } else {
-            sx[t] = x[o];
-        }
     }
     if (upd.P && blockIdx.x == 0) {
+        // the first row panel also takes x += st d and a += st Kid for the
+        // slice's columns (operands of one component requested together)
+        for (int j0 = 0; j0 < ns; j0 += SJ * NT)
+#pragma unroll
+            for (int c = 0; c < D; ++c) {
+                double xo[SJ], dd[SJ], ao[SJ], kd[SJ];
+#pragma unroll
+                for (int s = 0; s < SJ; ++s) {
+                    const int j = j0 + s * NT + (int)threadIdx.x;
+                    if (j < ns) {
+                        const size_t o = (size_t)c * nc + jb + j;
+                        xo[s] = upd.xs[o];
+                        dd[s] = upd.d[o];
+                        if (upd.as) { ao[s] = upd.as[o]; kd[s] = upd.Kid[o]; }
+                    }
+                }
+#pragma unroll
+                for (int s = 0; s < SJ; ++s) {
+                    const int j = j0 + s * NT + (int)threadIdx.x;
+                    if (j < ns) {
+                        const size_t o = (size_t)c * nc + jb + j;
+                        upd.xs[o] = 1.0 * xo[s] + st * dd[s];
+                        if (upd.as) upd.as[o] = ao[s] + st * kd[s];
+                    }
+                }
+            }
         const double t = block_sum(rrp, redd);   // (its barriers also order the staging)
         if (threadIdx.x == 0) upd.Prr[blockIdx.y] = t;
     }
@@ -1448,16 +1514,7 @@ __global__ __launch_bounds__(NT) void kstore_mv_k(int nr, int nc, const T *__res
         // products and their transposes: 0.81-0.83 of HBM peak for the
         // kernel alone against 0.72-0.75 for contiguous quarter-slices per
         // wavefront; 0.79 inside the solve.
-        const T *base = M + (size_t)blockIdx.x * PR * nc + (size_t)jb * PR + R * lane;
-        V4 m0[NV], m1[NV];
         int j = wv;
-        if (j < ns)
-#pragma unroll
-            for (int v = 0; v < NV; ++v) m0[v] = ks_load(reinterpret_cast<const V4 *>(base + (size_t)j * PR + v * EV));
-        if (j + W < ns)
-#pragma unroll
-            for (int v = 0; v < NV; ++v)
-                m1[v] = ks_load(reinterpret_cast<const V4 *>(base + (size_t)(j + W) * PR + v * EV));
         for (; j < ns; j += 2 * W) {
 #pragma unroll
             for (int c = 0; c < D; ++c) {
