@@ -1283,7 +1283,7 @@ struct Engine : EngineBase {
         upd.rIn = rIn; upd.Ad = Ad.p; upd.d = d.p; upd.Kid = Kid.p;
         upd.rOut = rOut; upd.xs = x.p; upd.as = a.p; upd.Prr = P2.p; upd.stop = ctl.p;
         if (ev) CK(hipEventRecord(ev[0], s));
-        kstore_mv_k<D, double><<<g, NT, ks_lds<double>(V, slicesK), s>>>(
+        kstore_mv_k<D, double, true><<<g, NT, ks_lds<double>(V, slicesK), s>>>(
             V, V, Kst.p, rIn, partS.p, ctl.p, nullptr, nullptr, nullptr, 0, nullptr, 0, nullptr, nullptr, P3.p, upd);
         CHECK_LAUNCH();
         if (ev) CK(hipEventRecord(ev[1], s));

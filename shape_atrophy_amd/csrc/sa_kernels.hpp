@@ -970,32 +970,40 @@ __global__ __launch_bounds__(NT) void pcg_dir_stored_k(int n, int slices, const 
                                                        double *__restrict__ d, Ctl *ctl) {
     __shared__ double red[NT / 64];
     if (cld(&ctl->done)) return;
-    const double rr = reduce_partials(Prr, nPrr, red);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    // every operand is requested before the partials are reduced
+    double kr = 0.0, rv = 0.0, kv = 0.0, dv = 0.0;
+    if (i < n) {
+        rv = r[i];
+        if (it >= 0) { kv = Kid[i]; dv = d[i]; }
+#pragma unroll 8
+        for (int k = 0; k < slices; ++k) kr += part[(size_t)k * n + i];
+    }
+    const double rzOld = it >= 0 ? cld(&hist[it]) : 0.0;
+    double pr = 0.0, pd = 0.0;
+    for (int j = threadIdx.x; j < nPrr; j += NT) pr += Prr[j];
+    for (int j = threadIdx.x; j < nPdot; j += NT) pd += Pdot[j];
+    const double rr = block_sum(pr, red);
     if (rr < cld(&ctl->tol)) {
         if (blockIdx.x == 0 && threadIdx.x == 0) { cst(&ctl->done, 1); cst(&ctl->iters, it + 1); }
         return;
     }
-    const double rKr = reduce_partials(Pdot, nPdot, red) + eps * rr;
+    const double rKr = block_sum(pd, red) + eps * rr;
     if (rKr == 0.0) {   // ipcg.cu:48-53, 83-88
         if (blockIdx.x == 0 && threadIdx.x == 0) { cst(&ctl->status, 1); cst(&ctl->done, 1); cst(&ctl->iters, it + 1); }
         return;
     }
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    double kr = 0.0;
-    if (i < n) {
-        for (int k = 0; k < slices; ++k) kr += part[(size_t)k * n + i];
-        kr += eps * r[i];
-    }
+    kr += eps * rv;
     if (it < 0) {
         if (blockIdx.x == 0 && threadIdx.x == 0) cst(&hist[0], rKr);
         if (i < n) d[i] = kr;
         return;
     }
-    const double be = rKr / cld(&hist[it]);
+    const double be = rKr / rzOld;
     if (blockIdx.x == 0 && threadIdx.x == 0) cst(&hist[it + 1], rKr);
     if (i < n) {
-        Kid[i] = 1.0 * r[i] + be * Kid[i];
-        d[i] = 1.0 * kr + be * d[i];
+        Kid[i] = 1.0 * rv + be * kv;
+        d[i] = 1.0 * kr + be * dv;
     }
 }
 
@@ -1341,7 +1349,7 @@ struct KsUpd {
     Ctl *stop = nullptr;
 };
 
-template <int D, typename T>
+template <int D, typename T, bool STEP = false>
 __global__ __launch_bounds__(NT) void kstore_mv_k(int nr, int nc, const T *__restrict__ M,
                                                   const double *__restrict__ x, double *__restrict__ part,
                                                   const Ctl *ctl, Ctl *adv = nullptr,
@@ -1389,22 +1397,34 @@ __global__ __launch_bounds__(NT) void kstore_mv_k(int nr, int nc, const T *__res
     // columns j0 + s NT + threadIdx.x (s < SJ) requested together: the
     // first pass before the partials below are reduced
     constexpr int SJ = 2;
+    // the CG step (upd) is taken by the STEP instantiation only (FP64)
+    static_assert(!STEP || sizeof(T) == 8, "the CG step is taken by the FP64 product");
+    constexpr bool UPD = STEP;
+    const bool upd0 = UPD && upd.P && blockIdx.x == 0;
     double va[SJ][D], vb[SJ][D];
+    double px[UPD ? SJ : 1][D], pd[UPD ? SJ : 1][D], pa[UPD ? SJ : 1][D], pk[UPD ? SJ : 1][D];
     // (block-uniform column bases, so the loads take 32-bit lane offsets)
-    const double *__restrict__ srcA = dz ? dz : upd.P ? upd.rIn : x;
+    const double *__restrict__ srcA = dz ? dz : UPD && upd.P ? upd.rIn : x;
     const double *__restrict__ srcB = dz ? x : upd.Ad;
-    const bool two = dz || upd.P;
+    const bool two = dz || (UPD && upd.P);
     auto fetch = [&](int j0) {
 #pragma unroll
         for (int c = 0; c < D; ++c) {
-            const double *__restrict__ a = srcA + (size_t)c * nc + jb;
-            const double *__restrict__ b = srcB + (size_t)c * nc + jb;
+            const size_t o0 = (size_t)c * nc + jb;
 #pragma unroll
             for (int s = 0; s < SJ; ++s) {
                 const int j = j0 + s * NT + (int)threadIdx.x;
                 if (j < ns) {
-                    va[s][c] = a[j];
-                    if (two) vb[s][c] = b[j];
+                    va[s][c] = srcA[o0 + j];
+                    if (two) vb[s][c] = srcB[o0 + j];
+                    if constexpr (UPD) {
+                        // the first row panel's x += st d and a += st Kid
+                        if (upd0) {
+                            px[s][c] = upd.xs[o0 + j];
+                            pd[s][c] = upd.d[o0 + j];
+                            if (upd.as) { pa[s][c] = upd.as[o0 + j]; pk[s][c] = upd.Kid[o0 + j]; }
+                        }
+                    }
                 }
             }
         }
@@ -1433,7 +1453,7 @@ __global__ __launch_bounds__(NT) void kstore_mv_k(int nr, int nc, const T *__res
         if (it >= cld(&adv->maxIt)) { cst(&adv->done, 1); cst(&adv->status, 2); cst(&adv->iters, it); }
     }
     double st = 0.0;
-    if (upd.P) {
+    if (UPD && upd.P) {
         // pcg_update_k's step (ipcg.cu:63-68 on zero curvature)
         const double dAd = reduce_partials(upd.P, upd.nP, redd);
         if (dAd == 0.0) {
@@ -1453,46 +1473,23 @@ __global__ __launch_bounds__(NT) void kstore_mv_k(int nr, int nc, const T *__res
             for (int s = 0; s < SJ; ++s) {
                 const int j = j0 + s * NT + (int)threadIdx.x;
                 if (j >= ns) continue;
+                const size_t o = (size_t)c * nc + jb + j;
                 double v = va[s][c];
                 if (dz) v = v + be * vb[s][c];
-                else if (upd.P) v = 1.0 * v - st * vb[s][c];
+                else if (UPD && upd.P) v = 1.0 * v - st * vb[s][c];
                 sx[c * ns + j] = v;
-                if (blockIdx.x == 0) {
-                    if (dz) dout[(size_t)c * nc + jb + j] = v;
-                    else if (upd.P) {
-                        upd.rOut[(size_t)c * nc + jb + j] = v;
+                if (dz && blockIdx.x == 0) dout[o] = v;
+                if constexpr (UPD) {
+                    if (upd0) {
+                        upd.rOut[o] = v;
+                        upd.xs[o] = 1.0 * px[s][c] + st * pd[s][c];
+                        if (upd.as) upd.as[o] = pa[s][c] + st * pk[s][c];
                         rrp += v * v;
                     }
                 }
             }
     }
-    if (upd.P && blockIdx.x == 0) {
-        // the first row panel also takes x += st d and a += st Kid for the
-        // slice's columns (operands of one component requested together)
-        for (int j0 = 0; j0 < ns; j0 += SJ * NT)
-#pragma unroll
-            for (int c = 0; c < D; ++c) {
-                double xo[SJ], dd[SJ], ao[SJ], kd[SJ];
-#pragma unroll
-                for (int s = 0; s < SJ; ++s) {
-                    const int j = j0 + s * NT + (int)threadIdx.x;
-                    if (j < ns) {
-                        const size_t o = (size_t)c * nc + jb + j;
-                        xo[s] = upd.xs[o];
-                        dd[s] = upd.d[o];
-                        if (upd.as) { ao[s] = upd.as[o]; kd[s] = upd.Kid[o]; }
-                    }
-                }
-#pragma unroll
-                for (int s = 0; s < SJ; ++s) {
-                    const int j = j0 + s * NT + (int)threadIdx.x;
-                    if (j < ns) {
-                        const size_t o = (size_t)c * nc + jb + j;
-                        upd.xs[o] = 1.0 * xo[s] + st * dd[s];
-                        if (upd.as) upd.as[o] = ao[s] + st * kd[s];
-                    }
-                }
-            }
+    if (upd0) {
         const double t = block_sum(rrp, redd);   // (its barriers also order the staging)
         if (threadIdx.x == 0) upd.Prr[blockIdx.y] = t;
     }
@@ -1575,6 +1572,22 @@ __global__ __launch_bounds__(NT) void kstore_mv_k(int nr, int nc, const T *__res
             }
         }
     }
+    // FP64: wavefront 0 requests its rows of the dot product's vectors
+    // before the barrier below (the FP32 products take no dot product)
+    constexpr bool DOT = sizeof(T) == 8;
+    double e0[DOT ? R : 1][D], e1[DOT ? R : 1][D];
+    if constexpr (DOT) {
+        if (wv == 0 && (UPD ? upd.P != nullptr : dotv != nullptr))
+#pragma unroll
+            for (int c = 0; c < D; ++c)
+#pragma unroll
+                for (int k = 0; k < R; ++k)
+                    if (r0 + k < nr) {
+                        const size_t o = (size_t)c * nr + r0 + k;
+                        if (UPD) { e0[k][c] = upd.rIn[o]; e1[k][c] = upd.Ad[o]; }
+                        else e0[k][c] = dotv[o];
+                    }
+    }
     // wavefront order: ((w0 + w1) + w2) + w3
     if (wv > 0) {
 #pragma unroll
@@ -1595,9 +1608,10 @@ __global__ __launch_bounds__(NT) void kstore_mv_k(int nr, int nc, const T *__res
                 for (int w = 0; w < W - 1; ++w) sum += red[(w * D + c) * PR + lane * R + k];
                 if (r0 + k < nr) {
                     po[c * nr + r0 + k] = sum;
-                    const size_t o = (size_t)c * nr + r0 + k;
-                    if (upd.P) dp += (1.0 * upd.rIn[o] - st * upd.Ad[o]) * sum;
-                    else if (dotv) dp += dotv[o] * sum;
+                    if constexpr (DOT) {
+                        if (UPD && upd.P) dp += (1.0 * e0[k][c] - st * e1[k][c]) * sum;
+                        else if (!UPD && dotv) dp += e0[k][c] * sum;
+                    }
                 }
             }
         if (Pdot) {
@@ -1828,8 +1842,18 @@ __global__ __launch_bounds__(NTS) void slice_combine_k(int C, int V, int slices,
     const int it = recD ? cld(&ctl->it) : recMax;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int i = blockIdx.x * 64 + lane;
+    // wavefront 0 requests its operands (K_cc d through the control-point
+    // index, d, r) before the slice partials are summed
+    double kd = 0.0, dv = 0.0, rv = 0.0;
+    if (wv == 0 && i < n) {
+        const int c = i / C;
+        kd = w[(size_t)c * V + idx[i - c * C]];
+        dv = d[i];
+        if (it < recMax) rv = r[i];
+    }
     double s = 0.0;
     if (i < n)
+#pragma unroll 4
         for (int k = wv; k < slices; k += SLW) s += part[(size_t)k * n + i];
     red[wv][lane] = s;
     __syncthreads();
@@ -1838,16 +1862,14 @@ __global__ __launch_bounds__(NTS) void slice_combine_k(int C, int V, int slices,
         double y = red[0][lane];
 #pragma unroll
         for (int ww = 1; ww < SLW; ++ww) y += red[ww][lane];
-        const int c = i / C;
-        const double kd = w[(size_t)c * V + idx[i - c * C]];
-        const double sd = y + ldmWgt * (kd + eps * d[i]);
+        const double sd = y + ldmWgt * (kd + eps * dv);
         out[i] = sd;
-        ss = d[i] * sd;
+        ss = dv * sd;
         if (it < recMax) {
             const size_t o = (size_t)it * n + i;
-            recD[o] = d[i];
+            recD[o] = dv;
             recSD[o] = sd;
-            recR[o] = r[i];
+            recR[o] = rv;
         }
     }
     const double t = block_sum(ss, red2);
@@ -1872,22 +1894,35 @@ __global__ __launch_bounds__(NTS) void slice_dot_k(int n, int slices, const doub
     __shared__ double cw[SPEC_KMAX];
     if (cld(&ctl->done)) return;
     const int it = cld(&ctl->it);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int i = blockIdx.x * 64 + lane;
+    // the slice partials and this thread's spectral basis entries are
+    // requested before the reductions (they depend on neither)
+    double s = 0.0;
+    if (i < n)
+#pragma unroll 4
+        for (int q = wv; q < slices; q += SLW) s += part[(size_t)q * n + i];
+    constexpr int UB = (SPEC_KMAX + SLW - 1) / SLW;
+    double uv[UB];
+#pragma unroll
+    for (int t = 0; t < UB; ++t) {
+        const int b = wv + t * SLW;
+        uv[t] = U && i < n && b < k ? U[(size_t)b * n + i] : 0.0;
+    }
+    double ws = 0.0;
+    const int b = threadIdx.x % SPEC_KMAX, g = threadIdx.x / SPEC_KMAX;
+    if (U && b < k)
+#pragma unroll 8
+        for (int j = g; j < nPw; j += G) ws += wp[(size_t)j * k + b];
     const double rr = reduce_partials(Prr, nPrr, red2);
     if (rr < cld(&ctl->tol)) {
         if (blockIdx.x == 0 && threadIdx.x == 0) { cst(&ctl->done, 1); cst(&ctl->iters, it + 1); }
         return;
     }
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int i = blockIdx.x * 64 + lane;
     if (U) {
         // w = U^T r from the nPw block partials: thread (g, b) sums partials
         // g, g + G, ...; the G sums are added in order (the same in every block)
-        const int b = threadIdx.x % SPEC_KMAX, g = threadIdx.x / SPEC_KMAX;
-        if (b < k) {
-            double s = 0.0;
-            for (int j = g; j < nPw; j += G) s += wp[(size_t)j * k + b];
-            cwp[g][b] = s;
-        }
+        if (b < k) cwp[g][b] = ws;
         __syncthreads();
         if (threadIdx.x < k) {
             double s = cwp[0][threadIdx.x];
@@ -1897,15 +1932,13 @@ __global__ __launch_bounds__(NTS) void slice_dot_k(int n, int slices, const doub
         }
         __syncthreads();
     }
-    double s = 0.0;
-    if (i < n) {
-        for (int q = wv; q < slices; q += SLW) s += part[(size_t)q * n + i];
+    if (i < n && U) {
         // wavefront wv's share of the spectral term: columns wv, wv + SLW, ...
-        if (U) {
-            double t = 0.0;
-            for (int b = wv; b < k; b += SLW) t += U[(size_t)b * n + i] * cw[b];
-            s += t;
-        }
+        double t = 0.0;
+#pragma unroll
+        for (int q = 0; q < UB; ++q)
+            if (wv + q * SLW < k) t += uv[q] * cw[wv + q * SLW];
+        s += t;
     }
     red[wv][lane] = s;
     __syncthreads();
@@ -1931,6 +1964,19 @@ __global__ __launch_bounds__(NT) void kfin_k(int V, int slices, const double *__
                                              int it, Ctl *ctl) {
     __shared__ double red[NT / 64];
     if (ctl && cld(&ctl->done)) return;
+    // the slice partials and x are requested before the stopping test
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double sv[D], xv[D];
+    if (i < V) {
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            double s = 0.0;
+#pragma unroll 8
+            for (int k = 0; k < slices; ++k) s += part[(size_t)k * D * V + (size_t)c * V + i];
+            sv[c] = s;
+            xv[c] = x[c * V + i];
+        }
+    }
     if constexpr (CHECK) {
         double rr = reduce_partials(Prr, nPrr, red);
         if (rr < cld(&ctl->tol)) {
@@ -1938,17 +1984,13 @@ __global__ __launch_bounds__(NT) void kfin_k(int V, int slices, const double *__
             return;
         }
     }
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     double dotp = 0.0;
     if (i < V) {
 #pragma unroll
         for (int c = 0; c < D; ++c) {
-            double s = 0.0;
-            for (int k = 0; k < slices; ++k) s += part[(size_t)k * D * V + (size_t)c * V + i];
-            const double xi = x[c * V + i];
-            s += eps * xi;
+            const double s = sv[c] + eps * xv[c];
             y[c * V + i] = s;
-            dotp += xi * s;
+            dotp += xv[c] * s;
         }
     }
     if constexpr (CHECK) {
@@ -1986,6 +2028,7 @@ __global__ __launch_bounds__(64 * W) void slice_rows_k(int n, int slices, const 
     const int i = blockIdx.x * 64 + lane;
     double s = 0.0;
     if (i < n)
+#pragma unroll 4
         for (int k = wv; k < slices; k += W) s += part[(size_t)k * n + i];
     red[wv][lane] = s;
     __syncthreads();
@@ -2214,6 +2257,18 @@ __global__ __launch_bounds__(NT) void ctl_update_k(int n, const double *__restri
     __shared__ double redw[NT / 64][SPW_COLS];
     if (cld(&ctl->done)) return;
     const int it = cld(&ctl->it);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int b0 = blockIdx.y * SPW_COLS;
+    // the operands are requested before the partials are reduced
+    const double rz = cld(&hist[it]);
+    double rv = 0.0, sv = 0.0, av = 0.0, dv = 0.0, uv[SPW_COLS];
+    if (i < n) {
+        rv = rin[i];
+        sv = Sd[i];
+        if (blockIdx.y == 0) { av = a[i]; dv = d[i]; }
+    }
+#pragma unroll
+    for (int c = 0; c < SPW_COLS; ++c) uv[c] = U && i < n && b0 + c < k ? U[(size_t)(b0 + c) * n + i] : 0.0;
     double dSd = reduce_partials(P, nP, red);
     if (dSd == 0.0) {
         if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
@@ -2221,12 +2276,11 @@ __global__ __launch_bounds__(NT) void ctl_update_k(int n, const double *__restri
         }
         return;
     }
-    double st = cld(&hist[it]) / dSd;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const double ri = i < n ? rin[i] - st * Sd[i] : 0.0;
+    const double st = rz / dSd;
+    const double ri = i < n ? rv - st * sv : 0.0;
     if (blockIdx.y == 0) {
         if (i < n) {
-            a[i] += st * d[i];
+            a[i] = av + st * dv;
             rout[i] = ri;
         }
         double t = block_sum(ri * ri, red);
@@ -2234,10 +2288,9 @@ __global__ __launch_bounds__(NT) void ctl_update_k(int n, const double *__restri
     }
     if (U) {
         const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-        const int b0 = blockIdx.y * SPW_COLS;
 #pragma unroll
         for (int c = 0; c < SPW_COLS; ++c) {
-            const double v = wave_sum(i < n && b0 + c < k ? U[(size_t)(b0 + c) * n + i] * ri : 0.0);
+            const double v = wave_sum(uv[c] * ri);
             if (lane == 0) redw[wv][c] = v;
         }
         __syncthreads();
@@ -2421,14 +2474,18 @@ __global__ __launch_bounds__(NT) void rd_update_k(int V, const double *__restric
                                                   double *__restrict__ P2, Ctl *ctl) {
     __shared__ double red[NT / 64];
     if (cld(&ctl->done)) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    // the operands are requested before the partials are reduced
+    const double rz = cld(&hist[it]);
+    double xv = 0.0, dv = 0.0, rv = 0.0, av = 0.0, dg = 1.0;
+    if (i < V) { xv = x[i]; dv = d[i]; rv = r[i]; av = Ad[i]; dg = A[diagIdx[i]]; }
     double dAd = reduce_partials(P, nP, red);
-    double al = cld(&hist[it]) / dAd;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double al = rz / dAd;
     double s_rr = 0.0, s_rz = 0.0;
     if (i < V) {
-        x[i] += al * d[i];
-        double ri = r[i] - al * Ad[i];
-        double zi = ri / A[diagIdx[i]];
+        x[i] = xv + al * dv;
+        double ri = rv - al * av;
+        double zi = ri / dg;
         r[i] = ri;
         zv[i] = zi;
         s_rr = ri * ri;
@@ -2469,23 +2526,32 @@ __global__ __launch_bounds__(NT) void rd_spmv_dir_k(int V, const int *__restrict
                                                     double *__restrict__ Ad, double *__restrict__ Pd, Ctl *ctl) {
     __shared__ double red[NT / 64];
     if (cld(&ctl->done)) return;
-    const double rr = reduce_partials(P, nP, red);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    // the row's bounds and own entries and the step history are requested
+    // before the partials are reduced
+    int z0 = 0, z1 = 0;
+    double zi = 0.0, di0 = 0.0;
+    if (i < V) { z0 = rowPtr[i]; z1 = rowPtr[i + 1]; zi = zv[i]; di0 = dOld[i]; }
+    const double rzOld = cld(&hist[it]);
+    double pr = 0.0, pz = 0.0;
+    for (int j = threadIdx.x; j < nP; j += NT) { pr += P[j]; pz += P[MAXP + j]; }
+    const double rr = block_sum(pr, red);
     if (rr < cld(&ctl->tol)) {
         if (blockIdx.x == 0 && threadIdx.x == 0) { cst(&ctl->done, 1); cst(&ctl->iters, it + 1); }
         return;
     }
-    const double rz = reduce_partials(P + MAXP, nP, red);
-    const double be = rz / cld(&hist[it]);
+    const double rz = block_sum(pz, red);
+    const double be = rz / rzOld;
     if (blockIdx.x == 0 && threadIdx.x == 0) cst(&hist[it + 1], rz);
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     double ss = 0.0;
     if (i < V) {
         double s = 0.0;
-        for (int z = rowPtr[i]; z < rowPtr[i + 1]; ++z) {
+#pragma unroll 8
+        for (int z = z0; z < z1; ++z) {
             const int c = colIdx[z];
             s += A[z] * (zv[c] + be * dOld[c]);
         }
-        const double di = zv[i] + be * dOld[i];
+        const double di = zi + be * di0;
         dNew[i] = di;
         Ad[i] = s;
         ss = di * s;
