@@ -1366,6 +1366,13 @@ __global__ __launch_bounds__(NT) void kstore_mv_k(int nr, int nc, const T *__res
 #define SA_KS_U 4
 #endif
     constexpr int U = SA_KS_U;   // FP64: columns whose loads are issued together
+    // FP32: columns of each wavefront in flight, a ring of KP loads.  At C2
+    // on one box: 2 -> 169.0 us per product (16.77 steps/s), 4 -> 163.5 us
+    // (17.16), 8 -> 169.6 us (16.66; 86 VGPRs, 5 waves per SIMD)
+#ifndef SA_KS_P
+#define SA_KS_P 4
+#endif
+    constexpr int KP = SA_KS_P;
     using V4 = typename std::conditional<sizeof(T) == 4, float4, double2>::type;
     constexpr int NV = R * (int)sizeof(T) / 16;   // 16-byte loads per lane and column
     constexpr int EV = 16 / (int)sizeof(T);       // elements per 16-byte load
@@ -1380,18 +1387,17 @@ __global__ __launch_bounds__(NT) void kstore_mv_k(int nr, int nc, const T *__res
     const int jb = blockIdx.y * per, je = min(nc, jb + per);
     const int ns = max(0, je - jb);
     const T *base = M + (size_t)blockIdx.x * PR * nc + (size_t)jb * PR + R * lane;
-    // FP32: each wavefront's first two columns are requested before the
-    // input is staged, so the stream starts while the staging's loads and
-    // reductions are in flight
-    V4 m0[NV], m1[NV];
+    // FP32: each wavefront's first KP columns (the load ring below) are
+    // requested before the input is staged, so the stream starts while the
+    // staging's loads and reductions are in flight
+    V4 m[KP][NV];
     if constexpr (sizeof(T) == 4) {
-        if (wv < ns)
 #pragma unroll
-            for (int v = 0; v < NV; ++v) m0[v] = ks_load(reinterpret_cast<const V4 *>(base + (size_t)wv * PR + v * EV));
-        if (wv + W < ns)
+        for (int k = 0; k < KP; ++k)
+            if (wv + k * W < ns)
 #pragma unroll
-            for (int v = 0; v < NV; ++v)
-                m1[v] = ks_load(reinterpret_cast<const V4 *>(base + (size_t)(wv + W) * PR + v * EV));
+                for (int v = 0; v < NV; ++v)
+                    m[k][v] = ks_load(reinterpret_cast<const V4 *>(base + (size_t)(wv + k * W) * PR + v * EV));
     }
     // the staged input's operands (x; z and d_old; r and Ad), a thread's
     // columns j0 + s NT + threadIdx.x (s < SJ) requested together: the
@@ -1503,37 +1509,30 @@ __global__ __launch_bounds__(NT) void kstore_mv_k(int nr, int nc, const T *__res
     if constexpr (sizeof(T) == 4) {
         // FP32: the wavefronts of a block take interleaved columns (w, w + W,
         // ...), so the block reads one contiguous run of W KiB at a time, and
-        // each wavefront keeps its next column's load in flight while it uses
-        // the current one.  The DRAM then sees one sequential stream per
+        // each wavefront keeps its next KP - 1 columns' loads in flight while
+        // it uses the current one.  The DRAM then sees one sequential stream per
         // block instead of one per wavefront.  The wavefront partials reuse
         // the LDS of x, so a block needs at most 24 KiB and the tile table's
         // launch fits the chip in one round.  Measured on the 51,210 x 5,000
         // products and their transposes: 0.81-0.83 of HBM peak for the
         // kernel alone against 0.72-0.75 for contiguous quarter-slices per
         // wavefront; 0.79 inside the solve.
-        int j = wv;
-        for (; j < ns; j += 2 * W) {
+        for (int j = wv; j < ns; j += KP * W) {
 #pragma unroll
-            for (int c = 0; c < D; ++c) {
-                const double xv = sx[c * ns + j];
+            for (int k = 0; k < KP; ++k) {
+                const int jj = j + k * W;
+                if (jj < ns) {
 #pragma unroll
-                for (int v = 0; v < NV; ++v) fma_rows(a, v * EV, c, m0[v], xv);
-            }
-            if (j + 2 * W < ns)
+                    for (int c = 0; c < D; ++c) {
+                        const double xv = sx[c * ns + jj];
 #pragma unroll
-                for (int v = 0; v < NV; ++v)
-                    m0[v] = ks_load(reinterpret_cast<const V4 *>(base + (size_t)(j + 2 * W) * PR + v * EV));
-            if (j + W < ns) {
+                        for (int v = 0; v < NV; ++v) fma_rows(a, v * EV, c, m[k][v], xv);
+                    }
+                    if (jj + KP * W < ns)
 #pragma unroll
-                for (int c = 0; c < D; ++c) {
-                    const double xv = sx[c * ns + j + W];
-#pragma unroll
-                    for (int v = 0; v < NV; ++v) fma_rows(a, v * EV, c, m1[v], xv);
+                        for (int v = 0; v < NV; ++v)
+                            m[k][v] = ks_load(reinterpret_cast<const V4 *>(base + (size_t)(jj + KP * W) * PR + v * EV));
                 }
-                if (j + 3 * W < ns)
-#pragma unroll
-                    for (int v = 0; v < NV; ++v)
-                        m1[v] = ks_load(reinterpret_cast<const V4 *>(base + (size_t)(j + 3 * W) * PR + v * EV));
             }
         }
         __syncthreads();   // every wavefront is done with x
