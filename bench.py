@@ -428,15 +428,15 @@ def mode_tune(args, api, W):
     rows, table = [], []
     names = {"xc": "kxc", "cx": "kcx", "cc": "precond", "kk": "kmatvec"}
     op_done = set()
-    f32_only = args.tune_only == "f32"
+    f32_only, f64_only = args.tune_only == "f32", args.tune_only == "f64"
     kk_only = args.tune_only == "kk"
     for desc, p in tune_shapes(W):
         d, V = p["q0"].shape
         nc = len(p["ctlIdx"]) if p.get("ctlIdx") is not None else 0
         E = p["elm"]["elmVtxMat"].shape[1]
-        if (f32_only and not nc) or (kk_only and nc):
+        if ((f32_only or f64_only) and not nc) or (kk_only and nc):
             continue
-        if (E, V) not in op_done and not f32_only and not kk_only:
+        if (E, V) not in op_done and args.tune_only == "all":
             # the fused operator apply: one launch shape per session
             op_done.add((E, V))
             times = {}
@@ -455,7 +455,7 @@ def mode_tune(args, api, W):
                          "us": {str(t): round(v, 2) for t, v in times.items()}, "target": pick})
             print(f"[tune] {desc} operator (E {E}, V {V}): " + " ".join(f"{t}:{v:.1f}" for t, v in times.items())
                   + f" -> {pick}", file=sys.stderr, flush=True)
-        precs = ("mixed",) if f32_only else ("mixed", "fp64") if nc else ("fp64",)
+        precs = ("mixed",) if f32_only else ("fp64",) if f64_only else ("mixed", "fp64") if nc else ("fp64",)
         for prec in precs:
             f32 = prec == "mixed"
             prods = ("xc", "cx", "cc") if nc else ("kk",)
@@ -512,9 +512,9 @@ def main():
                     choices=["single", "batch", "serve", "precision", "fit", "products", "tune", "operator"],
                     help="single: one solve of --config; batch: 64 uneven meshes concurrently; serve: Poisson stream")
     ap.add_argument("--tune-reps", type=int, default=30, help="product launches per timing of --mode tune")
-    ap.add_argument("--tune-only", default="all", choices=["all", "f32", "kk"],
+    ap.add_argument("--tune-only", default="all", choices=["all", "f32", "f64", "kk"],
                     help="--mode tune: every product and the operator, only the FP32 products of the mixed form, "
-                         "or only the all-landmark FP64 products")
+                         "only the FP64 control-point products, or only the all-landmark FP64 products")
     ap.add_argument("--workers", type=int, default=8)
     ap.add_argument("--batch-n", type=int, default=64)
     ap.add_argument("--pool", type=int, default=12)
