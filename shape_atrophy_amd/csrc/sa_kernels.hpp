@@ -1366,9 +1366,10 @@ __global__ __launch_bounds__(NT) void kstore_mv_k(int nr, int nc, const T *__res
 #define SA_KS_U 4
 #endif
     constexpr int U = SA_KS_U;   // FP64: columns whose loads are issued together
-    // FP32: columns of each wavefront in flight, a ring of KP loads.  At C2
-    // on one box: 2 -> 169.0 us per product (16.77 steps/s), 4 -> 163.5 us
-    // (17.16), 8 -> 169.6 us (16.66; 86 VGPRs, 5 waves per SIMD)
+    // FP32: columns of each wavefront in flight, a ring of KP loads.  The
+    // 51,210 x 5,000 products of the mixed solve on one box: 2 -> 169.0 us
+    // per product (16.77 steps/s), 4 -> 163.5 us (17.16), 8 -> 169.6 us
+    // (16.66; 86 VGPRs, 5 waves per SIMD)
 #ifndef SA_KS_P
 #define SA_KS_P 4
 #endif
