@@ -1560,16 +1560,23 @@ __global__ __launch_bounds__(NT) void kstore_mv_k(int nr, int nc, const T *__res
                     for (int v = 0; v < NV; ++v) fma_rows(a, v * EV, c, m[u][v], xv);
                 }
         }
-        for (; j < we; ++j, col += PR) {
-            V4 m[NV];
+        if (j < we) {
+            // the last (we - j < U) columns: their loads also go out together
+            V4 m[U][NV];
 #pragma unroll
-            for (int v = 0; v < NV; ++v) m[v] = *reinterpret_cast<const V4 *>(col + v * EV);
+            for (int u = 0; u < U; ++u)
+                if (j + u < we)
 #pragma unroll
-            for (int c = 0; c < D; ++c) {
-                const double xv = sx[c * ns + j];
+                    for (int v = 0; v < NV; ++v) m[u][v] = ks_load(reinterpret_cast<const V4 *>(col + u * PR + v * EV));
 #pragma unroll
-                for (int v = 0; v < NV; ++v) fma_rows(a, v * EV, c, m[v], xv);
-            }
+            for (int u = 0; u < U; ++u)
+                if (j + u < we)
+#pragma unroll
+                    for (int c = 0; c < D; ++c) {
+                        const double xv = sx[c * ns + j + u];
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) fma_rows(a, v * EV, c, m[u][v], xv);
+                    }
         }
     }
     // FP64: wavefront 0 requests its rows of the dot product's vectors
