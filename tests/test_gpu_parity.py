@@ -135,6 +135,26 @@ def test_full_solve_matrix_free_all_landmarks(which, oracle, monkeypatch):
     assert abs(r["obj"] - o["obj"]) <= 1e-9 * abs(o["obj"])
 
 
+def test_full_solve_all_landmarks_staging_in_passes(oracle, monkeypatch):
+    """The product that takes the all-landmark CG step stages its column
+    slice 512 columns per pass.  kk=16 leaves 4 slices of 803 columns on
+    this 3,210-landmark shell, so every launch takes two passes (and the
+    first row panel's x and alpha updates both).  Full solve against the
+    oracle, at the full-solve tolerance."""
+    monkeypatch.setenv("SA_KS_TARGET", "kk=16")
+    p = W.shell_problem(level=3, n_layers=5, time_steps=3)
+    p["name"] = "shell3d_level3_4_slices"
+    s = _session(p)
+    r = s.run(p["tauVar"][:, 0], p["tauPrm"], want_traj=True)
+    s.close()
+    o = oracle.objfcn(p, pcg_init=1, rd_solve=1)
+    assert r["status"] == 0 and o["status"] == 0
+    disp = np.linalg.norm(o["lmkStk"][-1] - o["lmkStk"][0])
+    assert np.linalg.norm(r["lmkStk"] - o["lmkStk"]) / (disp * np.sqrt(len(o["lmkStk"]))) < 1e-9
+    assert rel(r["tauMat"], o["tauMat"]) < 1e-9
+    assert abs(r["obj"] - o["obj"]) <= 1e-9 * abs(o["obj"])
+
+
 def test_probe_varifold(prob, oracle):
     s = _session(prob)
     rng = np.random.default_rng(9)
