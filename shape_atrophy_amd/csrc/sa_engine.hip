@@ -769,6 +769,11 @@ struct Engine : EngineBase {
     bool mixedOff = false;       // a step of this run stalled: the rest run in FP64
     bool precEveryStep = false;  // a stale preconditioner failed in this run: refresh every step
     bool forceRefresh = false;   // refresh at the next step (after a stalled mixed step)
+    // set while an inner FP32 solve of the mixed refinement runs: ctl_pcg_run
+    // then stops (status 3) when |r|^2 has not halved over STALL_WIN
+    // iterations, once STALL_MIN are done
+    bool stallCheck = false;
+    static constexpr int STALL_MIN = 200, STALL_WIN = 100;
     int failReason = 0;          // SA_FAIL_* of a failed run
 
     // mixed form, fallback to FP64: the FP64 pair takes the memory of the
@@ -2048,7 +2053,7 @@ struct Engine : EngineBase {
         }
         if (recActive) {
             recActive = false;
-            if (st != 1) spec_harvest(md, std::min(iters, SPEC_M));
+            if (st != 1 && st != 3) spec_harvest(md, std::min(iters, SPEC_M));
         }
         return st;
     }
@@ -2246,6 +2251,7 @@ struct Engine : EngineBase {
         Ctl &h = hctl[0];
         int it0 = 0;
         int chunk = std::max(4, std::min(64, chunkHint));
+        std::vector<std::pair<int, double>> rrAt;   // stallCheck: (iterations done, |r|^2) per chunk
         while (true) {
             // the counter check that ends a solve whose budget is spent runs
             // at the start of an iteration: iteration maxIt is launched too
@@ -2272,6 +2278,23 @@ struct Engine : EngineBase {
                 }
             }
             if (h.done) break;
+            if (stallCheck) {
+                // an inner solve of the mixed form whose |r|^2 fell less than
+                // half over the last STALL_WIN iterations (past STALL_MIN) no
+                // longer converges: the FP32 operator is too far from the FP64
+                // one, and the step is finished by the FP64 form at once
+                // instead of after the inner solve's whole budget
+                rrAt.emplace_back(itEnd, h.rr);
+                if (itEnd >= STALL_MIN) {
+                    double ref = -1.0;
+                    for (const auto &pr : rrAt)
+                        if (pr.first <= itEnd - STALL_WIN) ref = pr.second;
+                    if (ref >= 0.0 && !(h.rr < 0.5 * ref)) {
+                        iters = itEnd;
+                        return 3;
+                    }
+                }
+            }
             it0 = itEnd;
             chunk = 8;
         }
@@ -2410,8 +2433,16 @@ struct Engine : EngineBase {
                 // iterations (+200; 1/8 of the budget on a run's first step):
                 // an FP32 operator that does not converge counts as a stall
                 const int capIn = lastIters > 0 ? 3 * lastIters + 200 : std::max(1000, itrMax / 8);
-                int st = ctl_solve(cgMode, q, tauNow, geoNow, cRout.p, cDelta.p, true, inner,
+                int st;
+                {
+                    struct StallCheck {
+                        bool &f;
+                        explicit StallCheck(bool &b) : f(b) { f = true; }
+                        ~StallCheck() { f = false; }
+                    } sc(stallCheck);
+                    st = ctl_solve(cgMode, q, tauNow, geoNow, cRout.p, cDelta.p, true, inner,
                                    std::max(1, std::min(itrMax - iters, capIn)), its);
+                }
                 iters += its;
                 if (sw < 4) sweepIters[sw] = its;
                 if (st != 0) { stalled = true; break; }   // breakdown or no convergence of the FP32 solve

@@ -20,6 +20,7 @@ struct Ctl {            // solver control block, device resident
     int done, status, iters;
     int it, maxIt;      // control-point CG: current iteration (-1 before the first) and budget
     int pad;
+    double rr;          // control-point CG: |r|^2 after the last iteration
 };
 
 // ---------------------------------------------------------------------------
@@ -587,6 +588,7 @@ __global__ __launch_bounds__(64) void jacobi_dot_k(int n, const double *__restri
     if (cld(&ctl->done)) return;
     const int it = cld(&ctl->it);
     const double rr = reduce_partials(Prr, nPrr, red);
+    if (blockIdx.x == 0 && threadIdx.x == 0) cst(&ctl->rr, rr);
     if (rr < cld(&ctl->tol)) {
         if (blockIdx.x == 0 && threadIdx.x == 0) { cst(&ctl->done, 1); cst(&ctl->iters, it + 1); }
         return;
@@ -1883,8 +1885,9 @@ __global__ __launch_bounds__(NTS) void slice_combine_k(int C, int V, int slices,
     if (threadIdx.x == 0) P[blockIdx.x] = t;
 }
 
-// slice_dot_k: after the residual update, stop on |r|^2 (partials Prr) below
-// the tolerance; else z = M r from the preconditioner product's partials
+// slice_dot_k: after the residual update, record |r|^2 (partials Prr) in
+// the control block and stop when it is below the tolerance; else z = M r
+// from the preconditioner product's partials
 // (plus, with U, the spectral term U diag(cf) w, w = U^T r summed from the
 // nPw block partials wp of ctl_update_k) and the block partial of r.z
 __global__ __launch_bounds__(NTS) void slice_dot_k(int n, int slices, const double *__restrict__ part,
@@ -1922,6 +1925,7 @@ __global__ __launch_bounds__(NTS) void slice_dot_k(int n, int slices, const doub
 #pragma unroll 8
         for (int j = g; j < nPw; j += G) ws += wp[(size_t)j * k + b];
     const double rr = reduce_partials(Prr, nPrr, red2);
+    if (blockIdx.x == 0 && threadIdx.x == 0) cst(&ctl->rr, rr);
     if (rr < cld(&ctl->tol)) {
         if (blockIdx.x == 0 && threadIdx.x == 0) { cst(&ctl->done, 1); cst(&ctl->iters, it + 1); }
         return;
