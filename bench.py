@@ -341,6 +341,44 @@ def mode_products(args, api, W):
             **out}
 
 
+def mode_meshes(args, api, W):
+    """Three meshes of the 64-mesh batch solved one at a time on their own
+    sessions (no server, one stream): a 3D shell of 5k-9k landmarks, a 2D
+    ring of 1k-3k and a 3D shell of 20k-40k, mixed form, --steps steps
+    (default 10) after a warm-up solve.  Per mesh: wall time per CG
+    iteration; under rocprofv3 --kernel-trace, the per-kernel times of
+    small meshes."""
+    steps = args.steps if args.steps is not None else 10
+    probs = W.batch_problems(n=args.batch_n, time_steps=steps)
+    sizes = [p["q0"].shape[1] for p in probs]
+    out = []
+    for want_d, lo, hi in ((3, 5000, 9000), (2, 1000, 3000), (3, 20000, 40000)):
+        k = next((k for k, p in enumerate(probs) if p["dim"] == want_d and lo <= sizes[k] <= hi), None)
+        if k is None:
+            continue
+        p = probs[k]
+        s = api.Session(p, precision=args.precision)
+        try:
+            tv, tp = p["tauVar"][:, 0], p["tauPrm"]
+            s.run(tv, tp)
+            t0 = time.perf_counter()
+            r = s.run(tv, tp)
+            wall = time.perf_counter() - t0
+        finally:
+            s.close()
+        row = {"mesh": k, "dim": p["dim"], "landmarks": sizes[k], "control_points": len(p["ctlIdx"]),
+               "steps": r["steps"], "seconds": round(wall, 4), "pcg_iters": r["pcg_iters"],
+               "refine_sweeps": r["refine"], "fp64_steps": r["fp64_steps"],
+               "us_per_cg_iteration": round(1e6 * wall / max(r["pcg_iters"], 1), 1)}
+        print(f"[bench] mesh {row}", file=sys.stderr, flush=True)
+        out.append(row)
+    return {"metric": "single-mesh solves from the batch: wall time per CG iteration", "value": None,
+            "unit": "us per CG iteration", "n_gpus": 1, "steps": steps, "warmup": 1, "ms_per_step": None,
+            "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64+f32",
+            "data": "synthetic", "config": {"workload": "three meshes of the 64-mesh batch, one at a time",
+                                            "precision": args.precision}, "meshes": out}
+
+
 def mode_operator(args, api, W):
     """The operator apply (E + btmWgt P) x of one workload, as the CG runs it:
     the fused launch (op_fused_k, the default) against the two-launch apply
@@ -509,7 +547,7 @@ def main():
                     help="time without the kernel timer's events (no roofline, no operator or varifold timing)")
     ap.add_argument("--cpu-iters", type=int, default=0, help="CG iterations of the CPU sample (0: sized to the workload)")
     ap.add_argument("--mode", default="single",
-                    choices=["single", "batch", "serve", "precision", "fit", "products", "tune", "operator"],
+                    choices=["single", "batch", "serve", "precision", "fit", "products", "tune", "operator", "meshes"],
                     help="single: one solve of --config; batch: 64 uneven meshes concurrently; serve: Poisson stream")
     ap.add_argument("--tune-reps", type=int, default=30, help="product launches per timing of --mode tune")
     ap.add_argument("--tune-only", default="all", choices=["all", "f32", "f64", "kk"],
@@ -543,7 +581,7 @@ def main():
         from shape_atrophy_amd import workloads as W
         res = {"batch": mode_batch, "serve": mode_serve, "precision": mode_precision,
                "fit": mode_fit, "products": mode_products, "tune": mode_tune,
-               "operator": mode_operator}[args.mode](args, api, W)
+               "operator": mode_operator, "meshes": mode_meshes}[args.mode](args, api, W)
         print(json.dumps(res))
         return
     steps = args.steps if args.steps is not None else 50   # the configured solve: 50 time steps
