@@ -2314,24 +2314,6 @@ __global__ __launch_bounds__(NT) void ctl_update_k(int n, const double *__restri
     }
 }
 
-// after z = M^{-1} r: stop on |r|^2 < tol, else partial r.z
-__global__ __launch_bounds__(NT) void ctl_dot_k(int n, const double *__restrict__ Prr, int nP,
-                                                const double *__restrict__ r, const double *__restrict__ z,
-                                                double *__restrict__ P3, Ctl *ctl) {
-    __shared__ double red[NT / 64];
-    if (cld(&ctl->done)) return;
-    const int it = cld(&ctl->it);
-    double rr = reduce_partials(Prr, nP, red);
-    if (rr < cld(&ctl->tol)) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) { cst(&ctl->done, 1); cst(&ctl->iters, it + 1); }
-        return;
-    }
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    double ss = (i < n) ? r[i] * z[i] : 0.0;
-    double t = block_sum(ss, red);
-    if (threadIdx.x == 0) P3[blockIdx.x] = t;
-}
-
 // beta = rz_new / rz_old; d = z + beta d (it < 0: d = z)
 __global__ __launch_bounds__(NT) void ctl_dir_k(int n, const double *__restrict__ P, int nP, double *__restrict__ hist,
                                                 const double *__restrict__ z, double *__restrict__ d, Ctl *ctl) {
@@ -2507,23 +2489,8 @@ __global__ __launch_bounds__(NT) void rd_update_k(int V, const double *__restric
     if (threadIdx.x == 0) { P2[blockIdx.x] = a0; P2[MAXP + blockIdx.x] = a1; }
 }
 
-__global__ __launch_bounds__(NT) void rd_dir_k(int V, const double *__restrict__ P, int nP, double *__restrict__ hist,
-                                               int it, const double *__restrict__ zv, double *__restrict__ d, Ctl *ctl) {
-    __shared__ double red[NT / 64];
-    if (cld(&ctl->done)) return;
-    double rr = reduce_partials(P, nP, red);
-    if (rr < cld(&ctl->tol)) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) { cst(&ctl->done, 1); cst(&ctl->iters, it + 1); }
-        return;
-    }
-    double rz = reduce_partials(P + MAXP, nP, red);
-    double be = rz / cld(&hist[it]);
-    if (blockIdx.x == 0 && threadIdx.x == 0) cst(&hist[it + 1], rz);
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < V) d[i] = zv[i] + be * d[i];
-}
-
-// rd_dir_k of iteration it and rd_spmv_k of iteration it + 1 in one launch:
+// the direction update of iteration it (beta, d = z + beta d) and rd_spmv_k of
+// iteration it + 1 in one launch:
 // the stopping test and beta = rz / hist[it] from the partials of
 // rd_update_k (P: r.r, P + MAXP: r.z); each row forms the new direction at
 // its columns, zv + beta dOld (the same value every row that reads it
