@@ -2063,13 +2063,22 @@ __global__ __launch_bounds__(64 * W) void slice_rows_k(int n, int slices, const 
 // update leave it alone.  Fixed summation order: deterministic.
 constexpr int CHB = 64;
 
+// lane l's copy of x (a wavefront-uniform value, through scalar registers)
+__device__ inline double readlane_d(double x, int l) {
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readlane((int)b, l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 // one wavefront: lane i keeps row i of the diagonal block in registers
-// (r[l] = L[i][l]); step j publishes the pivot and then column j through
-// LDS, and every lane below updates its own row (the same operations, in the
-// same order, as a right-looking update of the whole block)
+// (r[l] = L[i][l]).  Step j reads the pivot and then column j straight from
+// the lanes that hold them (readlane: no LDS round trips), and every lane
+// updates its row: the same operations, in the same order, as a
+// right-looking update of the whole block.  The update runs on every lane
+// and every column after j; what it writes above a lane's diagonal is
+// never read as a pivot or column value, and is never stored.
 __global__ __launch_bounds__(CHB) void chol_diag_k(int n, int k0, double *__restrict__ A, int lda, int *info) {
-    __shared__ double col[CHB];
-    __shared__ double piv;
     if (cld(info) != 0) return;
     const int nb = min(CHB, n - k0), i = threadIdx.x;
     double r[CHB];
@@ -2079,9 +2088,7 @@ __global__ __launch_bounds__(CHB) void chol_diag_k(int n, int k0, double *__rest
 #pragma unroll
     for (int j = 0; j < CHB; ++j) {
         if (j < nb && ok) {
-            if (i == j) piv = r[j];
-            __syncthreads();
-            const double d = piv;
+            const double d = readlane_d(r[j], j);   // the same in every lane
             if (!(d > 0.0) || !isfinite(d)) {
                 if (i == 0) cst(info, k0 + j + 1);
                 ok = false;
@@ -2089,14 +2096,8 @@ __global__ __launch_bounds__(CHB) void chol_diag_k(int n, int k0, double *__rest
                 const double p = sqrt(d);
                 if (i == j) r[j] = p;
                 else if (i > j) r[j] = r[j] / p;
-                if (i > j) col[i] = r[j];
-                __syncthreads();
-                if (i > j) {
 #pragma unroll
-                    for (int l = j + 1; l < CHB; ++l)
-                        if (l <= i) r[l] -= r[j] * col[l];
-                }
-                __syncthreads();
+                for (int l = j + 1; l < CHB; ++l) r[l] -= r[j] * readlane_d(r[j], l);
             }
         }
     }
