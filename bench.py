@@ -461,8 +461,8 @@ def mode_tune(args, api, W):
     product, the device time of one product (kernel and slice sum, the mean
     of --tune-reps launches, best of three) at each block target, and for
     each mesh the fused operator apply at each launch shape; the table keeps
-    what tune_pick chooses.  Writes the table to
-    gpurun_out/tiles.txt (copy it beside the library to use it)."""
+    what tune_pick chooses.  Writes the table to --tune-out (copy it beside
+    the library, or name it in SA_TILE_TABLE, to use it)."""
     rows, table = [], []
     names = {"xc": "kxc", "cx": "kcx", "cc": "precond", "kk": "kmatvec"}
     op_done = set()
@@ -515,8 +515,9 @@ def mode_tune(args, api, W):
                 print(f"[tune] {desc} {prec} {prod} {shape}: " +
                       " ".join(f"{t}:{v:.1f}" for t, v in times.items()) + f" -> {pick}", file=sys.stderr, flush=True)
     os.environ.pop("SA_KS_TARGET", None)
-    os.makedirs("gpurun_out", exist_ok=True)
-    with open(os.path.join("gpurun_out", "tiles.txt"), "w") as f:
+    if os.path.dirname(args.tune_out):
+        os.makedirs(os.path.dirname(args.tune_out), exist_ok=True)
+    with open(args.tune_out, "w") as f:
         f.write("# product type rows cols target-blocks; op: elements landmarks launch-shape  (python bench.py --mode tune)\n")
         f.write("\n".join(table) + "\n")
     return {"metric": "tile sweep of the stored kernel products", "value": None, "unit": "us per product",
@@ -550,6 +551,7 @@ def main():
                     choices=["single", "batch", "serve", "precision", "fit", "products", "tune", "operator", "meshes"],
                     help="single: one solve of --config; batch: 64 uneven meshes concurrently; serve: Poisson stream")
     ap.add_argument("--tune-reps", type=int, default=30, help="product launches per timing of --mode tune")
+    ap.add_argument("--tune-out", default="tiles.tuned.txt", help="where --mode tune writes its tile table")
     ap.add_argument("--tune-only", default="all", choices=["all", "f32", "f64", "kk"],
                     help="--mode tune: every product and the operator, only the FP32 products of the mixed form, "
                          "only the FP64 control-point products, or only the all-landmark FP64 products")
@@ -573,7 +575,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the solve runs on one MI355X (asked for --gpus {args.gpus}); "
-              "sharding across GPUs is not part of this build", file=sys.stderr)
+              "the library has no multi-GPU sharding", file=sys.stderr)
         sys.exit(2)
 
     from shape_atrophy_amd import api
