@@ -1516,7 +1516,8 @@ struct Engine : EngineBase {
             int itEnd = std::min(itrMax, it0 + chunk);
             if (timer.on) timer.ensure(itEnd - it0);
             for (int it = it0; it < itEnd; ++it) {
-                hipEvent_t *ev = timer.on ? &timer.ev[6 * (it - it0)] : nullptr;
+                // the kernel timer samples every TIMER_EVERY-th iteration
+                hipEvent_t *ev = timer.on && it % TIMER_EVERY == 0 ? &timer.ev[6 * (it - it0)] : nullptr;
                 if (ev) CK(hipEventRecord(ev[2], s));
                 op_apply<0>(q, d.p, tauNow, geoNow, Kid.p, d.p, Ad.p, nullptr, P1.p);
                 if (ev) CK(hipEventRecord(ev[3], s));
@@ -1535,6 +1536,7 @@ struct Engine : EngineBase {
                 // its product ran iff it + 1 < iters
                 int lastOp = h.done ? h.iters : itEnd;
                 for (int it = it0; it < itEnd; ++it) {
+                    if (it % TIMER_EVERY) continue;
                     hipEvent_t *ev = &timer.ev[6 * (it - it0)];
                     float ms;
                     if (it < lastOp) {
@@ -2032,7 +2034,12 @@ struct Engine : EngineBase {
         CHECK_LAUNCH();
     }
 
-    static constexpr int TIMER_EVERY = 4;
+    // the kernel timer's events serialise the launches around them: sampling
+    // every 16th CG iteration costs under 0.5% of a solve on the 51k-landmark
+    // shell (17.42 against 17.42-17.50 steps/s untimed) and about 6% on a
+    // 2,000-landmark ring, whose launches are short (every 4th: 1.3% on the
+    // shell; every iteration: 38% on the ring)
+    static constexpr int TIMER_EVERY = 16;
 
     // PCG on S x = rhs (|rhs|^2 partials in P0, gC of them), preconditioned by
     // X^T X; x in/out (x0 = 0 when zeroStart); stop when |r|^2 < tolSqu |rhs|^2.
