@@ -308,6 +308,16 @@ struct SaError : std::runtime_error {
 
 #define CHECK_LAUNCH() CK(hipGetLastError())
 
+// the preconditioner product's matrix loads: ordinary (0, the default) or
+// non-temporal (1) like the other stored products.  M (100 MB in FP32 at
+// 5,000 control points) then stays in the 256 MB last-level cache from one
+// CG iteration to the next, past the non-temporal streams of K_xc and K_cx:
+// 17.51 against 17.41 steps/s on the 51k-landmark shell, mixed form
+#ifndef SA_PRE_NT
+#define SA_PRE_NT 0
+#endif
+constexpr bool PRE_NT = SA_PRE_NT != 0;
+
 inline int gridOf(long n, int nt = NT) { return (int)((n + nt - 1) / nt); }
 
 // Set while a constructor only sizes a session (sa_session_footprint): every
@@ -1856,11 +1866,11 @@ struct Engine : EngineBase {
         if ((md == KMode::Stored32 || md == KMode::Mfma32) && Mpre32.p) {
             sl = slicesM32;
             dim3 g(gridOf(NC, panel_rows<float>()), sl);
-            kstore_mv_k<D, float><<<g, NT, ks_lds<float>(NC, sl), s>>>(NC, NC, Mpre32.p, r, partS.p, ctl.p);
+            kstore_mv_k<D, float, false, PRE_NT><<<g, NT, ks_lds<float>(NC, sl), s>>>(NC, NC, Mpre32.p, r, partS.p, ctl.p);
         } else {
             sl = slicesM;
             dim3 g(gridOf(NC, panel_rows<double>()), sl);
-            kstore_mv_k<D, double><<<g, NT, ks_lds<double>(NC, sl), s>>>(NC, NC, Mpre.p, r, partS.p, ctl.p);
+            kstore_mv_k<D, double, false, PRE_NT><<<g, NT, ks_lds<double>(NC, sl), s>>>(NC, NC, Mpre.p, r, partS.p, ctl.p);
         }
         CHECK_LAUNCH();
         slice_dot_k<<<gridOf(nc, 64), NTS, 0, s>>>(nc, sl, partS.p, P2.p, gridOf(nc), r, z, P3.p,
@@ -2821,10 +2831,10 @@ struct Engine : EngineBase {
                 return timed([&] {
                     if (f32) {
                         dim3 g(gridOf(NC, panel_rows<float>()), sl);
-                        kstore_mv_k<D, float><<<g, NT, ks_lds<float>(NC, sl), s>>>(NC, NC, Mpre32.p, cR.p, partS.p, ctl.p);
+                        kstore_mv_k<D, float, false, PRE_NT><<<g, NT, ks_lds<float>(NC, sl), s>>>(NC, NC, Mpre32.p, cR.p, partS.p, ctl.p);
                     } else {
                         dim3 g(gridOf(NC, panel_rows<double>()), sl);
-                        kstore_mv_k<D, double><<<g, NT, ks_lds<double>(NC, sl), s>>>(NC, NC, Mpre.p, cR.p, partS.p, ctl.p);
+                        kstore_mv_k<D, double, false, PRE_NT><<<g, NT, ks_lds<double>(NC, sl), s>>>(NC, NC, Mpre.p, cR.p, partS.p, ctl.p);
                     }
                     CHECK_LAUNCH();
                     slice_rows(D * NC, sl, partS.p, cZ.p, ctl.p);

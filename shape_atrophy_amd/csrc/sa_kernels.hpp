@@ -1303,21 +1303,25 @@ constexpr int KS_BLOCKS = SA_KS_BLOCKS;   // target blocks per stored product
 #endif
 typedef float ks_f4 __attribute__((ext_vector_type(4)));
 typedef double ks_d2 __attribute__((ext_vector_type(2)));
+// (NTL = false: an ordinary load, for a matrix small enough to stay cached
+// from one product to the next)
+template <bool NTL = true>
 __device__ inline float4 ks_load(const float4 *p) {
-#if SA_KS_NT
-    ks_f4 v = __builtin_nontemporal_load(reinterpret_cast<const ks_f4 *>(p));
-    return make_float4(v.x, v.y, v.z, v.w);
-#else
-    return *p;
-#endif
+    if constexpr (NTL && SA_KS_NT) {
+        ks_f4 v = __builtin_nontemporal_load(reinterpret_cast<const ks_f4 *>(p));
+        return make_float4(v.x, v.y, v.z, v.w);
+    } else {
+        return *p;
+    }
 }
+template <bool NTL = true>
 __device__ inline double2 ks_load(const double2 *p) {
-#if SA_KS_NT
-    ks_d2 v = __builtin_nontemporal_load(reinterpret_cast<const ks_d2 *>(p));
-    return make_double2(v.x, v.y);
-#else
-    return *p;
-#endif
+    if constexpr (NTL && SA_KS_NT) {
+        ks_d2 v = __builtin_nontemporal_load(reinterpret_cast<const ks_d2 *>(p));
+        return make_double2(v.x, v.y);
+    } else {
+        return *p;
+    }
 }
 
 // a[r0 + k][c] += m.k * xv for the elements of one 16-byte load
@@ -1351,7 +1355,7 @@ struct KsUpd {
     Ctl *stop = nullptr;
 };
 
-template <int D, typename T, bool STEP = false>
+template <int D, typename T, bool STEP = false, bool NTL = true>
 __global__ __launch_bounds__(NT) void kstore_mv_k(int nr, int nc, const T *__restrict__ M,
                                                   const double *__restrict__ x, double *__restrict__ part,
                                                   const Ctl *ctl, Ctl *adv = nullptr,
@@ -1400,7 +1404,7 @@ __global__ __launch_bounds__(NT) void kstore_mv_k(int nr, int nc, const T *__res
             if (wv + k * W < ns)
 #pragma unroll
                 for (int v = 0; v < NV; ++v)
-                    m[k][v] = ks_load(reinterpret_cast<const V4 *>(base + (size_t)(wv + k * W) * PR + v * EV));
+                    m[k][v] = ks_load<NTL>(reinterpret_cast<const V4 *>(base + (size_t)(wv + k * W) * PR + v * EV));
     }
     // the staged input's operands (x; z and d_old; r and Ad), a thread's
     // columns j0 + s NT + threadIdx.x (s < SJ) requested together: the
@@ -1534,7 +1538,7 @@ __global__ __launch_bounds__(NT) void kstore_mv_k(int nr, int nc, const T *__res
                     if (jj + KP * W < ns)
 #pragma unroll
                         for (int v = 0; v < NV; ++v)
-                            m[k][v] = ks_load(reinterpret_cast<const V4 *>(base + (size_t)(jj + KP * W) * PR + v * EV));
+                            m[k][v] = ks_load<NTL>(reinterpret_cast<const V4 *>(base + (size_t)(jj + KP * W) * PR + v * EV));
                 }
             }
         }
@@ -1552,7 +1556,7 @@ __global__ __launch_bounds__(NT) void kstore_mv_k(int nr, int nc, const T *__res
 #pragma unroll
             for (int u = 0; u < U; ++u)
 #pragma unroll
-                for (int v = 0; v < NV; ++v) m[u][v] = ks_load(reinterpret_cast<const V4 *>(col + u * PR + v * EV));
+                for (int v = 0; v < NV; ++v) m[u][v] = ks_load<NTL>(reinterpret_cast<const V4 *>(col + u * PR + v * EV));
 #pragma unroll
             for (int u = 0; u < U; ++u)
 #pragma unroll
@@ -1569,7 +1573,7 @@ __global__ __launch_bounds__(NT) void kstore_mv_k(int nr, int nc, const T *__res
             for (int u = 0; u < U; ++u)
                 if (j + u < we)
 #pragma unroll
-                    for (int v = 0; v < NV; ++v) m[u][v] = ks_load(reinterpret_cast<const V4 *>(col + u * PR + v * EV));
+                    for (int v = 0; v < NV; ++v) m[u][v] = ks_load<NTL>(reinterpret_cast<const V4 *>(col + u * PR + v * EV));
 #pragma unroll
             for (int u = 0; u < U; ++u)
                 if (j + u < we)
