@@ -317,6 +317,13 @@ struct SaError : std::runtime_error {
 #define SA_PRE_NT 0
 #endif
 constexpr bool PRE_NT = SA_PRE_NT != 0;
+// all-landmark form: a stored K of at most this many MiB is read with
+// ordinary loads in the CG iterations (0: always non-temporal).  The
+// 2,000-landmark ring's 32 MB product: 11.2 against 12.7 us a launch; its
+// whole solve, launch-latency bound, 116.5-117.6 against 114.0-117.9 steps/s
+#ifndef SA_KST_CACHED_MB
+#define SA_KST_CACHED_MB 64
+#endif
 
 inline int gridOf(long n, int nt = NT) { return (int)((n + nt - 1) / nt); }
 
@@ -1298,8 +1305,15 @@ struct Engine : EngineBase {
         upd.rIn = rIn; upd.Ad = Ad.p; upd.d = d.p; upd.Kid = Kid.p;
         upd.rOut = rOut; upd.xs = x.p; upd.as = a.p; upd.Prr = P2.p; upd.stop = ctl.p;
         if (ev) CK(hipEventRecord(ev[0], s));
-        kstore_mv_k<D, double, true><<<g, NT, ks_lds<double>(V, slicesK), s>>>(
-            V, V, Kst.p, rIn, partS.p, ctl.p, nullptr, nullptr, nullptr, 0, nullptr, 0, nullptr, nullptr, P3.p, upd);
+        // a stored K of at most SA_KST_CACHED_MB is read with ordinary loads:
+        // it stays in the last-level cache from one iteration to the next
+        const bool cached = (double)V * V * sizeof(double) <= SA_KST_CACHED_MB * 1048576.0;
+#define SA_KSTEP(NTL)                                                                                    \
+    kstore_mv_k<D, double, true, NTL><<<g, NT, ks_lds<double>(V, slicesK), s>>>(                         \
+        V, V, Kst.p, rIn, partS.p, ctl.p, nullptr, nullptr, nullptr, 0, nullptr, 0, nullptr, nullptr, P3.p, upd)
+        if (cached) SA_KSTEP(false);
+        else SA_KSTEP(true);
+#undef SA_KSTEP
         CHECK_LAUNCH();
         if (ev) CK(hipEventRecord(ev[1], s));
         pcg_dir_stored_k<<<gN, NT, 0, s>>>(n, slicesK, partS.p, knlEps, P2.p, (int)g.y, P3.p, (int)(g.x * g.y),
